@@ -1,0 +1,327 @@
+#!/usr/bin/env python3
+"""Throughput of the eqtlbma_bf Bayes-factor engine on one MI355X.
+
+Workload (the batched-ABF configuration): 20,000 genes, 44 subgroups drawn
+from 500 individuals with uneven sizes N_s ~ U(80, 400), cis SNPs per gene
+~ Poisson(1500) clipped to [200, 6000], FP32 dosages E[g] + N(0, 0.05^2)
+clipped to [0, 2], 3 Gaussian covariates in the model, 15% eGenes,
+--analys join --bfs sin (consistent + 44 singleton configurations) on a
+50-point grid for both grids.  Synthetic data generated on the device;
+inputs are resident in HBM before the timed region.
+
+A step is one pass of the association engine (summary statistics, raw and
+grid-averaged Bayes factors for every pair) over the whole workload, issued
+as chunks of --batch-genes genes into reused output buffers.
+
+Also reported: the plain FP64 one-gene-per-launch path on a gene subset,
+the error of the mixed path against the FP64 CPU oracle on a sample, and the
+oracle's CPU throughput on that sample (cpu_baseline).
+"""
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+METRIC = "SNP-gene BFs/s + gene perm p-values/s on 1 MI355X vs plain GPU, at stated error"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--genes", type=int, default=20000)
+    ap.add_argument("--batch-genes", type=int, default=2000)
+    ap.add_argument("--plain-genes", type=int, default=60)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--seed", type=int, default=1001)
+    return ap.parse_args()
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+class Workload:
+    """Device-resident synthetic data of the batched configuration."""
+
+    def __init__(self, torch, G, seed, S=44, N=500, Q=3):
+        dev = torch.device("cuda:0")
+        self.torch, self.dev = torch, dev
+        self.G, self.S, self.N, self.Q = G, S, N, Q
+        rng = np.random.default_rng(seed)
+        P = np.clip(rng.poisson(1500, size=G), 200, 6000).astype(np.int64)
+        self.P = P
+        self.offs = np.concatenate([[0], np.cumsum(P)]).astype(np.int64)
+        n_snps = int(self.offs[-1])
+        self.n_snps = n_snps
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(seed)
+        t0 = time.time()
+        self.geno = torch.empty((N, n_snps), dtype=torch.float32, device=dev)
+        step = 1 << 20
+        for a in range(0, n_snps, step):
+            b = min(n_snps, a + step)
+            maf = torch.rand(b - a, device=dev, generator=gen) * 0.45 + 0.05
+            g = (torch.rand((N, b - a), device=dev, generator=gen) < maf).float()
+            g += (torch.rand((N, b - a), device=dev, generator=gen) < maf).float()
+            g += torch.randn((N, b - a), device=dev, generator=gen) * 0.05
+            self.geno[:, a:b] = g.clamp_(0.0, 2.0)
+            del g, maf
+        # membership: N_s ~ U(80, 400) individuals per subgroup
+        geno_in = np.zeros((S, N), np.uint8)
+        for s in range(S):
+            ns = rng.integers(80, 401)
+            geno_in[s, rng.choice(N, size=ns, replace=False)] = 1
+        self.geno_in = torch.from_numpy(geno_in).to(dev)
+        self.covars = torch.randn((S, Q, N), device=dev, generator=gen, dtype=torch.float64)
+        self.covar_count = torch.full((S,), Q, dtype=torch.int32, device=dev)
+        self.covar_in = torch.ones((S, N), dtype=torch.uint8, device=dev)
+        self.gene_in = torch.ones((G, S), dtype=torch.uint8, device=dev)
+        self.gset = torch.zeros((S,), dtype=torch.int32, device=dev)
+        self.snp_in = torch.ones((n_snps, S), dtype=torch.uint8, device=dev)
+        self.pair_snp = torch.arange(n_snps, dtype=torch.int32, device=dev)
+        # expression: noise + covariate effects + one eQTL in 15% of genes
+        y = torch.randn((G, S, N), device=dev, generator=gen, dtype=torch.float64)
+        y += 0.3 * self.covars.sum(dim=1)[None]
+        eg = rng.choice(G, size=int(round(0.15 * G)), replace=False)
+        for g in eg:
+            j = int(self.offs[g] + rng.integers(P[g]))
+            beta = rng.normal(0, 0.5)
+            act = rng.random(S) < 0.5
+            if not act.any():
+                act[rng.integers(S)] = True
+            idx = torch.from_numpy(np.nonzero(act)[0]).to(dev)
+            y[g, idx] += beta * self.geno[:, j].double()[None, :]
+        y[:, ~self.geno_in.bool()] = float("nan")
+        self.y = y
+        torch.cuda.synchronize()
+        log("generated %d genes, %d pairs, %.1f GB genotypes in %.1f s" %
+            (G, n_snps, self.geno.numel() * 4 / 1e9, time.time() - t0))
+        # per-chunk pair offsets (relative) on the device
+        self.batches = []
+
+    def make_batches(self, bg):
+        torch = self.torch
+        self.batches = []
+        for g0 in range(0, self.G, bg):
+            g1 = min(self.G, g0 + bg)
+            rel = self.offs[g0:g1 + 1] - self.offs[g0]
+            self.batches.append((g0, g1, torch.from_numpy(rel).to(self.dev)))
+        return max(int(self.offs[g1] - self.offs[g0]) for g0, g1, _ in self.batches)
+
+    def chunk(self, abi, g0, g1, rel_off):
+        c = abi.Chunk()
+        S, N = self.S, self.N
+        c.n_genes, c.n_subgroups, c.n_samples, c.n_covars_max = g1 - g0, S, N, self.Q
+        c.n_pairs = int(self.offs[g1] - self.offs[g0])
+        c.n_snps = self.n_snps
+        c.explevels = self.y.data_ptr() + g0 * S * N * 8
+        c.gene_in_sbgrp = self.gene_in.data_ptr() + g0 * S
+        c.geno_in_sbgrp = self.geno_in.data_ptr()
+        c.covar_count = self.covar_count.data_ptr()
+        c.covars = self.covars.data_ptr()
+        c.covar_in_sbgrp = self.covar_in.data_ptr()
+        c.n_geno_sets = 1
+        c.geno_set_of_sbgrp = self.gset.data_ptr()
+        c.genotypes = self.geno.data_ptr()
+        c.geno_dtype = abi.F32
+        c.geno_ld = self.n_snps
+        c.geno_set_stride = N * self.n_snps
+        c.snp_in_sbgrp = self.snp_in.data_ptr()
+        c.pair_offset = rel_off.data_ptr()
+        c.pair_snp = self.pair_snp.data_ptr() + int(self.offs[g0]) * 4
+        return c
+
+    def host_chunk(self, abi, g0, g1):
+        """Host copy of genes [g0, g1) with their own SNP columns."""
+        p0, p1 = int(self.offs[g0]), int(self.offs[g1])
+        return abi.HostChunk(
+            explevels=self.y[g0:g1].cpu().numpy(),
+            gene_in_sbgrp=self.gene_in[g0:g1].cpu().numpy(),
+            geno_in_sbgrp=self.geno_in.cpu().numpy(),
+            genotypes=np.ascontiguousarray(self.geno[:, p0:p1].cpu().numpy()),
+            snp_in_sbgrp=self.snp_in[p0:p1].cpu().numpy(),
+            pair_offset=self.offs[g0:g1 + 1] - p0,
+            pair_snp=np.arange(p1 - p0, dtype=np.int32),
+            covars=self.covars.cpu().numpy(), covar_count=self.covar_count.cpu().numpy(),
+            covar_in_sbgrp=self.covar_in.cpu().numpy())
+
+
+def alg_bytes(P_g, S, N, GL, GS):
+    """Algorithmic HBM bytes of the association pass for one gene: per pair
+    the genotype row (4N), the raw Bayes factors in FP32 (3 GL + S GS), the
+    grid-averaged ones in FP64 (3 + S + 1), plus the gene's expression
+    levels once (8 S N)."""
+    per_pair = 4 * N + 4 * (3 * GL + S * GS) + 8 * (3 + S + 1)
+    return P_g * per_pair + 8 * S * N
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the engine shards genes across GPUs with no "
+              "collective on the hot path (see DESIGN.md); --gpus %d is not run." % args.gpus)
+        sys.exit(2)
+    import torch
+    from eqtlbma_amd import _abi, synth
+    from eqtlbma_amd.engine import Engine, lib
+
+    grid = synth.make_grid("general", n_aes=10)  # 50 points: 10 effect sizes x 5 homogeneities
+    GL = GS = grid.shape[0]
+    wl = Workload(torch, args.genes, args.seed)
+    S, N = wl.S, wl.N
+    max_np = wl.make_batches(args.batch_genes)
+    n_raw = 3 * GL + S * GS
+    n_w = 3 + S + 2
+    dev = wl.dev
+    out_n = torch.empty((S, max_np), dtype=torch.int32, device=dev)
+    out_raw = torch.empty((n_raw, max_np), dtype=torch.float32, device=dev)
+    out_w = torch.empty((n_w, max_np), dtype=torch.float64, device=dev)
+    eng = Engine(0)
+    eng.set_grids(grid, grid)
+    opts = _abi.AssocOpts(_abi.ANALYSIS["join"], _abi.BFS["sin"], _abi.PREC["mixed"], _abi.F32)
+    o = _abi.AssocOut()
+    o.n, o.sstats, o.stdsstats = out_n.data_ptr(), None, None
+    o.raw_abf, o.w_abf = out_raw.data_ptr(), out_w.data_ptr()
+    chunks = [wl.chunk(_abi, g0, g1, rel) for g0, g1, rel in wl.batches]
+    stream = torch.cuda.current_stream()
+    sptr = C.c_void_p(stream.cuda_stream)
+
+    def step():
+        for c in chunks:
+            eng.assoc_uvlr_device(c, opts, o, sptr)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    L = lib()
+    L.eqbf_profile.argtypes = [C.c_void_p, C.c_int]
+    L.eqbf_profile_read.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]
+    L.eqbf_profile(eng._ctx, 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    L.eqbf_profile(eng._ctx, 0)
+    names = C.create_string_buffer(64 * 16)
+    tot = np.zeros(16)
+    cnt = np.zeros(16, np.int64)
+    nk = C.c_int32(0)
+    L.eqbf_profile_read(eng._ctx, names, tot.ctypes.data, cnt.ctypes.data, 16, C.byref(nk))
+    kern = {}
+    for k in range(nk.value):
+        nm = names.raw[64 * k:64 * k + 64].split(b"\0")[0].decode()
+        kern[nm] = (tot[k], int(cnt[k]))
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    pairs = int(wl.offs[-1])
+    value = pairs / (ms_step / 1e3)
+    total_bytes = sum(alg_bytes(int(p), S, N, GL, GS) for p in wl.P)
+    mk_ms, mk_n = kern.get("assoc_mixed_kernel", (float("nan"), 1))
+    launches_per_step = len(chunks)
+    bytes_per_launch = total_bytes / launches_per_step
+    avg_ms = mk_ms / max(mk_n, 1)
+    achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9
+    log("step %.2f ms, %.3e pairs/s; kernels: %s" % (ms_step, value, {k: (round(v[0] / max(v[1], 1), 3), v[1]) for k, v in kern.items()}))
+
+    result = {
+        "metric": METRIC, "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64 sums + f32 Bayes-factor terms (mixed)", "data": "synthetic",
+        "config": {"workload": "batched ABFs: %d genes, 44 subgroups of N_s~U(80,400) from 500 "
+                               "individuals, cis SNPs~Pois(1500) in [200,6000], FP32 dosages, 3 covariates, "
+                               "--bfs sin, 50-point grid" % args.genes,
+                   "pairs": pairs, "genes": args.genes, "subgroups": S, "samples": N,
+                   "grid_points": GL, "chunk_genes": args.batch_genes},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "assoc_mixed_kernel", "avg_launch_ms": avg_ms,
+                     "alg_bytes_per_launch": bytes_per_launch},
+        "kernels_ms_per_launch": {k: v[0] / max(v[1], 1) for k, v in kern.items()},
+    }
+
+    # ---- parity sample and CPU baseline (oracle, one core) ----
+    if not args.no_cpu_baseline:
+        from oracle import oracle as orc
+        k = 1
+        hc = wl.host_chunk(_abi, 0, k)
+        t = time.perf_counter()
+        ref = orc.assoc_uvlr(hc, "join", "sin", grid, grid)
+        dt = time.perf_counter() - t
+        per_gene = dt / k
+        k2 = max(1, min(40, int(args.cpu_seconds / max(per_gene, 1e-3))))
+        if k2 > k:
+            hc = wl.host_chunk(_abi, 0, k2)
+            t = time.perf_counter()
+            ref = orc.assoc_uvlr(hc, "join", "sin", grid, grid)
+            dt = time.perf_counter() - t
+            k = k2
+        cpu_pairs = hc.n_pairs
+        result["cpu_baseline"] = {"value": cpu_pairs / dt, "unit": "pairs/s", "cores": 1, "kind": "port",
+                                  "sample": "first %d genes of the workload (%d pairs), oracle FP64/long-double "
+                                            "restatement, single thread, %.1f s" % (k, cpu_pairs, dt)}
+        gpu = eng.assoc_uvlr(hc, analysis="join", bfs="sin", precision="mixed", raw_dtype=np.float32)
+        has = ref.n > 0
+        bet, se = ref.sstats[2], ref.sstats[3]
+        ok = has & ~np.isnan(bet)
+        e_beta = np.max(np.abs(gpu.sstats[2][ok] - bet[ok]) / np.maximum(np.abs(bet[ok]), se[ok]))
+        e_se = np.max(np.abs(gpu.sstats[3][ok] - se[ok]) / se[ok])
+        fin = np.isfinite(ref.raw)
+        e_raw = np.max(np.abs(gpu.raw.astype(np.float64)[fin] - ref.raw[fin]))
+        fin = np.isfinite(ref.w)
+        e_w = np.max(np.abs(gpu.w[fin] - ref.w[fin]))
+        pv = ref.sstats[4]
+        okp = ok & (pv > 1e-290)
+        e_p = np.max(np.abs(gpu.sstats[4][okp] - pv[okp]) / pv[okp])
+        result["error_vs_fp64_oracle"] = {
+            "sample_pairs": int(hc.n_pairs), "betahat_rel_to_max(|b|,se)": float(e_beta),
+            "se_rel": float(e_se), "pval_rel": float(e_p), "log10_abf_raw_abs": float(e_raw),
+            "log10_abf_weighted_abs": float(e_w)}
+        log("cpu oracle %.1f pairs/s; errors %s" % (cpu_pairs / dt, result["error_vs_fp64_oracle"]))
+
+    # ---- plain FP64 one-gene-per-launch path on a subset ----
+    if not args.no_plain and args.plain_genes > 0:
+        gp = min(args.plain_genes, wl.G)
+        rel = torch.from_numpy(wl.offs[:gp + 1].copy()).to(dev)
+        c = wl.chunk(_abi, 0, gp, rel)
+        popts = _abi.AssocOpts(_abi.ANALYSIS["join"], _abi.BFS["sin"], _abi.PREC["fp64_plain"], _abi.F32)
+        eng.assoc_uvlr_device(c, popts, o, sptr)  # warm
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        eng.assoc_uvlr_device(c, popts, o, sptr)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        plain_rate = c.n_pairs / dt
+        # mixed path on the same subset, same call shape
+        eng.assoc_uvlr_device(c, opts, o, sptr)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        eng.assoc_uvlr_device(c, opts, o, sptr)
+        torch.cuda.synchronize()
+        dtm = time.perf_counter() - t
+        result["plain_gpu"] = {"value": plain_rate, "unit": "pairs/s",
+                               "sample": "first %d genes (%d pairs), FP64, one launch per gene" % (gp, c.n_pairs),
+                               "mixed_same_sample": c.n_pairs / dtm,
+                               "speedup_mixed_vs_plain_same_sample": (c.n_pairs / dtm) / plain_rate,
+                               "speedup_full_run_vs_plain": value / plain_rate}
+        log("plain %.3e pairs/s" % plain_rate)
+
+    eng.close()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

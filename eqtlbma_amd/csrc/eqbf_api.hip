@@ -1,0 +1,108 @@
+/*
+ * C-ABI entry points of the association test (include/eqbf.h): the
+ * device-pointer variant enqueues on a stream; the host-pointer variant stages
+ * the chunk into context-owned device buffers, runs, and copies back.
+ */
+#include <cstring>
+
+#include "eqbf_internal.h"
+
+static int check_flag(eqbf_ctx* ctx, hipStream_t st) {
+  int flag = 0;
+  EQBF_HIP(ctx, hipMemcpyAsync(&flag, ctx->d_flag.ptr, sizeof(int), hipMemcpyDeviceToHost, st));
+  EQBF_HIP(ctx, hipStreamSynchronize(st));
+  if (flag & FLAG_MISSING_COVAR)
+    return ctx_fail(ctx, EQBF_ERR_DATA,
+                    "missing covariate for a sample with genotype and expression level "
+                    "(gene_snp_pair.cpp:138-144)");
+  return EQBF_OK;
+}
+
+extern "C" int eqbf_assoc_uvlr_device(eqbf_ctx* ctx, const eqbf_chunk* c, const eqbf_assoc_opts* o,
+                                      eqbf_assoc_out* out, void* stream) {
+  if (!ctx || !c || !o || !out) return ctx_fail(ctx, EQBF_ERR_ARG, "null argument");
+  EQBF_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  return assoc_uvlr_run(ctx, c, o, out, st);
+}
+
+namespace {
+struct Stager {
+  eqbf_ctx* ctx;
+  hipStream_t st;
+  int rc = EQBF_OK;
+  template <class T>
+  T* put(const char* name, const T* host, size_t count) {
+    if (!host || count == 0) return nullptr;
+    T* d = (T*)ctx_buf(ctx, name, count * sizeof(T));
+    if (!d) {
+      rc = EQBF_ERR_HIP;
+      return nullptr;
+    }
+    if (hipMemcpyAsync(d, host, count * sizeof(T), hipMemcpyHostToDevice, st) != hipSuccess) rc = EQBF_ERR_HIP;
+    return d;
+  }
+  template <class T>
+  T* alloc(const char* name, size_t count, bool want) {
+    if (!want || count == 0) return nullptr;
+    T* d = (T*)ctx_buf(ctx, name, count * sizeof(T));
+    if (!d) rc = EQBF_ERR_HIP;
+    return d;
+  }
+};
+}  // namespace
+
+extern "C" int eqbf_assoc_uvlr(eqbf_ctx* ctx, const eqbf_chunk* c, const eqbf_assoc_opts* o, eqbf_assoc_out* out) {
+  if (!ctx || !c || !o || !out) return ctx_fail(ctx, EQBF_ERR_ARG, "null argument");
+  EQBF_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const int64_t G = c->n_genes, S = c->n_subgroups, N = c->n_samples, NP = c->n_pairs;
+  const int64_t Qm = c->n_covars_max;
+  if (G < 0 || S < 1 || N < 1 || NP < 0) return ctx_fail(ctx, EQBF_ERR_ARG, "invalid chunk sizes");
+  const size_t esz = c->geno_dtype == EQBF_F32 ? 4 : 8;
+  const size_t geno_elems = (size_t)(c->n_geno_sets - 1) * c->geno_set_stride + (size_t)N * c->geno_ld;
+  Stager sg{ctx, st};
+  eqbf_chunk d = *c;
+  d.explevels = sg.put("h_expl", c->explevels, (size_t)(G * S * N));
+  d.gene_in_sbgrp = sg.put("h_genein", c->gene_in_sbgrp, (size_t)(G * S));
+  d.geno_in_sbgrp = sg.put("h_genoin", c->geno_in_sbgrp, (size_t)(S * N));
+  d.covar_count = sg.put("h_covcnt", c->covar_count, (size_t)S);
+  d.covars = Qm > 0 ? sg.put("h_covars", c->covars, (size_t)(S * Qm * N)) : nullptr;
+  d.covar_in_sbgrp = sg.put("h_covin", c->covar_in_sbgrp, (size_t)(S * N));
+  d.geno_set_of_sbgrp = sg.put("h_gset", c->geno_set_of_sbgrp, (size_t)S);
+  d.genotypes = sg.put("h_geno", (const uint8_t*)c->genotypes, geno_elems * esz);
+  d.snp_in_sbgrp = sg.put("h_snpin", c->snp_in_sbgrp, (size_t)(c->n_snps * S));
+  d.pair_offset = sg.put("h_poff", c->pair_offset, (size_t)(G + 1));
+  d.pair_snp = sg.put("h_psnp", c->pair_snp, (size_t)NP);
+  if (sg.rc) return ctx_fail(ctx, sg.rc, "staging the chunk failed");
+  const bool join = o->analysis == EQBF_ANALYSIS_JOIN;
+  const int GL = (int)ctx->phiL.size(), GS = (int)ctx->phiS.size();
+  int64_t n_raw = join ? eqbf_n_raw(o->bfs, (int)S, GL, GS) : 0;
+  int64_t n_w = join ? eqbf_n_weighted(o->bfs, (int)S) : 0;
+  const size_t rsz = o->raw_dtype == EQBF_F64 ? 8 : 4;
+  eqbf_assoc_out dout;
+  memset(&dout, 0, sizeof dout);
+  dout.n = sg.alloc<int32_t>("o_n", (size_t)(S * NP), true);
+  dout.sstats = sg.alloc<double>("o_ss", (size_t)(5 * S * NP), out->sstats != nullptr);
+  dout.stdsstats = sg.alloc<double>("o_std", (size_t)(3 * S * NP), out->stdsstats != nullptr);
+  dout.raw_abf = sg.alloc<uint8_t>("o_raw", (size_t)(n_raw * NP) * rsz, join && out->raw_abf != nullptr);
+  dout.w_abf = sg.alloc<double>("o_w", (size_t)(n_w * NP), join);
+  if (sg.rc) return ctx_fail(ctx, sg.rc, "allocating outputs failed");
+  int rc = assoc_uvlr_run(ctx, &d, o, &dout, st);
+  if (rc) return rc;
+  rc = check_flag(ctx, st);
+  if (rc) return rc;
+  if (NP > 0) {
+    EQBF_HIP(ctx, hipMemcpyAsync(out->n, dout.n, S * NP * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    if (out->sstats)
+      EQBF_HIP(ctx, hipMemcpyAsync(out->sstats, dout.sstats, 5 * S * NP * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (out->stdsstats)
+      EQBF_HIP(ctx, hipMemcpyAsync(out->stdsstats, dout.stdsstats, 3 * S * NP * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (join && out->raw_abf)
+      EQBF_HIP(ctx, hipMemcpyAsync(out->raw_abf, dout.raw_abf, n_raw * NP * rsz, hipMemcpyDeviceToHost, st));
+    if (join && out->w_abf)
+      EQBF_HIP(ctx, hipMemcpyAsync(out->w_abf, dout.w_abf, n_w * NP * sizeof(double), hipMemcpyDeviceToHost, st));
+  }
+  EQBF_HIP(ctx, hipStreamSynchronize(st));
+  return EQBF_OK;
+}

@@ -1,0 +1,984 @@
+/*
+ * Association tests with the uvlr error model: per-subgroup OLS summary
+ * statistics and Wen-Stephens Bayes factors for every gene-SNP pair of a
+ * chunk.
+ *
+ * Reference path (eqtlbma 1.3.3):
+ *   Gene::TestForAssociations           gene.cpp:263-332
+ *   GeneSnpPair::FillStlContainers      gene_snp_pair.cpp:79-170
+ *   GeneSnpPair::CalcSstatsOneSbgrp     gene_snp_pair.cpp:175-208
+ *   utils::FitSingleGeneWithSingleSnp   utils_math.cpp:166-209
+ *   StandardizeSstatsAndCorrectSmallSampleSize  gene_snp_pair.cpp:256-290
+ *   CalcLog10AbfUvlr / CalcAbfsUvlr*    gene_snp_pair.cpp:297-622
+ *   utils::log10_weighted_sum           utils_math.cpp:100-159
+ *
+ * Algebra.  For subgroup s of gene g the members are the samples with a
+ * genotype column in s and a non-missing expression value.  With U an
+ * orthonormal basis of [1, covariates] over the members and y~ = y - U U'y,
+ * the OLS fit of y on [1, covariates, x] has (Frisch-Waugh-Lovell)
+ *   x~'x~ = x'x - |U'x|^2,  betahat = x'y~ / x~'x~,
+ *   rss = y~'y~ - (x'y~)^2 / x~'x~,  se^2 = rss/(N-P) / x~'x~,
+ * so a pair needs only the sums x'x, x'y~ and U'x over the members.  The prep
+ * kernel builds U and y~ once per (gene, subgroup); the pair kernels stream
+ * the genotypes once and accumulate those sums in FP64.
+ *
+ * Two paths:
+ *   mixed (EQBF_PREC_MIXED): one persistent launch over a packed list of
+ *     64-pair tiles of every gene of the chunk, XCD-aware; FP64 sums and
+ *     residual sums of squares; t-tail by tabulated cubic Hermite in FP64;
+ *     Bayes-factor terms in FP32; grid averages with FP64 accumulation.
+ *   plain (EQBF_PREC_FP64_PLAIN): one launch per gene, one thread per pair,
+ *     every quantity in FP64 with the reference's loop structure.
+ */
+#include <algorithm>
+#include <cstring>
+
+#include "eqbf_internal.h"
+
+namespace {
+
+constexpr int NW = 4;     /* waves per workgroup in the mixed kernel */
+constexpr int TILE = 64;  /* pairs per tile (one per lane)           */
+constexpr float kHalfInvLn10 = 0.21714724095162590f; /* 1 / (2 ln 10) */
+
+enum { F_ABSENT = 0, F_NORMAL = 1, F_ZERO = 2, F_NAN = 3 };
+
+struct AssocParams {
+  const PrepRec* rec;
+  const double* B;       /* [G][N][S][KU+1]: y~, U columns              */
+  const uint64_t* memb;  /* [G][N] bit s = sample is a member of s      */
+  const int32_t* idx;    /* [G][S][N] compact member lists (plain path) */
+  const int4* tiles;     /* (gene, first pair, count, 0)                */
+  const int* n_tiles;
+  const double* tz;
+  const double* grid;    /* phiL, omaL, phiS, omaS                      */
+  const uint64_t* masks; /* configurations                              */
+  int32_t* n_out;
+  double* sstats;
+  double* stdsstats;
+  void* raw;
+  double* w;
+  int* flag;
+  int GL, GS, n_cfg, n_w, KU, bfs, join, raw_f64, multiset, CH;
+};
+
+__device__ inline double geno_load(const eqbf_chunk& c, int set, int64_t i, int64_t j) {
+  int64_t off = (int64_t)set * c.geno_set_stride + i * c.geno_ld + j;
+  if (c.geno_dtype == EQBF_F32) return (double)((const float*)c.genotypes)[off];
+  return ((const double*)c.genotypes)[off];
+}
+
+/* ---- block helpers ----------------------------------------------------------- */
+__device__ inline double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ double block_sum(double v, double* red) {
+  v = wave_sum(v);
+  int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) red[w] = v;
+  __syncthreads();
+  double t = 0;
+  for (int k = 0; k < (int)(blockDim.x >> 6); ++k) t += red[k];
+  return t;
+}
+
+__global__ void zero_u64_kernel(uint64_t* p, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
+/* ---- prep: one workgroup per (gene, subgroup) ---------------------------------- */
+__global__ void __launch_bounds__(256) prep_kernel(const eqbf_chunk c, PrepRec* __restrict__ rec,
+                                                   double* __restrict__ B, uint64_t* __restrict__ memb,
+                                                   int32_t* __restrict__ idxs, int* __restrict__ flag, int KU) {
+  __shared__ double red[8];
+  __shared__ int wcnt[4];
+  __shared__ int kept[17];
+  const int S = c.n_subgroups, N = c.n_samples;
+  const int g = blockIdx.x / S, s = blockIdx.x % S;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  PrepRec r;
+  r.n = 0;
+  r.flags = 0;
+  r.q = c.covar_count[s];
+  r.rank = 0;
+  r.yy = 0;
+  r.tss = 0;
+  if (!c.gene_in_sbgrp[(int64_t)g * S + s]) {
+    if (tid == 0) rec[(int64_t)g * S + s] = r;
+    return;
+  }
+  const double* y = c.explevels + ((int64_t)g * S + s) * N;
+  const uint8_t* gin = c.geno_in_sbgrp + (int64_t)s * N;
+  int32_t* idx = idxs + ((int64_t)g * S + s) * N;
+  /* membership and compaction in sample order (gene_snp_pair.cpp:98-116) */
+  int count = 0;
+  for (int base = 0; base < N; base += 256) {
+    int i = base + tid;
+    bool m = (i < N) && !isnan(y[i]) && gin[i];
+    uint64_t bal = __ballot(m);
+    int pre = __popcll(bal & ((1ull << lane) - 1ull));
+    if (lane == 0) wcnt[wv] = __popcll(bal);
+    __syncthreads();
+    int off = count;
+    for (int k = 0; k < wv; ++k) off += wcnt[k];
+    if (m) idx[off + pre] = i;
+    int tot = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+    __syncthreads();
+    count += tot;
+  }
+  __syncthreads();
+  const int n = count, Q = r.q;
+  r.n = n;
+  if (n == 0) {
+    if (tid == 0) rec[(int64_t)g * S + s] = r;
+    return;
+  }
+  if (Q > 0) {
+    int miss = 0;
+    for (int k = tid; k < n; k += 256) miss |= !c.covar_in_sbgrp[(int64_t)s * N + idx[k]];
+    if (__syncthreads_or(miss)) {
+      if (tid == 0) {
+        atomicOr(flag, FLAG_MISSING_COVAR);
+        r.n = 0;
+        rec[(int64_t)g * S + s] = r;
+      }
+      return;
+    }
+  }
+  if (n < Q + 3) { /* N < P + 1 with P = Q + 2 (utils_math.cpp:175-178) */
+    if (tid == 0) {
+      r.flags = PREP_TOO_FEW;
+      rec[(int64_t)g * S + s] = r;
+    }
+    return;
+  }
+  const int KC = KU + 1;
+  auto Bat = [&](int i, int col) -> double& { return B[(((int64_t)g * N + i) * S + s) * KC + col]; };
+  /* orthonormal basis of [1, covariates] by two-pass modified Gram-Schmidt */
+  int rank = 0;
+  for (int j = 0; j <= Q; ++j) {
+    double nn = 0;
+    for (int k = tid; k < n; k += 256) {
+      int i = idx[k];
+      double v = (j == 0) ? 1.0 : c.covars[((int64_t)s * c.n_covars_max + (j - 1)) * N + i];
+      Bat(i, 1 + j) = v;
+      nn += v * v;
+    }
+    double n0 = sqrt(block_sum(nn, red));
+    for (int pass = 0; pass < 2; ++pass)
+      for (int l = 0; l < j; ++l) {
+        if (!kept[l]) continue;
+        double d = 0;
+        for (int k = tid; k < n; k += 256) {
+          int i = idx[k];
+          d += Bat(i, 1 + l) * Bat(i, 1 + j);
+        }
+        d = block_sum(d, red);
+        for (int k = tid; k < n; k += 256) {
+          int i = idx[k];
+          Bat(i, 1 + j) -= d * Bat(i, 1 + l);
+        }
+      }
+    double nv = 0;
+    for (int k = tid; k < n; k += 256) {
+      double v = Bat(idx[k], 1 + j);
+      nv += v * v;
+    }
+    nv = sqrt(block_sum(nv, red));
+    bool keep = nv > 1e-9 * n0 && nv > 0;
+    double sc = keep ? 1.0 / nv : 0.0;
+    for (int k = tid; k < n; k += 256) Bat(idx[k], 1 + j) *= sc;
+    if (tid == 0) kept[j] = keep;
+    rank += keep;
+    __syncthreads();
+  }
+  for (int j = Q + 1; j < KU; ++j)
+    for (int k = tid; k < n; k += 256) Bat(idx[k], 1 + j) = 0.0;
+  /* y~ = y - U U'y (two passes), y~'y~, and the total sum of squares */
+  double sy = 0;
+  for (int k = tid; k < n; k += 256) {
+    int i = idx[k];
+    Bat(i, 0) = y[i];
+    sy += y[i];
+  }
+  double ybar = block_sum(sy, red) / n;
+  double ts = 0;
+  for (int k = tid; k < n; k += 256) {
+    double d = y[idx[k]] - ybar;
+    ts += d * d;
+  }
+  double tss = block_sum(ts, red);
+  for (int pass = 0; pass < 2; ++pass)
+    for (int l = 0; l <= Q; ++l) {
+      if (!kept[l]) continue;
+      double d = 0;
+      for (int k = tid; k < n; k += 256) {
+        int i = idx[k];
+        d += Bat(i, 1 + l) * Bat(i, 0);
+      }
+      d = block_sum(d, red);
+      for (int k = tid; k < n; k += 256) {
+        int i = idx[k];
+        Bat(i, 0) -= d * Bat(i, 1 + l);
+      }
+    }
+  double yy = 0;
+  for (int k = tid; k < n; k += 256) {
+    double v = Bat(idx[k], 0);
+    yy += v * v;
+  }
+  yy = block_sum(yy, red);
+  for (int k = tid; k < n; k += 256) atomicOr((unsigned long long*)&memb[(int64_t)g * N + idx[k]], 1ull << s);
+  if (tid == 0) {
+    r.rank = rank;
+    r.yy = yy;
+    r.tss = tss;
+    rec[(int64_t)g * S + s] = r;
+  }
+}
+
+/* ---- packed work list: tiles of up to 64 pairs of one gene ---------------------- */
+__global__ void __launch_bounds__(1024) tiles_kernel(const int64_t* __restrict__ pair_offset, int G,
+                                                     int4* __restrict__ tiles, int* __restrict__ n_tiles) {
+  __shared__ int part[1024];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < G; base += 1024) {
+    int g = base + threadIdx.x;
+    int nt = 0;
+    if (g < G) nt = (int)((pair_offset[g + 1] - pair_offset[g] + TILE - 1) / TILE);
+    part[threadIdx.x] = nt;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) { /* inclusive scan */
+      int v = (threadIdx.x >= (unsigned)o) ? part[threadIdx.x - o] : 0;
+      __syncthreads();
+      part[threadIdx.x] += v;
+      __syncthreads();
+    }
+    int start = carry + part[threadIdx.x] - nt;
+    if (g < G) {
+      int64_t p0 = pair_offset[g], p1 = pair_offset[g + 1];
+      for (int k = 0; k < nt; ++k) {
+        int64_t a = p0 + (int64_t)k * TILE;
+        int cnt = (int)min((int64_t)TILE, p1 - a);
+        tiles[start + k] = make_int4(g, (int)a, cnt, 0);
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += part[1023];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *n_tiles = carry;
+}
+
+/* ---- Bayes factor of one configuration at one grid point (FP32) ---------------- */
+/* Follows CalcLog10AbfUvlr (gene_snp_pair.cpp:297-356): subgroups with
+ * |t| < 1e-8 are skipped and absent ones are inactive.  An active subgroup
+ * whose statistics are NaN makes the variance sum NaN, so the reference's
+ * test `varbbarhat < infinity` fails and the ABF is 0 (:336-354). */
+__device__ inline float abf_eval(uint64_t mask, int S, const float* sv, const float* sbp, const float* sz2,
+                                 const float* slv, const uint8_t* sfl, int lane, float phi2, float oma2) {
+  float L = 0.f, A = 0.f, Bs = 0.f;
+  bool nan = false;
+  for (int s = 0; s < S; ++s) {
+    if (!((mask >> s) & 1ull)) continue;
+    int f = sfl[s * TILE + lane];
+    if (f == F_ABSENT || f == F_ZERO) continue;
+    if (f == F_NAN) {
+      nan = true;
+      break;
+    }
+    float v = sv[s * TILE + lane];
+    float vp = v + phi2;
+    float r = __builtin_amdgcn_rcpf(vp);
+    L += 0.5f * (slv[s * TILE + lane] - __log2f(vp) * 0.30102999566398120f) + kHalfInvLn10 * sz2[s * TILE + lane] * phi2 * r;
+    A += sbp[s * TILE + lane] * r;
+    Bs += r;
+  }
+  if (nan) return 0.f;
+  if (Bs == 0.f) return 0.f;
+  float bbar = A / Bs;
+  if (bbar == 0.f) return 0.f;
+  float V = 1.f / Bs;
+  float T2 = bbar * bbar / V;
+  float lbar = 0.f;
+  if (T2 != 0.f) {
+    float Vo = V + oma2;
+    lbar = 0.5f * (__log2f(V) - __log2f(Vo)) * 0.30102999566398120f + kHalfInvLn10 * T2 * oma2 / Vo;
+  }
+  return lbar + L;
+}
+
+/* Online log10 of the mean of 10^v (utils::log10_weighted_sum semantics:
+ * NaN when the first value is NaN, later NaNs skipped). */
+struct L10Acc {
+  float m;
+  double s;
+  bool first, nan0;
+  __device__ void init() {
+    m = 0.f;
+    s = 0.0;
+    first = true;
+    nan0 = false;
+  }
+  __device__ void add(float v) {
+    if (first) {
+      first = false;
+      if (isnan(v)) {
+        nan0 = true;
+        return;
+      }
+      m = v;
+      s = 1.0;
+      return;
+    }
+    if (isnan(v)) return;
+    if (v > m) {
+      s = s * (double)exp2f((m - v) * 3.32192809488736235f) + 1.0;
+      m = v;
+    } else {
+      s += (double)exp2f((v - m) * 3.32192809488736235f);
+    }
+  }
+};
+
+__device__ inline double l10_finish(float m, double s, bool nan, double weight) {
+  if (nan) return NAN;
+  double res = (double)m + log10(s * weight);
+  if (fabs(res) <= 2.220446049250313e-16) res = 0.0;
+  return res;
+}
+
+/* ---- mixed path: persistent kernel over tiles ----------------------------------- */
+template <int SPW, int KU>
+__global__ void __launch_bounds__(256) assoc_mixed_kernel(const eqbf_chunk c, const AssocParams kp) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int S = c.n_subgroups, N = c.n_samples;
+  const int64_t NP = c.n_pairs;
+  const int KC = KU + 1;
+  const int CH = kp.CH;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  /* LDS: during the sum phase two staging buffers of CH samples of B rows;
+   * afterwards the standardised statistics of the tile and the partial grid
+   * sums of the consistent configurations. */
+  double* bstage = (double*)smem;
+  const int stage_elems = CH * S * KC;
+  float* sv = (float*)smem;
+  float* sbp = sv + S * TILE;
+  float* sz2 = sbp + S * TILE;
+  float* slv = sz2 + S * TILE;
+  uint8_t* sfl = (uint8_t*)(slv + S * TILE);
+  double* part = (double*)(smem + (((size_t)S * TILE * 17 + 15) / 16) * 16); /* [3][4][2][64] */
+
+  const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3, per_xcd = gridDim.x >> 3;
+  const int T = *kp.n_tiles;
+  const int span = (T + 7) / 8;
+  const int t0 = xcd * span, t1 = min(T, t0 + span);
+  const double* grid = kp.grid;
+  const double *phiL = grid, *omaL = grid + kp.GL, *phiS = grid + 2 * kp.GL, *omaS = phiS + kp.GS;
+
+  for (int t = t0 + slot; t < t1; t += per_xcd) {
+    const int4 tile = kp.tiles[t];
+    const int g = tile.x;
+    const int64_t p0 = tile.y;
+    const int cnt = tile.z;
+    const bool valid = lane < cnt;
+    const int64_t p = p0 + (valid ? lane : 0);
+    const int64_t snp = c.pair_snp[p];
+    uint64_t wmask = 0;
+#pragma unroll
+    for (int sl = 0; sl < SPW; ++sl) {
+      int s = w + NW * sl;
+      if (s < S) wmask |= 1ull << s;
+    }
+    double acc[SPW][KU + 2];
+#pragma unroll
+    for (int sl = 0; sl < SPW; ++sl)
+#pragma unroll
+      for (int k = 0; k < KU + 2; ++k) acc[sl][k] = 0.0;
+
+    const double* Bg = kp.B + (int64_t)g * N * S * KC;
+    const uint64_t* mg = kp.memb + (int64_t)g * N;
+    const int nchunk = (N + CH - 1) / CH;
+    /* stage chunk 0 */
+    {
+      int lim = min(CH, N) * S * KC;
+      for (int e = tid; e < lim; e += 256) bstage[e] = Bg[e];
+    }
+    __syncthreads();
+    for (int ck = 0; ck < nchunk; ++ck) {
+      const int i0 = ck * CH;
+      const int ni = min(CH, N - i0);
+      double* cur = bstage + (ck & 1) * stage_elems;
+      double* nxt = bstage + ((ck + 1) & 1) * stage_elems;
+      /* prefetch the next chunk into registers */
+      double pre[8];
+      int nnext = 0;
+      if (ck + 1 < nchunk) {
+        nnext = min(CH, N - i0 - CH) * S * KC;
+        const double* src = Bg + (int64_t)(i0 + CH) * S * KC;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          int e = tid + u * 256;
+          pre[u] = (e < nnext) ? src[e] : 0.0;
+        }
+      }
+      for (int ii = 0; ii < ni; ++ii) {
+        const int i = i0 + ii;
+        const uint64_t m = mg[i] & wmask;
+        if (!m) continue;
+        double x = geno_load(c, kp.multiset ? 0 : 0, i, snp);
+#pragma unroll
+        for (int sl = 0; sl < SPW; ++sl) {
+          const int s = w + NW * sl;
+          if ((m >> s) & 1ull) {
+            double xs = x;
+            if (kp.multiset) xs = geno_load(c, c.geno_set_of_sbgrp[s], i, snp);
+            const double* b = cur + (ii * S + s) * KC;
+            acc[sl][0] += xs * xs;
+            acc[sl][1] += xs * b[0];
+#pragma unroll
+            for (int q = 0; q < KU; ++q) acc[sl][2 + q] += xs * b[1 + q];
+          }
+        }
+      }
+      __syncthreads();
+      if (ck + 1 < nchunk) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          int e = tid + u * 256;
+          if (e < nnext) nxt[e] = pre[u];
+        }
+        /* chunks larger than 8*256 elements are copied directly */
+        const double* src = Bg + (int64_t)(i0 + CH) * S * KC;
+        for (int e = tid + 8 * 256; e < nnext; e += 256) nxt[e] = src[e];
+      }
+      __syncthreads();
+    }
+    /* ---- per (pair, subgroup) statistics (FP64) ---- */
+#pragma unroll
+    for (int sl = 0; sl < SPW; ++sl) {
+      const int s = w + NW * sl;
+      if (s >= S) continue;
+      const PrepRec r = kp.rec[(int64_t)g * S + s];
+      const bool has = r.n > 0 && c.snp_in_sbgrp[snp * S + s];
+      int f = F_ABSENT;
+      double o5[5] = {NAN, NAN, NAN, NAN, NAN};
+      double st3[3] = {NAN, NAN, NAN};
+      int nout = 0;
+      if (has) {
+        nout = r.n;
+        f = F_NAN;
+        if (!(r.flags & PREP_TOO_FEW)) {
+          const double sxx = acc[sl][0], sxy = acc[sl][1];
+          double proj = 0;
+#pragma unroll
+          for (int q = 0; q < KU; ++q) proj += acc[sl][2 + q] * acc[sl][2 + q];
+          const double xx = sxx - proj;
+          const int n = r.n, P = r.q + 2;
+          if (xx > 1e-12 * sxx) {
+            const double beta = sxy / xx;
+            const double rss = r.yy - sxy * beta;
+            const int rank = r.rank + 1;
+            const double s2 = rss / (double)(n - P);
+            const double se = sqrt(s2 / xx);
+            const double sig = sqrt(rss / (double)(n - rank));
+            const double tstat = beta / se;
+            const double wv = dm::tz_eval(kp.tz, n - rank, fabs(tstat));
+            const double wz = (n - rank == n - P) ? wv : dm::tz_eval(kp.tz, n - P, fabs(tstat));
+            o5[0] = 1.0 - rss / r.tss;
+            o5[1] = sig;
+            o5[2] = beta;
+            o5[3] = se;
+            o5[4] = 2.0 * exp(-wv);
+            /* StandardizeSstatsAndCorrectSmallSampleSize */
+            const double z = dm::gauss_pinv_from_w(wz);
+            const double bhat = beta / sig, sebhat = se / sig;
+            if (!isnan(bhat / sebhat)) {
+              if (fabs(z) > 1e-8) {
+                const double sig2 = fabs(beta) / (fabs(z) * sebhat);
+                const double b2 = beta / sig2;
+                st3[0] = b2;
+                st3[1] = fabs(b2 / z);
+                st3[2] = z;
+                f = F_NORMAL;
+              } else {
+                st3[0] = 0.0;
+                st3[1] = INFINITY;
+                st3[2] = z;
+                f = F_ZERO;
+              }
+            }
+          } else { /* genotype collinear with [1, covariates]: not estimable */
+            o5[0] = 1.0 - r.yy / r.tss;
+            o5[1] = sqrt(r.yy / (double)(n - r.rank));
+          }
+        }
+      }
+      if (valid) {
+        kp.n_out[(int64_t)s * NP + p] = nout;
+        if (kp.sstats)
+          for (int k = 0; k < 5; ++k) kp.sstats[((int64_t)k * S + s) * NP + p] = o5[k];
+        if (kp.stdsstats)
+          for (int k = 0; k < 3; ++k) kp.stdsstats[((int64_t)k * S + s) * NP + p] = st3[k];
+      }
+      float v = (float)(st3[1] * st3[1]);
+      sv[s * TILE + lane] = v;
+      sbp[s * TILE + lane] = (float)st3[0];
+      sz2[s * TILE + lane] = (float)(st3[2] * st3[2]);
+      slv[s * TILE + lane] = (f == F_NORMAL) ? __log2f(v) * 0.30102999566398120f : 0.f;
+      sfl[s * TILE + lane] = (uint8_t)f;
+    }
+    __syncthreads();
+    if (kp.join) {
+      /* ---- Bayes factors: 12 quarter-jobs of the consistent configuration
+       * (gen, gen-fix, gen-maxh x 4 grid ranges) then one job per
+       * configuration (CalcAbfsUvlrFor*, gene_snp_pair.cpp:364-550). ---- */
+      const uint64_t all = (S >= 64) ? ~0ull : ((1ull << S) - 1ull);
+      const int njobs = 12 + kp.n_cfg;
+      for (int j = w; j < njobs; j += NW) {
+        L10Acc a;
+        a.init();
+        if (j < 12) {
+          const int type = j >> 2, qtr = j & 3;
+          const int ga = (kp.GL * qtr) >> 2, gb = (kp.GL * (qtr + 1)) >> 2;
+          for (int gi = ga; gi < gb; ++gi) {
+            float ph = (float)phiL[gi], om = (float)omaL[gi];
+            float p2 = type == 0 ? ph : (type == 1 ? 0.f : (float)(phiL[gi] + omaL[gi]));
+            float o2 = type == 0 ? om : (type == 1 ? (float)(phiL[gi] + omaL[gi]) : 0.f);
+            float l = abf_eval(all, S, sv, sbp, sz2, slv, sfl, lane, p2, o2);
+            if (valid && kp.raw) {
+              int64_t ro = (int64_t)(type * kp.GL + gi) * NP + p;
+              if (kp.raw_f64)
+                ((double*)kp.raw)[ro] = l;
+              else
+                ((float*)kp.raw)[ro] = l;
+            }
+            a.add(l);
+          }
+          double* pp = part + ((type * 4 + qtr) * 2) * TILE;
+          /* partial: max (NaN flags the first value NaN) and rescaled sum */
+          pp[lane] = a.first ? -INFINITY : (a.nan0 ? NAN : (double)a.m);
+          pp[TILE + lane] = a.s;
+        } else {
+          const int cfg = j - 12;
+          const uint64_t mk = kp.masks[cfg];
+          for (int gi = 0; gi < kp.GS; ++gi) {
+            float l = abf_eval(mk, S, sv, sbp, sz2, slv, sfl, lane, (float)phiS[gi], (float)omaS[gi]);
+            if (valid && kp.raw) {
+              int64_t ro = (int64_t)(3 * kp.GL + cfg * kp.GS + gi) * NP + p;
+              if (kp.raw_f64)
+                ((double*)kp.raw)[ro] = l;
+              else
+                ((float*)kp.raw)[ro] = l;
+            }
+            a.add(l);
+          }
+          if (valid) kp.w[(int64_t)(3 + cfg) * NP + p] = l10_finish(a.m, a.s, a.nan0, 1.0 / kp.GS);
+        }
+      }
+      __syncthreads();
+      if (w < 3) { /* combine the four grid ranges of gen / gen-fix / gen-maxh */
+        const int type = w;
+        double M = -INFINITY;
+        bool nan = false;
+        for (int q = 0; q < 4; ++q) {
+          double m = part[((type * 4 + q) * 2) * TILE + lane];
+          if (isnan(m)) nan = true;
+          else if (m > M) M = m;
+        }
+        double ssum = 0;
+        for (int q = 0; q < 4; ++q) {
+          double m = part[((type * 4 + q) * 2) * TILE + lane];
+          double sq = part[((type * 4 + q) * 2 + 1) * TILE + lane];
+          if (!isnan(m) && m > -INFINITY) ssum += sq * (double)exp2f((float)(m - M) * 3.32192809488736235f);
+        }
+        if (valid) kp.w[(int64_t)type * NP + p] = l10_finish((float)M, ssum, nan, 1.0 / kp.GL);
+      }
+    }
+    __syncthreads(); /* the next tile's staging reuses this LDS */
+  }
+}
+
+/* gen-sin (CalcBMAlite, gene_snp_pair.cpp:552-570) and all (CalcBMA
+ * :572-602) from the grid-averaged values; one thread per pair, FP64. */
+__global__ void bma_kernel(double* __restrict__ w, int64_t NP, int S, int n_cfg, int bfs,
+                           const uint64_t* __restrict__ masks, const double* __restrict__ inv_choose) {
+  int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= NP) return;
+  if (bfs == EQBF_BFS_GEN) { /* not computed with --bfs gen */
+    w[(int64_t)3 * NP + p] = NAN;
+    w[(int64_t)4 * NP + p] = NAN;
+    return;
+  }
+  /* BMAlite: singletons 1..S then gen, weights 1/(2S) and 1/2 */
+  {
+    double v0 = w[3 * NP + p];
+    double m = v0;
+    for (int s = 0; s < S; ++s) {
+      double v = w[(int64_t)(3 + s) * NP + p];
+      if (v > m) m = v;
+    }
+    double vg = w[p];
+    if (vg > m) m = vg;
+    double sum = 0;
+    for (int s = 0; s < S; ++s) {
+      double v = w[(int64_t)(3 + s) * NP + p];
+      if (!isnan(v)) sum += (0.5 / S) * exp10(v - m);
+    }
+    if (!isnan(vg)) sum += 0.5 * exp10(vg - m);
+    double res = m + log10(sum);
+    if (fabs(res) <= 2.220446049250313e-16) res = 0.0;
+    w[(int64_t)(3 + n_cfg) * NP + p] = res;
+  }
+  if (bfs == EQBF_BFS_ALL) {
+    double m = w[3 * NP + p];
+    for (int k = 0; k < n_cfg; ++k) {
+      double v = w[(int64_t)(3 + k) * NP + p];
+      if (v > m) m = v;
+    }
+    double sum = 0;
+    for (int k = 0; k < n_cfg; ++k) {
+      double v = w[(int64_t)(3 + k) * NP + p];
+      if (!isnan(v)) sum += (1.0 / S) * inv_choose[__popcll(masks[k])] * exp10(v - m);
+    }
+    double res = m + log10(sum);
+    if (fabs(res) <= 2.220446049250313e-16) res = 0.0;
+    w[(int64_t)(3 + n_cfg + 1) * NP + p] = res;
+  } else {
+    w[(int64_t)(3 + n_cfg + 1) * NP + p] = NAN;
+  }
+}
+
+/* ---- plain FP64 path: one launch per gene, one thread per pair ------------------ */
+__device__ double abf_plain(uint64_t mask, int S, const double* st_b, const double* st_se, const double* st_t,
+                            const uint8_t* has, double phi2, double oma2) {
+  /* CalcLog10AbfUvlr with the reference's expressions */
+  double bn = 0, bd = 0, vb = 0, single = 0;
+  bool any_single = false;
+  for (int s = 0; s < S; ++s) {
+    if (!((mask >> s) & 1ull) || !has[s]) continue;
+    double bhat = st_b[s], varbhat = pow(st_se[s], 2), t = st_t[s];
+    double l;
+    if (fabs(t) < 1e-8) {
+      l = 0;
+    } else {
+      bn += bhat / (varbhat + phi2);
+      bd += 1 / (varbhat + phi2);
+      vb += 1 / (varbhat + phi2);
+      l = 0.5 * log10(varbhat) - 0.5 * log10(varbhat + phi2) + (0.5 * pow(t, 2) * phi2 / (varbhat + phi2)) / log(10.0);
+    }
+    single += l;
+    any_single = true;
+  }
+  (void)any_single;
+  double bbar = (bd != 0.0) ? bn / bd : 0.0;
+  vb = (vb != 0.0) ? 1 / vb : INFINITY;
+  if (bbar != 0.0 && vb < INFINITY) {
+    double T2 = pow(bbar, 2.0) / vb;
+    double lb = (T2 != 0) ? 0.5 * log10(vb) - 0.5 * log10(vb + oma2) + (0.5 * T2 * oma2 / (vb + oma2)) / log(10.0) : 0;
+    return lb + single;
+  }
+  return 0.0;
+}
+
+__device__ double l10ws_plain(const double* v, int n) {
+  double mx = v[0];
+  for (int i = 0; i < n; ++i)
+    if (v[i] > mx) mx = v[i];
+  double w = 1 / ((double)n), sum = 0;
+  for (int i = 0; i < n; ++i)
+    if (!isnan(v[i])) sum += w * pow(10.0, v[i] - mx);
+  double res = mx + log10(sum);
+  if (fabs(res) <= 2.220446049250313e-16) res = 0.0;
+  return res;
+}
+
+#define PLAIN_SMAX 64
+#define PLAIN_GMAX 128
+
+__global__ void __launch_bounds__(64) assoc_plain_kernel(const eqbf_chunk c, const AssocParams kp, int g) {
+  const int S = c.n_subgroups, N = c.n_samples;
+  const int64_t NP = c.n_pairs;
+  const int64_t p = c.pair_offset[g] + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= c.pair_offset[g + 1]) return;
+  const int64_t snp = c.pair_snp[p];
+  const int KC = kp.KU + 1;
+  double st_b[PLAIN_SMAX], st_se[PLAIN_SMAX], st_t[PLAIN_SMAX];
+  uint8_t has[PLAIN_SMAX];
+  for (int s = 0; s < S; ++s) {
+    const PrepRec r = kp.rec[(int64_t)g * S + s];
+    has[s] = r.n > 0 && c.snp_in_sbgrp[snp * S + s];
+    st_b[s] = st_se[s] = st_t[s] = NAN;
+    double o5[5] = {NAN, NAN, NAN, NAN, NAN};
+    if (has[s] && !(r.flags & PREP_TOO_FEW)) {
+      const int32_t* idx = kp.idx + ((int64_t)g * S + s) * N;
+      double sxx = 0, sxy = 0, a[17];
+      for (int q = 0; q < kp.KU; ++q) a[q] = 0;
+      const int set = c.geno_set_of_sbgrp[s];
+      for (int k = 0; k < r.n; ++k) {
+        const int i = idx[k];
+        const double x = geno_load(c, set, i, snp);
+        const double* b = kp.B + (((int64_t)g * N + i) * S + s) * KC;
+        sxx += x * x;
+        sxy += x * b[0];
+        for (int q = 0; q < kp.KU; ++q) a[q] += x * b[1 + q];
+      }
+      double proj = 0;
+      for (int q = 0; q < kp.KU; ++q) proj += a[q] * a[q];
+      const double xx = sxx - proj;
+      const int n = r.n, P = r.q + 2;
+      if (xx > 1e-12 * sxx) {
+        const double beta = sxy / xx;
+        const double rss = r.yy - sxy * beta;
+        const int rank = r.rank + 1;
+        const double se = sqrt(rss / (double)(n - P) / xx);
+        const double sig = sqrt(rss / (double)(n - rank));
+        o5[0] = 1.0 - rss / r.tss;
+        o5[1] = sig;
+        o5[2] = beta;
+        o5[3] = se;
+        o5[4] = 2.0 * dm::t_lower_neg(fabs(beta / se), (double)(n - rank));
+        /* StandardizeSstatsAndCorrectSmallSampleSize (gene_snp_pair.cpp:256-290) */
+        double bhat = beta / sig, sebhat = se / sig, t = bhat / sebhat;
+        if (!isnan(t)) {
+          double nu = (double)n - 2 - r.q;
+          t = dm::gauss_pinv(dm::t_lower_neg(fabs(bhat / sebhat), nu));
+          if (fabs(t) > 1e-8) {
+            double sig2 = fabs(beta) / (fabs(t) * sebhat);
+            bhat = beta / sig2;
+            sebhat = fabs(bhat / t);
+          } else {
+            bhat = 0;
+            sebhat = INFINITY;
+          }
+          st_b[s] = bhat;
+          st_se[s] = sebhat;
+          st_t[s] = t;
+        }
+      } else {
+        o5[0] = 1.0 - r.yy / r.tss;
+        o5[1] = sqrt(r.yy / (double)(n - r.rank));
+      }
+    }
+    kp.n_out[(int64_t)s * NP + p] = has[s] ? r.n : 0;
+    if (kp.sstats)
+      for (int k = 0; k < 5; ++k) kp.sstats[((int64_t)k * S + s) * NP + p] = o5[k];
+    if (kp.stdsstats) {
+      kp.stdsstats[((int64_t)0 * S + s) * NP + p] = st_b[s];
+      kp.stdsstats[((int64_t)1 * S + s) * NP + p] = st_se[s];
+      kp.stdsstats[((int64_t)2 * S + s) * NP + p] = st_t[s];
+    }
+  }
+  if (!kp.join) return;
+  const double* phiL = kp.grid;
+  const double* omaL = phiL + kp.GL;
+  const double* phiS = phiL + 2 * kp.GL;
+  const double* omaS = phiS + kp.GS;
+  const uint64_t all = (S >= 64) ? ~0ull : ((1ull << S) - 1ull);
+  double buf[PLAIN_GMAX];
+  for (int type = 0; type < 3; ++type) {
+    for (int gi = 0; gi < kp.GL; ++gi) {
+      double p2 = type == 0 ? phiL[gi] : (type == 1 ? 0.0 : phiL[gi] + omaL[gi]);
+      double o2 = type == 0 ? omaL[gi] : (type == 1 ? phiL[gi] + omaL[gi] : 0.0);
+      buf[gi] = abf_plain(all, S, st_b, st_se, st_t, has, p2, o2);
+      if (kp.raw) {
+        int64_t ro = (int64_t)(type * kp.GL + gi) * NP + p;
+        if (kp.raw_f64)
+          ((double*)kp.raw)[ro] = buf[gi];
+        else
+          ((float*)kp.raw)[ro] = (float)buf[gi];
+      }
+    }
+    kp.w[(int64_t)type * NP + p] = l10ws_plain(buf, kp.GL);
+  }
+  for (int cfg = 0; cfg < kp.n_cfg; ++cfg) {
+    const uint64_t mk = kp.masks[cfg];
+    bool any = false;
+    for (int s = 0; s < S; ++s) any |= ((mk >> s) & 1ull) && has[s];
+    for (int gi = 0; gi < kp.GS; ++gi) {
+      buf[gi] = any ? abf_plain(mk, S, st_b, st_se, st_t, has, phiS[gi], omaS[gi]) : 0.0;
+      if (kp.raw) {
+        int64_t ro = (int64_t)(3 * kp.GL + cfg * kp.GS + gi) * NP + p;
+        if (kp.raw_f64)
+          ((double*)kp.raw)[ro] = buf[gi];
+        else
+          ((float*)kp.raw)[ro] = (float)buf[gi];
+      }
+    }
+    kp.w[(int64_t)(3 + cfg) * NP + p] = l10ws_plain(buf, kp.GS);
+  }
+}
+
+template <int SPW, int KU>
+void launch_mixed(const eqbf_chunk& c, const AssocParams& kp, int grid, size_t lds, hipStream_t st) {
+  hipLaunchKernelGGL((assoc_mixed_kernel<SPW, KU>), dim3(grid), dim3(256), lds, st, c, kp);
+}
+
+template <int KU>
+int dispatch_spw(int spw, const eqbf_chunk& c, const AssocParams& kp, int grid, size_t lds, hipStream_t st) {
+  switch (spw) {
+    case 1: launch_mixed<1, KU>(c, kp, grid, lds, st); return 0;
+    case 2: launch_mixed<2, KU>(c, kp, grid, lds, st); return 0;
+    case 4: launch_mixed<4, KU>(c, kp, grid, lds, st); return 0;
+    case 8: launch_mixed<8, KU>(c, kp, grid, lds, st); return 0;
+    case 12: launch_mixed<12, KU>(c, kp, grid, lds, st); return 0;
+    case 16: launch_mixed<16, KU>(c, kp, grid, lds, st); return 0;
+  }
+  return -1;
+}
+
+}  // namespace
+
+/* ---- orchestration ------------------------------------------------------------- */
+int assoc_uvlr_run(eqbf_ctx* ctx, const eqbf_chunk* cp, const eqbf_assoc_opts* o, eqbf_assoc_out* out,
+                   hipStream_t st) {
+  const eqbf_chunk& c = *cp;
+  const int S = c.n_subgroups, N = c.n_samples, G = c.n_genes;
+  if (S < 1 || S > 64 || N < 1 || G < 0 || c.n_covars_max < 0 || c.n_covars_max > 16)
+    return ctx_fail(ctx, EQBF_ERR_ARG, "chunk sizes out of range (S=%d N=%d G=%d Q=%d)", S, N, G, c.n_covars_max);
+  if (!out || !out->n || (o->analysis == EQBF_ANALYSIS_JOIN && !out->w_abf))
+    return ctx_fail(ctx, EQBF_ERR_ARG, "missing output buffers");
+  const bool join = o->analysis == EQBF_ANALYSIS_JOIN;
+  const int GL = (int)ctx->phiL.size(), GS = (int)ctx->phiS.size();
+  if (join && GL < 1) return ctx_fail(ctx, EQBF_ERR_ARG, "--analys join needs the large grid (eqbf_set_grids)");
+  if (join && o->bfs != EQBF_BFS_GEN && GS < 1)
+    return ctx_fail(ctx, EQBF_ERR_ARG, "--bfs sin/all needs the small grid (eqbf_set_grids)");
+  int64_t n_cfg = join ? eqbf_n_configs(o->bfs, S) : 0;
+  if (n_cfg < 0) return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED, "--bfs all with %d subgroups", S);
+  if (G == 0 || c.n_pairs == 0) return EQBF_OK;
+  if (o->precision == EQBF_PREC_FP64_PLAIN && (GL > PLAIN_GMAX || GS > PLAIN_GMAX))
+    return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED, "plain path supports grids of at most %d points", PLAIN_GMAX);
+  int KU = 1;
+  while (KU < c.n_covars_max + 1) KU = (KU == 1) ? 4 : KU * 2;
+  if (KU > 16) KU = 17;
+  if (o->precision == EQBF_PREC_MIXED && KU > 8)
+    return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED, "mixed path supports at most 7 covariates");
+  const int KC = KU + 1;
+
+  PrepRec* rec = (PrepRec*)ctx_buf(ctx, "rec", (size_t)G * S * sizeof(PrepRec));
+  double* B = (double*)ctx_buf(ctx, "B", (size_t)G * N * S * KC * sizeof(double));
+  uint64_t* memb = (uint64_t*)ctx_buf(ctx, "memb", (size_t)G * N * sizeof(uint64_t));
+  int32_t* idx = (int32_t*)ctx_buf(ctx, "idx", (size_t)G * S * N * sizeof(int32_t));
+  const size_t max_tiles = (size_t)(c.n_pairs / TILE) + (size_t)G + 1;
+  int4* tiles = (int4*)ctx_buf(ctx, "tiles", max_tiles * sizeof(int4));
+  int* n_tiles = (int*)ctx_buf(ctx, "ntiles", 256);
+  uint64_t* d_masks = (uint64_t*)ctx_buf(ctx, "masks", (size_t)std::max<int64_t>(n_cfg, 1) * sizeof(uint64_t) + 65 * sizeof(double));
+  if (!rec || !B || !memb || !idx || !tiles || !n_tiles || !d_masks) return EQBF_ERR_HIP;
+  double* d_inv_choose = (double*)(d_masks + std::max<int64_t>(n_cfg, 1));
+  if (n_cfg > 0) {
+    std::vector<uint64_t> hm(n_cfg);
+    eqbf_config_masks(o->bfs, S, hm.data());
+    std::vector<double> ic(65, 0.0);
+    for (int k = 1; k <= S; ++k) {
+      double ch = 1;
+      for (int i = 1; i <= k; ++i) ch = ch * (S - k + i) / i;
+      ic[k] = 1.0 / ch;
+    }
+    EQBF_HIP(ctx, hipMemcpyAsync(d_masks, hm.data(), n_cfg * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+    EQBF_HIP(ctx, hipMemcpyAsync(d_inv_choose, ic.data(), 65 * sizeof(double), hipMemcpyHostToDevice, st));
+    EQBF_HIP(ctx, hipStreamSynchronize(st)); /* host vectors go out of scope */
+  }
+  int rc = ctx_ensure_tz(ctx, N, st);
+  if (rc) return rc;
+
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(1024), dim3(256), 0, st, memb, (int64_t)G * N);
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, (uint64_t*)ctx->d_flag.ptr, (int64_t)1);
+  {
+    ProfScope ps(ctx, "prep_kernel", st);
+    hipLaunchKernelGGL(prep_kernel, dim3(G * S), dim3(256), 0, st, c, rec, B, memb, idx, (int*)ctx->d_flag.ptr, KU);
+  }
+  EQBF_HIP(ctx, hipGetLastError());
+
+  AssocParams kp;
+  memset(&kp, 0, sizeof kp);
+  kp.rec = rec;
+  kp.B = B;
+  kp.memb = memb;
+  kp.idx = idx;
+  kp.tiles = tiles;
+  kp.n_tiles = n_tiles;
+  kp.tz = ctx_tz_table(ctx);
+  kp.grid = (const double*)ctx->d_grid.ptr;
+  kp.masks = d_masks;
+  kp.n_out = out->n;
+  kp.sstats = out->sstats;
+  kp.stdsstats = out->stdsstats;
+  kp.raw = join ? out->raw_abf : nullptr;
+  kp.w = out->w_abf;
+  kp.flag = (int*)ctx->d_flag.ptr;
+  kp.GL = GL;
+  kp.GS = GS;
+  kp.n_cfg = (int)n_cfg;
+  kp.n_w = (int)(3 + n_cfg + 2);
+  kp.KU = KU;
+  kp.bfs = o->bfs;
+  kp.join = join;
+  kp.raw_f64 = o->raw_dtype == EQBF_F64;
+  kp.multiset = c.n_geno_sets > 1;
+
+  if (o->precision == EQBF_PREC_FP64_PLAIN) {
+    if (S > PLAIN_SMAX) return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED, "plain path supports at most 64 subgroups");
+    std::vector<int64_t> off(G + 1);
+    EQBF_HIP(ctx, hipMemcpyAsync(off.data(), c.pair_offset, (G + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    EQBF_HIP(ctx, hipStreamSynchronize(st));
+    for (int g = 0; g < G; ++g) {
+      int64_t np = off[g + 1] - off[g];
+      if (np <= 0) continue;
+      ProfScope ps(ctx, "assoc_plain_kernel", st);
+      hipLaunchKernelGGL(assoc_plain_kernel, dim3((unsigned)((np + 63) / 64)), dim3(64), 0, st, c, kp, g);
+    }
+    EQBF_HIP(ctx, hipGetLastError());
+    if (join && n_cfg > 0) {
+      hipLaunchKernelGGL(bma_kernel, dim3((unsigned)((c.n_pairs + 255) / 256)), dim3(256), 0, st, out->w_abf,
+                         c.n_pairs, S, (int)n_cfg, o->bfs, d_masks, d_inv_choose);
+    } else if (join) {
+      hipLaunchKernelGGL(bma_kernel, dim3((unsigned)((c.n_pairs + 255) / 256)), dim3(256), 0, st, out->w_abf,
+                         c.n_pairs, S, 0, EQBF_BFS_GEN, d_masks, d_inv_choose);
+    }
+    EQBF_HIP(ctx, hipGetLastError());
+    return EQBF_OK;
+  }
+
+  /* mixed path */
+  if (join && n_cfg > 60000) return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED, "too many configurations");
+  {
+    ProfScope ps(ctx, "tiles_kernel", st);
+    hipLaunchKernelGGL(tiles_kernel, dim3(1), dim3(1024), 0, st, c.pair_offset, G, tiles, n_tiles);
+  }
+  int spw = (S + NW - 1) / NW;
+  int spw_t = spw <= 1 ? 1 : spw <= 2 ? 2 : spw <= 4 ? 4 : spw <= 8 ? 8 : spw <= 12 ? 12 : 16;
+  /* LDS: max(staging of two CH-sample chunks, statistics + partial sums) */
+  int CH = 8;
+  while (CH > 1 && (size_t)2 * CH * S * KC * sizeof(double) > 48 * 1024) CH >>= 1;
+  kp.CH = CH;
+  size_t lds_stage = (size_t)2 * CH * S * KC * sizeof(double);
+  size_t lds_stats = (((size_t)S * TILE * 17 + 15) / 16) * 16 + (size_t)3 * 4 * 2 * TILE * sizeof(double);
+  size_t lds = std::max(lds_stage, lds_stats);
+  int grid = ctx->n_cu * 2;
+  grid = (grid + 7) / 8 * 8;
+  int r2;
+  {
+  ProfScope ps(ctx, "assoc_mixed_kernel", st);
+  if (KU == 1)
+    r2 = dispatch_spw<1>(spw_t, c, kp, grid, lds, st);
+  else if (KU == 4)
+    r2 = dispatch_spw<4>(spw_t, c, kp, grid, lds, st);
+  else
+    r2 = dispatch_spw<8>(spw_t, c, kp, grid, lds, st);
+  }
+  if (r2) return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED, "no kernel variant for S=%d", S);
+  EQBF_HIP(ctx, hipGetLastError());
+  if (join) {
+    ProfScope ps(ctx, "bma_kernel", st);
+    hipLaunchKernelGGL(bma_kernel, dim3((unsigned)((c.n_pairs + 255) / 256)), dim3(256), 0, st, out->w_abf,
+                       c.n_pairs, S, (int)n_cfg, n_cfg > 0 ? o->bfs : EQBF_BFS_GEN, d_masks, d_inv_choose);
+    EQBF_HIP(ctx, hipGetLastError());
+  }
+  return EQBF_OK;
+}

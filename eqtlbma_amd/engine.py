@@ -1,0 +1,113 @@
+"""Python binding over the C ABI of libeqbf_hip.so (include/eqbf.h).
+
+The library must have been built (python -m eqtlbma_amd.build); there is no
+fallback: a missing or unloadable library raises immediately.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import LIB_PATH, _abi
+
+_lib = None
+
+
+class EngineError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("eqbf error %d: %s" % (code, msg))
+        self.code = code
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libeqbf_hip.so is not built (%s); run python -m eqtlbma_amd.build" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        vp, i32, i64 = C.c_void_p, C.c_int32, C.c_int64
+        L.eqbf_create.argtypes = [C.POINTER(vp), C.c_int]
+        L.eqbf_destroy.argtypes = [vp]
+        L.eqbf_last_error.argtypes = [vp]
+        L.eqbf_last_error.restype = C.c_char_p
+        L.eqbf_sync.argtypes = [vp]
+        L.eqbf_set_grids.argtypes = [vp, vp, vp, i32, vp, vp, i32]
+        L.eqbf_n_configs.argtypes = [i32, i32]
+        L.eqbf_n_configs.restype = i64
+        L.eqbf_config_masks.argtypes = [i32, i32, vp]
+        L.eqbf_n_raw.argtypes = [i32, i32, i32, i32]
+        L.eqbf_n_raw.restype = i64
+        L.eqbf_n_weighted.argtypes = [i32, i32]
+        L.eqbf_n_weighted.restype = i64
+        L.eqbf_assoc_uvlr.argtypes = [vp, vp, vp, vp]
+        L.eqbf_assoc_uvlr_device.argtypes = [vp, vp, vp, vp, vp]
+        L.eqbf_debug_ttail.argtypes = [vp, vp, vp, i32, vp, vp, vp, vp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+class Engine:
+    """One context on one HIP device."""
+
+    def __init__(self, device=0):
+        self._ctx = C.c_void_p()
+        rc = lib().eqbf_create(C.byref(self._ctx), device)
+        if rc != 0:
+            raise EngineError(rc, "eqbf_create(device=%d) failed" % device)
+        self.GL = self.GS = 0
+
+    def close(self):
+        if self._ctx:
+            lib().eqbf_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise EngineError(rc, lib().eqbf_last_error(self._ctx).decode())
+
+    def set_grids(self, gridL, gridS=None):
+        gL = np.ascontiguousarray(gridL, dtype=np.float64)
+        gS = np.ascontiguousarray(gridS if gridS is not None else np.zeros((0, 2)), dtype=np.float64)
+        self._gL = (np.ascontiguousarray(gL[:, 0]), np.ascontiguousarray(gL[:, 1]))
+        self._gS = (np.ascontiguousarray(gS[:, 0]), np.ascontiguousarray(gS[:, 1]))
+        self._check(lib().eqbf_set_grids(self._ctx, _p(self._gL[0]), _p(self._gL[1]), gL.shape[0],
+                                         _p(self._gS[0]), _p(self._gS[1]), gS.shape[0]))
+        self.GL, self.GS = gL.shape[0], gS.shape[0]
+
+    def assoc_uvlr(self, hc, analysis="join", bfs="gen", precision="mixed", raw_dtype=np.float64,
+                   want_sstats=True, want_std=True, want_raw=True):
+        out = _abi.HostAssocOut(hc.S, hc.n_pairs, bfs, self.GL, self.GS, raw_dtype=raw_dtype,
+                                want_sstats=want_sstats, want_std=want_std, want_raw=want_raw)
+        opts = _abi.AssocOpts(_abi.ANALYSIS[analysis], _abi.BFS[bfs], _abi.PREC[precision],
+                              _abi.F64 if raw_dtype == np.float64 else _abi.F32)
+        st = hc.struct()
+        o = out.struct()
+        self._check(lib().eqbf_assoc_uvlr(self._ctx, C.byref(st), C.byref(opts), C.byref(o)))
+        return out
+
+    def assoc_uvlr_device(self, chunk_struct, opts, out_struct, stream=None):
+        """Device-pointer entry: chunk_struct / out_struct are _abi.Chunk /
+        _abi.AssocOut whose pointers are device addresses."""
+        self._check(lib().eqbf_assoc_uvlr_device(self._ctx, C.byref(chunk_struct), C.byref(opts),
+                                                 C.byref(out_struct), stream))
+
+    def sync(self):
+        self._check(lib().eqbf_sync(self._ctx))
+
+    def debug_ttail(self, t, nu):
+        t = np.ascontiguousarray(t, dtype=np.float64)
+        nu = np.ascontiguousarray(nu, dtype=np.int32)
+        n = t.size
+        outs = [np.zeros(n) for _ in range(4)]
+        self._check(lib().eqbf_debug_ttail(self._ctx, _p(t), _p(nu), n, *[_p(o) for o in outs]))
+        return outs

@@ -1,0 +1,179 @@
+/*
+ * eqbf.h -- C ABI of the MI355X Bayes-factor engine behind eqtlbma_bf.
+ *
+ * The reference (eqtlbma 1.3.3) has no plugin API: its only public surface is
+ * the eqtlbma_bf command line.  The seam this library replaces is the pair of
+ * per-chunk calls inside run():
+ *   testForAssociations(...)   src/eqtlbma_bf.cpp:707-771   (called :1551-1555)
+ *   makePermutations(...)      src/eqtlbma_bf.cpp:867-917   (called :1558-1561)
+ * and the GeneSnpPair / Gene getters the writers read afterwards
+ * (src/eqtlbma/quantgen/gene_snp_pair.cpp:1425-1470, gene.cpp:719-725).
+ *
+ * Conventions
+ *  - Every function returns EQBF_OK (0) or an error code; the message is in
+ *    eqbf_last_error(ctx).  The reference prints to stderr and exits with
+ *    EXIT_FAILURE instead (utils_io.cpp:214-219); the host program maps a
+ *    non-zero code to exactly that.
+ *  - Numerical failure is data, not an error: it comes back as NaN, as in the
+ *    reference (utils_math.cpp:175-178).
+ *  - The caller owns every buffer it passes.  A context owns its device memory,
+ *    its stream and its cached tables.  One context per host thread.
+ *  - "_device" entry points take device pointers (already resident in HBM) and
+ *    enqueue on the given stream (NULL = the context's stream) without
+ *    synchronising; the other entry points take host pointers, stage them and
+ *    return when the results are back in host memory.
+ *  - All per-pair outputs are structure-of-arrays with leading dimension
+ *    n_pairs, so that consecutive pairs are consecutive in memory.
+ *
+ * Layout of the inputs (eqbf_chunk)
+ *  - subgroups are indexed 0..S-1 in lexicographic name order and samples
+ *    0..N-1 in lexicographic name order, as the reference orders them
+ *    (data_loader.cpp:141-143, samples.cpp:67).
+ *  - explevels[g][s][i] is the expression level of gene g in subgroup s for
+ *    sample i, NaN when the sample has no column in that subgroup's expression
+ *    file or the value is NA (gene.cpp:104-118).
+ *  - genotypes: element (set, i, j) is at genotypes[set*geno_set_stride +
+ *    i*geno_ld + j] for sample i and SNP j; subgroup s reads set
+ *    geno_set_of_sbgrp[s] (subgroups sharing one genotype file share a set,
+ *    data_loader.cpp:691-715).
+ *  - the pairs of gene g are pair_offset[g] .. pair_offset[g+1]-1 and pair p
+ *    tests SNP pair_snp[p] (cis SNPs in position order, gene.cpp:140-157).
+ */
+#ifndef EQBF_H
+#define EQBF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ------------------------------------------------------- */
+#define EQBF_OK               0
+#define EQBF_ERR_ARG          1  /* invalid argument or size                  */
+#define EQBF_ERR_HIP          2  /* HIP runtime failure                       */
+#define EQBF_ERR_DATA         3  /* input the reference rejects with exit(1)  */
+#define EQBF_ERR_UNSUPPORTED  4  /* valid option this build does not run      */
+
+/* ---- option values (eqtlbma_bf.cpp:262-299, defaults :1586-1597) ------- */
+enum { EQBF_ANALYSIS_SEP = 0, EQBF_ANALYSIS_JOIN = 1 };         /* --analys */
+enum { EQBF_BFS_GEN = 0, EQBF_BFS_SIN = 1, EQBF_BFS_ALL = 2 };  /* --bfs    */
+enum { EQBF_PBF_GEN = 0, EQBF_PBF_GENSIN = 1, EQBF_PBF_ALL = 2 };/* --pbf   */
+/* precision / batching mode:
+ *  FP64_PLAIN: every quantity in FP64, one kernel launch per gene, reference
+ *              loop structure (the "plain GPU" comparison path);
+ *  MIXED:      batched persistent launch over all genes of the chunk, FP64
+ *              sums and residual sums of squares, FP32 Bayes-factor terms,
+ *              FP64 log10-sum-exp. */
+enum { EQBF_PREC_FP64_PLAIN = 0, EQBF_PREC_MIXED = 1 };
+enum { EQBF_F32 = 0, EQBF_F64 = 1 };
+
+/* ---- opaque context ------------------------------------------------------ */
+typedef struct eqbf_ctx eqbf_ctx;
+
+/* Create a context on HIP device `device` (ordinal). */
+int eqbf_create(eqbf_ctx** out, int device);
+void eqbf_destroy(eqbf_ctx* ctx);
+const char* eqbf_last_error(const eqbf_ctx* ctx);
+/* Block until every operation enqueued on the context's stream is done. */
+int eqbf_sync(eqbf_ctx* ctx);
+
+/* Grids of prior variances (phi2, oma2); replaces Grid::Grid (grid.cpp:28-67).
+ * The large grid is used for the consistent configuration (gen, gen-fix,
+ * gen-maxh), the small one for singletons / all configurations. */
+int eqbf_set_grids(eqbf_ctx* ctx,
+                   const double* phi2_large, const double* oma2_large, int32_t n_large,
+                   const double* phi2_small, const double* oma2_small, int32_t n_small);
+
+/* ---- inputs ----------------------------------------------------------------*/
+typedef struct eqbf_chunk {
+  int32_t n_genes;
+  int32_t n_subgroups;          /* S  (<= 64)                                 */
+  int32_t n_samples;            /* N  (union of all samples)                  */
+  int32_t n_covars_max;         /* Q_max covariate slots per subgroup (<= 16) */
+  int64_t n_pairs;
+  int64_t n_snps;               /* columns of the genotype matrices           */
+  const double*  explevels;     /* [n_genes][S][N]                            */
+  const uint8_t* gene_in_sbgrp; /* [n_genes][S] gene has expression in s      */
+  const uint8_t* geno_in_sbgrp; /* [S][N] sample has a genotype column in s   */
+  const int32_t* covar_count;   /* [S] number of covariates of s              */
+  const double*  covars;        /* [S][Q_max][N]                              */
+  const uint8_t* covar_in_sbgrp;/* [S][N] sample has covariate values in s    */
+  int32_t        n_geno_sets;
+  const int32_t* geno_set_of_sbgrp; /* [S]                                    */
+  const void*    genotypes;     /* see layout note above                      */
+  int32_t        geno_dtype;    /* EQBF_F32 or EQBF_F64                       */
+  int64_t        geno_ld;       /* >= n_snps                                  */
+  int64_t        geno_set_stride;
+  const uint8_t* snp_in_sbgrp;  /* [n_snps][S] SNP kept in s (no NA, MAF ok)  */
+  const int64_t* pair_offset;   /* [n_genes+1]                                */
+  const int32_t* pair_snp;      /* [n_pairs]                                  */
+} eqbf_chunk;
+
+typedef struct eqbf_assoc_opts {
+  int32_t analysis;    /* EQBF_ANALYSIS_*                                     */
+  int32_t bfs;         /* EQBF_BFS_*  (join only)                             */
+  int32_t precision;   /* EQBF_PREC_*                                         */
+  int32_t raw_dtype;   /* element type of out->raw_abf: EQBF_F32 / EQBF_F64   */
+} eqbf_assoc_opts;
+
+/* Number of configurations written after the consistent ones:
+ * gen -> 0, sin -> S, all -> 2^S - 1 (gsl_combination order: size 1..S,
+ * lexicographic within a size; gene_snp_pair.cpp:469-550). */
+int64_t eqbf_n_configs(int32_t bfs, int32_t n_subgroups);
+/* Bit masks (bit s = subgroup s active) of those configurations, in order. */
+int eqbf_config_masks(int32_t bfs, int32_t n_subgroups, uint64_t* masks);
+/* Raw Bayes factors per pair: 3*n_large (gen, gen-fix, gen-maxh) followed by
+ * n_configs*n_small. */
+int64_t eqbf_n_raw(int32_t bfs, int32_t n_subgroups, int32_t n_large, int32_t n_small);
+/* Grid-averaged Bayes factors per pair: gen, gen-fix, gen-maxh, then one per
+ * configuration, then gen-sin (BMAlite) and all (BMA); NaN when not computed. */
+int64_t eqbf_n_weighted(int32_t bfs, int32_t n_subgroups);
+
+/* ---- outputs (structure of arrays, leading dimension n_pairs) ------------- */
+typedef struct eqbf_assoc_out {
+  int32_t* n;         /* [S][n_pairs] sample size, 0 = no result in s
+                         (GeneSnpPair::GetSampleSize, gene_snp_pair.cpp:1425) */
+  double*  sstats;    /* [5][S][n_pairs] pve, sigmahat, betahat.geno,
+                         sebetahat.geno, betapval.geno (:1430-1453); may be NULL */
+  double*  stdsstats; /* [3][S][n_pairs] standardised b, se, t after the
+                         small-sample correction (:256-290); may be NULL       */
+  void*    raw_abf;   /* [n_raw][n_pairs] log10 ABFs (:1455-1465); may be NULL */
+  double*  w_abf;     /* [n_weighted][n_pairs] (:1467-1470); join only         */
+} eqbf_assoc_out;
+
+/* Association test of every pair of the chunk with the uvlr error model:
+ * per-subgroup OLS summary statistics (GeneSnpPair::CalcSstatsOneSbgrp,
+ * gene_snp_pair.cpp:175-239 -> utils::FitSingleGeneWithSingleSnp,
+ * utils_math.cpp:166-209), and for --analys join the Bayes factors
+ * (GeneSnpPair::CalcAbfsUvlr, gene_snp_pair.cpp:604-622).
+ * Replaces testForAssociations, eqtlbma_bf.cpp:707-771, for uvlr/sep. */
+int eqbf_assoc_uvlr(eqbf_ctx* ctx, const eqbf_chunk* chunk,
+                    const eqbf_assoc_opts* opts, eqbf_assoc_out* out);
+int eqbf_assoc_uvlr_device(eqbf_ctx* ctx, const eqbf_chunk* chunk_dev,
+                           const eqbf_assoc_opts* opts, eqbf_assoc_out* out_dev,
+                           void* hip_stream);
+
+/* ---- live kernel timing -------------------------------------------------------
+ * No counterpart in the reference.  When enabled, every kernel launch of the
+ * context is bracketed by HIP events on the stream it is launched on; the
+ * totals per kernel name are read back with eqbf_profile_read (which
+ * synchronises the context's streams).  names: n_max entries of 64 chars. */
+int eqbf_profile(eqbf_ctx* ctx, int enable);
+int eqbf_profile_read(eqbf_ctx* ctx, char* names, double* total_ms, int64_t* launches, int32_t n_max,
+                      int32_t* n_out);
+
+/* ---- diagnostics ------------------------------------------------------------
+ * No counterpart in the reference.  Evaluates, on the device, the Student-t
+ * tail w = -ln P(T_nu <= -|t|) from the tabulated interpolant (w_tab) and
+ * from the direct continued fraction (w_cf), and the corresponding normal
+ * quantiles z (as GSL's gsl_cdf_gaussian_Pinv(gsl_cdf_tdist_P(-|t|, nu)),
+ * gene_snp_pair.cpp:274).  Host arrays of length n. */
+int eqbf_debug_ttail(eqbf_ctx* ctx, const double* t, const int32_t* nu, int32_t n,
+                     double* w_tab, double* w_cf, double* z_tab, double* z_cf);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EQBF_H */

@@ -1,0 +1,273 @@
+"""ctypes access to the CPU oracle (test infrastructure only).
+
+Only the tests, __graft_entry__.smoke() and the CPU-baseline leg of bench.py
+use this module, as the checker.  The product never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from eqtlbma_amd import _abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liborc_eqbf.so")
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = C.CDLL(_LIB)
+        d, i32, i64, u32, u64, vp = (C.c_double, C.c_int32, C.c_int64, C.c_uint32,
+                                    C.c_uint64, C.c_void_p)
+        L.orc_mt_seed.argtypes = [vp, u64]
+        L.orc_mt_next.argtypes = [vp]
+        L.orc_mt_next.restype = u32
+        L.orc_uniform_int.argtypes = [vp, u32]
+        L.orc_uniform_int.restype = u32
+        L.orc_shuffle.argtypes = [vp, vp, i64]
+        L.orc_flat.argtypes = [vp, d, d]
+        L.orc_flat.restype = d
+        L.orc_perm_tables.argtypes = [u64, i64, i64, i64, vp]
+        for f in ("orc_tdist_P", "orc_tdist_Q", "orc_fdist_Q_1"):
+            getattr(L, f).argtypes = [d, d]
+            getattr(L, f).restype = d
+        for f in ("orc_gauss_Pinv", "orc_chisq_Qinv_1"):
+            getattr(L, f).argtypes = [d]
+            getattr(L, f).restype = d
+        L.orc_fit_ols.argtypes = [i64, i64, vp, vp, vp]
+        L.orc_log10_weighted_sum.argtypes = [vp, i64]
+        L.orc_log10_weighted_sum.restype = d
+        L.orc_log10_weighted_sum_w.argtypes = [vp, vp, i64]
+        L.orc_log10_weighted_sum_w.restype = d
+        L.orc_l10abf_uvlr.argtypes = [i32, vp, vp, d, d]
+        L.orc_l10abf_uvlr.restype = d
+        L.orc_standardize.argtypes = [d, d, d, d, d, vp]
+        L.orc_config_masks.argtypes = [i32, i32, vp]
+        L.orc_config_masks.restype = i64
+        L.orc_assoc_uvlr.argtypes = [vp, i32, i32, vp, vp, i32, vp, vp, i32,
+                                     vp, vp, vp, vp, vp]
+        L.orc_assoc_uvlr.restype = C.c_int
+        L.orc_perm_join_stats.argtypes = [vp, i64, i32, i32, vp, vp, i32, vp, vp, i32,
+                                          vp, i64, vp]
+        L.orc_perm_join_stats.restype = C.c_int
+        L.orc_true_stat.argtypes = [vp, i64, i32]
+        L.orc_true_stat.restype = d
+        L.orc_perm_count.argtypes = [vp, d, vp, i32, i64, vp, vp, vp, vp]
+        L.orc_perm_count.restype = i64
+        L.orc_perm_sep_stats.argtypes = [vp, i64, i32, vp, i64, vp]
+        L.orc_perm_sep_stats.restype = C.c_int
+        L.orc_perm_sep_count.argtypes = [vp, d, vp, i32, i64, vp, vp, vp]
+        L.orc_perm_sep_count.restype = i64
+        L.orc_mvlr_abfs.argtypes = [i32, i32, i32, vp, vp, vp, d, vp, vp, vp, i32, vp]
+        L.orc_mvlr_sigma.argtypes = [i32, i32, i32, vp, vp, vp, d, vp, vp, vp]
+        L.orc_assoc_mvlr.argtypes = [vp, i32, d, vp, vp, i32, vp, vp, i32, vp, vp, vp]
+        L.orc_assoc_mvlr.restype = C.c_int
+        L.orc_last_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data
+
+
+class MT:
+    """GSL-semantics mt19937 stream (gsl_rng_set, uniform_int, shuffle, flat)."""
+
+    def __init__(self, seed):
+        self._st = (C.c_uint32 * 625)()
+        lib().orc_mt_seed(C.addressof(self._st), seed)
+
+    def next(self):
+        return lib().orc_mt_next(C.addressof(self._st))
+
+    def uniform_int(self, n):
+        return lib().orc_uniform_int(C.addressof(self._st), n)
+
+    def shuffle(self, a):
+        a = np.ascontiguousarray(a, dtype=np.int64)
+        lib().orc_shuffle(C.addressof(self._st), _p(a), a.size)
+        return a
+
+    def flat(self, a, b):
+        return lib().orc_flat(C.addressof(self._st), a, b)
+
+    @property
+    def addr(self):
+        return C.addressof(self._st)
+
+
+def perm_tables(seed, n_samples, n_genes, nperm):
+    t = np.zeros((n_genes, nperm, n_samples), np.int32)
+    lib().orc_perm_tables(seed, n_samples, n_genes, nperm, _p(t))
+    return t
+
+
+def tdist_P(t, nu):
+    return lib().orc_tdist_P(float(t), float(nu))
+
+
+def tdist_Q(t, nu):
+    return lib().orc_tdist_Q(float(t), float(nu))
+
+
+def gauss_Pinv(p):
+    return lib().orc_gauss_Pinv(float(p))
+
+
+def fdist_Q_1(F, nu):
+    return lib().orc_fdist_Q_1(float(F), float(nu))
+
+
+def chisq_Qinv_1(q):
+    return lib().orc_chisq_Qinv_1(float(q))
+
+
+def fit_ols(X, y):
+    """X: N x P with the genotype in the last column.  Returns
+    (pve, sigmahat, betahat, sebetahat, betapval)."""
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    out = np.zeros(5)
+    lib().orc_fit_ols(X.shape[0], X.shape[1], _p(X), _p(y), _p(out))
+    return out
+
+
+def log10_weighted_sum(v, w=None):
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    if w is None:
+        return lib().orc_log10_weighted_sum(_p(v), v.size)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    return lib().orc_log10_weighted_sum_w(_p(v), _p(w), v.size)
+
+
+def l10abf_uvlr(gamma, stdsstats, phi2, oma2):
+    g = np.ascontiguousarray(gamma, dtype=np.int32)
+    st = np.ascontiguousarray(stdsstats, dtype=np.float64)
+    return lib().orc_l10abf_uvlr(g.size, _p(g), _p(st), phi2, oma2)
+
+
+def standardize(N, Q, sigmahat, betahat, sebetahat):
+    out = np.zeros(3)
+    lib().orc_standardize(N, Q, sigmahat, betahat, sebetahat, _p(out))
+    return out
+
+
+def assoc_uvlr(hc, analysis, bfs, gridL, gridS):
+    """Run the oracle's association test on a HostChunk; returns HostAssocOut."""
+    gL = np.ascontiguousarray(gridL, dtype=np.float64)
+    gS = np.ascontiguousarray(gridS if gridS is not None else gridL, dtype=np.float64)
+    out = _abi.HostAssocOut(hc.S, hc.n_pairs, bfs, gL.shape[0], gS.shape[0])
+    st = hc.struct()
+    phiL, omaL = np.ascontiguousarray(gL[:, 0]), np.ascontiguousarray(gL[:, 1])
+    phiS, omaS = np.ascontiguousarray(gS[:, 0]), np.ascontiguousarray(gS[:, 1])
+    rc = lib().orc_assoc_uvlr(C.addressof(st), _abi.ANALYSIS[analysis], _abi.BFS[bfs],
+                              _p(phiL), _p(omaL), gL.shape[0], _p(phiS), _p(omaS), gS.shape[0],
+                              _p(out.n), _p(out.sstats), _p(out.stdsstats), _p(out.raw), _p(out.w))
+    if rc != 0:
+        raise RuntimeError("oracle: %s" % lib().orc_last_error().decode())
+    return out
+
+
+def assoc_mvlr(hc, bfs, fiterr, gridL, gridS):
+    gL = np.ascontiguousarray(gridL, dtype=np.float64)
+    gS = np.ascontiguousarray(gridS if gridS is not None else gridL, dtype=np.float64)
+    out = _abi.HostAssocOut(hc.S, hc.n_pairs, bfs, gL.shape[0], gS.shape[0],
+                            want_sstats=False, want_std=False)
+    st = hc.struct()
+    phiL, omaL = np.ascontiguousarray(gL[:, 0]), np.ascontiguousarray(gL[:, 1])
+    phiS, omaS = np.ascontiguousarray(gS[:, 0]), np.ascontiguousarray(gS[:, 1])
+    rc = lib().orc_assoc_mvlr(C.addressof(st), _abi.BFS[bfs], fiterr, _p(phiL), _p(omaL),
+                              gL.shape[0], _p(phiS), _p(omaS), gS.shape[0], _p(out.n),
+                              _p(out.raw), _p(out.w))
+    if rc != 0:
+        raise RuntimeError("oracle: %s" % lib().orc_last_error().decode())
+    return out
+
+
+def mvlr_abfs(Y, x, Xc, fiterr, gamma, grid):
+    """Y: s x n, x: n, Xc: qcov x n; grid: K x 2 (phi2, oma2)."""
+    Y = np.ascontiguousarray(Y, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    Xc = np.ascontiguousarray(Xc if Xc is not None else np.zeros((0, Y.shape[1])), dtype=np.float64)
+    g = np.ascontiguousarray(gamma, dtype=np.int32)
+    grid = np.asarray(grid, dtype=np.float64)
+    phi = np.ascontiguousarray(grid[:, 0])
+    oma = np.ascontiguousarray(grid[:, 1])
+    out = np.zeros(grid.shape[0])
+    lib().orc_mvlr_abfs(Y.shape[1], Y.shape[0], Xc.shape[0], _p(Y), _p(x), _p(Xc), fiterr,
+                        _p(g), _p(phi), _p(oma), grid.shape[0], _p(out))
+    return out
+
+
+def mvlr_sigma(Y, x, Xc, fiterr, gamma):
+    Y = np.ascontiguousarray(Y, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    Xc = np.ascontiguousarray(Xc if Xc is not None else np.zeros((0, Y.shape[1])), dtype=np.float64)
+    g = np.ascontiguousarray(gamma, dtype=np.int32)
+    s = Y.shape[0]
+    S1 = np.zeros((s, s))
+    S0 = np.zeros((s, s))
+    lib().orc_mvlr_sigma(Y.shape[1], s, Xc.shape[0], _p(Y), _p(x), _p(Xc), fiterr, _p(g),
+                         _p(S1), _p(S0))
+    return S1, S0
+
+
+def perm_join_stats(hc, g, pbf, use_max_bf, gridL, gridS, table):
+    gL = np.ascontiguousarray(gridL, dtype=np.float64)
+    gS = np.ascontiguousarray(gridS, dtype=np.float64)
+    phiL, omaL = np.ascontiguousarray(gL[:, 0]), np.ascontiguousarray(gL[:, 1])
+    phiS, omaS = np.ascontiguousarray(gS[:, 0]), np.ascontiguousarray(gS[:, 1])
+    table = np.ascontiguousarray(table, dtype=np.int32)
+    nperm = table.shape[0]
+    stats = np.zeros(nperm)
+    st = hc.struct()
+    rc = lib().orc_perm_join_stats(C.addressof(st), g, _abi.PBF[pbf], int(use_max_bf),
+                                   _p(phiL), _p(omaL), gL.shape[0], _p(phiS), _p(omaS),
+                                   gS.shape[0], _p(table), nperm, _p(stats))
+    if rc != 0:
+        raise RuntimeError("oracle: %s" % lib().orc_last_error().decode())
+    return stats
+
+
+def perm_sep_stats(hc, g, sbgrp, table):
+    table = np.ascontiguousarray(table, dtype=np.int32)
+    stats = np.zeros(table.shape[0])
+    st = hc.struct()
+    rc = lib().orc_perm_sep_stats(C.addressof(st), g, sbgrp, _p(table), table.shape[0], _p(stats))
+    if rc != 0:
+        raise RuntimeError("oracle: %s" % lib().orc_last_error().decode())
+    return stats
+
+
+def true_stat(wpbf, use_max_bf):
+    w = np.ascontiguousarray(wpbf, dtype=np.float64)
+    return lib().orc_true_stat(_p(w), w.size, int(use_max_bf))
+
+
+def perm_count(stats, true_stat_value, nb_permutations, trick=0, trick_cutoff=10, rng_trick=None,
+               sep=False):
+    """Replays the reference's counting loop.  Returns (pval, nperm_used,
+    median, nb_permutations_after, drawn); median is None for sep."""
+    stats = np.ascontiguousarray(stats, dtype=np.float64)
+    nb = C.c_int64(nb_permutations)
+    pv = C.c_double(0)
+    used = C.c_int64(0)
+    med = C.c_double(0)
+    addr = rng_trick.addr if rng_trick is not None else None
+    if sep:
+        drawn = lib().orc_perm_sep_count(_p(stats), true_stat_value, C.addressof(nb), trick,
+                                         trick_cutoff, addr, C.addressof(pv), C.addressof(used))
+        return pv.value, used.value, None, nb.value, drawn
+    drawn = lib().orc_perm_count(_p(stats), true_stat_value, C.addressof(nb), trick, trick_cutoff,
+                                 addr, C.addressof(pv), C.addressof(used), C.addressof(med))
+    return pv.value, used.value, med.value, nb.value, drawn
