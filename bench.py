@@ -45,6 +45,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--diag", action="store_true", help="per-phase split of the association kernel")
     ap.add_argument("--seed", type=int, default=1001)
     return ap.parse_args()
 
@@ -252,6 +253,30 @@ def main():
                      "alg_bytes_per_launch": bytes_per_launch},
         "kernels_ms_per_launch": {k: v[0] / max(v[1], 1) for k, v in kern.items()},
     }
+
+    # ---- per-phase split of the dominant kernel on one chunk ----
+    if args.diag:
+        c0 = chunks[0]
+        diag = {}
+        for name, an, bfs in [("sumstats_only", "sep", "gen"), ("gen", "join", "gen"), ("sin", "join", "sin")]:
+            dopts = _abi.AssocOpts(_abi.ANALYSIS[an], _abi.BFS[bfs], _abi.PREC["mixed"], _abi.F32)
+            eng.assoc_uvlr_device(c0, dopts, o, sptr)
+            torch.cuda.synchronize()
+            L.eqbf_profile(eng._ctx, 1)
+            for _ in range(3):
+                eng.assoc_uvlr_device(c0, dopts, o, sptr)
+            torch.cuda.synchronize()
+            L.eqbf_profile(eng._ctx, 0)
+            L.eqbf_profile_read(eng._ctx, names, tot.ctypes.data, cnt.ctypes.data, 16, C.byref(nk))
+            cur = {}
+            for k in range(nk.value):
+                nm = names.raw[64 * k:64 * k + 64].split(b"\0")[0].decode()
+                cur[nm] = (tot[k], int(cnt[k]))
+            diag[name] = {k: (v[0] - kern.get(k, (0, 0))[0]) / max(v[1] - kern.get(k, (0, 0))[1], 1)
+                          for k, v in cur.items()}
+            kern = cur
+        result["diag_ms_per_launch_first_chunk"] = diag
+        log("diag %s" % diag)
 
     # ---- parity sample and CPU baseline (oracle, one core) ----
     if not args.no_cpu_baseline:

@@ -39,12 +39,15 @@ def _run(cmd):
 
 def build_lib(jobs=8):
     os.makedirs(BUILD, exist_ok=True)
-    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
+    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip"))) + sorted(glob.glob(os.path.join(CSRC, "*.cpp")))
     objs = []
 
     def comp(src):
         obj = os.path.join(BUILD, os.path.basename(src) + ".o")
-        _run([HIPCC] + HIP_FLAGS + ["-c", src, "-o", obj])
+        if src.endswith(".cpp"):  # host-only C++
+            _run(["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-I" + INCLUDE, "-c", src, "-o", obj])
+        else:
+            _run([HIPCC] + HIP_FLAGS + ["-c", src, "-o", obj])
         return obj
 
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:

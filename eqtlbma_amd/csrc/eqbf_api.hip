@@ -52,29 +52,71 @@ struct Stager {
 };
 }  // namespace
 
+/* Copies a host chunk into context-owned device buffers. */
+static int stage_chunk(eqbf_ctx* ctx, Stager& sg, const eqbf_chunk* c, eqbf_chunk* d) {
+  const int64_t G = c->n_genes, S = c->n_subgroups, N = c->n_samples, NP = c->n_pairs;
+  const int64_t Qm = c->n_covars_max;
+  if (G < 0 || S < 1 || N < 1 || NP < 0 || c->n_geno_sets < 1)
+    return ctx_fail(ctx, EQBF_ERR_ARG, "invalid chunk sizes");
+  const size_t esz = c->geno_dtype == EQBF_F32 ? 4 : 8;
+  const size_t geno_elems = (size_t)(c->n_geno_sets - 1) * c->geno_set_stride + (size_t)N * c->geno_ld;
+  *d = *c;
+  d->explevels = sg.put("h_expl", c->explevels, (size_t)(G * S * N));
+  d->gene_in_sbgrp = sg.put("h_genein", c->gene_in_sbgrp, (size_t)(G * S));
+  d->geno_in_sbgrp = sg.put("h_genoin", c->geno_in_sbgrp, (size_t)(S * N));
+  d->covar_count = sg.put("h_covcnt", c->covar_count, (size_t)S);
+  d->covars = Qm > 0 ? sg.put("h_covars", c->covars, (size_t)(S * Qm * N)) : nullptr;
+  d->covar_in_sbgrp = sg.put("h_covin", c->covar_in_sbgrp, (size_t)(S * N));
+  d->geno_set_of_sbgrp = sg.put("h_gset", c->geno_set_of_sbgrp, (size_t)S);
+  d->genotypes = sg.put("h_geno", (const uint8_t*)c->genotypes, geno_elems * esz);
+  d->snp_in_sbgrp = sg.put("h_snpin", c->snp_in_sbgrp, (size_t)(c->n_snps * S));
+  d->pair_offset = sg.put("h_poff", c->pair_offset, (size_t)(G + 1));
+  d->pair_snp = sg.put("h_psnp", c->pair_snp, (size_t)NP);
+  if (sg.rc) return ctx_fail(ctx, sg.rc, "staging the chunk failed");
+  return EQBF_OK;
+}
+
+extern "C" int eqbf_perm_stats_device(eqbf_ctx* ctx, const eqbf_chunk* c, const eqbf_perm_opts* po,
+                                      const int32_t* perm_idx_dev, int32_t n_tables, const int32_t* gene_table,
+                                      int64_t nperm, double* stats_dev, void* stream) {
+  if (!ctx || !c || !po || !perm_idx_dev || !gene_table || !stats_dev || n_tables < 0)
+    return ctx_fail(ctx, EQBF_ERR_ARG, "null argument");
+  EQBF_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  return perm_stats_run(ctx, c, po, perm_idx_dev, n_tables, gene_table, nperm, stats_dev, st);
+}
+
+extern "C" int eqbf_perm_stats(eqbf_ctx* ctx, const eqbf_chunk* c, const eqbf_perm_opts* po, const int32_t* perm_idx,
+                               int32_t n_tables, const int32_t* gene_table, int64_t nperm, double* stats) {
+  if (!ctx || !c || !po || !perm_idx || !gene_table || !stats || n_tables < 0)
+    return ctx_fail(ctx, EQBF_ERR_ARG, "null argument");
+  EQBF_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  Stager sg{ctx, st};
+  eqbf_chunk d;
+  int rc = stage_chunk(ctx, sg, c, &d);
+  if (rc) return rc;
+  const int32_t* dtab = sg.put("h_ptab", perm_idx, (size_t)n_tables * nperm * c->n_samples);
+  double* dstats = sg.alloc<double>("o_pstats", (size_t)c->n_genes * nperm, true);
+  if (sg.rc || (n_tables > 0 && !dtab)) return ctx_fail(ctx, EQBF_ERR_HIP, "staging the tables failed");
+  rc = perm_stats_run(ctx, &d, po, dtab, n_tables, gene_table, nperm, dstats, st);
+  if (rc) return rc;
+  rc = check_flag(ctx, st);
+  if (rc) return rc;
+  EQBF_HIP(ctx, hipMemcpyAsync(stats, dstats, (size_t)c->n_genes * nperm * sizeof(double), hipMemcpyDeviceToHost, st));
+  EQBF_HIP(ctx, hipStreamSynchronize(st));
+  return EQBF_OK;
+}
+
 extern "C" int eqbf_assoc_uvlr(eqbf_ctx* ctx, const eqbf_chunk* c, const eqbf_assoc_opts* o, eqbf_assoc_out* out) {
   if (!ctx || !c || !o || !out) return ctx_fail(ctx, EQBF_ERR_ARG, "null argument");
   EQBF_HIP(ctx, hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
-  const int64_t G = c->n_genes, S = c->n_subgroups, N = c->n_samples, NP = c->n_pairs;
-  const int64_t Qm = c->n_covars_max;
-  if (G < 0 || S < 1 || N < 1 || NP < 0) return ctx_fail(ctx, EQBF_ERR_ARG, "invalid chunk sizes");
-  const size_t esz = c->geno_dtype == EQBF_F32 ? 4 : 8;
-  const size_t geno_elems = (size_t)(c->n_geno_sets - 1) * c->geno_set_stride + (size_t)N * c->geno_ld;
+  const int64_t S = c->n_subgroups, NP = c->n_pairs;
   Stager sg{ctx, st};
-  eqbf_chunk d = *c;
-  d.explevels = sg.put("h_expl", c->explevels, (size_t)(G * S * N));
-  d.gene_in_sbgrp = sg.put("h_genein", c->gene_in_sbgrp, (size_t)(G * S));
-  d.geno_in_sbgrp = sg.put("h_genoin", c->geno_in_sbgrp, (size_t)(S * N));
-  d.covar_count = sg.put("h_covcnt", c->covar_count, (size_t)S);
-  d.covars = Qm > 0 ? sg.put("h_covars", c->covars, (size_t)(S * Qm * N)) : nullptr;
-  d.covar_in_sbgrp = sg.put("h_covin", c->covar_in_sbgrp, (size_t)(S * N));
-  d.geno_set_of_sbgrp = sg.put("h_gset", c->geno_set_of_sbgrp, (size_t)S);
-  d.genotypes = sg.put("h_geno", (const uint8_t*)c->genotypes, geno_elems * esz);
-  d.snp_in_sbgrp = sg.put("h_snpin", c->snp_in_sbgrp, (size_t)(c->n_snps * S));
-  d.pair_offset = sg.put("h_poff", c->pair_offset, (size_t)(G + 1));
-  d.pair_snp = sg.put("h_psnp", c->pair_snp, (size_t)NP);
-  if (sg.rc) return ctx_fail(ctx, sg.rc, "staging the chunk failed");
+  eqbf_chunk d;
+  int rc0 = stage_chunk(ctx, sg, c, &d);
+  if (rc0) return rc0;
   const bool join = o->analysis == EQBF_ANALYSIS_JOIN;
   const int GL = (int)ctx->phiL.size(), GS = (int)ctx->phiS.size();
   int64_t n_raw = join ? eqbf_n_raw(o->bfs, (int)S, GL, GS) : 0;

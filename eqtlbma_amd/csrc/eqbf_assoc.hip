@@ -37,7 +37,6 @@
 
 namespace {
 
-constexpr int NW = 4;     /* waves per workgroup in the mixed kernel */
 constexpr int TILE = 64;  /* pairs per tile (one per lane)           */
 constexpr float kHalfInvLn10 = 0.21714724095162590f; /* 1 / (2 ln 10) */
 
@@ -240,6 +239,143 @@ __global__ void __launch_bounds__(256) prep_kernel(const eqbf_chunk c, PrepRec* 
   }
 }
 
+/* ---- prep, one wave per (gene, subgroup): columns held in LDS ------------------ */
+/* Same computation as prep_kernel (which remains for very large N): member
+ * list, orthonormal basis of [1, covariates] by two-pass modified
+ * Gram-Schmidt, y~ = y - U U'y, y~'y~ and the total sum of squares; all
+ * reductions are wave-level, the coefficient rows are written once. */
+__global__ void __launch_bounds__(64) prep_wave_kernel(const eqbf_chunk c, PrepRec* __restrict__ rec,
+                                                       double* __restrict__ B, uint64_t* __restrict__ memb,
+                                                       int32_t* __restrict__ idxs, int* __restrict__ flag, int KU) {
+  extern __shared__ __align__(16) unsigned char psm[];
+  const int S = c.n_subgroups, N = c.n_samples;
+  const int g = blockIdx.x / S, s = blockIdx.x % S;
+  const int lane = threadIdx.x;
+  const int KC = KU + 1;
+  double* col = (double*)psm;                 /* [KC][N]: y~, U columns over members */
+  int* lidx = (int*)(col + (size_t)KC * N);   /* [N] member sample indices          */
+  PrepRec r;
+  r.n = 0;
+  r.flags = 0;
+  r.q = c.covar_count[s];
+  r.rank = 0;
+  r.yy = 0;
+  r.tss = 0;
+  if (!c.gene_in_sbgrp[(int64_t)g * S + s]) {
+    if (lane == 0) rec[(int64_t)g * S + s] = r;
+    return;
+  }
+  const double* y = c.explevels + ((int64_t)g * S + s) * N;
+  const uint8_t* gin = c.geno_in_sbgrp + (int64_t)s * N;
+  int32_t* idx = idxs + ((int64_t)g * S + s) * N;
+  int n = 0;
+  for (int base = 0; base < N; base += 64) {
+    const int i = base + lane;
+    const bool m = (i < N) && !isnan(y[i]) && gin[i];
+    const uint64_t bal = __ballot(m);
+    const int pre = __popcll(bal & ((1ull << lane) - 1ull));
+    if (m) {
+      lidx[n + pre] = i;
+      idx[n + pre] = i;
+    }
+    n += __popcll(bal);
+  }
+  __syncthreads();
+  const int Q = r.q;
+  r.n = n;
+  if (n == 0) {
+    if (lane == 0) rec[(int64_t)g * S + s] = r;
+    return;
+  }
+  if (Q > 0) {
+    int miss = 0;
+    for (int k = lane; k < n; k += 64) miss |= !c.covar_in_sbgrp[(int64_t)s * N + lidx[k]];
+    if (__ballot(miss)) {
+      if (lane == 0) {
+        atomicOr(flag, FLAG_MISSING_COVAR);
+        r.n = 0;
+        rec[(int64_t)g * S + s] = r;
+      }
+      return;
+    }
+  }
+  if (n < Q + 3) { /* N < P + 1 with P = Q + 2 (utils_math.cpp:175-178) */
+    if (lane == 0) {
+      r.flags = PREP_TOO_FEW;
+      rec[(int64_t)g * S + s] = r;
+    }
+    return;
+  }
+  /* load y and the columns [1, covariates] of the members */
+  double sy = 0;
+  for (int k = lane; k < n; k += 64) {
+    const int i = lidx[k];
+    col[k] = y[i];
+    sy += y[i];
+    col[(size_t)1 * N + k] = 1.0;
+    for (int j = 1; j <= Q; ++j) col[(size_t)(1 + j) * N + k] = c.covars[((int64_t)s * c.n_covars_max + (j - 1)) * N + i];
+    for (int j = Q + 1; j < KU; ++j) col[(size_t)(1 + j) * N + k] = 0.0;
+  }
+  const double ybar = wave_sum(sy) / n;
+  double ts = 0;
+  for (int k = lane; k < n; k += 64) {
+    const double d = col[k] - ybar;
+    ts += d * d;
+  }
+  const double tss = wave_sum(ts);
+  uint32_t kept = 0;
+  int rank = 0;
+  for (int j = 0; j <= Q; ++j) {
+    double* cj = col + (size_t)(1 + j) * N;
+    double nn = 0;
+    for (int k = lane; k < n; k += 64) nn += cj[k] * cj[k];
+    const double n0 = sqrt(wave_sum(nn));
+    for (int pass = 0; pass < 2; ++pass)
+      for (int l = 0; l < j; ++l) {
+        if (!((kept >> l) & 1u)) continue;
+        const double* cl = col + (size_t)(1 + l) * N;
+        double d = 0;
+        for (int k = lane; k < n; k += 64) d += cl[k] * cj[k];
+        d = wave_sum(d);
+        for (int k = lane; k < n; k += 64) cj[k] -= d * cl[k];
+      }
+    double nv = 0;
+    for (int k = lane; k < n; k += 64) nv += cj[k] * cj[k];
+    nv = sqrt(wave_sum(nv));
+    const bool keep = nv > 1e-9 * n0 && nv > 0;
+    const double sc = keep ? 1.0 / nv : 0.0;
+    for (int k = lane; k < n; k += 64) cj[k] *= sc;
+    if (keep) {
+      kept |= 1u << j;
+      ++rank;
+    }
+  }
+  for (int pass = 0; pass < 2; ++pass)
+    for (int l = 0; l <= Q; ++l) {
+      if (!((kept >> l) & 1u)) continue;
+      const double* cl = col + (size_t)(1 + l) * N;
+      double d = 0;
+      for (int k = lane; k < n; k += 64) d += cl[k] * col[k];
+      d = wave_sum(d);
+      for (int k = lane; k < n; k += 64) col[k] -= d * cl[k];
+    }
+  double yy = 0;
+  for (int k = lane; k < n; k += 64) yy += col[k] * col[k];
+  yy = wave_sum(yy);
+  for (int k = lane; k < n; k += 64) {
+    const int i = lidx[k];
+    double* dst = B + (((int64_t)g * N + i) * S + s) * KC;
+    for (int cc = 0; cc < KC; ++cc) dst[cc] = col[(size_t)cc * N + k];
+    atomicOr((unsigned long long*)&memb[(int64_t)g * N + i], 1ull << s);
+  }
+  if (lane == 0) {
+    r.rank = rank;
+    r.yy = yy;
+    r.tss = tss;
+    rec[(int64_t)g * S + s] = r;
+  }
+}
+
 /* ---- packed work list: tiles of up to 64 pairs of one gene ---------------------- */
 __global__ void __launch_bounds__(1024) tiles_kernel(const int64_t* __restrict__ pair_offset, int G,
                                                      int4* __restrict__ tiles, int* __restrict__ n_tiles) {
@@ -275,42 +411,63 @@ __global__ void __launch_bounds__(1024) tiles_kernel(const int64_t* __restrict__
   if (threadIdx.x == 0) *n_tiles = carry;
 }
 
-/* ---- Bayes factor of one configuration at one grid point (FP32) ---------------- */
-/* Follows CalcLog10AbfUvlr (gene_snp_pair.cpp:297-356): subgroups with
- * |t| < 1e-8 are skipped and absent ones are inactive.  An active subgroup
- * whose statistics are NaN makes the variance sum NaN, so the reference's
- * test `varbbarhat < infinity` fails and the ABF is 0 (:336-354). */
-__device__ inline float abf_eval(uint64_t mask, int S, const float* sv, const float* sbp, const float* sz2,
-                                 const float* slv, const uint8_t* sfl, int lane, float phi2, float oma2) {
-  float L = 0.f, A = 0.f, Bs = 0.f;
-  bool nan = false;
-  for (int s = 0; s < S; ++s) {
-    if (!((mask >> s) & 1ull)) continue;
-    int f = sfl[s * TILE + lane];
-    if (f == F_ABSENT || f == F_ZERO) continue;
-    if (f == F_NAN) {
-      nan = true;
-      break;
-    }
-    float v = sv[s * TILE + lane];
-    float vp = v + phi2;
-    float r = __builtin_amdgcn_rcpf(vp);
-    L += 0.5f * (slv[s * TILE + lane] - __log2f(vp) * 0.30102999566398120f) + kHalfInvLn10 * sz2[s * TILE + lane] * phi2 * r;
-    A += sbp[s * TILE + lane] * r;
-    Bs += r;
-  }
-  if (nan) return 0.f;
-  if (Bs == 0.f) return 0.f;
-  float bbar = A / Bs;
-  if (bbar == 0.f) return 0.f;
-  float V = 1.f / Bs;
-  float T2 = bbar * bbar / V;
+/* ---- Bayes-factor phase helpers (FP32 terms, FP64 grid averages) ------------ */
+__device__ inline float lg10(float x) { return __log2f(x) * 0.30102999566398120f; }
+
+/* Last step of CalcLog10AbfUvlr (gene_snp_pair.cpp:331-355) from the sums over
+ * the active subgroups: L = sum of single-subgroup terms, A = sum b/(v+phi2),
+ * Bs = sum 1/(v+phi2).  bbar = A/Bs and V = 1/Bs; the ABF is 0 when no
+ * subgroup contributes (Bs == 0), when bbar == 0, or when V is not finite. */
+__device__ inline float abf_finish(float L, float A, float Bs, float oma2) {
+  if (!(Bs > 0.f)) return 0.f;
+  const float bbar = A / Bs;
+  const float V = 1.f / Bs;
+  if (bbar == 0.f || !(V < INFINITY)) return 0.f;
+  const float T2 = bbar * bbar / V;
   float lbar = 0.f;
   if (T2 != 0.f) {
-    float Vo = V + oma2;
-    lbar = 0.5f * (__log2f(V) - __log2f(Vo)) * 0.30102999566398120f + kHalfInvLn10 * T2 * oma2 / Vo;
+    const float Vo = V + oma2;
+    lbar = 0.5f * (lg10(V) - lg10(Vo)) + kHalfInvLn10 * T2 * oma2 / Vo;
   }
   return lbar + L;
+}
+
+/* ABFs of configuration `mask` at up to BLK grid points (p2[k], o2[k]).
+ * Subgroup-outer loop: each subgroup's statistics are read once from LDS and
+ * applied to every point of the block.  Subgroups with |t| < 1e-8 are
+ * skipped and absent ones are inactive; an active subgroup with NaN
+ * statistics makes the variance sum NaN, so the ABF is 0 (:336-354). */
+template <int BLK>
+__device__ __noinline__ void abf_block(uint64_t mask, const float* sv, const float* sbp, const float* sz2,
+                                 const float* slv, const uint8_t* sfl, int lane, const float* p2,
+                                 const float* o2, int nb, float* out) {
+  float Lr[BLK], Ar[BLK], Br[BLK];
+#pragma unroll
+  for (int k = 0; k < BLK; ++k) Lr[k] = Ar[k] = Br[k] = 0.f;
+  bool nan = false;
+  uint64_t m = mask;
+  while (m) {
+    const int s = __ffsll((unsigned long long)m) - 1;
+    m &= m - 1ull;
+    const int f = sfl[s * TILE + lane];
+    if (f == F_NAN) nan = true;
+    if (f != F_NORMAL) continue;
+    const float v = sv[s * TILE + lane], bp = sbp[s * TILE + lane];
+    const float z2 = sz2[s * TILE + lane], lv = slv[s * TILE + lane];
+#pragma unroll
+    for (int k = 0; k < BLK; ++k) {
+      if (k < nb) {
+        const float vp = v + p2[k];
+        const float r = __builtin_amdgcn_rcpf(vp);
+        Lr[k] += 0.5f * (lv - lg10(vp)) + kHalfInvLn10 * z2 * p2[k] * r;
+        Ar[k] += bp * r;
+        Br[k] += r;
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < BLK; ++k)
+    if (k < nb) out[k] = nan ? 0.f : abf_finish(Lr[k], Ar[k], Br[k], o2[k]);
 }
 
 /* Online log10 of the mean of 10^v (utils::log10_weighted_sum semantics:
@@ -353,21 +510,110 @@ __device__ inline double l10_finish(float m, double s, bool nan, double weight) 
   return res;
 }
 
+__device__ inline void store_raw(const AssocParams& kp, int64_t row, int64_t NP, int64_t p, float l) {
+  const int64_t ro = row * NP + p;
+  if (kp.raw_f64)
+    ((double*)kp.raw)[ro] = l;
+  else
+    ((float*)kp.raw)[ro] = l;
+}
+
+/* Summary statistics of one (pair, subgroup) from its sums x'x, x'y~ and
+ * |U'x|^2 (FitSingleGeneWithSingleSnp, utils_math.cpp:166-209, then
+ * StandardizeSstatsAndCorrectSmallSampleSize, gene_snp_pair.cpp:256-290);
+ * writes the outputs and the tile's LDS copy of the standardised values.
+ * Kept out of line so that its temporaries do not stack on the live sums. */
+__device__ __noinline__ void pair_stats(const eqbf_chunk& c, const AssocParams& kp, double sxx, double sxy,
+                                        double proj, int g, int s, int64_t p, int64_t snp, bool valid, int lane,
+                                        float* sv, float* sbp, float* sz2, float* slv, uint8_t* sfl) {
+  const int S = c.n_subgroups;
+  const int64_t NP = c.n_pairs;
+  const PrepRec r = kp.rec[(int64_t)g * S + s];
+  const bool has = r.n > 0 && c.snp_in_sbgrp[snp * S + s];
+  int f = F_ABSENT;
+  double o5[5] = {NAN, NAN, NAN, NAN, NAN};
+  double st3[3] = {NAN, NAN, NAN};
+  int nout = 0;
+  if (has) {
+    nout = r.n;
+    f = F_NAN;
+    if (!(r.flags & PREP_TOO_FEW)) {
+      const double xx = sxx - proj;
+      const int n = r.n, P = r.q + 2;
+      if (xx > 1e-12 * sxx) {
+        const double beta = sxy / xx;
+        const double rss = r.yy - sxy * beta;
+        const int rank = r.rank + 1;
+        const double se = sqrt(rss / (double)(n - P) / xx);
+        const double sig = sqrt(rss / (double)(n - rank));
+        const double tstat = beta / se;
+        const double wv = dm::tz_eval(kp.tz, n - rank, fabs(tstat));
+        const double wz = (n - rank == n - P) ? wv : dm::tz_eval(kp.tz, n - P, fabs(tstat));
+        o5[0] = 1.0 - rss / r.tss;
+        o5[1] = sig;
+        o5[2] = beta;
+        o5[3] = se;
+        o5[4] = 2.0 * exp(-wv);
+        const double z = dm::gauss_pinv_from_w(wz);
+        const double bhat = beta / sig, sebhat = se / sig;
+        if (!isnan(bhat / sebhat)) {
+          if (fabs(z) > 1e-8) {
+            const double sig2 = fabs(beta) / (fabs(z) * sebhat);
+            const double b2 = beta / sig2;
+            st3[0] = b2;
+            st3[1] = fabs(b2 / z);
+            st3[2] = z;
+            f = F_NORMAL;
+          } else {
+            st3[0] = 0.0;
+            st3[1] = INFINITY;
+            st3[2] = z;
+            f = F_ZERO;
+          }
+        }
+      } else { /* genotype collinear with [1, covariates]: not estimable */
+        o5[0] = 1.0 - r.yy / r.tss;
+        o5[1] = sqrt(r.yy / (double)(n - r.rank));
+      }
+    }
+  }
+  if (valid) {
+    kp.n_out[(int64_t)s * NP + p] = nout;
+    if (kp.sstats)
+      for (int k = 0; k < 5; ++k) kp.sstats[((int64_t)k * S + s) * NP + p] = o5[k];
+    if (kp.stdsstats)
+      for (int k = 0; k < 3; ++k) kp.stdsstats[((int64_t)k * S + s) * NP + p] = st3[k];
+  }
+  const float v = (float)(st3[1] * st3[1]);
+  sv[s * TILE + lane] = v;
+  sbp[s * TILE + lane] = (float)st3[0];
+  sz2[s * TILE + lane] = (float)(st3[2] * st3[2]);
+  slv[s * TILE + lane] = (f == F_NORMAL) ? lg10(v) : 0.f;
+  sfl[s * TILE + lane] = (uint8_t)f;
+}
+
 /* ---- mixed path: persistent kernel over tiles ----------------------------------- */
-template <int SPW, int KU>
-__global__ void __launch_bounds__(256) assoc_mixed_kernel(const eqbf_chunk c, const AssocParams kp) {
+constexpr int CHMAX = 8;   /* samples per staged chunk                 */
+constexpr int UPRE = 4;    /* staged doubles prefetched per thread     */
+constexpr int XPRE = 2;    /* genotype samples prefetched per lane     */
+constexpr int BLK = 8;     /* grid points per register block           */
+
+template <int NWV, int SPW, int KU>
+__global__ void __launch_bounds__(NWV * 64) assoc_mixed_kernel(const eqbf_chunk c, const AssocParams kp) {
   extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int NT = NWV * 64;
+  constexpr int KC = KU + 1;
   const int S = c.n_subgroups, N = c.n_samples;
   const int64_t NP = c.n_pairs;
-  const int KC = KU + 1;
   const int CH = kp.CH;
   const int tid = threadIdx.x, lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-  /* LDS: during the sum phase two staging buffers of CH samples of B rows;
-   * afterwards the standardised statistics of the tile and the partial grid
-   * sums of the consistent configurations. */
-  double* bstage = (double*)smem;
-  const int stage_elems = CH * S * KC;
+  /* LDS, sum phase: two staging buffers, each CH samples of B rows followed
+   * by the CH membership words.  Afterwards: standardised statistics of the
+   * tile [S][64] and the partial grid sums of the consistent configuration. */
+  const int rows_d = CH * S * KC;
+  const int stage_d = rows_d + CH + CH * TILE / 2; /* + genotypes as floats */
+  double* bst = (double*)smem;
   float* sv = (float*)smem;
   float* sbp = sv + S * TILE;
   float* sz2 = sbp + S * TILE;
@@ -393,7 +639,7 @@ __global__ void __launch_bounds__(256) assoc_mixed_kernel(const eqbf_chunk c, co
     uint64_t wmask = 0;
 #pragma unroll
     for (int sl = 0; sl < SPW; ++sl) {
-      int s = w + NW * sl;
+      const int s = w + NWV * sl;
       if (s < S) wmask |= 1ull << s;
     }
     double acc[SPW][KU + 2];
@@ -401,41 +647,56 @@ __global__ void __launch_bounds__(256) assoc_mixed_kernel(const eqbf_chunk c, co
     for (int sl = 0; sl < SPW; ++sl)
 #pragma unroll
       for (int k = 0; k < KU + 2; ++k) acc[sl][k] = 0.0;
-
     const double* Bg = kp.B + (int64_t)g * N * S * KC;
     const uint64_t* mg = kp.memb + (int64_t)g * N;
     const int nchunk = (N + CH - 1) / CH;
-    /* stage chunk 0 */
+    /* stage chunk 0: B rows, membership words and the tile's genotypes
+     * (sample k of the chunk is loaded by wave k mod NWV) */
     {
-      int lim = min(CH, N) * S * KC;
-      for (int e = tid; e < lim; e += 256) bstage[e] = Bg[e];
+      const int ni = min(CH, N);
+      double* b0 = bst;
+      for (int e = tid; e < ni * S * KC; e += NT) b0[e] = Bg[e];
+      for (int e = tid; e < ni; e += NT) ((uint64_t*)(b0 + rows_d))[e] = mg[e];
+      float* x0 = (float*)(b0 + rows_d + CH);
+      for (int k = w; k < ni; k += NWV) x0[k * TILE + lane] = (float)geno_load(c, 0, k, snp);
     }
     __syncthreads();
     for (int ck = 0; ck < nchunk; ++ck) {
       const int i0 = ck * CH;
       const int ni = min(CH, N - i0);
-      double* cur = bstage + (ck & 1) * stage_elems;
-      double* nxt = bstage + ((ck + 1) & 1) * stage_elems;
+      const double* cur = bst + (ck & 1) * stage_d;
+      double* nxt = bst + ((ck + 1) & 1) * stage_d;
       /* prefetch the next chunk into registers */
-      double pre[8];
-      int nnext = 0;
+      double pre[UPRE];
+      float xpre[XPRE];
+      int nn = 0, nrow = 0, ni2 = 0;
       if (ck + 1 < nchunk) {
-        nnext = min(CH, N - i0 - CH) * S * KC;
+        ni2 = min(CH, N - i0 - CH);
+        nrow = ni2 * S * KC;
+        nn = nrow + ni2;
         const double* src = Bg + (int64_t)(i0 + CH) * S * KC;
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          int e = tid + u * 256;
-          pre[u] = (e < nnext) ? src[e] : 0.0;
+        for (int u = 0; u < UPRE; ++u) {
+          const int e = tid + u * NT;
+          pre[u] = (e < nrow) ? src[e] : (e < nn ? __longlong_as_double((long long)mg[i0 + CH + e - nrow]) : 0.0);
+        }
+#pragma unroll
+        for (int u = 0; u < XPRE; ++u) {
+          const int k = w + u * NWV;
+          xpre[u] = (k < ni2) ? (float)geno_load(c, 0, i0 + CH + k, snp) : 0.f;
         }
       }
+      const uint64_t* mcur = (const uint64_t*)(cur + rows_d);
+      const float* xcur = (const float*)(cur + rows_d + CH);
+#pragma unroll 1
       for (int ii = 0; ii < ni; ++ii) {
-        const int i = i0 + ii;
-        const uint64_t m = mg[i] & wmask;
+        const uint64_t m = mcur[ii] & wmask;
         if (!m) continue;
-        double x = geno_load(c, kp.multiset ? 0 : 0, i, snp);
+        const double x = (double)xcur[ii * TILE + lane];
+        const int i = i0 + ii;
 #pragma unroll
         for (int sl = 0; sl < SPW; ++sl) {
-          const int s = w + NW * sl;
+          const int s = w + NWV * sl;
           if ((m >> s) & 1ull) {
             double xs = x;
             if (kp.multiset) xs = geno_load(c, c.geno_set_of_sbgrp[s], i, snp);
@@ -449,135 +710,132 @@ __global__ void __launch_bounds__(256) assoc_mixed_kernel(const eqbf_chunk c, co
       }
       __syncthreads();
       if (ck + 1 < nchunk) {
+        /* rows at [0, nrow), membership words at [rows_d, rows_d + ni2) */
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          int e = tid + u * 256;
-          if (e < nnext) nxt[e] = pre[u];
+        for (int u = 0; u < UPRE; ++u) {
+          const int e = tid + u * NT;
+          if (e < nn) nxt[e < nrow ? e : rows_d + (e - nrow)] = pre[u];
         }
-        /* chunks larger than 8*256 elements are copied directly */
         const double* src = Bg + (int64_t)(i0 + CH) * S * KC;
-        for (int e = tid + 8 * 256; e < nnext; e += 256) nxt[e] = src[e];
+        for (int e = tid + UPRE * NT; e < nn; e += NT) {
+          if (e < nrow)
+            nxt[e] = src[e];
+          else
+            nxt[rows_d + (e - nrow)] = __longlong_as_double((long long)mg[i0 + CH + e - nrow]);
+        }
+        float* xn = (float*)(nxt + rows_d + CH);
+#pragma unroll
+        for (int u = 0; u < XPRE; ++u) {
+          const int k = w + u * NWV;
+          if (k < ni2) xn[k * TILE + lane] = xpre[u];
+        }
       }
       __syncthreads();
     }
-    /* ---- per (pair, subgroup) statistics (FP64) ---- */
+    /* ---- per (pair, subgroup) statistics (FP64), one call per subgroup ---- */
 #pragma unroll
     for (int sl = 0; sl < SPW; ++sl) {
-      const int s = w + NW * sl;
+      const int s = w + NWV * sl;
       if (s >= S) continue;
-      const PrepRec r = kp.rec[(int64_t)g * S + s];
-      const bool has = r.n > 0 && c.snp_in_sbgrp[snp * S + s];
-      int f = F_ABSENT;
-      double o5[5] = {NAN, NAN, NAN, NAN, NAN};
-      double st3[3] = {NAN, NAN, NAN};
-      int nout = 0;
-      if (has) {
-        nout = r.n;
-        f = F_NAN;
-        if (!(r.flags & PREP_TOO_FEW)) {
-          const double sxx = acc[sl][0], sxy = acc[sl][1];
-          double proj = 0;
+      double proj = 0;
 #pragma unroll
-          for (int q = 0; q < KU; ++q) proj += acc[sl][2 + q] * acc[sl][2 + q];
-          const double xx = sxx - proj;
-          const int n = r.n, P = r.q + 2;
-          if (xx > 1e-12 * sxx) {
-            const double beta = sxy / xx;
-            const double rss = r.yy - sxy * beta;
-            const int rank = r.rank + 1;
-            const double s2 = rss / (double)(n - P);
-            const double se = sqrt(s2 / xx);
-            const double sig = sqrt(rss / (double)(n - rank));
-            const double tstat = beta / se;
-            const double wv = dm::tz_eval(kp.tz, n - rank, fabs(tstat));
-            const double wz = (n - rank == n - P) ? wv : dm::tz_eval(kp.tz, n - P, fabs(tstat));
-            o5[0] = 1.0 - rss / r.tss;
-            o5[1] = sig;
-            o5[2] = beta;
-            o5[3] = se;
-            o5[4] = 2.0 * exp(-wv);
-            /* StandardizeSstatsAndCorrectSmallSampleSize */
-            const double z = dm::gauss_pinv_from_w(wz);
-            const double bhat = beta / sig, sebhat = se / sig;
-            if (!isnan(bhat / sebhat)) {
-              if (fabs(z) > 1e-8) {
-                const double sig2 = fabs(beta) / (fabs(z) * sebhat);
-                const double b2 = beta / sig2;
-                st3[0] = b2;
-                st3[1] = fabs(b2 / z);
-                st3[2] = z;
-                f = F_NORMAL;
-              } else {
-                st3[0] = 0.0;
-                st3[1] = INFINITY;
-                st3[2] = z;
-                f = F_ZERO;
-              }
-            }
-          } else { /* genotype collinear with [1, covariates]: not estimable */
-            o5[0] = 1.0 - r.yy / r.tss;
-            o5[1] = sqrt(r.yy / (double)(n - r.rank));
-          }
-        }
-      }
-      if (valid) {
-        kp.n_out[(int64_t)s * NP + p] = nout;
-        if (kp.sstats)
-          for (int k = 0; k < 5; ++k) kp.sstats[((int64_t)k * S + s) * NP + p] = o5[k];
-        if (kp.stdsstats)
-          for (int k = 0; k < 3; ++k) kp.stdsstats[((int64_t)k * S + s) * NP + p] = st3[k];
-      }
-      float v = (float)(st3[1] * st3[1]);
-      sv[s * TILE + lane] = v;
-      sbp[s * TILE + lane] = (float)st3[0];
-      sz2[s * TILE + lane] = (float)(st3[2] * st3[2]);
-      slv[s * TILE + lane] = (f == F_NORMAL) ? __log2f(v) * 0.30102999566398120f : 0.f;
-      sfl[s * TILE + lane] = (uint8_t)f;
+      for (int q = 0; q < KU; ++q) proj += acc[sl][2 + q] * acc[sl][2 + q];
+      pair_stats(c, kp, acc[sl][0], acc[sl][1], proj, g, s, p, snp, valid, lane, sv, sbp, sz2, slv, sfl);
     }
     __syncthreads();
     if (kp.join) {
-      /* ---- Bayes factors: 12 quarter-jobs of the consistent configuration
-       * (gen, gen-fix, gen-maxh x 4 grid ranges) then one job per
-       * configuration (CalcAbfsUvlrFor*, gene_snp_pair.cpp:364-550). ---- */
+      /* ---- Bayes factors (CalcAbfsUvlrFor*, gene_snp_pair.cpp:364-550):
+       * 12 jobs for the consistent configuration (gen, gen-fix, gen-maxh x 4
+       * grid ranges) then one job per configuration. ---- */
       const uint64_t all = (S >= 64) ? ~0ull : ((1ull << S) - 1ull);
       const int njobs = 12 + kp.n_cfg;
-      for (int j = w; j < njobs; j += NW) {
+      for (int j = w; j < njobs; j += NWV) {
         L10Acc a;
         a.init();
         if (j < 12) {
           const int type = j >> 2, qtr = j & 3;
           const int ga = (kp.GL * qtr) >> 2, gb = (kp.GL * (qtr + 1)) >> 2;
-          for (int gi = ga; gi < gb; ++gi) {
-            float ph = (float)phiL[gi], om = (float)omaL[gi];
-            float p2 = type == 0 ? ph : (type == 1 ? 0.f : (float)(phiL[gi] + omaL[gi]));
-            float o2 = type == 0 ? om : (type == 1 ? (float)(phiL[gi] + omaL[gi]) : 0.f);
-            float l = abf_eval(all, S, sv, sbp, sz2, slv, sfl, lane, p2, o2);
-            if (valid && kp.raw) {
-              int64_t ro = (int64_t)(type * kp.GL + gi) * NP + p;
-              if (kp.raw_f64)
-                ((double*)kp.raw)[ro] = l;
-              else
-                ((float*)kp.raw)[ro] = l;
+          if (type == 1) {
+            /* gen-fix: phi2 = 0 makes every single-subgroup term 0 and the
+             * sums independent of the grid point */
+            float A = 0.f, Bs = 0.f;
+            bool nan = false;
+            for (int s = 0; s < S; ++s) {
+              const int f = sfl[s * TILE + lane];
+              if (f == F_NAN) nan = true;
+              if (f != F_NORMAL) continue;
+              const float r = __builtin_amdgcn_rcpf(sv[s * TILE + lane]);
+              A += sbp[s * TILE + lane] * r;
+              Bs += r;
             }
-            a.add(l);
+            for (int gi = ga; gi < gb; ++gi) {
+              const float l = nan ? 0.f : abf_finish(0.f, A, Bs, (float)(phiL[gi] + omaL[gi]));
+              if (valid && kp.raw) store_raw(kp, kp.GL + gi, NP, p, l);
+              a.add(l);
+            }
+          } else {
+            for (int g0 = ga; g0 < gb; g0 += BLK) {
+              const int nb = min(BLK, gb - g0);
+              float p2[BLK], o2[BLK], out[BLK];
+#pragma unroll
+              for (int k = 0; k < BLK; ++k) {
+                const int gi = min(g0 + k, kp.GL - 1);
+                p2[k] = type == 0 ? (float)phiL[gi] : (float)(phiL[gi] + omaL[gi]);
+                o2[k] = type == 0 ? (float)omaL[gi] : 0.f;
+              }
+              abf_block<BLK>(all, sv, sbp, sz2, slv, sfl, lane, p2, o2, nb, out);
+#pragma unroll
+              for (int k = 0; k < BLK; ++k)
+                if (k < nb) {
+                  if (valid && kp.raw) store_raw(kp, type * kp.GL + g0 + k, NP, p, out[k]);
+                  a.add(out[k]);
+                }
+            }
           }
           double* pp = part + ((type * 4 + qtr) * 2) * TILE;
-          /* partial: max (NaN flags the first value NaN) and rescaled sum */
           pp[lane] = a.first ? -INFINITY : (a.nan0 ? NAN : (double)a.m);
           pp[TILE + lane] = a.s;
         } else {
           const int cfg = j - 12;
           const uint64_t mk = kp.masks[cfg];
-          for (int gi = 0; gi < kp.GS; ++gi) {
-            float l = abf_eval(mk, S, sv, sbp, sz2, slv, sfl, lane, (float)phiS[gi], (float)omaS[gi]);
-            if (valid && kp.raw) {
-              int64_t ro = (int64_t)(3 * kp.GL + cfg * kp.GS + gi) * NP + p;
-              if (kp.raw_f64)
-                ((double*)kp.raw)[ro] = l;
-              else
-                ((float*)kp.raw)[ro] = l;
+          const int64_t row0 = 3 * (int64_t)kp.GL + (int64_t)cfg * kp.GS;
+          if (__popcll(mk) == 1) {
+            /* singleton: with one active subgroup bbar = b, V = v + phi2 and
+             * the two log terms telescope:
+             * ABF = (log10 v - log10(v+phi2+oma2))/2
+             *       + z^2 (phi2/(v+phi2) + v oma2/((v+phi2)(v+phi2+oma2)))/(2 ln 10) */
+            const int s = __ffsll((unsigned long long)mk) - 1;
+            const int f = sfl[s * TILE + lane];
+            const float v = sv[s * TILE + lane], z2 = sz2[s * TILE + lane], lv = slv[s * TILE + lane];
+            for (int gi = 0; gi < kp.GS; ++gi) {
+              float l = 0.f;
+              if (f == F_NORMAL) {
+                const float p2 = (float)phiS[gi], o2 = (float)omaS[gi];
+                const float vp = v + p2, va = vp + o2;
+                const float r1 = __builtin_amdgcn_rcpf(vp), r2 = __builtin_amdgcn_rcpf(va);
+                l = 0.5f * (lv - lg10(va)) + kHalfInvLn10 * z2 * (p2 * r1 + v * o2 * r1 * r2);
+              }
+              if (valid && kp.raw) store_raw(kp, row0 + gi, NP, p, l);
+              a.add(l);
             }
-            a.add(l);
+          } else {
+            for (int g0 = 0; g0 < kp.GS; g0 += BLK) {
+              const int nb = min(BLK, kp.GS - g0);
+              float p2[BLK], o2[BLK], out[BLK];
+#pragma unroll
+              for (int k = 0; k < BLK; ++k) {
+                const int gi = min(g0 + k, kp.GS - 1);
+                p2[k] = (float)phiS[gi];
+                o2[k] = (float)omaS[gi];
+              }
+              abf_block<BLK>(mk, sv, sbp, sz2, slv, sfl, lane, p2, o2, nb, out);
+#pragma unroll
+              for (int k = 0; k < BLK; ++k)
+                if (k < nb) {
+                  if (valid && kp.raw) store_raw(kp, row0 + g0 + k, NP, p, out[k]);
+                  a.add(out[k]);
+                }
+            }
           }
           if (valid) kp.w[(int64_t)(3 + cfg) * NP + p] = l10_finish(a.m, a.s, a.nan0, 1.0 / kp.GS);
         }
@@ -588,14 +846,16 @@ __global__ void __launch_bounds__(256) assoc_mixed_kernel(const eqbf_chunk c, co
         double M = -INFINITY;
         bool nan = false;
         for (int q = 0; q < 4; ++q) {
-          double m = part[((type * 4 + q) * 2) * TILE + lane];
-          if (isnan(m)) nan = true;
-          else if (m > M) M = m;
+          const double m = part[((type * 4 + q) * 2) * TILE + lane];
+          if (isnan(m))
+            nan = true;
+          else if (m > M)
+            M = m;
         }
         double ssum = 0;
         for (int q = 0; q < 4; ++q) {
-          double m = part[((type * 4 + q) * 2) * TILE + lane];
-          double sq = part[((type * 4 + q) * 2 + 1) * TILE + lane];
+          const double m = part[((type * 4 + q) * 2) * TILE + lane];
+          const double sq = part[((type * 4 + q) * 2 + 1) * TILE + lane];
           if (!isnan(m) && m > -INFINITY) ssum += sq * (double)exp2f((float)(m - M) * 3.32192809488736235f);
         }
         if (valid) kp.w[(int64_t)type * NP + p] = l10_finish((float)M, ssum, nan, 1.0 / kp.GL);
@@ -815,21 +1075,22 @@ __global__ void __launch_bounds__(64) assoc_plain_kernel(const eqbf_chunk c, con
   }
 }
 
-template <int SPW, int KU>
+template <int NWV, int SPW, int KU>
 void launch_mixed(const eqbf_chunk& c, const AssocParams& kp, int grid, size_t lds, hipStream_t st) {
-  hipLaunchKernelGGL((assoc_mixed_kernel<SPW, KU>), dim3(grid), dim3(256), lds, st, c, kp);
+  hipLaunchKernelGGL((assoc_mixed_kernel<NWV, SPW, KU>), dim3(grid), dim3(NWV * 64), lds, st, c, kp);
 }
 
+/* Kernel variant: 4 waves per workgroup for S <= 8, else 8 waves, each wave
+ * summing ceil(S / waves) subgroups. */
 template <int KU>
-int dispatch_spw(int spw, const eqbf_chunk& c, const AssocParams& kp, int grid, size_t lds, hipStream_t st) {
-  switch (spw) {
-    case 1: launch_mixed<1, KU>(c, kp, grid, lds, st); return 0;
-    case 2: launch_mixed<2, KU>(c, kp, grid, lds, st); return 0;
-    case 4: launch_mixed<4, KU>(c, kp, grid, lds, st); return 0;
-    case 8: launch_mixed<8, KU>(c, kp, grid, lds, st); return 0;
-    case 12: launch_mixed<12, KU>(c, kp, grid, lds, st); return 0;
-    case 16: launch_mixed<16, KU>(c, kp, grid, lds, st); return 0;
-  }
+int dispatch_mixed(int S, const eqbf_chunk& c, const AssocParams& kp, int grid, size_t lds, hipStream_t st) {
+  if (S <= 4) return launch_mixed<4, 1, KU>(c, kp, grid, lds, st), 0;
+  if (S <= 8) return launch_mixed<4, 2, KU>(c, kp, grid, lds, st), 0;
+  if (S <= 16) return launch_mixed<8, 2, KU>(c, kp, grid, lds, st), 0;
+  if (S <= 24) return launch_mixed<8, 3, KU>(c, kp, grid, lds, st), 0;
+  if (S <= 32) return launch_mixed<8, 4, KU>(c, kp, grid, lds, st), 0;
+  if (S <= 48) return launch_mixed<8, 6, KU>(c, kp, grid, lds, st), 0;
+  if (S <= 64) return launch_mixed<8, 8, KU>(c, kp, grid, lds, st), 0;
   return -1;
 }
 
@@ -891,7 +1152,12 @@ int assoc_uvlr_run(eqbf_ctx* ctx, const eqbf_chunk* cp, const eqbf_assoc_opts* o
   hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, (uint64_t*)ctx->d_flag.ptr, (int64_t)1);
   {
     ProfScope ps(ctx, "prep_kernel", st);
-    hipLaunchKernelGGL(prep_kernel, dim3(G * S), dim3(256), 0, st, c, rec, B, memb, idx, (int*)ctx->d_flag.ptr, KU);
+    const size_t plds = (size_t)N * (KC * sizeof(double) + sizeof(int));
+    if (plds <= 64 * 1024)
+      hipLaunchKernelGGL(prep_wave_kernel, dim3(G * S), dim3(64), plds, st, c, rec, B, memb, idx,
+                         (int*)ctx->d_flag.ptr, KU);
+    else
+      hipLaunchKernelGGL(prep_kernel, dim3(G * S), dim3(256), 0, st, c, rec, B, memb, idx, (int*)ctx->d_flag.ptr, KU);
   }
   EQBF_HIP(ctx, hipGetLastError());
 
@@ -951,26 +1217,24 @@ int assoc_uvlr_run(eqbf_ctx* ctx, const eqbf_chunk* cp, const eqbf_assoc_opts* o
     ProfScope ps(ctx, "tiles_kernel", st);
     hipLaunchKernelGGL(tiles_kernel, dim3(1), dim3(1024), 0, st, c.pair_offset, G, tiles, n_tiles);
   }
-  int spw = (S + NW - 1) / NW;
-  int spw_t = spw <= 1 ? 1 : spw <= 2 ? 2 : spw <= 4 ? 4 : spw <= 8 ? 8 : spw <= 12 ? 12 : 16;
-  /* LDS: max(staging of two CH-sample chunks, statistics + partial sums) */
-  int CH = 8;
-  while (CH > 1 && (size_t)2 * CH * S * KC * sizeof(double) > 48 * 1024) CH >>= 1;
+  /* LDS: max(two staging buffers of CH samples, statistics + partial sums) */
+  int CH = CHMAX;
+  while (CH > 1 && (size_t)2 * (CH * S * KC + CH + CH * TILE / 2) * sizeof(double) > 48 * 1024) CH >>= 1;
   kp.CH = CH;
-  size_t lds_stage = (size_t)2 * CH * S * KC * sizeof(double);
+  size_t lds_stage = (size_t)2 * (CH * S * KC + CH + CH * TILE / 2) * sizeof(double);
   size_t lds_stats = (((size_t)S * TILE * 17 + 15) / 16) * 16 + (size_t)3 * 4 * 2 * TILE * sizeof(double);
   size_t lds = std::max(lds_stage, lds_stats);
   int grid = ctx->n_cu * 2;
   grid = (grid + 7) / 8 * 8;
   int r2;
   {
-  ProfScope ps(ctx, "assoc_mixed_kernel", st);
-  if (KU == 1)
-    r2 = dispatch_spw<1>(spw_t, c, kp, grid, lds, st);
-  else if (KU == 4)
-    r2 = dispatch_spw<4>(spw_t, c, kp, grid, lds, st);
-  else
-    r2 = dispatch_spw<8>(spw_t, c, kp, grid, lds, st);
+    ProfScope ps(ctx, "assoc_mixed_kernel", st);
+    if (KU == 1)
+      r2 = dispatch_mixed<1>(S, c, kp, grid, lds, st);
+    else if (KU == 4)
+      r2 = dispatch_mixed<4>(S, c, kp, grid, lds, st);
+    else
+      r2 = dispatch_mixed<8>(S, c, kp, grid, lds, st);
   }
   if (r2) return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED, "no kernel variant for S=%d", S);
   EQBF_HIP(ctx, hipGetLastError());

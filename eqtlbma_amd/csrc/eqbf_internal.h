@@ -253,5 +253,7 @@ __device__ inline double tz_eval(const double* __restrict__ tz, int nu, double a
 /* ---- kernels launched from the C-ABI layer ------------------------------- */
 int assoc_uvlr_run(eqbf_ctx* ctx, const eqbf_chunk* c, const eqbf_assoc_opts* o, eqbf_assoc_out* out,
                    hipStream_t st);
+int perm_stats_run(eqbf_ctx* ctx, const eqbf_chunk* c, const eqbf_perm_opts* po, const int32_t* tables_dev,
+                   int32_t n_tables, const int32_t* gene_table, int64_t nperm, double* stats_dev, hipStream_t st);
 
 #endif

@@ -1,0 +1,241 @@
+/*
+ * Permutation statistics, general path.
+ *
+ * Reference loops (eqtlbma 1.3.3): Gene::MakePermutationsJoin
+ * (gene.cpp:598-717), MakePermutationsSepAllSubgroups (gene.cpp:493-570),
+ * MakePermutationsSepPerSubgroup (gene.cpp:380-450).  Under a permutation
+ * the expression level used for sample i is the one of sample perm(i), while
+ * genotypes and covariates stay with i (gene_snp_pair.cpp:104-149); which
+ * samples are used can therefore change from one permutation to the next.
+ *
+ * Each (gene, permutation) is laid out as a virtual gene whose expression
+ * levels are gathered through the permutation, and the virtual genes of a
+ * batch go through the association engine (prep, sums, Bayes factors) in
+ * one call.  A reduction per virtual gene then forms the statistic:
+ *   join: max over SNPs of the grid-averaged ABF[pbf] (std::max_element
+ *         semantics: NaN only when the first SNP's value is NaN,
+ *         gene.cpp:677-678) or its log10 average over SNPs
+ *         (utils::log10_weighted_sum, gene.cpp:689-690);
+ *   sep:  minimum p-value over SNPs (and subgroups for --permsep 1).
+ * This path is exact for any input (missing values, uneven membership,
+ * covariates); a faster contraction path for full membership is separate.
+ */
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "eqbf_internal.h"
+
+namespace {
+
+__global__ void perm_gather_kernel(const double* __restrict__ y, const uint8_t* __restrict__ gene_in,
+                                   const int32_t* __restrict__ tables, const int32_t* __restrict__ vg_gene,
+                                   const int32_t* __restrict__ vg_table, const int32_t* __restrict__ vg_perm,
+                                   int64_t nperm, int S, int N, double* __restrict__ y_v,
+                                   uint8_t* __restrict__ gene_in_v) {
+  const int v = blockIdx.x / S, s = blockIdx.x % S;
+  const int g = vg_gene[v];
+  const int32_t* perm = tables + ((int64_t)vg_table[v] * nperm + vg_perm[v]) * N;
+  const double* src = y + ((int64_t)g * S + s) * N;
+  double* dst = y_v + ((int64_t)v * S + s) * N;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) dst[i] = src[perm[i]];
+  if (threadIdx.x == 0) gene_in_v[(int64_t)v * S + s] = gene_in[(int64_t)g * S + s];
+}
+
+__global__ void perm_pairs_kernel(const int64_t* __restrict__ pair_offset, const int32_t* __restrict__ pair_snp,
+                                  const int32_t* __restrict__ vg_gene, const int64_t* __restrict__ off_v,
+                                  int32_t* __restrict__ pair_snp_v) {
+  const int v = blockIdx.x;
+  const int g = vg_gene[v];
+  const int64_t a = pair_offset[g], n = pair_offset[g + 1] - a, d = off_v[v];
+  for (int64_t k = threadIdx.x; k < n; k += blockDim.x) pair_snp_v[d + k] = pair_snp[a + k];
+}
+
+__device__ inline double wmax(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ inline double wmin(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ inline double wsum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+/* One wave per virtual gene. */
+__global__ void __launch_bounds__(64) perm_reduce_kernel(
+    const int64_t* __restrict__ off_v, const int32_t* __restrict__ vg_gene, const int32_t* __restrict__ vg_perm,
+    const int32_t* __restrict__ pair_snp_v, const int32_t* __restrict__ n_out, const double* __restrict__ sstats,
+    const double* __restrict__ w, const uint8_t* __restrict__ gene_in, const uint8_t* __restrict__ snp_in,
+    int64_t NP, int S, int join, int w_row, int use_max, int sep_sbgrp, int64_t nperm,
+    double* __restrict__ stats) {
+  const int v = blockIdx.x, lane = threadIdx.x;
+  const int g = vg_gene[v];
+  const int64_t a = off_v[v], b = off_v[v + 1];
+  auto val = [&](int64_t q) -> double {
+    if (join) return w[(int64_t)w_row * NP + q];
+    const int64_t snp = pair_snp_v[q];
+    if (sep_sbgrp >= 0) {
+      const int s = sep_sbgrp;
+      if (!(gene_in[(int64_t)g * S + s] && snp_in[snp * S + s])) return 1.0;
+      return n_out[(int64_t)s * NP + q] > 0 ? sstats[((int64_t)4 * S + s) * NP + q] : 1.0;
+    }
+    double mn = 1.0;
+    for (int s = 0; s < S; ++s) {
+      if (!(gene_in[(int64_t)g * S + s] && snp_in[snp * S + s]) || n_out[(int64_t)s * NP + q] <= 0) continue;
+      const double pv = sstats[((int64_t)4 * S + s) * NP + q];
+      if (pv < mn) mn = pv;
+    }
+    return mn;
+  };
+  double res;
+  if (b <= a) {
+    res = NAN;
+  } else {
+    const double v0 = val(a);
+    if (join && !use_max) {
+      /* utils::log10_weighted_sum over the SNPs */
+      double mx = -INFINITY;
+      for (int64_t q = a + lane; q < b; q += 64) {
+        const double x = val(q);
+        if (!isnan(x)) mx = fmax(mx, x);
+      }
+      mx = wmax(mx);
+      if (isnan(v0)) {
+        res = NAN;
+      } else {
+        double sum = 0;
+        for (int64_t q = a + lane; q < b; q += 64) {
+          const double x = val(q);
+          if (!isnan(x)) sum += exp10(x - mx);
+        }
+        sum = wsum(sum) / (double)(b - a);
+        res = mx + log10(sum);
+        if (fabs(res) <= 2.220446049250313e-16) res = 0.0;
+      }
+    } else {
+      /* std::max_element / std::min_element: NaN only if the first is NaN */
+      double m = join ? -INFINITY : INFINITY;
+      for (int64_t q = a + lane; q < b; q += 64) {
+        const double x = val(q);
+        if (!isnan(x)) m = join ? fmax(m, x) : fmin(m, x);
+      }
+      m = join ? wmax(m) : wmin(m);
+      res = isnan(v0) ? NAN : m;
+    }
+  }
+  if (lane == 0) stats[(int64_t)g * nperm + vg_perm[v]] = res;
+}
+
+}  // namespace
+
+static int pbf_bfs(int pbf) { return pbf == EQBF_PBF_GEN ? EQBF_BFS_GEN : (pbf == EQBF_PBF_GENSIN ? EQBF_BFS_SIN : EQBF_BFS_ALL); }
+
+int perm_stats_run(eqbf_ctx* ctx, const eqbf_chunk* cp, const eqbf_perm_opts* po, const int32_t* tables_dev,
+                   int32_t n_tables, const int32_t* gene_table, int64_t nperm, double* stats_dev, hipStream_t st) {
+  const eqbf_chunk& c = *cp;
+  const int S = c.n_subgroups, N = c.n_samples, G = c.n_genes;
+  const bool join = po->analysis == EQBF_ANALYSIS_JOIN;
+  if (nperm <= 0 || G <= 0) return EQBF_OK;
+  for (int g = 0; g < G; ++g)
+    if (gene_table[g] >= n_tables) return ctx_fail(ctx, EQBF_ERR_ARG, "gene_table[%d] = %d out of range", g, gene_table[g]);
+  std::vector<int64_t> off(G + 1);
+  EQBF_HIP(ctx, hipMemcpyAsync(off.data(), c.pair_offset, (G + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  EQBF_HIP(ctx, hipStreamSynchronize(st));
+  /* every statistic starts as NaN (skipped genes keep it) */
+  {
+    std::vector<double> nanrow((size_t)G * nperm, NAN);
+    EQBF_HIP(ctx, hipMemcpyAsync(stats_dev, nanrow.data(), nanrow.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    EQBF_HIP(ctx, hipStreamSynchronize(st));
+  }
+  const int bfs = join ? pbf_bfs(po->pbf) : EQBF_BFS_GEN;
+  const int64_t n_cfg = join ? eqbf_n_configs(bfs, S) : 0;
+  if (n_cfg < 0) return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED, "--pbf all with %d subgroups", S);
+  const int64_t n_w = 3 + n_cfg + 2;
+  const int w_row = !join ? 0 : (po->pbf == EQBF_PBF_GEN ? 0 : (po->pbf == EQBF_PBF_GENSIN ? (int)(3 + n_cfg) : (int)(3 + n_cfg + 1)));
+  /* batch: at most 64k virtual genes and a bounded number of virtual pairs */
+  const int64_t per_pair_bytes = (int64_t)S * 4 + (join ? n_w * 8 : (int64_t)5 * S * 8) + 4;
+  const int64_t pmax = std::max<int64_t>(1 << 16, std::min<int64_t>(8 << 20, ((int64_t)3 << 30) / per_pair_bytes));
+  std::vector<int32_t> vg_gene, vg_table, vg_perm;
+  std::vector<int64_t> off_v;
+  eqbf_assoc_opts ao;
+  ao.analysis = join ? EQBF_ANALYSIS_JOIN : EQBF_ANALYSIS_SEP;
+  ao.bfs = bfs;
+  ao.precision = po->precision;
+  ao.raw_dtype = EQBF_F32;
+  auto flush = [&]() -> int {
+    const int VB = (int)vg_gene.size();
+    if (VB == 0) return EQBF_OK;
+    const int64_t NPv = off_v.back();
+    int32_t* d_vg = (int32_t*)ctx_buf(ctx, "p_vg", (size_t)VB * 3 * sizeof(int32_t));
+    int64_t* d_off = (int64_t*)ctx_buf(ctx, "p_off", (size_t)(VB + 1) * sizeof(int64_t));
+    double* d_y = (double*)ctx_buf(ctx, "p_y", (size_t)VB * S * N * sizeof(double));
+    uint8_t* d_gin = (uint8_t*)ctx_buf(ctx, "p_gin", (size_t)VB * S);
+    int32_t* d_psnp = (int32_t*)ctx_buf(ctx, "p_psnp", (size_t)std::max<int64_t>(NPv, 1) * sizeof(int32_t));
+    int32_t* d_n = (int32_t*)ctx_buf(ctx, "p_n", (size_t)S * std::max<int64_t>(NPv, 1) * sizeof(int32_t));
+    double* d_w = join ? (double*)ctx_buf(ctx, "p_w", (size_t)n_w * std::max<int64_t>(NPv, 1) * sizeof(double)) : nullptr;
+    double* d_ss = !join ? (double*)ctx_buf(ctx, "p_ss", (size_t)5 * S * std::max<int64_t>(NPv, 1) * sizeof(double)) : nullptr;
+    if (!d_vg || !d_off || !d_y || !d_gin || !d_psnp || !d_n || (join && !d_w) || (!join && !d_ss)) return EQBF_ERR_HIP;
+    std::vector<int32_t> packed;
+    packed.insert(packed.end(), vg_gene.begin(), vg_gene.end());
+    packed.insert(packed.end(), vg_table.begin(), vg_table.end());
+    packed.insert(packed.end(), vg_perm.begin(), vg_perm.end());
+    EQBF_HIP(ctx, hipMemcpyAsync(d_vg, packed.data(), packed.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    EQBF_HIP(ctx, hipMemcpyAsync(d_off, off_v.data(), off_v.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    {
+      ProfScope ps(ctx, "perm_gather_kernel", st);
+      hipLaunchKernelGGL(perm_gather_kernel, dim3(VB * S), dim3(256), 0, st, c.explevels, c.gene_in_sbgrp,
+                         tables_dev, d_vg, d_vg + VB, d_vg + 2 * VB, nperm, S, N, d_y, d_gin);
+      hipLaunchKernelGGL(perm_pairs_kernel, dim3(VB), dim3(256), 0, st, c.pair_offset, c.pair_snp, d_vg, d_off,
+                         d_psnp);
+    }
+    EQBF_HIP(ctx, hipGetLastError());
+    eqbf_chunk vc = c;
+    vc.n_genes = VB;
+    vc.n_pairs = NPv;
+    vc.explevels = d_y;
+    vc.gene_in_sbgrp = d_gin;
+    vc.pair_offset = d_off;
+    vc.pair_snp = d_psnp;
+    eqbf_assoc_out vo;
+    memset(&vo, 0, sizeof vo);
+    vo.n = d_n;
+    vo.sstats = d_ss;
+    vo.w_abf = d_w;
+    int rc = assoc_uvlr_run(ctx, &vc, &ao, &vo, st);
+    if (rc) return rc;
+    {
+      ProfScope ps(ctx, "perm_reduce_kernel", st);
+      hipLaunchKernelGGL(perm_reduce_kernel, dim3(VB), dim3(64), 0, st, d_off, d_vg, d_vg + 2 * VB, d_psnp, d_n,
+                         d_ss, d_w, c.gene_in_sbgrp, c.snp_in_sbgrp, NPv, S, (int)join, w_row, po->use_max_bf,
+                         po->sep_sbgrp, nperm, stats_dev);
+    }
+    EQBF_HIP(ctx, hipGetLastError());
+    /* the host vectors are reused for the next batch */
+    EQBF_HIP(ctx, hipStreamSynchronize(st));
+    vg_gene.clear();
+    vg_table.clear();
+    vg_perm.clear();
+    off_v.assign(1, 0);
+    return EQBF_OK;
+  };
+  off_v.assign(1, 0);
+  for (int g = 0; g < G; ++g) {
+    if (gene_table[g] < 0) continue;
+    const int64_t np = off[g + 1] - off[g];
+    if (np <= 0) continue;
+    for (int64_t p = 0; p < nperm; ++p) {
+      if ((int64_t)vg_gene.size() >= 65536 || off_v.back() + np > pmax) {
+        int rc = flush();
+        if (rc) return rc;
+      }
+      vg_gene.push_back(g);
+      vg_table.push_back(gene_table[g]);
+      vg_perm.push_back((int32_t)p);
+      off_v.push_back(off_v.back() + np);
+    }
+  }
+  return flush();
+}

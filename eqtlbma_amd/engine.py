@@ -42,8 +42,69 @@ def lib():
         L.eqbf_assoc_uvlr.argtypes = [vp, vp, vp, vp]
         L.eqbf_assoc_uvlr_device.argtypes = [vp, vp, vp, vp, vp]
         L.eqbf_debug_ttail.argtypes = [vp, vp, vp, i32, vp, vp, vp, vp]
+        L.eqbf_perm_stats.argtypes = [vp, vp, vp, vp, i32, vp, i64, vp]
+        L.eqbf_perm_stats_device.argtypes = [vp, vp, vp, vp, i32, vp, i64, vp, vp]
+        L.eqbf_rng_seed.argtypes = [vp, C.c_uint64]
+        L.eqbf_rng_get.argtypes = [vp]
+        L.eqbf_rng_get.restype = C.c_uint32
+        L.eqbf_rng_uniform_int.argtypes = [vp, C.c_uint32]
+        L.eqbf_rng_uniform_int.restype = C.c_uint32
+        L.eqbf_rng_flat.argtypes = [vp, C.c_double, C.c_double]
+        L.eqbf_rng_flat.restype = C.c_double
+        L.eqbf_rng_shuffle.argtypes = [vp, vp, i64]
+        L.eqbf_perm_tables.argtypes = [vp, i64, i64, i64, vp]
+        L.eqbf_perm_pvalue.argtypes = [vp, C.c_double, i32, vp, i32, i64, vp, vp, vp, vp]
+        L.eqbf_perm_pvalue.restype = i64
         _lib = L
     return _lib
+
+
+class PermOpts(C.Structure):
+    _fields_ = [("analysis", C.c_int32), ("pbf", C.c_int32), ("use_max_bf", C.c_int32),
+                ("sep_sbgrp", C.c_int32), ("precision", C.c_int32)]
+
+
+class Rng:
+    """GSL-semantics mt19937 of the library (host side, no GPU needed)."""
+
+    def __init__(self, seed):
+        self._st = (C.c_uint32 * 625)()
+        lib().eqbf_rng_seed(C.addressof(self._st), seed)
+
+    @property
+    def addr(self):
+        return C.addressof(self._st)
+
+    def get(self):
+        return lib().eqbf_rng_get(self.addr)
+
+    def uniform_int(self, n):
+        return lib().eqbf_rng_uniform_int(self.addr, n)
+
+    def flat(self, a, b):
+        return lib().eqbf_rng_flat(self.addr, a, b)
+
+    def shuffle(self, a):
+        a = np.ascontiguousarray(a, dtype=np.int32)
+        lib().eqbf_rng_shuffle(self.addr, a.ctypes.data, a.size)
+        return a
+
+    def tables(self, n_samples, n_genes, nperm):
+        t = np.zeros((n_genes, nperm, n_samples), np.int32)
+        lib().eqbf_perm_tables(self.addr, n_samples, n_genes, nperm, t.ctypes.data)
+        return t
+
+
+def perm_pvalue(stats, true_stat, nb_permutations, sep=False, trick=0, trick_cutoff=10, rng_trick=None):
+    """Replays the reference's counting loop; returns (pval, nperm_used,
+    median, nb_permutations_after, drawn)."""
+    stats = np.ascontiguousarray(stats, dtype=np.float64)
+    nb = C.c_int64(nb_permutations)
+    pv, used, med = C.c_double(0), C.c_int64(0), C.c_double(0)
+    drawn = lib().eqbf_perm_pvalue(stats.ctypes.data, float(true_stat), int(sep), C.addressof(nb), trick,
+                                   trick_cutoff, rng_trick.addr if rng_trick is not None else None,
+                                   C.addressof(pv), C.addressof(used), C.addressof(med))
+    return pv.value, used.value, (None if sep else med.value), nb.value, drawn
 
 
 def _p(a):
@@ -100,6 +161,20 @@ class Engine:
         _abi.AssocOut whose pointers are device addresses."""
         self._check(lib().eqbf_assoc_uvlr_device(self._ctx, C.byref(chunk_struct), C.byref(opts),
                                                  C.byref(out_struct), stream))
+
+    def perm_stats(self, hc, tables, gene_table, analysis="join", pbf="all", use_max_bf=True, sep_sbgrp=-1,
+                   precision="mixed"):
+        """Statistic of every permutation: tables [n_tables][nperm][N],
+        gene_table [n_genes] (-1 skips a gene); returns [n_genes][nperm]."""
+        tables = np.ascontiguousarray(tables, dtype=np.int32)
+        gt = np.ascontiguousarray(gene_table, dtype=np.int32)
+        nperm = tables.shape[1]
+        out = np.full((hc.G, nperm), np.nan)
+        po = PermOpts(_abi.ANALYSIS[analysis], _abi.PBF[pbf], int(use_max_bf), sep_sbgrp, _abi.PREC[precision])
+        st = hc.struct()
+        self._check(lib().eqbf_perm_stats(self._ctx, C.byref(st), C.byref(po), _p(tables), tables.shape[0],
+                                          _p(gt), nperm, _p(out)))
+        return out
 
     def sync(self):
         self._check(lib().eqbf_sync(self._ctx))

@@ -155,6 +155,72 @@ int eqbf_assoc_uvlr_device(eqbf_ctx* ctx, const eqbf_chunk* chunk_dev,
                            const eqbf_assoc_opts* opts, eqbf_assoc_out* out_dev,
                            void* hip_stream);
 
+/* ---- permutations ------------------------------------------------------------
+ * Replaces makePermutations (eqtlbma_bf.cpp:867-917) and the permutation
+ * loops of Gene::MakePermutationsJoin (gene.cpp:598-717),
+ * MakePermutationsSepAllSubgroups (gene.cpp:493-570) and
+ * MakePermutationsSepPerSubgroup (gene.cpp:380-450).
+ *
+ * The permutation indices are made on the host exactly as the reference
+ * draws them (GSL mt19937, gsl_ran_shuffle; eqbf_perm_tables below); the
+ * device computes the test statistic of every permutation; the host replays
+ * the reference's counting loop (eqbf_perm_pvalue) so that the --trick
+ * rules and the NaN bookkeeping follow the reference step by step. */
+typedef struct eqbf_perm_opts {
+  int32_t analysis;    /* EQBF_ANALYSIS_JOIN: Bayes-factor statistic;
+                          EQBF_ANALYSIS_SEP: minimum p-value               */
+  int32_t pbf;         /* join: EQBF_PBF_* (--pbf)                         */
+  int32_t use_max_bf;  /* join: 1 = max over SNPs (--maxbf), 0 = average   */
+  int32_t sep_sbgrp;   /* sep: subgroup (--permsep 2) or -1 (--permsep 1)  */
+  int32_t precision;   /* EQBF_PREC_*                                      */
+} eqbf_perm_opts;
+
+/* Statistic of every permutation of every gene of the chunk.
+ * perm_idx:   [n_tables][nperm][n_samples]; sample i of permutation p of
+ *             table t takes the expression level of sample perm_idx[t][p][i]
+ *             (gene_snp_pair.cpp:104-115; genotypes and covariates stay with i)
+ * gene_table: [n_genes] table used by each gene, -1 to skip the gene
+ * stats:      [n_genes][nperm] output; NaN where the reference's statistic
+ *             is NaN (those permutations decrement nb_permutations). */
+int eqbf_perm_stats(eqbf_ctx* ctx, const eqbf_chunk* chunk, const eqbf_perm_opts* opts,
+                    const int32_t* perm_idx, int32_t n_tables, const int32_t* gene_table,
+                    int64_t nperm, double* stats);
+int eqbf_perm_stats_device(eqbf_ctx* ctx, const eqbf_chunk* chunk_dev, const eqbf_perm_opts* opts,
+                           const int32_t* perm_idx_dev, int32_t n_tables, const int32_t* gene_table,
+                           int64_t nperm, double* stats_dev, void* hip_stream);
+
+/* ---- host random numbers with GSL's semantics ----------------------------------
+ * gsl_rng_mt19937 + gsl_rng_set / gsl_rng_uniform_int / gsl_ran_shuffle /
+ * gsl_ran_flat as called at eqtlbma_bf.cpp:889-916, gene.cpp:358-362,410,
+ * 523,639.  GSL_RNG_TYPE is not consulted: the generator is mt19937, the
+ * reference's default (gsl_rng_env_setup, eqtlbma_bf.cpp:890). */
+typedef struct eqbf_rng {
+  uint32_t mt[624];
+  int32_t idx;
+} eqbf_rng;
+void eqbf_rng_seed(eqbf_rng* r, uint64_t seed);
+uint32_t eqbf_rng_get(eqbf_rng* r);
+uint32_t eqbf_rng_uniform_int(eqbf_rng* r, uint32_t n);
+double eqbf_rng_flat(eqbf_rng* r, double a, double b);
+void eqbf_rng_shuffle(eqbf_rng* r, int32_t* a, int64_t n);
+/* Index tables of n_genes analysed genes continuing the stream r: each gene
+ * starts from the identity and takes nperm cumulative shuffles
+ * (gene.cpp:617,638-639).  out: [n_genes][nperm][n_samples]. */
+void eqbf_perm_tables(eqbf_rng* r, int64_t n_samples, int64_t n_genes, int64_t nperm, int32_t* out);
+
+/* The reference's counting loop over one gene's permutation statistics
+ * (gene.cpp:637-716 join, :409-449 / :522-569 sep) and
+ * CalcPermutationPvalue (gene.cpp:348-364).  A permutation counts when its
+ * statistic is >= the true one (join) or <= (sep).  nb_permutations is the
+ * caller's shared counter, decremented on NaN statistics as the reference
+ * does; rng_trick may be NULL when trick == 0.  Returns the number of
+ * permutations drawn.  median (join only, may be NULL) is taken over the
+ * valid statistics: the reference reads one element past them
+ * (gene.cpp:713-714). */
+int64_t eqbf_perm_pvalue(const double* stats, double true_stat, int32_t sep, int64_t* nb_permutations,
+                         int32_t trick, int64_t trick_cutoff, eqbf_rng* rng_trick, double* pval,
+                         int64_t* nperm_used, double* median);
+
 /* ---- live kernel timing -------------------------------------------------------
  * No counterpart in the reference.  When enabled, every kernel launch of the
  * context is bracketed by HIP events on the stream it is launched on; the
