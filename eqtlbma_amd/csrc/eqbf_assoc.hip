@@ -1097,6 +1097,33 @@ int dispatch_mixed(int S, const eqbf_chunk& c, const AssocParams& kp, int grid, 
 }  // namespace
 
 /* ---- orchestration ------------------------------------------------------------- */
+int launch_bma(eqbf_ctx* ctx, double* w, int64_t NP, int S, int bfs, hipStream_t st) {
+  const int64_t n_cfg = eqbf_n_configs(bfs, S);
+  if (n_cfg < 0) return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED, "--bfs all with %d subgroups", S);
+  const int64_t nm = std::max<int64_t>(n_cfg, 1);
+  uint64_t* d_masks = (uint64_t*)ctx_buf(ctx, "bma_tab", (size_t)nm * sizeof(uint64_t) + 65 * sizeof(double));
+  if (!d_masks) return ctx_fail(ctx, EQBF_ERR_HIP, "allocating the BMA tables failed");
+  double* d_inv_choose = (double*)(d_masks + nm);
+  std::vector<uint64_t> hm(nm, 0);
+  if (n_cfg > 0) eqbf_config_masks(bfs, S, hm.data());
+  std::vector<double> ic(65, 0.0);
+  for (int k = 1; k <= S; ++k) {
+    double ch = 1;
+    for (int i = 1; i <= k; ++i) ch = ch * (S - k + i) / i;
+    ic[k] = 1.0 / ch;
+  }
+  EQBF_HIP(ctx, hipMemcpyAsync(d_masks, hm.data(), nm * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+  EQBF_HIP(ctx, hipMemcpyAsync(d_inv_choose, ic.data(), 65 * sizeof(double), hipMemcpyHostToDevice, st));
+  EQBF_HIP(ctx, hipStreamSynchronize(st)); /* host vectors go out of scope */
+  {
+    ProfScope ps(ctx, "bma_kernel", st);
+    hipLaunchKernelGGL(bma_kernel, dim3((unsigned)((NP + 255) / 256)), dim3(256), 0, st, w, NP, S, (int)n_cfg,
+                       n_cfg > 0 ? bfs : EQBF_BFS_GEN, d_masks, d_inv_choose);
+  }
+  EQBF_HIP(ctx, hipGetLastError());
+  return EQBF_OK;
+}
+
 int assoc_uvlr_run(eqbf_ctx* ctx, const eqbf_chunk* cp, const eqbf_assoc_opts* o, eqbf_assoc_out* out,
                    hipStream_t st) {
   const eqbf_chunk& c = *cp;

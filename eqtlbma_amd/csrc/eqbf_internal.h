@@ -98,6 +98,7 @@ struct PrepRec {
 };
 #define PREP_TOO_FEW 1
 #define FLAG_MISSING_COVAR 1
+#define FLAG_MVLR_SAMPLES 2  /* --error mvlr: subgroups with different samples */
 
 /* ---- device special functions (FP64) ------------------------------------- */
 namespace dm {
@@ -253,6 +254,9 @@ __device__ inline double tz_eval(const double* __restrict__ tz, int nu, double a
 /* ---- kernels launched from the C-ABI layer ------------------------------- */
 int assoc_uvlr_run(eqbf_ctx* ctx, const eqbf_chunk* c, const eqbf_assoc_opts* o, eqbf_assoc_out* out,
                    hipStream_t st);
+int mvlr_run(eqbf_ctx* ctx, const eqbf_chunk* c, const eqbf_mvlr_opts* o, eqbf_assoc_out* out, hipStream_t st);
+/* gen-sin and all rows of out->w_abf from the grid-averaged rows. */
+int launch_bma(eqbf_ctx* ctx, double* w, int64_t n_pairs, int S, int bfs, hipStream_t st);
 int perm_stats_run(eqbf_ctx* ctx, const eqbf_chunk* c, const eqbf_perm_opts* po, const int32_t* tables_dev,
                    int32_t n_tables, const int32_t* gene_table, int64_t nperm, double* stats_dev, hipStream_t st);
 

@@ -1,0 +1,742 @@
+/*
+ * Multivariate (MVLR) Bayes factors, --error mvlr, in FP64.
+ *
+ * Reference: GeneSnpPair::CalcAbfsMvlr and CalcAbfsMvlrFor*
+ * (gene_snp_pair.cpp:624-758) over class MVLR (MVLR.cpp:26-764) with one
+ * SNP per pair.  With T the projection off [1, covariates of the first
+ * subgroup], x~ = T x, y~_s = T y_s, k = x~'x~ and b_s = y~_s'x~:
+ *   Sigma0 = Y~'Y~ / n                      (compute_Sigma_null :176-195, m = q+s+1)
+ *   residual of y_s on [Xc, x] (gamma_s = 1) or on Xc (gamma_s = 0):
+ *     (E'E)_ij = (Y~'Y~)_ij - [gamma_i or gamma_j] b_i b_j / k
+ *   F -> chi-square factor for gamma_s = 1 (compute_residual :366-410):
+ *     T2 = (b_s^2/k) / sigma1^2 with sigma1 = rss_s/(n-q-1) (the reference
+ *     divides by the square of a variance), F = T2,
+ *     factor = F / chisq_Qinv(fdist_Q(F; 1, n-q-1); 1)
+ *            = T2 / Phi^-1(P(T_{n-q-1} <= -sqrt T2))^2  (1 when F < 1e-8,
+ *              0 when the F tail probability underflows in FP64)
+ *   Sigma = fiterr [ (H m/(m+n) + E'E/(m+n)) C ] + (1 - fiterr) Sigma0,
+ *     C = diag(sqrt(factor)), H = 1e-4 I (Sigma C, not C Sigma C: :268-273)
+ *   eVb = Sigma^-1 b, t = gamma o sqrt(diag Sigma), W = oma2 t t' + phi2 diag(t^2),
+ *   M = I + k Sigma^-1 W,  log10 ABF = (eVb' W M^-1 eVb - ln|det M|) / (2 ln 10)
+ *                                                    (:497-512, :559-605)
+ * so no n x n matrix is ever formed.  M is factorised with partial pivoting
+ * (as gsl_linalg_LU_decomp); ln|det M| is summed from the pivots and
+ * M^-1 eVb comes from one solve instead of an explicit inverse.
+ *
+ * Kernels (one chunk):
+ *   mvlr_prep_kernel   one wave per gene: common members, basis of
+ *                      [1, covariates], y~_s, Y~'Y~, Sigma0 and Sigma0^-1
+ *   mvlr_sums_kernel   one thread per pair: k and b (FP64 sums over members)
+ *   mvlr_cfg_kernel    one thread per (pair, configuration): Sigma, Sigma^-1,
+ *                      Sigma^-1 t, t, eVb (structure-of-arrays, coalesced)
+ *   mvlr_grid_kernel   one thread per (pair, raw row): M, LU, log10 ABF
+ *   mvlr_wavg_kernel   one thread per (pair, weighted row): grid average
+ *                      (utils::log10_weighted_sum)
+ * followed by the BMA kernel of the univariate path (gen-sin, all).
+ */
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "eqbf_internal.h"
+
+namespace {
+
+enum { MV_OK = 0, MV_SKIP = 1, MV_MISMATCH = 2, MV_FEW = 3 };
+struct MvRec {
+  int32_t n, flags, qcov, rank;
+};
+
+__device__ inline double wsum64(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+/* LU factorisation with partial pivoting of an SM x SM register matrix
+ * (largest |a| of the column, first one on ties).  piv[j] = row swapped
+ * with j at step j.  Returns ln|det A|. */
+template <int SM>
+__device__ inline double lu_factor(double (&A)[SM][SM], int (&piv)[SM]) {
+  double lnd = 0.0;
+#pragma unroll
+  for (int j = 0; j < SM; ++j) {
+    int p = j;
+    double best = fabs(A[j][j]);
+#pragma unroll
+    for (int i = j + 1; i < SM; ++i)
+      if (fabs(A[i][j]) > best) {
+        best = fabs(A[i][j]);
+        p = i;
+      }
+    piv[j] = p;
+#pragma unroll
+    for (int i = j + 1; i < SM; ++i) {
+      const bool sw = p == i;
+#pragma unroll
+      for (int c = 0; c < SM; ++c) {
+        const double a = A[j][c], b = A[i][c];
+        A[j][c] = sw ? b : a;
+        A[i][c] = sw ? a : b;
+      }
+    }
+    const double d = A[j][j];
+    lnd += log(fabs(d));
+#pragma unroll
+    for (int i = j + 1; i < SM; ++i) {
+      const double f = A[i][j] / d;
+      A[i][j] = f;
+#pragma unroll
+      for (int c = j + 1; c < SM; ++c) A[i][c] -= f * A[j][c];
+    }
+  }
+  return lnd;
+}
+
+/* Solves A x = rhs in place from the factorisation of lu_factor. */
+template <int SM>
+__device__ inline void lu_apply(const double (&A)[SM][SM], const int (&piv)[SM], double (&x)[SM]) {
+#pragma unroll
+  for (int j = 0; j < SM; ++j) {
+#pragma unroll
+    for (int i = j + 1; i < SM; ++i) {
+      const bool sw = piv[j] == i;
+      const double a = x[j], b = x[i];
+      x[j] = sw ? b : a;
+      x[i] = sw ? a : b;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < SM; ++i)
+#pragma unroll
+    for (int c = 0; c < i; ++c) x[i] -= A[i][c] * x[c];
+#pragma unroll
+  for (int i = SM - 1; i >= 0; --i) {
+    double s = x[i];
+#pragma unroll
+    for (int c = i + 1; c < SM; ++c) s -= A[i][c] * x[c];
+    x[i] = s / A[i][i];
+  }
+}
+
+__global__ void mv_zero_kernel(int* p) { *p = 0; }
+
+/* ---- per-gene preparation: one wave per gene ------------------------------------ */
+struct MvPrep {
+  MvRec* mrec;
+  int32_t* midx; /* [G][N] member sample indices            */
+  double* Ut;    /* [G][KU][N] orthonormal basis columns     */
+  double* Yt;    /* [G][S][N] y~_s over the members          */
+  double* YtY;   /* [G][S][S]                                */
+  double* S0;    /* [G][S][S] Sigma0                         */
+  double* S0inv; /* [G][S][S]                                */
+  int* flag;
+  int KU;
+};
+
+template <int SM>
+__global__ void __launch_bounds__(64) mvlr_prep_kernel(const eqbf_chunk c, const MvPrep pp) {
+  const int S = c.n_subgroups, N = c.n_samples, g = blockIdx.x, lane = threadIdx.x;
+  const int KU = pp.KU;
+  MvRec r;
+  r.n = 0;
+  r.flags = MV_OK;
+  r.qcov = min(max(c.covar_count[0], 0), c.n_covars_max);
+  r.rank = 0;
+  for (int s = 0; s < S; ++s)
+    if (!c.gene_in_sbgrp[(int64_t)g * S + s]) r.flags = MV_SKIP;
+  if (r.flags != MV_OK) {
+    if (lane == 0) pp.mrec[g] = r;
+    return;
+  }
+  /* members of subgroup 0 (expression level not NaN, genotyped); every other
+   * subgroup must have the same ones */
+  int32_t* idx = pp.midx + (int64_t)g * N;
+  int n = 0;
+  bool mismatch = false;
+  for (int base = 0; base < N; base += 64) {
+    const int i = base + lane;
+    bool m0 = false, diff = false;
+    if (i < N) {
+      m0 = !isnan(c.explevels[((int64_t)g * S) * N + i]) && c.geno_in_sbgrp[i];
+      for (int s = 1; s < S; ++s) {
+        const bool ms = !isnan(c.explevels[((int64_t)g * S + s) * N + i]) && c.geno_in_sbgrp[(int64_t)s * N + i];
+        diff |= ms != m0;
+      }
+    }
+    mismatch |= __ballot(diff) != 0ull;
+    const uint64_t bal = __ballot(m0);
+    if (m0) idx[n + __popcll(bal & ((1ull << lane) - 1ull))] = i;
+    n += __popcll(bal);
+  }
+  __syncthreads();
+  r.n = n;
+  const int Q = r.qcov;
+  if (mismatch)
+    r.flags = MV_MISMATCH;
+  else if (n < Q + 3) /* no residual degree of freedom for the F statistic */
+    r.flags = MV_FEW;
+  if (r.flags != MV_OK) {
+    if (lane == 0) pp.mrec[g] = r;
+    return;
+  }
+  if (Q > 0) { /* a member without covariates is a data error (gene_snp_pair.cpp:137-144) */
+    int miss = 0;
+    for (int k = lane; k < n; k += 64) miss |= !c.covar_in_sbgrp[idx[k]];
+    if (__ballot(miss) != 0ull) {
+      r.flags = MV_SKIP;
+      if (lane == 0) {
+        atomicOr(pp.flag, FLAG_MISSING_COVAR);
+        pp.mrec[g] = r;
+      }
+      return;
+    }
+  }
+  double* U = pp.Ut + (int64_t)g * KU * N;
+  double* Y = pp.Yt + (int64_t)g * S * N;
+  /* orthonormal basis of [1, covariates] by two-pass modified Gram-Schmidt;
+   * each lane only touches its own rows k = lane + 64 m */
+  uint32_t kept = 0;
+  for (int j = 0; j <= Q; ++j) {
+    double* cj = U + (int64_t)j * N;
+    for (int k = lane; k < n; k += 64)
+      cj[k] = (j == 0) ? 1.0 : c.covars[((int64_t)(j - 1)) * N + idx[k]];
+    double nn = 0;
+    for (int k = lane; k < n; k += 64) nn += cj[k] * cj[k];
+    const double n0 = sqrt(wsum64(nn));
+    for (int pass = 0; pass < 2; ++pass)
+      for (int l = 0; l < j; ++l) {
+        if (!((kept >> l) & 1u)) continue;
+        const double* cl = U + (int64_t)l * N;
+        double d = 0;
+        for (int k = lane; k < n; k += 64) d += cl[k] * cj[k];
+        d = wsum64(d);
+        for (int k = lane; k < n; k += 64) cj[k] -= d * cl[k];
+      }
+    double nv = 0;
+    for (int k = lane; k < n; k += 64) nv += cj[k] * cj[k];
+    nv = sqrt(wsum64(nv));
+    const bool keep = nv > 1e-9 * n0 && nv > 0.0;
+    const double sc = keep ? 1.0 / nv : 0.0;
+    for (int k = lane; k < n; k += 64) cj[k] *= sc;
+    if (keep) {
+      kept |= 1u << j;
+      ++r.rank;
+    }
+  }
+  for (int j = Q + 1; j < KU; ++j)
+    for (int k = lane; k < n; k += 64) U[(int64_t)j * N + k] = 0.0;
+  for (int s = 0; s < S; ++s) {
+    double* ys = Y + (int64_t)s * N;
+    for (int k = lane; k < n; k += 64) ys[k] = c.explevels[((int64_t)g * S + s) * N + idx[k]];
+    for (int pass = 0; pass < 2; ++pass)
+      for (int l = 0; l <= Q; ++l) {
+        if (!((kept >> l) & 1u)) continue;
+        const double* cl = U + (int64_t)l * N;
+        double d = 0;
+        for (int k = lane; k < n; k += 64) d += cl[k] * ys[k];
+        d = wsum64(d);
+        for (int k = lane; k < n; k += 64) ys[k] -= d * cl[k];
+      }
+  }
+  double* yty = pp.YtY + (int64_t)g * S * S;
+  for (int a = 0; a < S; ++a)
+    for (int b = a; b < S; ++b) {
+      const double* ya = Y + (int64_t)a * N;
+      const double* yb = Y + (int64_t)b * N;
+      double d = 0;
+      for (int k = lane; k < n; k += 64) d += ya[k] * yb[k];
+      d = wsum64(d);
+      if (lane == 0) {
+        yty[a * S + b] = d;
+        yty[b * S + a] = d;
+      }
+    }
+  if (lane == 0) {
+    /* Sigma0 = Y'TY / (n + m - q - s - 1) = Y'TY / n, m = q + s + 1 */
+    double A[SM][SM];
+    int piv[SM];
+#pragma unroll
+    for (int i = 0; i < SM; ++i)
+#pragma unroll
+      for (int j = 0; j < SM; ++j) A[i][j] = (i < S && j < S) ? yty[i * S + j] / (double)n : (i == j ? 1.0 : 0.0);
+    for (int i = 0; i < S; ++i)
+      for (int j = 0; j < S; ++j) pp.S0[((int64_t)g * S + i) * S + j] = yty[i * S + j] / (double)n;
+    (void)lu_factor<SM>(A, piv);
+    for (int col = 0; col < S; ++col) {
+      double e[SM];
+#pragma unroll
+      for (int i = 0; i < SM; ++i) e[i] = (i == col) ? 1.0 : 0.0;
+      lu_apply<SM>(A, piv, e);
+#pragma unroll
+      for (int i = 0; i < SM; ++i)
+        if (i < S) pp.S0inv[((int64_t)g * S + i) * S + col] = e[i];
+    }
+    pp.mrec[g] = r;
+  }
+}
+
+/* ---- per pair: k and b over the members ------------------------------------------ */
+struct MvSums {
+  const MvRec* mrec;
+  const int32_t* midx;
+  const double* Ut;
+  const double* Yt;
+  double* kb;         /* [S + 1][NP]: k, b_0 .. b_{S-1}          */
+  int32_t* pgene;     /* [NP]: gene of the pair, -1 = no result    */
+  int32_t* n_out;
+  double* sstats;
+  double* stdsstats;
+  int* flag;
+  int KU;
+};
+
+__device__ inline int gene_of_pair(const int64_t* __restrict__ off, int G, int64_t p) {
+  int lo = 0, hi = G - 1; /* largest g with off[g] <= p */
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= p)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+template <int SM>
+__global__ void __launch_bounds__(256) mvlr_sums_kernel(const eqbf_chunk c, const MvSums ms) {
+  const int64_t NP = c.n_pairs;
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= NP) return;
+  const int S = c.n_subgroups, N = c.n_samples;
+  const int g = gene_of_pair(c.pair_offset, c.n_genes, p);
+  const int64_t snp = c.pair_snp[p];
+  const MvRec r = ms.mrec[g];
+  bool all_geno = true;
+  for (int s = 0; s < S; ++s) all_geno &= c.snp_in_sbgrp[snp * S + s] != 0;
+  if (r.flags == MV_MISMATCH && all_geno) atomicOr(ms.flag, FLAG_MVLR_SAMPLES);
+  const bool has = r.flags != MV_SKIP && r.flags != MV_MISMATCH && all_geno;
+  /* the reference records the sample size of each subgroup and NaN summary
+   * statistics for --error mvlr (gene_snp_pair.cpp:154-158) */
+  for (int s = 0; s < S; ++s) {
+    ms.n_out[(int64_t)s * NP + p] = has ? r.n : 0;
+    if (ms.sstats)
+      for (int k = 0; k < 5; ++k) ms.sstats[((int64_t)k * S + s) * NP + p] = NAN;
+    if (ms.stdsstats)
+      for (int k = 0; k < 3; ++k) ms.stdsstats[((int64_t)k * S + s) * NP + p] = NAN;
+  }
+  if (!has || r.flags == MV_FEW) {
+    ms.pgene[p] = -1;
+    return;
+  }
+  const int n = r.n;
+  const int32_t* idx = ms.midx + (int64_t)g * N;
+  const double* U = ms.Ut + (int64_t)g * ms.KU * N;
+  const double* Y = ms.Yt + (int64_t)g * S * N;
+  const int64_t gbase = (int64_t)c.geno_set_of_sbgrp[0] * c.geno_set_stride + snp;
+  const bool f32 = c.geno_dtype == EQBF_F32;
+  double xx = 0, b[SM];
+#pragma unroll
+  for (int s = 0; s < SM; ++s) b[s] = 0;
+  for (int k = 0; k < n; ++k) {
+    const int64_t off = gbase + (int64_t)idx[k] * c.geno_ld;
+    const double x = f32 ? (double)((const float*)c.genotypes)[off] : ((const double*)c.genotypes)[off];
+    xx += x * x;
+#pragma unroll
+    for (int s = 0; s < SM; ++s)
+      if (s < S) b[s] += x * Y[(int64_t)s * N + k];
+  }
+  double proj = 0;
+  for (int q = 0; q < ms.KU; ++q) {
+    const double* uq = U + (int64_t)q * N;
+    double a = 0;
+    for (int k = 0; k < n; ++k) {
+      const int64_t off = gbase + (int64_t)idx[k] * c.geno_ld;
+      const double x = f32 ? (double)((const float*)c.genotypes)[off] : ((const double*)c.genotypes)[off];
+      a += x * uq[k];
+    }
+    proj += a * a;
+  }
+  ms.kb[p] = xx - proj;
+#pragma unroll
+  for (int s = 0; s < SM; ++s)
+    if (s < S) ms.kb[(int64_t)(1 + s) * NP + p] = b[s];
+  ms.pgene[p] = g;
+}
+
+/* ---- per (pair, configuration): Sigma, Sigma^-1, Sigma^-1 t, t, eVb -------------- */
+struct MvCfg {
+  const MvRec* mrec;
+  const double* YtY;
+  const double* S0;
+  const double* S0inv;
+  const double* kb;
+  const int32_t* pgene;
+  const double* tz;
+  const uint64_t* masks; /* [1 + n_cfg]: the consistent configuration first */
+  double* par;           /* [cfg in batch][PSZ][NP]                          */
+  double fiterr;
+  int cfg0, PSZ;
+};
+
+template <int SM>
+__global__ void __launch_bounds__(64) mvlr_cfg_kernel(const eqbf_chunk c, const MvCfg mc) {
+  const int64_t NP = c.n_pairs;
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= NP) return;
+  const int ci = blockIdx.y;
+  const int S = c.n_subgroups;
+  const int PSZ = mc.PSZ;
+  double* out = mc.par + (int64_t)ci * PSZ * NP + p;
+  const int g = mc.pgene[p];
+  if (g < 0) {
+    out[(int64_t)(PSZ - 1) * NP] = 1.0;
+    return;
+  }
+  const MvRec r = mc.mrec[g];
+  const uint64_t gm = mc.masks[mc.cfg0 + ci];
+  const double kk = mc.kb[p];
+  double b[SM];
+#pragma unroll
+  for (int s = 0; s < SM; ++s) b[s] = (s < S) ? mc.kb[(int64_t)(1 + s) * NP + p] : 0.0;
+  const double* YY = mc.YtY + (int64_t)g * S * S;
+  const double* S0 = mc.S0 + (int64_t)g * S * S;
+  double Sg[SM][SM];
+  const bool fit = !(mc.fiterr < 1e-6);
+  if (!fit) {
+#pragma unroll
+    for (int i = 0; i < SM; ++i)
+#pragma unroll
+      for (int j = 0; j < SM; ++j) Sg[i][j] = (i < S && j < S) ? S0[i * S + j] : (i == j ? 1.0 : 0.0);
+  } else {
+    const double nn = (double)r.n;
+    const double m = (double)(r.qcov + 1 + S + 1);
+    const int nu = r.n - (r.qcov + 1) - 1;
+    double cf[SM];
+#pragma unroll
+    for (int i = 0; i < SM; ++i) {
+      cf[i] = 1.0;
+      if (i < S && ((gm >> i) & 1ull)) {
+        const double b2k = b[i] * b[i] / kk;
+        const double s1 = (YY[i * S + i] - b2k) / (double)nu;
+        const double T2 = b2k / (s1 * s1);
+        if (!(T2 < 1e-8)) {
+          const double w = dm::tz_eval(mc.tz, nu, sqrt(T2)); /* -ln P(T_nu <= -sqrt T2) */
+          if (2.0 * exp(-w) == 0.0) {
+            cf[i] = 0.0; /* the F tail underflows: chi-square quantile +inf */
+          } else {
+            const double z = dm::gauss_pinv_from_w(w);
+            cf[i] = sqrt(T2 / (z * z));
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < SM; ++i)
+#pragma unroll
+      for (int j = 0; j < SM; ++j) {
+        if (i < S && j < S) {
+          const bool any = ((gm >> i) & 1ull) || ((gm >> j) & 1ull);
+          const double ee = YY[i * S + j] - (any ? b[i] * b[j] / kk : 0.0);
+          const double h = (i == j) ? 1e-4 : 0.0;
+          const double sm = (h * m / (m + nn) + ee / (m + nn)) * cf[j];
+          Sg[i][j] = mc.fiterr * sm + (1.0 - mc.fiterr) * S0[i * S + j];
+        } else {
+          Sg[i][j] = (i == j) ? 1.0 : 0.0;
+        }
+      }
+  }
+  double tv[SM];
+#pragma unroll
+  for (int j = 0; j < SM; ++j) tv[j] = (j < S && ((gm >> j) & 1ull)) ? sqrt(Sg[j][j]) : 0.0;
+  double u[SM], eVb[SM];
+#pragma unroll
+  for (int i = 0; i < SM; ++i) u[i] = eVb[i] = 0.0;
+  if (!fit) {
+    const double* Si = mc.S0inv + (int64_t)g * S * S;
+#pragma unroll
+    for (int i = 0; i < SM; ++i)
+      if (i < S) {
+#pragma unroll
+        for (int j = 0; j < SM; ++j)
+          if (j < S) {
+            const double v = Si[i * S + j];
+            out[(int64_t)(i * S + j) * NP] = v;
+            u[i] += v * tv[j];
+            eVb[i] += v * b[j];
+          }
+      }
+  } else {
+    int piv[SM];
+    (void)lu_factor<SM>(Sg, piv);
+    for (int col = 0; col < S; ++col) {
+      double e[SM];
+#pragma unroll
+      for (int i = 0; i < SM; ++i) e[i] = (i == col) ? 1.0 : 0.0;
+      lu_apply<SM>(Sg, piv, e);
+      double tc = 0, bc = 0;
+#pragma unroll
+      for (int j = 0; j < SM; ++j)
+        if (j == col) {
+          tc = tv[j];
+          bc = b[j];
+        }
+#pragma unroll
+      for (int i = 0; i < SM; ++i)
+        if (i < S) {
+          out[(int64_t)(i * S + col) * NP] = e[i];
+          u[i] += e[i] * tc;
+          eVb[i] += e[i] * bc;
+        }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < SM; ++i)
+    if (i < S) {
+      out[(int64_t)(S * S + i) * NP] = u[i];
+      out[(int64_t)(S * S + S + i) * NP] = tv[i];
+      out[(int64_t)(S * S + 2 * S + i) * NP] = eVb[i];
+    }
+  out[(int64_t)(S * S + 3 * S) * NP] = kk;
+  out[(int64_t)(PSZ - 1) * NP] = 0.0;
+}
+
+/* ---- per (pair, raw row): one log10 ABF ------------------------------------------ */
+struct MvGrid {
+  const double* par;
+  const double* grid; /* phiL, omaL, phiS, omaS */
+  double* raw;        /* [n_raw][NP]            */
+  int GL, GS, cfg0, PSZ;
+  int64_t row0, nrows, pair_blocks;
+};
+
+template <int SM>
+__global__ void __launch_bounds__(64) mvlr_grid_kernel(const eqbf_chunk c, const MvGrid mg) {
+  const int64_t NP = c.n_pairs;
+  const int64_t rr = blockIdx.x / mg.pair_blocks;
+  const int64_t p = (blockIdx.x - rr * mg.pair_blocks) * 64 + threadIdx.x;
+  if (p >= NP || rr >= mg.nrows) return;
+  const int64_t row = mg.row0 + rr;
+  const int S = c.n_subgroups;
+  int cfg, gi, type = 0;
+  double p2, o2;
+  if (row < 3 * (int64_t)mg.GL) {
+    cfg = 0;
+    type = (int)(row / mg.GL);
+    gi = (int)(row - (int64_t)type * mg.GL);
+    p2 = mg.grid[gi];
+    o2 = mg.grid[mg.GL + gi];
+    if (type == 1) { /* gen-fix: phi2 = 0, oma2 = phi2 + oma2 */
+      o2 = p2 + o2;
+      p2 = 0.0;
+    } else if (type == 2) { /* gen-maxh: phi2 = phi2 + oma2, oma2 = 0 */
+      p2 = p2 + o2;
+      o2 = 0.0;
+    }
+  } else {
+    const int64_t rs = row - 3 * (int64_t)mg.GL;
+    cfg = 1 + (int)(rs / mg.GS);
+    gi = (int)(rs - (int64_t)(cfg - 1) * mg.GS);
+    p2 = mg.grid[2 * mg.GL + gi];
+    o2 = mg.grid[2 * mg.GL + mg.GS + gi];
+  }
+  const int PSZ = mg.PSZ;
+  const double* pr = mg.par + (int64_t)(cfg - mg.cfg0) * PSZ * NP + p;
+  double l = NAN;
+  if (pr[(int64_t)(PSZ - 1) * NP] == 0.0) {
+    const double kk = pr[(int64_t)(S * S + 3 * S) * NP];
+    double t[SM], z[SM], u[SM];
+#pragma unroll
+    for (int i = 0; i < SM; ++i) {
+      t[i] = (i < S) ? pr[(int64_t)(S * S + S + i) * NP] : 0.0;
+      u[i] = (i < S) ? pr[(int64_t)(S * S + i) * NP] : 0.0;
+      z[i] = (i < S) ? pr[(int64_t)(S * S + 2 * S + i) * NP] : 0.0;
+    }
+    /* M = I + k Sigma^-1 (oma2 t t' + phi2 diag(t^2)) */
+    double M[SM][SM];
+#pragma unroll
+    for (int i = 0; i < SM; ++i)
+#pragma unroll
+      for (int j = 0; j < SM; ++j) {
+        double v = (i == j) ? 1.0 : 0.0;
+        if (i < S && j < S) v += kk * (o2 * u[i] * t[j] + p2 * pr[(int64_t)(i * S + j) * NP] * t[j] * t[j]);
+        M[i][j] = v;
+      }
+    double e[SM];
+#pragma unroll
+    for (int i = 0; i < SM; ++i) e[i] = z[i];
+    int piv[SM];
+    const double lnd = lu_factor<SM>(M, piv);
+    lu_apply<SM>(M, piv, z); /* z = M^-1 eVb */
+    /* eVb' W z with W = oma2 t t' + phi2 diag(t^2) */
+    double et = 0, tz = 0, q2 = 0;
+#pragma unroll
+    for (int i = 0; i < SM; ++i) {
+      et += e[i] * t[i];
+      tz += t[i] * z[i];
+      q2 += e[i] * t[i] * t[i] * z[i];
+    }
+    const double quad = o2 * et * tz + p2 * q2;
+    l = (0.5 * quad - 0.5 * lnd) / 2.302585092994045684;
+  }
+  mg.raw[row * NP + p] = l;
+}
+
+/* ---- per (pair, weighted row): utils::log10_weighted_sum over the grid ------------ */
+__global__ void __launch_bounds__(256) mvlr_wavg_kernel(const double* __restrict__ raw, double* __restrict__ w,
+                                                         int64_t NP, int GL, int GS, int64_t wrow0, int64_t nw) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= nw * NP) return;
+  const int64_t wr = wrow0 + i / NP, p = i % NP;
+  const int G = wr < 3 ? GL : GS;
+  const int64_t r0 = wr < 3 ? wr * GL : 3 * (int64_t)GL + (wr - 3) * GS;
+  const double* v = raw + r0 * NP + p;
+  double mx = v[0];
+  for (int k = 1; k < G; ++k) {
+    const double x = v[(int64_t)k * NP];
+    if (x > mx) mx = x;
+  }
+  const double wt = 1.0 / (double)G;
+  double sum = 0;
+  for (int k = 0; k < G; ++k) {
+    const double x = v[(int64_t)k * NP];
+    if (!isnan(x)) sum += wt * exp10(x - mx);
+  }
+  double res = mx + log10(sum);
+  if (fabs(res) <= 2.220446049250313e-16) res = 0.0;
+  w[wr * NP + p] = res;
+}
+
+template <int SM>
+int mvlr_run_sm(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_mvlr_opts* o, eqbf_assoc_out* out, hipStream_t st) {
+  const int S = c.n_subgroups, N = c.n_samples, G = c.n_genes;
+  const int64_t NP = c.n_pairs;
+  const int GL = (int)ctx->phiL.size(), GS = (int)ctx->phiS.size();
+  const int64_t n_cfg = eqbf_n_configs(o->bfs, S);
+  const int KU = 1 + c.n_covars_max;
+  MvPrep pp;
+  pp.mrec = (MvRec*)ctx_buf(ctx, "mv_rec", (size_t)G * sizeof(MvRec));
+  pp.midx = (int32_t*)ctx_buf(ctx, "mv_idx", (size_t)G * N * sizeof(int32_t));
+  pp.Ut = (double*)ctx_buf(ctx, "mv_U", (size_t)G * KU * N * sizeof(double));
+  pp.Yt = (double*)ctx_buf(ctx, "mv_Y", (size_t)G * S * N * sizeof(double));
+  pp.YtY = (double*)ctx_buf(ctx, "mv_YtY", (size_t)G * S * S * sizeof(double));
+  pp.S0 = (double*)ctx_buf(ctx, "mv_S0", (size_t)G * S * S * sizeof(double));
+  pp.S0inv = (double*)ctx_buf(ctx, "mv_S0i", (size_t)G * S * S * sizeof(double));
+  pp.KU = KU;
+  pp.flag = (int*)ctx->d_flag.ptr;
+  double* kb = (double*)ctx_buf(ctx, "mv_kb", (size_t)(S + 1) * NP * sizeof(double));
+  int32_t* pgene = (int32_t*)ctx_buf(ctx, "mv_pgene", (size_t)NP * sizeof(int32_t));
+  /* configuration masks: the consistent one (all subgroups) first */
+  std::vector<uint64_t> hm(1 + n_cfg);
+  hm[0] = (S >= 64) ? ~0ull : ((1ull << S) - 1ull);
+  if (n_cfg > 0) eqbf_config_masks(o->bfs, S, hm.data() + 1);
+  uint64_t* dmask = (uint64_t*)ctx_buf(ctx, "mv_masks", hm.size() * sizeof(uint64_t));
+  if (!pp.mrec || !pp.midx || !pp.Ut || !pp.Yt || !pp.YtY || !pp.S0 || !pp.S0inv || !kb || !pgene || !dmask)
+    return ctx_fail(ctx, EQBF_ERR_HIP, "allocating mvlr buffers failed");
+  EQBF_HIP(ctx, hipMemcpyAsync(dmask, hm.data(), hm.size() * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+  EQBF_HIP(ctx, hipStreamSynchronize(st)); /* the host vector goes out of scope */
+  int rc = ctx_ensure_tz(ctx, N, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(mv_zero_kernel, dim3(1), dim3(1), 0, st, (int*)ctx->d_flag.ptr);
+  {
+    ProfScope ps(ctx, "mvlr_prep_kernel", st);
+    hipLaunchKernelGGL(mvlr_prep_kernel<SM>, dim3(G), dim3(64), 0, st, c, pp);
+  }
+  MvSums ms;
+  ms.mrec = pp.mrec;
+  ms.midx = pp.midx;
+  ms.Ut = pp.Ut;
+  ms.Yt = pp.Yt;
+  ms.kb = kb;
+  ms.pgene = pgene;
+  ms.n_out = out->n;
+  ms.sstats = out->sstats;
+  ms.stdsstats = out->stdsstats;
+  ms.flag = (int*)ctx->d_flag.ptr;
+  ms.KU = KU;
+  {
+    ProfScope ps(ctx, "mvlr_sums_kernel", st);
+    hipLaunchKernelGGL(mvlr_sums_kernel<SM>, dim3((unsigned)((NP + 255) / 256)), dim3(256), 0, st, c, ms);
+  }
+  EQBF_HIP(ctx, hipGetLastError());
+  /* configurations in batches bounded by the parameter buffer (and by the
+   * 65535 limit of the grid's y dimension) */
+  const int PSZ = S * S + 3 * S + 2;
+  const int64_t total = 1 + n_cfg;
+  int64_t per = std::max<int64_t>(1, ((int64_t)1 << 30) / std::max<int64_t>(1, NP * PSZ * 8));
+  per = std::min<int64_t>(std::min<int64_t>(per, 65535), total);
+  double* par = (double*)ctx_buf(ctx, "mv_par", (size_t)per * NP * PSZ * sizeof(double));
+  if (!par) return ctx_fail(ctx, EQBF_ERR_HIP, "allocating mvlr buffers failed");
+  const int64_t pair_blocks = (NP + 63) / 64;
+  for (int64_t c0 = 0; c0 < total; c0 += per) {
+    const int nb = (int)std::min<int64_t>(per, total - c0);
+    MvCfg mc;
+    mc.mrec = pp.mrec;
+    mc.YtY = pp.YtY;
+    mc.S0 = pp.S0;
+    mc.S0inv = pp.S0inv;
+    mc.kb = kb;
+    mc.pgene = pgene;
+    mc.tz = ctx_tz_table(ctx);
+    mc.masks = dmask;
+    mc.par = par;
+    mc.fiterr = o->fiterr;
+    mc.cfg0 = (int)c0;
+    mc.PSZ = PSZ;
+    {
+      ProfScope ps(ctx, "mvlr_cfg_kernel", st);
+      hipLaunchKernelGGL(mvlr_cfg_kernel<SM>, dim3((unsigned)pair_blocks, (unsigned)nb), dim3(64), 0, st, c, mc);
+    }
+    /* raw rows of configurations [c0, c0 + nb) and their weighted rows */
+    const int64_t c1 = c0 + nb;
+    const int64_t row0 = c0 == 0 ? 0 : 3 * (int64_t)GL + (c0 - 1) * GS;
+    const int64_t row1 = 3 * (int64_t)GL + (c1 - 1) * GS;
+    const int64_t wrow0 = c0 == 0 ? 0 : 3 + (c0 - 1);
+    const int64_t wrow1 = 3 + (c1 - 1);
+    for (int64_t r0 = row0; r0 < row1;) {
+      const int64_t nr = std::min<int64_t>(row1 - r0, std::max<int64_t>(1, ((int64_t)1 << 30) / pair_blocks));
+      MvGrid mg;
+      mg.par = par;
+      mg.grid = (const double*)ctx->d_grid.ptr;
+      mg.raw = (double*)out->raw_abf;
+      mg.GL = GL;
+      mg.GS = GS;
+      mg.cfg0 = (int)c0;
+      mg.PSZ = PSZ;
+      mg.row0 = r0;
+      mg.nrows = nr;
+      mg.pair_blocks = pair_blocks;
+      {
+        ProfScope ps(ctx, "mvlr_grid_kernel", st);
+        hipLaunchKernelGGL(mvlr_grid_kernel<SM>, dim3((unsigned)(nr * pair_blocks)), dim3(64), 0, st, c, mg);
+      }
+      r0 += nr;
+    }
+    const int64_t nw = wrow1 - wrow0;
+    hipLaunchKernelGGL(mvlr_wavg_kernel, dim3((unsigned)((nw * NP + 255) / 256)), dim3(256), 0, st,
+                       (const double*)out->raw_abf, out->w_abf, NP, GL, GS, wrow0, nw);
+    EQBF_HIP(ctx, hipGetLastError());
+  }
+  return launch_bma(ctx, out->w_abf, NP, S, o->bfs, st);
+}
+
+}  // namespace
+
+int mvlr_run(eqbf_ctx* ctx, const eqbf_chunk* cp, const eqbf_mvlr_opts* o, eqbf_assoc_out* out, hipStream_t st) {
+  const eqbf_chunk& c = *cp;
+  const int S = c.n_subgroups, N = c.n_samples, G = c.n_genes;
+  const int GL = (int)ctx->phiL.size(), GS = (int)ctx->phiS.size();
+  if (S < 1 || S > 16) return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED, "--error mvlr supports 1 to 16 subgroups (S=%d)", S);
+  if (N < 1 || G < 0 || c.n_pairs < 0 || c.n_covars_max < 0 || c.n_covars_max > 16)
+    return ctx_fail(ctx, EQBF_ERR_ARG, "chunk sizes out of range (N=%d G=%d Q=%d)", N, G, c.n_covars_max);
+  if (o->bfs != EQBF_BFS_GEN && o->bfs != EQBF_BFS_SIN && o->bfs != EQBF_BFS_ALL)
+    return ctx_fail(ctx, EQBF_ERR_ARG, "unknown --bfs value %d", o->bfs);
+  if (!(o->fiterr >= 0.0 && o->fiterr <= 1.0)) return ctx_fail(ctx, EQBF_ERR_ARG, "--fiterr must be in [0, 1]");
+  if (GL < 1 || (o->bfs != EQBF_BFS_GEN && GS < 1))
+    return ctx_fail(ctx, EQBF_ERR_ARG, "--error mvlr needs the grids (eqbf_set_grids)");
+  if (!out || !out->n || !out->w_abf || !out->raw_abf) return ctx_fail(ctx, EQBF_ERR_ARG, "missing output buffers");
+  if (G == 0 || c.n_pairs == 0) return EQBF_OK;
+  if (S <= 4) return mvlr_run_sm<4>(ctx, c, o, out, st);
+  if (S <= 8) return mvlr_run_sm<8>(ctx, c, o, out, st);
+  if (S <= 12) return mvlr_run_sm<12>(ctx, c, o, out, st);
+  return mvlr_run_sm<16>(ctx, c, o, out, st);
+}

@@ -41,6 +41,8 @@ def lib():
         L.eqbf_n_weighted.restype = i64
         L.eqbf_assoc_uvlr.argtypes = [vp, vp, vp, vp]
         L.eqbf_assoc_uvlr_device.argtypes = [vp, vp, vp, vp, vp]
+        L.eqbf_assoc_mvlr.argtypes = [vp, vp, vp, vp]
+        L.eqbf_assoc_mvlr_device.argtypes = [vp, vp, vp, vp, vp]
         L.eqbf_debug_ttail.argtypes = [vp, vp, vp, i32, vp, vp, vp, vp]
         L.eqbf_perm_stats.argtypes = [vp, vp, vp, vp, i32, vp, i64, vp]
         L.eqbf_perm_stats_device.argtypes = [vp, vp, vp, vp, i32, vp, i64, vp, vp]
@@ -62,6 +64,10 @@ def lib():
 class PermOpts(C.Structure):
     _fields_ = [("analysis", C.c_int32), ("pbf", C.c_int32), ("use_max_bf", C.c_int32),
                 ("sep_sbgrp", C.c_int32), ("precision", C.c_int32)]
+
+
+class MvlrOpts(C.Structure):
+    _fields_ = [("bfs", C.c_int32), ("fiterr", C.c_double)]
 
 
 class Rng:
@@ -160,6 +166,20 @@ class Engine:
         """Device-pointer entry: chunk_struct / out_struct are _abi.Chunk /
         _abi.AssocOut whose pointers are device addresses."""
         self._check(lib().eqbf_assoc_uvlr_device(self._ctx, C.byref(chunk_struct), C.byref(opts),
+                                                 C.byref(out_struct), stream))
+
+    def assoc_mvlr(self, hc, bfs="gen", fiterr=0.0, want_sstats=False):
+        """--error mvlr Bayes factors (FP64); raw log10 ABFs are FP64."""
+        out = _abi.HostAssocOut(hc.S, hc.n_pairs, bfs, self.GL, self.GS, raw_dtype=np.float64,
+                                want_sstats=want_sstats, want_std=want_sstats, want_raw=True)
+        opts = MvlrOpts(_abi.BFS[bfs], float(fiterr))
+        st = hc.struct()
+        o = out.struct()
+        self._check(lib().eqbf_assoc_mvlr(self._ctx, C.byref(st), C.byref(opts), C.byref(o)))
+        return out
+
+    def assoc_mvlr_device(self, chunk_struct, opts, out_struct, stream=None):
+        self._check(lib().eqbf_assoc_mvlr_device(self._ctx, C.byref(chunk_struct), C.byref(opts),
                                                  C.byref(out_struct), stream))
 
     def perm_stats(self, hc, tables, gene_table, analysis="join", pbf="all", use_max_bf=True, sep_sbgrp=-1,

@@ -155,6 +155,29 @@ int eqbf_assoc_uvlr_device(eqbf_ctx* ctx, const eqbf_chunk* chunk_dev,
                            const eqbf_assoc_opts* opts, eqbf_assoc_out* out_dev,
                            void* hip_stream);
 
+/* ---- multivariate (MVLR) Bayes factors ------------------------------------------
+ * --error mvlr: GeneSnpPair::CalcAbfsMvlr (gene_snp_pair.cpp:730-758) and
+ * class MVLR (MVLR.cpp:26-764) for one SNP per pair.  Every subgroup must use
+ * the same samples (the reference requires one genotype file and reads the
+ * phenotype vectors by position, data_loader.cpp:133-140, MVLR.cpp:29-41);
+ * the covariates are those of the first subgroup (gene_snp_pair.cpp:120-151).
+ * Genes without expression in every subgroup and SNPs without genotypes in
+ * every subgroup are skipped (n = 0, NaN Bayes factors), as gene.cpp:314-321
+ * and eqtlbma_bf.cpp:755-762 do.  Computed in FP64.  Outputs use the layout
+ * of eqbf_assoc_out with raw_abf in FP64 (required, like n and w_abf); as in
+ * the reference, n holds the sample size of each subgroup and the optional
+ * sstats / stdsstats are NaN (gene_snp_pair.cpp:154-158).  Up to 16
+ * subgroups and 16 covariates.  Errors: EQBF_ERR_UNSUPPORTED when the
+ * subgroups of an analysed pair have different samples (reported by the
+ * host-pointer entry), EQBF_ERR_DATA for a missing covariate. */
+typedef struct eqbf_mvlr_opts {
+  int32_t bfs;       /* EQBF_BFS_*                                        */
+  double fiterr;     /* --fiterr, proportion of the ML error covariance   */
+} eqbf_mvlr_opts;
+int eqbf_assoc_mvlr(eqbf_ctx* ctx, const eqbf_chunk* chunk, const eqbf_mvlr_opts* opts, eqbf_assoc_out* out);
+int eqbf_assoc_mvlr_device(eqbf_ctx* ctx, const eqbf_chunk* chunk_dev, const eqbf_mvlr_opts* opts,
+                           eqbf_assoc_out* out_dev, void* hip_stream);
+
 /* ---- permutations ------------------------------------------------------------
  * Replaces makePermutations (eqtlbma_bf.cpp:867-917) and the permutation
  * loops of Gene::MakePermutationsJoin (gene.cpp:598-717),
