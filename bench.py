@@ -47,6 +47,9 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--diag", action="store_true", help="per-phase split of the association kernel")
     ap.add_argument("--seed", type=int, default=1001)
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB (a kernel of known "
+                         "traffic, to scale the HBM byte counters of a profiler run)")
     return ap.parse_args()
 
 
@@ -192,6 +195,13 @@ def main():
     out_w = torch.empty((n_w, max_np), dtype=torch.float64, device=dev)
     eng = Engine(0)
     eng.set_grids(grid, grid)
+    if args.calib_copy_mib > 0:
+        src = torch.ones(args.calib_copy_mib << 18, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.mul(src, 1.0, out=dst)  # an elementwise kernel: reads and writes every byte once
+        torch.cuda.synchronize()
+        log("calibration copy: %d MiB read + %d MiB written" % (args.calib_copy_mib, args.calib_copy_mib))
+        del src, dst
     opts = _abi.AssocOpts(_abi.ANALYSIS["join"], _abi.BFS["sin"], _abi.PREC["mixed"], _abi.F32)
     o = _abi.AssocOut()
     o.n, o.sstats, o.stdsstats = out_n.data_ptr(), None, None
