@@ -63,8 +63,9 @@ def build_cli():
     if not srcs:
         return None
     tmp = CLI + ".tmp"
-    _run(["g++", "-O2", "-std=c++17", "-I" + INCLUDE, "-o", tmp] + srcs +
-         ["-L" + HERE, "-leqbf_hip", "-Wl,-rpath,$ORIGIN", "-lz", "-lpthread"])
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I" + INCLUDE, "-o", tmp] + srcs +
+         ["-L" + HERE, "-leqbf_hip", "-Wl,-rpath,$ORIGIN", "-L/opt/rocm/lib", "-Wl,-rpath-link,/opt/rocm/lib",
+          "-lz", "-lpthread"])
     os.replace(tmp, CLI)
     return CLI
 

@@ -1,0 +1,66 @@
+"""The eqtlbma_bf host program without a GPU: help, the reference's option
+checks (eqtlbma_bf.cpp:432-693) and the early exit when nothing is left to
+analyse (run() returns before any output, :1503-1504)."""
+import os
+import subprocess
+
+import pytest
+
+from eqtlbma_amd import CLI_PATH
+import cli_case
+
+
+def _run(args, cwd):
+    return subprocess.run([CLI_PATH] + args, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+                          timeout=60)
+
+
+@pytest.fixture(scope="module")
+def root(tmp_path_factory):
+    r = str(tmp_path_factory.mktemp("clihost"))
+    cli_case.make_case(r)
+    return r
+
+
+def _base(root, out):
+    return ["--geno", os.path.join(root, "list_geno.txt"), "--scoord", os.path.join(root, "snps.bed"),
+            "--exp", os.path.join(root, "list_exp.txt"), "--gcoord", os.path.join(root, "genes.bed"),
+            "--out", out, "-v", "0"]
+
+
+def test_help_and_version():
+    r = _run(["--help"], "/tmp")
+    assert r.returncode == 0 and "--gridL" in r.stdout and "--pbf" in r.stdout
+    assert _run(["-V"], "/tmp").returncode == 0
+
+
+@pytest.mark.parametrize("extra,msg", [
+    ([], "missing compulsory option --analys"),
+    (["--analys", "both"], "--analys both is not valid"),
+    (["--analys", "join"], "missing compulsory option --gridL with --analys join"),
+    (["--analys", "join", "--gridL", "GL", "--bfs", "sin"], "--gridS is required with --analys join and --bfs sin"),
+    (["--analys", "join", "--gridL", "GL", "--bfs", "one"], "--bfs one is not valid"),
+    (["--analys", "join", "--gridL", "GL", "--error", "xx"], "--error xx is not valid"),
+    (["--analys", "sep", "--nperm", "10"], "--permsep should be '1' or '2'"),
+    (["--analys", "join", "--gridL", "GL", "--nperm", "10"], "--pbf should be different than 'none'"),
+    (["--analys", "join", "--gridL", "GL", "--nperm", "10", "--pbf", "all"], "--pbf should be 'gen'"),
+    (["--analys", "join", "--gridL", "GL", "--gridS", "GS", "--bfs", "sin", "--nperm", "10", "--pbf", "all"],
+     "--pbf should be 'gen' or 'gen-sin'"),
+    (["--analys", "sep", "--nperm", "5", "--permsep", "1", "--trick", "1", "--tricut", "10"],
+     "--tricut 10 is larger than --nperm 5"),
+    (["--analys", "sep", "--wrtsize", "0"], "--wrtsize should be greater than 1"),
+    (["--analys", "sep", "--anchor", "TES"], "--anchor should be TSS or TSS+TES"),
+])
+def test_option_checks(root, extra, msg):
+    extra = [os.path.join(root, "gridL.txt") if a == "GL" else os.path.join(root, "gridS.txt") if a == "GS" else a
+             for a in extra]
+    r = _run(_base(root, os.path.join(root, "o")) + extra, root)
+    assert r.returncode == 1
+    assert msg in r.stderr
+
+
+def test_no_subgroup_left_writes_nothing(root):
+    out = os.path.join(root, "none")
+    r = _run(_base(root, out) + ["--analys", "sep", "--sbgrp", "nosuch"], root)
+    assert r.returncode == 0, r.stderr
+    assert not [f for f in os.listdir(root) if f.startswith("none_")]
