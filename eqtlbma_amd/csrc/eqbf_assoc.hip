@@ -40,7 +40,6 @@ namespace {
 constexpr int TILE = 64;  /* pairs per tile (one per lane)           */
 constexpr float kHalfInvLn10 = 0.21714724095162590f; /* 1 / (2 ln 10) */
 
-enum { F_ABSENT = 0, F_NORMAL = 1, F_ZERO = 2, F_NAN = 3 };
 
 struct AssocParams {
   const PrepRec* rec;
@@ -530,55 +529,10 @@ __device__ __noinline__ void pair_stats(const eqbf_chunk& c, const AssocParams& 
   const int64_t NP = c.n_pairs;
   const PrepRec r = kp.rec[(int64_t)g * S + s];
   const bool has = r.n > 0 && c.snp_in_sbgrp[snp * S + s];
-  int f = F_ABSENT;
-  double o5[5] = {NAN, NAN, NAN, NAN, NAN};
-  double st3[3] = {NAN, NAN, NAN};
-  int nout = 0;
-  if (has) {
-    nout = r.n;
-    f = F_NAN;
-    if (!(r.flags & PREP_TOO_FEW)) {
-      const double xx = sxx - proj;
-      const int n = r.n, P = r.q + 2;
-      if (xx > 1e-12 * sxx) {
-        const double beta = sxy / xx;
-        const double rss = r.yy - sxy * beta;
-        const int rank = r.rank + 1;
-        const double se = sqrt(rss / (double)(n - P) / xx);
-        const double sig = sqrt(rss / (double)(n - rank));
-        const double tstat = beta / se;
-        const double wv = dm::tz_eval(kp.tz, n - rank, fabs(tstat));
-        const double wz = (n - rank == n - P) ? wv : dm::tz_eval(kp.tz, n - P, fabs(tstat));
-        o5[0] = 1.0 - rss / r.tss;
-        o5[1] = sig;
-        o5[2] = beta;
-        o5[3] = se;
-        o5[4] = 2.0 * exp(-wv);
-        const double z = dm::gauss_pinv_from_w(wz);
-        const double bhat = beta / sig, sebhat = se / sig;
-        if (!isnan(bhat / sebhat)) {
-          if (fabs(z) > 1e-8) {
-            const double sig2 = fabs(beta) / (fabs(z) * sebhat);
-            const double b2 = beta / sig2;
-            st3[0] = b2;
-            st3[1] = fabs(b2 / z);
-            st3[2] = z;
-            f = F_NORMAL;
-          } else {
-            st3[0] = 0.0;
-            st3[1] = INFINITY;
-            st3[2] = z;
-            f = F_ZERO;
-          }
-        }
-      } else { /* genotype collinear with [1, covariates]: not estimable */
-        o5[0] = 1.0 - r.yy / r.tss;
-        o5[1] = sqrt(r.yy / (double)(n - r.rank));
-      }
-    }
-  }
+  double o5[5], st3[3];
+  const int f = uvlr_stats(r, has, sxx, sxy, proj, kp.tz, o5, st3);
   if (valid) {
-    kp.n_out[(int64_t)s * NP + p] = nout;
+    kp.n_out[(int64_t)s * NP + p] = has ? r.n : 0;
     if (kp.sstats)
       for (int k = 0; k < 5; ++k) kp.sstats[((int64_t)k * S + s) * NP + p] = o5[k];
     if (kp.stdsstats)
@@ -1175,7 +1129,10 @@ int assoc_uvlr_run(eqbf_ctx* ctx, const eqbf_chunk* cp, const eqbf_assoc_opts* o
   int rc = ctx_ensure_tz(ctx, N, st);
   if (rc) return rc;
 
+  const bool fast = o->precision == EQBF_PREC_MIXED && uvlr_fast_eligible(c, KU);
   hipLaunchKernelGGL(zero_u64_kernel, dim3(1024), dim3(256), 0, st, memb, (int64_t)G * N);
+  /* the batched path multiplies over every sample: rows of non-members are zero */
+  if (fast) hipLaunchKernelGGL(zero_u64_kernel, dim3(4096), dim3(256), 0, st, (uint64_t*)B, (int64_t)G * N * S * KC);
   hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, (uint64_t*)ctx->d_flag.ptr, (int64_t)1);
   {
     ProfScope ps(ctx, "prep_kernel", st);
@@ -1243,6 +1200,17 @@ int assoc_uvlr_run(eqbf_ctx* ctx, const eqbf_chunk* cp, const eqbf_assoc_opts* o
   {
     ProfScope ps(ctx, "tiles_kernel", st);
     hipLaunchKernelGGL(tiles_kernel, dim3(1), dim3(1024), 0, st, c.pair_offset, G, tiles, n_tiles);
+  }
+  if (fast) {
+    rc = uvlr_fast_run(ctx, c, o, out, st, rec, B, memb, tiles, n_tiles, KU, d_masks, (int)n_cfg);
+    if (rc) return rc;
+    if (join) {
+      ProfScope ps(ctx, "bma_kernel", st);
+      hipLaunchKernelGGL(bma_kernel, dim3((unsigned)((c.n_pairs + 255) / 256)), dim3(256), 0, st, out->w_abf,
+                         c.n_pairs, S, (int)n_cfg, n_cfg > 0 ? o->bfs : EQBF_BFS_GEN, d_masks, d_inv_choose);
+      EQBF_HIP(ctx, hipGetLastError());
+    }
+    return EQBF_OK;
   }
   /* LDS: max(two staging buffers of CH samples, statistics + partial sums) */
   int CH = CHMAX;

@@ -251,9 +251,67 @@ __device__ inline double tz_eval(const double* __restrict__ tz, int nu, double a
 
 }  // namespace dm
 
+/* ---- per (pair, subgroup) summary statistics -------------------------------
+ * From the sums x'x, x'y~ and |U'x|^2 over the members:
+ * FitSingleGeneWithSingleSnp (utils_math.cpp:166-209) then
+ * StandardizeSstatsAndCorrectSmallSampleSize (gene_snp_pair.cpp:256-290).
+ * o5: pve, sigmahat, betahat, sebetahat, p; st3: standardised b, se, z.
+ * Returns the subgroup's role in the Bayes factors: */
+enum { F_ABSENT = 0, F_NORMAL = 1, F_ZERO = 2, F_NAN = 3 };
+__device__ inline int uvlr_stats(const PrepRec& r, bool has, double sxx, double sxy, double proj,
+                                 const double* __restrict__ tz, double (&o5)[5], double (&st3)[3]) {
+  for (int k = 0; k < 5; ++k) o5[k] = NAN;
+  for (int k = 0; k < 3; ++k) st3[k] = NAN;
+  if (!has) return F_ABSENT;
+  if (r.flags & PREP_TOO_FEW) return F_NAN;
+  const double xx = sxx - proj;
+  const int n = r.n, P = r.q + 2;
+  if (!(xx > 1e-12 * sxx)) { /* genotype collinear with [1, covariates]: not estimable */
+    o5[0] = 1.0 - r.yy / r.tss;
+    o5[1] = sqrt(r.yy / (double)(n - r.rank));
+    return F_NAN;
+  }
+  const double beta = sxy / xx;
+  const double rss = r.yy - sxy * beta;
+  const int rank = r.rank + 1;
+  const double se = sqrt(rss / (double)(n - P) / xx);
+  const double sig = sqrt(rss / (double)(n - rank));
+  const double tstat = beta / se;
+  const double wv = dm::tz_eval(tz, n - rank, fabs(tstat));
+  const double wz = (n - rank == n - P) ? wv : dm::tz_eval(tz, n - P, fabs(tstat));
+  o5[0] = 1.0 - rss / r.tss;
+  o5[1] = sig;
+  o5[2] = beta;
+  o5[3] = se;
+  o5[4] = 2.0 * exp(-wv);
+  const double z = dm::gauss_pinv_from_w(wz);
+  const double bhat = beta / sig, sebhat = se / sig;
+  if (isnan(bhat / sebhat)) return F_NAN;
+  if (fabs(z) > 1e-8) {
+    const double sig2 = fabs(beta) / (fabs(z) * sebhat);
+    const double b2 = beta / sig2;
+    st3[0] = b2;
+    st3[1] = fabs(b2 / z);
+    st3[2] = z;
+    return F_NORMAL;
+  }
+  st3[0] = 0.0;
+  st3[1] = INFINITY;
+  st3[2] = z;
+  return F_ZERO;
+}
+
 /* ---- kernels launched from the C-ABI layer ------------------------------- */
 int assoc_uvlr_run(eqbf_ctx* ctx, const eqbf_chunk* c, const eqbf_assoc_opts* o, eqbf_assoc_out* out,
                    hipStream_t st);
+/* packed work list of 64-pair tiles of every gene (one block scan) */
+void launch_tiles(const int64_t* pair_offset, int G, int4* tiles, int* n_tiles, hipStream_t st);
+void launch_zero_u64(uint64_t* p, int64_t n, hipStream_t st);
+/* batched mixed-precision uvlr path (one genotype set, matrix-core sums) */
+bool uvlr_fast_eligible(const eqbf_chunk& c, int KU);
+int uvlr_fast_run(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_assoc_opts* o, eqbf_assoc_out* out, hipStream_t st,
+                  const PrepRec* rec, const double* B, const uint64_t* memb, const int4* tiles, const int* n_tiles,
+                  int KU, const uint64_t* d_masks, int n_cfg);
 int mvlr_run(eqbf_ctx* ctx, const eqbf_chunk* c, const eqbf_mvlr_opts* o, eqbf_assoc_out* out, hipStream_t st);
 /* gen-sin and all rows of out->w_abf from the grid-averaged rows. */
 int launch_bma(eqbf_ctx* ctx, double* w, int64_t n_pairs, int S, int bfs, hipStream_t st);

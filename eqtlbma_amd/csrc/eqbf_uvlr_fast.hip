@@ -1,0 +1,595 @@
+/*
+ * Mixed-precision uvlr association, batched path for one genotype set:
+ * two kernels after the prep.
+ *
+ * uvlr_sums_kernel -- the sums of every (pair, subgroup) as a dense FP64
+ *   matrix product on the matrix cores.  For the 64 pairs (SNPs) of a tile
+ *   and the N samples of the chunk:
+ *     C_x[snp][col] = sum_i x[i][snp] * B[i][col]   col = (s, k): y~_s, U_s,k
+ *     C_m[snp][s]   = sum_i x[i][snp]^2 * m_s[i]    m_s = membership of s
+ *   B holds zeros outside the members (the prep writes only members' rows
+ *   into a zeroed buffer), so the products over all N samples are the sums
+ *   over each subgroup's members.  v_mfma_f64_16x16x4f64: A = 16 SNPs x 4
+ *   samples, B = 4 samples x 16 columns, C = 16 x 16 in four FP64 per lane
+ *   (lane l: A[l % 16][l / 16], B[l / 16][l % 16], D[l / 16 + 4 r][l % 16]).
+ *   Eight waves: wave w owns SNP rows 16*(w&3).. and half of the column
+ *   tiles.  Samples are staged through LDS in chunks (double buffered, the
+ *   next chunk prefetched into registers).  The epilogue moves the sums of a
+ *   range of subgroups through LDS and computes the summary statistics
+ *   (FitSingleGeneWithSingleSnp, utils_math.cpp:166-209, and the
+ *   standardisation of gene_snp_pair.cpp:256-290) one (pair, subgroup) per
+ *   thread, writing the outputs and the standardised values the Bayes
+ *   factors need.
+ *
+ * uvlr_abf_kernel -- the Bayes factors (CalcAbfsUvlrFor*,
+ *   gene_snp_pair.cpp:364-550) for 64 pairs per workgroup, one lane per
+ *   pair, FP32 terms and FP64 grid sums.  The consistent configuration is
+ *   evaluated once per distinct phi2 of the gen / gen-fix / gen-maxh grids:
+ *   the single-subgroup terms and the sums sum b/(v+phi2), sum 1/(v+phi2)
+ *   depend on phi2 only, and every grid point with that phi2 is finished from
+ *   them with its own omega2.  Singletons use the closed form; other
+ *   configurations sweep their active subgroups per grid point.
+ */
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "eqbf_internal.h"
+
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int NT_SUMS = 512; /* threads per sums workgroup (8 waves) */
+constexpr int BPRE = 8;      /* B elements prefetched per thread      */
+constexpr float kHalfInvLn10f = 0.21714724095162590f;
+
+struct SumsParams {
+  const PrepRec* rec;
+  const double* B;      /* [G][N][CX], CX = S * KC                      */
+  const uint64_t* memb; /* [G][N]                                       */
+  const int4* tiles;
+  const int* n_tiles;
+  const double* tz;
+  int32_t* n_out;
+  double* sstats;
+  double* stdsstats;
+  float* st_v;   /* [S][NP] standardised variance s'^2 (join only)      */
+  float* st_b;   /* [S][NP] standardised effect b'                      */
+  float* st_z2;  /* [S][NP] z^2                                         */
+  uint8_t* st_f; /* [S][NP] role of the subgroup (F_*)                  */
+  int KU, KC, CX, NXT, KCH, SPP;
+};
+
+__device__ inline double geno_at(const eqbf_chunk& c, int64_t i, int64_t j) {
+  const int64_t off = i * c.geno_ld + j;
+  if (c.geno_dtype == EQBF_F32) return (double)((const float*)c.genotypes)[off];
+  return ((const double*)c.genotypes)[off];
+}
+
+/* Each wave owns NXW x-tiles (columns y~, U) and NMW m-tiles (membership)
+ * of the tile's 64 x (columns) product; the column tiles of the two halves
+ * of the workgroup are padded to those counts (surplus tiles read padding
+ * and are discarded), so the product loop has no branches. */
+template <int NXW, int NMW, bool MULTI>
+__global__ void __launch_bounds__(NT_SUMS) uvlr_sums_kernel(const eqbf_chunk c, const SumsParams sp) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int CXPP = 2 * NXW * 16; /* staged B row length */
+  const int S = c.n_subgroups, N = c.n_samples;
+  const int64_t NP = c.n_pairs;
+  const int tid = threadIdx.x, lane = tid & 63, l16 = lane & 15, kq = lane >> 4;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int rb = w & 3, half = w >> 2;
+  const int KCH = sp.KCH, CX = sp.CX, KC = sp.KC;
+  const int stage_d = KCH * 64 + KCH * CXPP + KCH; /* doubles per buffer */
+  double* const buf = (double*)smem;
+  const int nchunk = (N + KCH - 1) / KCH;
+  const int belem = KCH * CX;
+  const int tx0 = half * NXW, tm0 = half * NMW;
+
+  const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3, per_xcd = gridDim.x >> 3;
+  const int T = *sp.n_tiles;
+  const int span = (T + 7) / 8;
+  const int t0 = xcd * span, t1 = min(T, t0 + span);
+  for (int t = t0 + slot; t < t1; t += per_xcd) {
+    const int4 tile = sp.tiles[t];
+    const int g = tile.x;
+    const int64_t p0 = tile.y;
+    const int cnt = tile.z;
+    /* thread tid stages x of SNP column xj for samples (tid >> 6) + 8u */
+    const int xj = tid & 63;
+    const int64_t xsnp = c.pair_snp[p0 + min(xj, cnt - 1)];
+    const double* Bg = sp.B + (int64_t)g * N * CX;
+    const uint64_t* mg = sp.memb + (int64_t)g * N;
+    d4 ax[NXW], am[NMW];
+#pragma unroll
+    for (int k = 0; k < NXW; ++k) ax[k] = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < NMW; ++k) am[k] = d4{0.0, 0.0, 0.0, 0.0};
+
+    {
+      double* xs = buf;
+      double* bs = buf + KCH * 64;
+      uint64_t* mb = (uint64_t*)(bs + KCH * CXPP);
+      for (int kk = tid >> 6; kk < KCH; kk += NT_SUMS / 64) xs[kk * 64 + xj] = (kk < N) ? geno_at(c, kk, xsnp) : 0.0;
+      for (int e = tid; e < belem; e += NT_SUMS) {
+        const int r = e / CX;
+        bs[r * CXPP + (e - r * CX)] = (r < N) ? Bg[e] : 0.0;
+      }
+      if (tid < KCH) mb[tid] = (tid < N) ? mg[tid] : 0ull;
+    }
+    __syncthreads();
+    /* per-thread walk of the B chunk: element e = tid + u * NT_SUMS */
+    const int q512 = NT_SUMS / CX, r512 = NT_SUMS - q512 * CX;
+    const int r_first = tid / CX, c_first = tid - r_first * CX;
+    for (int ck = 0; ck < nchunk; ++ck) {
+      const double* cur = buf + (ck & 1) * stage_d;
+      double* nxt = buf + ((ck + 1) & 1) * stage_d;
+      const bool more = ck + 1 < nchunk;
+      const int i1 = (ck + 1) * KCH;
+      double bpre[BPRE], xpre[2];
+      uint64_t mpre = 0;
+      if (more) {
+        const double* src = Bg + (int64_t)i1 * CX;
+        int r = r_first, cc = c_first;
+#pragma unroll
+        for (int u = 0; u < BPRE; ++u) {
+          bpre[u] = (r < KCH && i1 + r < N) ? src[r * CX + cc] : 0.0;
+          r += q512;
+          cc += r512;
+          if (cc >= CX) {
+            cc -= CX;
+            ++r;
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const int kk = (tid >> 6) + u * (NT_SUMS / 64);
+          xpre[u] = (kk < KCH && i1 + kk < N) ? geno_at(c, i1 + kk, xsnp) : 0.0;
+        }
+        if (tid < KCH) mpre = (i1 + tid < N) ? mg[i1 + tid] : 0ull;
+      }
+      /* products of the current chunk: four samples per step */
+      const double* xs = cur;
+      const double* bs = cur + KCH * 64;
+      const uint64_t* mb = (const uint64_t*)(bs + KCH * CXPP);
+      for (int kk = 0; kk < KCH; kk += 4) {
+        const int krow = kk + kq;
+        const double a = xs[krow * 64 + rb * 16 + l16];
+        const double a2 = a * a;
+        const uint64_t mbits = mb[krow];
+        double bx[NXW];
+#pragma unroll
+        for (int k = 0; k < NXW; ++k) bx[k] = bs[krow * CXPP + (tx0 + k) * 16 + l16];
+#pragma unroll
+        for (int k = 0; k < NXW; ++k) ax[k] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bx[k], ax[k], 0, 0, 0);
+#pragma unroll
+        for (int k = 0; k < NMW; ++k) {
+          const int s = (tm0 + k) * 16 + l16;
+          const double bm = (s < S && ((mbits >> (s & 63)) & 1ull)) ? 1.0 : 0.0;
+          am[k] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, bm, am[k], 0, 0, 0);
+        }
+      }
+      __syncthreads();
+      if (more) {
+        double* xs2 = nxt;
+        double* bs2 = nxt + KCH * 64;
+        uint64_t* mb2 = (uint64_t*)(bs2 + KCH * CXPP);
+        int r = r_first, cc = c_first;
+#pragma unroll
+        for (int u = 0; u < BPRE; ++u) {
+          if (r < KCH) bs2[r * CXPP + cc] = bpre[u];
+          r += q512;
+          cc += r512;
+          if (cc >= CX) {
+            cc -= CX;
+            ++r;
+          }
+        }
+        for (int e = tid + BPRE * NT_SUMS; e < belem; e += NT_SUMS) {
+          const int rr = e / CX;
+          bs2[rr * CXPP + (e - rr * CX)] = (i1 + rr < N) ? Bg[(int64_t)i1 * CX + e] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const int kk = (tid >> 6) + u * (NT_SUMS / 64);
+          if (kk < KCH) xs2[kk * 64 + xj] = xpre[u];
+        }
+        for (int kk = (tid >> 6) + 2 * (NT_SUMS / 64); kk < KCH; kk += NT_SUMS / 64)
+          xs2[kk * 64 + xj] = (i1 + kk < N) ? geno_at(c, i1 + kk, xsnp) : 0.0;
+        if (tid < KCH) mb2[tid] = mpre;
+      }
+      __syncthreads();
+    }
+
+    /* ---- epilogue: sums of SPP subgroups at a time through LDS (one pass
+     * when MULTI is false); lane holds D[(lane >> 4) + 4 r][lane & 15] ---- */
+    double* E = buf;
+    for (int sa = 0; sa < S; sa += MULTI ? sp.SPP : S) {
+      const int sb = MULTI ? min(S, sa + sp.SPP) : S, ns = sb - sa, NL = ns * KC, ecols = NL + ns;
+#pragma unroll
+      for (int k = 0; k < NXW; ++k) {
+        const int gc = (tx0 + k) * 16 + l16;
+        const int s = gc / KC;
+        if (gc < CX && s >= sa && s < sb) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) E[(rb * 16 + kq + 4 * r) * ecols + (s - sa) * KC + (gc - s * KC)] = ax[k][r];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < NMW; ++k) {
+        const int s = (tm0 + k) * 16 + l16;
+        if (s >= sa && s < sb) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) E[(rb * 16 + kq + 4 * r) * ecols + NL + (s - sa)] = am[k][r];
+        }
+      }
+      __syncthreads();
+      for (int it = tid; it < 64 * ns; it += NT_SUMS) {
+        const int j = it & 63, s = sa + (it >> 6);
+        if (j >= cnt) continue;
+        const double* e = E + j * ecols + (s - sa) * KC;
+        double proj = 0;
+        for (int q = 0; q < sp.KU; ++q) proj += e[1 + q] * e[1 + q];
+        const int64_t p = p0 + j, o = (int64_t)s * NP + p;
+        const int64_t snp = c.pair_snp[p];
+        const PrepRec r = sp.rec[(int64_t)g * S + s];
+        const bool has = r.n > 0 && c.snp_in_sbgrp[snp * S + s];
+        double o5[5], st3[3];
+        const int f = uvlr_stats(r, has, E[j * ecols + NL + (s - sa)], e[0], proj, sp.tz, o5, st3);
+        sp.n_out[o] = has ? r.n : 0;
+        if (sp.sstats)
+          for (int k = 0; k < 5; ++k) sp.sstats[(int64_t)k * S * NP + o] = o5[k];
+        if (sp.stdsstats)
+          for (int k = 0; k < 3; ++k) sp.stdsstats[(int64_t)k * S * NP + o] = st3[k];
+        if (sp.st_f) {
+          sp.st_v[o] = (float)(st3[1] * st3[1]);
+          sp.st_b[o] = (float)st3[0];
+          sp.st_z2[o] = (float)(st3[2] * st3[2]);
+          sp.st_f[o] = (uint8_t)f;
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+/* ---- Bayes factors ------------------------------------------------------------ */
+struct PhiJob {
+  float phi2;
+  int e0, e1; /* entries [e0, e1) */
+};
+struct PhiEntry {
+  int type, gi;
+  float oma2;
+};
+
+struct AbfParams {
+  const float* st_v;
+  const float* st_b;
+  const float* st_z2;
+  const uint8_t* st_f;
+  const PhiJob* jobs;
+  const PhiEntry* ents;
+  const uint64_t* masks;
+  const double* grid; /* phiL, omaL, phiS, omaS */
+  void* raw;
+  double* w;
+  int n_jobs, n_cfg, GL, GS, raw_f64;
+};
+
+__device__ inline float lg10f(float x) { return __log2f(x) * 0.30102999566398120f; }
+
+/* Finishes one ABF from the single-subgroup sum L and A = sum b/(v+phi2),
+ * Bs = sum 1/(v+phi2) (CalcLog10AbfUvlr, gene_snp_pair.cpp:336-354). */
+__device__ inline float abf_fin(float L, float A, float Bs, float oma2) {
+  if (!(Bs > 0.f)) return 0.f;
+  const float bbar = A / Bs, V = 1.f / Bs;
+  if (bbar == 0.f || !(V < INFINITY)) return 0.f;
+  const float T2 = bbar * bbar / V;
+  float lbar = 0.f;
+  if (T2 != 0.f) {
+    const float Vo = V + oma2;
+    lbar = 0.5f * lg10f(V / Vo) + kHalfInvLn10f * T2 * oma2 / Vo;
+  }
+  return lbar + L;
+}
+
+/* log10 of the mean of 10^v over a grid: running maximum and rescaled sum */
+struct Acc {
+  float m;
+  double s;
+  __device__ void init() {
+    m = -INFINITY;
+    s = 0.0;
+  }
+  __device__ void add(float v) {
+    if (v > m) {
+      s = s * (double)exp2f((m - v) * 3.32192809488736235f) + 1.0;
+      m = v;
+    } else {
+      s += (double)exp2f((v - m) * 3.32192809488736235f);
+    }
+  }
+};
+
+__device__ inline double acc_finish(float m, double s, double weight) {
+  double res = (double)m + log10(s * weight);
+  if (fabs(res) <= 2.220446049250313e-16) res = 0.0;
+  return res;
+}
+
+__device__ inline void put_raw(const AbfParams& ap, int64_t row, int64_t NP, int64_t p, float l) {
+  if (!ap.raw) return;
+  if (ap.raw_f64)
+    ((double*)ap.raw)[row * NP + p] = l;
+  else
+    ((float*)ap.raw)[row * NP + p] = l;
+}
+
+constexpr int NT_ABF = 256;
+
+/* Bayes factors of 64 pairs, one lane per pair (four waves share the jobs). */
+__global__ void __launch_bounds__(NT_ABF) uvlr_abf_kernel(const int S, const int64_t NP, const AbfParams ap) {
+  extern __shared__ __align__(16) unsigned char sm2[];
+  float* lv = (float*)sm2;        /* [S][64] standardised variance s'^2 */
+  float* lb = lv + S * 64;        /* [S][64] standardised effect b'     */
+  float* lz = lb + S * 64;        /* [S][64] z^2                        */
+  uint8_t* lf = (uint8_t*)(lz + S * 64);
+  double* part = (double*)(sm2 + ((S * 64 * 13 + 15) / 16) * 16); /* [4 waves][3 types][2][64] */
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int64_t p0 = (int64_t)blockIdx.x * 64;
+  const int cnt = (int)min((int64_t)64, NP - p0);
+  for (int e = tid; e < S * 64; e += NT_ABF) {
+    const int s = e >> 6, j = e & 63;
+    const int64_t o = (int64_t)s * NP + p0 + min(j, cnt - 1);
+    lv[e] = ap.st_v[o];
+    lb[e] = ap.st_b[o];
+    lz[e] = ap.st_z2[o];
+    lf[e] = ap.st_f[o];
+  }
+  __syncthreads();
+  const bool valid = lane < cnt;
+  const int64_t p = p0 + (valid ? lane : 0);
+  /* nan: an active subgroup with NaN statistics sets every ABF to 0 */
+  bool any_nan = false;
+  for (int s = 0; s < S; ++s) any_nan |= lf[s * 64 + lane] == F_NAN;
+
+  Acc ta0, ta1, ta2;
+  ta0.init();
+  ta1.init();
+  ta2.init();
+  const double* phiS = ap.grid + 2 * ap.GL;
+  const double* omaS = phiS + ap.GS;
+  const int njobs = ap.n_jobs + ap.n_cfg;
+  for (int j = w; j < njobs; j += NT_ABF / 64) {
+    if (j < ap.n_jobs) {
+      /* consistent configuration at one phi2 */
+      const PhiJob jb = ap.jobs[j];
+      const float p2 = jb.phi2;
+      float L = 0.f, A = 0.f, Bs = 0.f;
+      for (int s = 0; s < S; ++s) {
+        if (lf[s * 64 + lane] != F_NORMAL) continue;
+        const float v = lv[s * 64 + lane];
+        const float r = __builtin_amdgcn_rcpf(v + p2);
+        if (p2 != 0.f) L += 0.5f * lg10f(v * r) + kHalfInvLn10f * lz[s * 64 + lane] * p2 * r;
+        A += lb[s * 64 + lane] * r;
+        Bs += r;
+      }
+      for (int e = jb.e0; e < jb.e1; ++e) {
+        const PhiEntry en = ap.ents[e];
+        const float l = any_nan ? 0.f : abf_fin(L, A, Bs, en.oma2);
+        if (valid) put_raw(ap, (int64_t)en.type * ap.GL + en.gi, NP, p, l);
+        if (en.type == 0)
+          ta0.add(l);
+        else if (en.type == 1)
+          ta1.add(l);
+        else
+          ta2.add(l);
+      }
+    } else {
+      const int cfg = j - ap.n_jobs;
+      const uint64_t mk = ap.masks[cfg];
+      const int64_t row0 = 3 * (int64_t)ap.GL + (int64_t)cfg * ap.GS;
+      Acc a;
+      a.init();
+      if (__popcll(mk) == 1) {
+        /* singleton: bbar = b, V = v + phi2, and the log terms telescope */
+        const int s = __ffsll((unsigned long long)mk) - 1;
+        const int f = lf[s * 64 + lane];
+        const float v = lv[s * 64 + lane], z2 = lz[s * 64 + lane];
+        for (int gi = 0; gi < ap.GS; ++gi) {
+          float l = 0.f;
+          if (f == F_NORMAL) {
+            const float p2 = (float)phiS[gi], o2 = (float)omaS[gi];
+            const float vp = v + p2, va = vp + o2;
+            const float r1 = __builtin_amdgcn_rcpf(vp), r2 = __builtin_amdgcn_rcpf(va);
+            l = 0.5f * lg10f(v * r2) + kHalfInvLn10f * z2 * (p2 * r1 + v * o2 * r1 * r2);
+          }
+          if (valid) put_raw(ap, row0 + gi, NP, p, l);
+          a.add(l);
+        }
+      } else {
+        bool nan = false;
+        for (int s = 0; s < S; ++s)
+          if (((mk >> s) & 1ull) && lf[s * 64 + lane] == F_NAN) nan = true;
+        for (int gi = 0; gi < ap.GS; ++gi) {
+          const float p2 = (float)phiS[gi], o2 = (float)omaS[gi];
+          float L = 0.f, A = 0.f, Bs = 0.f;
+          uint64_t m = mk;
+          while (m) {
+            const int s = __ffsll((unsigned long long)m) - 1;
+            m &= m - 1ull;
+            if (lf[s * 64 + lane] != F_NORMAL) continue;
+            const float v = lv[s * 64 + lane];
+            const float r = __builtin_amdgcn_rcpf(v + p2);
+            if (p2 != 0.f) L += 0.5f * lg10f(v * r) + kHalfInvLn10f * lz[s * 64 + lane] * p2 * r;
+            A += lb[s * 64 + lane] * r;
+            Bs += r;
+          }
+          const float l = nan ? 0.f : abf_fin(L, A, Bs, o2);
+          if (valid) put_raw(ap, row0 + gi, NP, p, l);
+          a.add(l);
+        }
+      }
+      if (valid) ap.w[(int64_t)(3 + cfg) * NP + p] = acc_finish(a.m, a.s, 1.0 / ap.GS);
+    }
+  }
+  /* combine the consistent-configuration partial sums of the four waves */
+  part[((w * 3 + 0) * 2) * 64 + lane] = (double)ta0.m;
+  part[((w * 3 + 0) * 2 + 1) * 64 + lane] = ta0.s;
+  part[((w * 3 + 1) * 2) * 64 + lane] = (double)ta1.m;
+  part[((w * 3 + 1) * 2 + 1) * 64 + lane] = ta1.s;
+  part[((w * 3 + 2) * 2) * 64 + lane] = (double)ta2.m;
+  part[((w * 3 + 2) * 2 + 1) * 64 + lane] = ta2.s;
+  __syncthreads();
+  if (w < 3 && valid) {
+    const int k = w;
+    double M = -INFINITY;
+    for (int ww = 0; ww < NT_ABF / 64; ++ww) M = fmax(M, part[((ww * 3 + k) * 2) * 64 + lane]);
+    double ssum = 0;
+    for (int ww = 0; ww < NT_ABF / 64; ++ww) {
+      const double m = part[((ww * 3 + k) * 2) * 64 + lane];
+      if (m > -INFINITY) ssum += part[((ww * 3 + k) * 2 + 1) * 64 + lane] * exp10(m - M);
+    }
+    ap.w[(int64_t)k * NP + p] = acc_finish((float)M, ssum, 1.0 / ap.GL);
+  }
+}
+
+template <int NXW, int NMW>
+void launch_sums(const eqbf_chunk& c, const SumsParams& sp, int grid, size_t lds, hipStream_t st) {
+  if (sp.SPP >= c.n_subgroups)
+    hipLaunchKernelGGL((uvlr_sums_kernel<NXW, NMW, false>), dim3(grid), dim3(NT_SUMS), lds, st, c, sp);
+  else
+    hipLaunchKernelGGL((uvlr_sums_kernel<NXW, NMW, true>), dim3(grid), dim3(NT_SUMS), lds, st, c, sp);
+}
+
+/* column tiles per wave: x-tiles padded to one of a few counts */
+template <int NMW>
+int dispatch_sums(int nxw, const eqbf_chunk& c, const SumsParams& sp, int grid, size_t lds, hipStream_t st) {
+  if (nxw <= 2) launch_sums<2, NMW>(c, sp, grid, lds, st);
+  else if (nxw <= 4) launch_sums<4, NMW>(c, sp, grid, lds, st);
+  else if (nxw <= 7) launch_sums<7, NMW>(c, sp, grid, lds, st);
+  else if (nxw <= 10) launch_sums<10, NMW>(c, sp, grid, lds, st);
+  else if (nxw <= 14) launch_sums<14, NMW>(c, sp, grid, lds, st);
+  else if (nxw <= 18) launch_sums<18, NMW>(c, sp, grid, lds, st);
+  else return 1;
+  return 0;
+}
+
+int sums_nxw(int nxw) { return nxw <= 2 ? 2 : nxw <= 4 ? 4 : nxw <= 7 ? 7 : nxw <= 10 ? 10 : nxw <= 14 ? 14 : 18; }
+
+}  // namespace
+
+bool uvlr_fast_eligible(const eqbf_chunk& c, int KU) {
+  return c.n_geno_sets == 1 && c.n_subgroups <= 64 && KU <= 8;
+}
+
+
+int uvlr_fast_run(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_assoc_opts* o, eqbf_assoc_out* out, hipStream_t st,
+                  const PrepRec* rec, const double* B, const uint64_t* memb, const int4* tiles, const int* n_tiles,
+                  int KU, const uint64_t* d_masks, int n_cfg) {
+  const int S = c.n_subgroups, G = c.n_genes;
+  const int64_t NP = c.n_pairs;
+  const bool join = o->analysis == EQBF_ANALYSIS_JOIN;
+  SumsParams sp;
+  memset(&sp, 0, sizeof sp);
+  sp.rec = rec;
+  sp.B = B;
+  sp.memb = memb;
+  sp.tiles = tiles;
+  sp.n_tiles = n_tiles;
+  sp.KU = KU;
+  sp.KC = KU + 1;
+  sp.CX = S * sp.KC;
+  sp.NXT = (sp.CX + 15) / 16;
+  sp.tz = ctx_tz_table(ctx);
+  sp.n_out = out->n;
+  sp.sstats = out->sstats;
+  sp.stdsstats = out->stdsstats;
+  if (join) {
+    uint8_t* base = (uint8_t*)ctx_buf(ctx, "abf_in", (size_t)S * NP * 13 + 64);
+    if (!base) return ctx_fail(ctx, EQBF_ERR_HIP, "allocating the Bayes-factor inputs failed");
+    sp.st_v = (float*)base;
+    sp.st_b = sp.st_v + (size_t)S * NP;
+    sp.st_z2 = sp.st_b + (size_t)S * NP;
+    sp.st_f = (uint8_t*)(sp.st_z2 + (size_t)S * NP);
+  }
+  const int nxw = sums_nxw((sp.NXT + 1) / 2), nmw = ((S + 15) / 16 + 1) / 2;
+  if ((sp.NXT + 1) / 2 > 18) return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED, "too many subgroup columns for the batched path");
+  const int CXPP = 2 * nxw * 16;
+  /* LDS: two staging buffers of KCH samples within 80 KB */
+  int KCH = (int)((80 * 1024) / (2 * 8 * (size_t)(65 + CXPP)));
+  KCH = std::min(16, KCH & ~3);
+  if (KCH < 4) return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED, "too many subgroup columns for the batched path");
+  sp.KCH = KCH;
+  /* the epilogue takes every subgroup at once when 160 KB allow it (the
+   * accumulators are then dead while the statistics are computed) */
+  const size_t lds_stage = (size_t)2 * 8 * KCH * (65 + CXPP);
+  const size_t lds_epi = (size_t)64 * 8 * S * (sp.KC + 1);
+  const size_t lds = std::max(lds_stage, std::min<size_t>(lds_epi, 160 * 1024));
+  sp.SPP = (int)std::min<size_t>(S, lds / (64 * 8 * (size_t)(sp.KC + 1)));
+  int grid = ctx->n_cu * 2;
+  grid = (grid + 7) / 8 * 8;
+  {
+    ProfScope ps(ctx, "uvlr_sums_kernel", st);
+    const int bad = nmw <= 1 ? dispatch_sums<1>(nxw, c, sp, grid, lds, st) : dispatch_sums<2>(nxw, c, sp, grid, lds, st);
+    if (bad) return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED, "too many subgroup columns for the batched path");
+  }
+  EQBF_HIP(ctx, hipGetLastError());
+  if (!join) return EQBF_OK;
+
+  /* jobs of the consistent configuration: distinct phi2 of gen, gen-fix,
+   * gen-maxh with the grid points each one finishes */
+  const int GL = (int)ctx->phiL.size(), GS = (int)ctx->phiS.size();
+  std::vector<PhiJob> jobs;
+  std::vector<PhiEntry> ents;
+  {
+    std::vector<std::pair<float, PhiEntry>> all;
+    for (int gi = 0; gi < GL; ++gi) {
+      const double p = ctx->phiL[gi], om = ctx->omaL[gi];
+      all.push_back({(float)p, PhiEntry{0, gi, (float)om}});
+      all.push_back({0.f, PhiEntry{1, gi, (float)(p + om)}});
+      all.push_back({(float)(p + om), PhiEntry{2, gi, 0.f}});
+    }
+    std::stable_sort(all.begin(), all.end(),
+                     [](const std::pair<float, PhiEntry>& a, const std::pair<float, PhiEntry>& b) { return a.first < b.first; });
+    for (size_t k = 0; k < all.size(); ++k) {
+      if (jobs.empty() || jobs.back().phi2 != all[k].first) jobs.push_back(PhiJob{all[k].first, (int)k, (int)k});
+      ents.push_back(all[k].second);
+      jobs.back().e1 = (int)k + 1;
+    }
+  }
+  const size_t jb_bytes = jobs.size() * sizeof(PhiJob), en_bytes = ents.size() * sizeof(PhiEntry);
+  uint8_t* jt = (uint8_t*)ctx_buf(ctx, "abf_jobs", jb_bytes + en_bytes + 64);
+  if (!jt) return ctx_fail(ctx, EQBF_ERR_HIP, "allocating the Bayes-factor jobs failed");
+  EQBF_HIP(ctx, hipMemcpyAsync(jt, jobs.data(), jb_bytes, hipMemcpyHostToDevice, st));
+  EQBF_HIP(ctx, hipMemcpyAsync(jt + jb_bytes, ents.data(), en_bytes, hipMemcpyHostToDevice, st));
+  EQBF_HIP(ctx, hipStreamSynchronize(st)); /* host vectors go out of scope */
+  AbfParams ap;
+  memset(&ap, 0, sizeof ap);
+  ap.st_v = sp.st_v;
+  ap.st_b = sp.st_b;
+  ap.st_z2 = sp.st_z2;
+  ap.st_f = sp.st_f;
+  ap.jobs = (const PhiJob*)jt;
+  ap.ents = (const PhiEntry*)(jt + jb_bytes);
+  ap.masks = d_masks;
+  ap.grid = (const double*)ctx->d_grid.ptr;
+  ap.raw = out->raw_abf;
+  ap.w = out->w_abf;
+  ap.n_jobs = (int)jobs.size();
+  ap.n_cfg = n_cfg;
+  ap.GL = GL;
+  ap.GS = GS;
+  ap.raw_f64 = o->raw_dtype == EQBF_F64;
+  const size_t lds2 = ((size_t)(S * 64 * 13 + 15) / 16) * 16 + (size_t)4 * 3 * 2 * 64 * 8;
+  {
+    ProfScope ps(ctx, "uvlr_abf_kernel", st);
+    hipLaunchKernelGGL(uvlr_abf_kernel, dim3((unsigned)((NP + 63) / 64)), dim3(NT_ABF), lds2, st, S, NP, ap);
+  }
+  EQBF_HIP(ctx, hipGetLastError());
+  (void)G;
+  return EQBF_OK;
+}
