@@ -32,6 +32,23 @@ sys.path.insert(0, REPO)
 
 METRIC = "SNP-gene BFs/s + gene perm p-values/s on 1 MI355X vs plain GPU, at stated error"
 HBM_PEAK_GBS = 8000.0
+FP64_MATRIX_PEAK_TFLOPS = 78.6
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "hbm_traffic.json")
+
+
+def _load_traffic(kernel, args):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 counters
+    (FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size) for this
+    workload, if such a measurement is recorded."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    e = t.get(kernel)
+    if not e or e.get("batch_genes") != args.batch_genes:
+        return None
+    return e.get("bytes_per_launch")
 
 
 def parse():
@@ -172,6 +189,16 @@ def alg_bytes(P_g, S, N, GL, GS):
     return P_g * per_pair + 8 * S * N
 
 
+def abf_bytes(P_g, S, GL, GS):
+    """Algorithmic HBM bytes of the Bayes-factor kernel for one gene: per
+    pair the standardised inputs of every subgroup (3 FP32 + 1 byte), the
+    raw Bayes factors in FP32 and the grid-averaged ones in FP64."""
+    return P_g * (13 * S + 4 * (3 * GL + S * GS) + 8 * (3 + S))
+
+
+SUMS_FLOPS_PER_MEMBER = 2 * (4 + 2)  # x.x, x.y~ and x.U for 4 basis columns (3 covariates + 1)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -240,12 +267,36 @@ def main():
     ms_step = (t1 - t0) * 1e3 / args.steps
     pairs = int(wl.offs[-1])
     value = pairs / (ms_step / 1e3)
-    total_bytes = sum(alg_bytes(int(p), S, N, GL, GS) for p in wl.P)
-    mk_ms, mk_n = kern.get("assoc_mixed_kernel", (float("nan"), 1))
     launches_per_step = len(chunks)
-    bytes_per_launch = total_bytes / launches_per_step
-    avg_ms = mk_ms / max(mk_n, 1)
-    achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9
+    members = int(wl.geno_in.sum().item())  # sum over subgroups of N_s (no missing expression levels)
+    dominant = max(kern, key=lambda k: kern[k][0])
+    dm_ms, dm_n = kern[dominant]
+    avg_ms = dm_ms / max(dm_n, 1)
+    if dominant == "uvlr_sums_kernel":
+        # FP64 matrix-core sums: algorithmic flops = 12 per (pair, member of a subgroup)
+        flops_per_launch = pairs * members * SUMS_FLOPS_PER_MEMBER / launches_per_step
+        achieved = flops_per_launch / (avg_ms / 1e3) / 1e12
+        roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_MATRIX_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / FP64_MATRIX_PEAK_TFLOPS, "traffic": None, "kernel": dominant,
+                "avg_launch_ms": avg_ms, "alg_flops_per_launch": flops_per_launch,
+                "alg_flops_per_pair": members * SUMS_FLOPS_PER_MEMBER}
+    else:
+        total_bytes = sum(alg_bytes(int(p), S, N, GL, GS) for p in wl.P)
+        bytes_per_launch = total_bytes / launches_per_step
+        achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dominant, "avg_launch_ms": avg_ms,
+                "alg_bytes_per_launch": bytes_per_launch}
+    others = {}
+    if "uvlr_abf_kernel" in kern:
+        a_ms = kern["uvlr_abf_kernel"][0] / max(kern["uvlr_abf_kernel"][1], 1)
+        a_bytes = sum(abf_bytes(int(p), S, GL, GS) for p in wl.P) / launches_per_step
+        others["uvlr_abf_kernel"] = {"bound": "hbm", "achieved": a_bytes / (a_ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
+                                     "unit": "GB/s", "frac": a_bytes / (a_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                                     "avg_launch_ms": a_ms, "alg_bytes_per_launch": a_bytes}
+    traffic = _load_traffic(dominant, args)
+    if traffic is not None:
+        roof["traffic"] = traffic
     log("step %.2f ms, %.3e pairs/s; kernels: %s" % (ms_step, value, {k: (round(v[0] / max(v[1], 1), 3), v[1]) for k, v in kern.items()}))
 
     result = {
@@ -257,10 +308,8 @@ def main():
                                "--bfs sin, 50-point grid" % args.genes,
                    "pairs": pairs, "genes": args.genes, "subgroups": S, "samples": N,
                    "grid_points": GL, "chunk_genes": args.batch_genes},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "assoc_mixed_kernel", "avg_launch_ms": avg_ms,
-                     "alg_bytes_per_launch": bytes_per_launch},
+        "roofline": roof,
+        "roofline_other_kernels": others,
         "kernels_ms_per_launch": {k: v[0] / max(v[1], 1) for k, v in kern.items()},
     }
 

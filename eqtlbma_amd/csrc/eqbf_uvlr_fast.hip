@@ -295,21 +295,27 @@ __device__ inline float abf_fin(float L, float A, float Bs, float oma2) {
   return lbar + L;
 }
 
-/* log10 of the mean of 10^v over a grid: running maximum and rescaled sum */
+/* log10 of the mean of 10^v over a grid: running maximum m and the sum of
+ * 10^(v - m) rescaled when the maximum moves (FP32, branch-free). */
+constexpr float kLog2_10 = 3.32192809488736235f;
 struct Acc {
-  float m;
-  double s;
+  float m, s;
   __device__ void init() {
     m = -INFINITY;
-    s = 0.0;
+    s = 0.f;
   }
   __device__ void add(float v) {
-    if (v > m) {
-      s = s * (double)exp2f((m - v) * 3.32192809488736235f) + 1.0;
-      m = v;
-    } else {
-      s += (double)exp2f((v - m) * 3.32192809488736235f);
-    }
+    const bool up = v > m;
+    const float e = exp2f(-fabsf(v - m) * kLog2_10);
+    s = up ? fmaf(s, e, 1.f) : s + e;
+    m = up ? v : m;
+  }
+  /* merges another accumulator into this one */
+  __device__ void merge(const Acc& o) {
+    const float M = fmaxf(m, o.m);
+    if (M == -INFINITY) return;
+    s = s * exp2f((m - M) * kLog2_10) + o.s * exp2f((o.m - M) * kLog2_10);
+    m = M;
   }
 };
 
@@ -399,6 +405,9 @@ __global__ void __launch_bounds__(NT_ABF) uvlr_abf_kernel(const int S, const int
         const int s = __ffsll((unsigned long long)mk) - 1;
         const int f = lf[s * 64 + lane];
         const float v = lv[s * 64 + lane], z2 = lz[s * 64 + lane];
+        Acc a1;
+        a1.init();
+#pragma unroll 2
         for (int gi = 0; gi < ap.GS; ++gi) {
           float l = 0.f;
           if (f == F_NORMAL) {
@@ -408,8 +417,12 @@ __global__ void __launch_bounds__(NT_ABF) uvlr_abf_kernel(const int S, const int
             l = 0.5f * lg10f(v * r2) + kHalfInvLn10f * z2 * (p2 * r1 + v * o2 * r1 * r2);
           }
           if (valid) put_raw(ap, row0 + gi, NP, p, l);
-          a.add(l);
+          if (gi & 1)
+            a1.add(l);
+          else
+            a.add(l);
         }
+        a.merge(a1);
       } else {
         bool nan = false;
         for (int s = 0; s < S; ++s)
@@ -433,16 +446,16 @@ __global__ void __launch_bounds__(NT_ABF) uvlr_abf_kernel(const int S, const int
           a.add(l);
         }
       }
-      if (valid) ap.w[(int64_t)(3 + cfg) * NP + p] = acc_finish(a.m, a.s, 1.0 / ap.GS);
+      if (valid) ap.w[(int64_t)(3 + cfg) * NP + p] = acc_finish(a.m, (double)a.s, 1.0 / ap.GS);
     }
   }
   /* combine the consistent-configuration partial sums of the four waves */
   part[((w * 3 + 0) * 2) * 64 + lane] = (double)ta0.m;
-  part[((w * 3 + 0) * 2 + 1) * 64 + lane] = ta0.s;
+  part[((w * 3 + 0) * 2 + 1) * 64 + lane] = (double)ta0.s;
   part[((w * 3 + 1) * 2) * 64 + lane] = (double)ta1.m;
-  part[((w * 3 + 1) * 2 + 1) * 64 + lane] = ta1.s;
+  part[((w * 3 + 1) * 2 + 1) * 64 + lane] = (double)ta1.s;
   part[((w * 3 + 2) * 2) * 64 + lane] = (double)ta2.m;
-  part[((w * 3 + 2) * 2 + 1) * 64 + lane] = ta2.s;
+  part[((w * 3 + 2) * 2 + 1) * 64 + lane] = (double)ta2.s;
   __syncthreads();
   if (w < 3 && valid) {
     const int k = w;
