@@ -64,6 +64,10 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--diag", action="store_true", help="per-phase split of the association kernel")
     ap.add_argument("--seed", type=int, default=1001)
+    ap.add_argument("--perm-genes", type=int, default=5000)
+    ap.add_argument("--perm-nperm", type=int, default=10000)
+    ap.add_argument("--no-perm", action="store_true", help="skip the permutation workload")
+    ap.add_argument("--only-perm", action="store_true", help="run only the permutation workload")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a kernel of known "
                          "traffic, to scale the HBM byte counters of a profiler run)")
@@ -180,6 +184,230 @@ class Workload:
             covar_in_sbgrp=self.covar_in.cpu().numpy())
 
 
+class PermWorkload:
+    """Device-resident synthetic data of the permutation configuration: S
+    subgroups sharing all N individuals, hard-call genotypes 0/1/2 under HWE
+    with MAF~U(0.05, 0.5), y = N(0,1) plus one eQTL (beta~N(0, 0.5^2) in a
+    random non-empty set of subgroups) in 10% of genes, each gene with its
+    own cis SNPs (P~Poisson(1000) clipped to [500, 1500])."""
+
+    def __init__(self, torch, G, seed, S=5, N=300):
+        dev = torch.device("cuda:0")
+        self.torch, self.dev, self.G, self.S, self.N = torch, dev, G, S, N
+        rng = np.random.default_rng(seed)
+        P = np.clip(rng.poisson(1000, size=G), 500, 1500).astype(np.int64)
+        self.P = P
+        self.offs = np.concatenate([[0], np.cumsum(P)]).astype(np.int64)
+        n_snps = int(self.offs[-1])
+        self.n_snps = n_snps
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(seed)
+        t0 = time.time()
+        self.geno = torch.empty((N, n_snps), dtype=torch.float32, device=dev)
+        step = 1 << 20
+        for a in range(0, n_snps, step):
+            b = min(n_snps, a + step)
+            maf = torch.rand(b - a, device=dev, generator=gen) * 0.45 + 0.05
+            g = (torch.rand((N, b - a), device=dev, generator=gen) < maf).float()
+            g += (torch.rand((N, b - a), device=dev, generator=gen) < maf).float()
+            self.geno[:, a:b] = g
+            del g, maf
+        y = torch.randn((G, S, N), device=dev, generator=gen, dtype=torch.float64)
+        for gi in rng.choice(G, size=int(round(0.1 * G)), replace=False):
+            j = int(self.offs[gi] + rng.integers(P[gi]))
+            beta = rng.normal(0, 0.5)
+            mask = int(rng.integers(1, 1 << S))
+            idx = torch.tensor([s for s in range(S) if (mask >> s) & 1], device=dev)
+            y[gi, idx] += beta * self.geno[:, j].double()[None, :]
+        self.y = y - y.mean(dim=2, keepdim=True)
+        self.gene_in = torch.ones((G, S), dtype=torch.uint8, device=dev)
+        self.geno_in = torch.ones((S, N), dtype=torch.uint8, device=dev)
+        self.covars = torch.zeros((S, 1, N), dtype=torch.float64, device=dev)
+        self.covar_count = torch.zeros((S,), dtype=torch.int32, device=dev)
+        self.covar_in = torch.ones((S, N), dtype=torch.uint8, device=dev)
+        self.gset = torch.zeros((S,), dtype=torch.int32, device=dev)
+        self.snp_in = torch.ones((n_snps, S), dtype=torch.uint8, device=dev)
+        self.pair_snp = torch.arange(n_snps, dtype=torch.int32, device=dev)
+        self.rel = torch.from_numpy(self.offs.copy()).to(dev)
+        torch.cuda.synchronize()
+        log("perm workload: %d genes, %d pairs, %.1f GB genotypes in %.1f s" %
+            (G, n_snps, self.geno.numel() * 4 / 1e9, time.time() - t0))
+
+    def chunk(self, abi, g0, g1):
+        c = abi.Chunk()
+        S, N = self.S, self.N
+        c.n_genes, c.n_subgroups, c.n_samples, c.n_covars_max = g1 - g0, S, N, 0
+        c.n_pairs = int(self.offs[g1] - self.offs[g0])
+        c.n_snps = self.n_snps
+        c.explevels = self.y.data_ptr() + g0 * S * N * 8
+        c.gene_in_sbgrp = self.gene_in.data_ptr() + g0 * S
+        c.geno_in_sbgrp = self.geno_in.data_ptr()
+        c.covar_count = self.covar_count.data_ptr()
+        c.covars = self.covars.data_ptr()
+        c.covar_in_sbgrp = self.covar_in.data_ptr()
+        c.n_geno_sets = 1
+        c.geno_set_of_sbgrp = self.gset.data_ptr()
+        c.genotypes = self.geno.data_ptr()
+        c.geno_dtype = abi.F32
+        c.geno_ld = self.n_snps
+        c.geno_set_stride = N * self.n_snps
+        c.snp_in_sbgrp = self.snp_in.data_ptr()
+        # pair offsets relative to gene g0: a shifted copy
+        rel = self.rel[g0:g1 + 1] - int(self.offs[g0])
+        self._keep = rel
+        c.pair_offset = rel.data_ptr()
+        c.pair_snp = self.pair_snp.data_ptr() + int(self.offs[g0]) * 4
+        return c
+
+    def host_chunk(self, abi, g0, g1):
+        p0, p1 = int(self.offs[g0]), int(self.offs[g1])
+        return abi.HostChunk(
+            explevels=self.y[g0:g1].cpu().numpy(), gene_in_sbgrp=self.gene_in[g0:g1].cpu().numpy(),
+            geno_in_sbgrp=self.geno_in.cpu().numpy(),
+            genotypes=np.ascontiguousarray(self.geno[:, p0:p1].cpu().numpy()),
+            snp_in_sbgrp=self.snp_in[p0:p1].cpu().numpy(), pair_offset=self.offs[g0:g1 + 1] - p0,
+            pair_snp=np.arange(p1 - p0, dtype=np.int32))
+
+
+WRTSIZE = 10  # genes per chunk of the reference's run loop (--wrtsize default); the RNG restarts per chunk
+
+
+def perm_leg(args, torch, result):
+    """Gene-level permutation p-values (--analys join --pbf all --maxbf,
+    joint shuffle, seed 7): true statistics from the association pass, the
+    statistic of every permutation, and the reference's counting loop."""
+    from eqtlbma_amd import _abi, synth
+    from eqtlbma_amd.engine import Engine, PermOpts, Rng, perm_pvalue
+    gL, gS = synth.make_grid("general"), synth.make_grid("small")
+    G, nperm = args.perm_genes, args.perm_nperm
+    wl = PermWorkload(torch, G, 1002)
+    S, N = wl.S, wl.N
+    dev = wl.dev
+    eng = Engine(0)
+    eng.set_grids(gL, gS)
+    n_cfg = 2 ** S - 1
+    n_w = 3 + n_cfg + 2
+    n_raw = 3 * gL.shape[0] + n_cfg * gS.shape[0]
+    NP = int(wl.offs[-1])
+    out_n = torch.empty((S, NP), dtype=torch.int32, device=dev)
+    out_w = torch.empty((n_w, NP), dtype=torch.float64, device=dev)
+    out_raw = torch.empty((n_raw, 1), dtype=torch.float32, device=dev)
+    o = _abi.AssocOut()
+    o.n, o.sstats, o.stdsstats, o.raw_abf, o.w_abf = out_n.data_ptr(), None, None, None, out_w.data_ptr()
+    aopts = _abi.AssocOpts(_abi.ANALYSIS["join"], _abi.BFS["all"], _abi.PREC["mixed"], _abi.F32)
+    popts = PermOpts(_abi.ANALYSIS["join"], _abi.PBF["all"], 1, -1, _abi.PREC["mixed"])
+    t = time.perf_counter()
+    tables = Rng(7).tables(N, WRTSIZE, nperm)  # [WRTSIZE][nperm][N]: one RNG stream per chunk of genes
+    t_tab = time.perf_counter() - t
+    tables_d = torch.from_numpy(tables).to(dev)
+    stats = torch.empty((G, nperm), dtype=torch.float64, device=dev)
+    gene_table = (np.arange(G) % WRTSIZE).astype(np.int32)
+    stream = torch.cuda.current_stream()
+    sptr = C.c_void_p(stream.cuda_stream)
+    w_all_row = 3 + n_cfg + 1
+
+    def run(g1):
+        c = wl.chunk(_abi, 0, g1)
+        eng.assoc_uvlr_device(c, aopts, o, sptr)
+        eng.perm_stats_device(c, popts, C.c_void_p(tables_d.data_ptr()), WRTSIZE, gene_table[:g1], nperm,
+                              C.c_void_p(stats.data_ptr()), sptr)
+        w = out_w[w_all_row, :int(wl.offs[g1])].cpu().numpy()
+        st = stats[:g1].cpu().numpy()
+        pv = np.empty(g1)
+        for g in range(g1):
+            true = np.max(w[wl.offs[g]:wl.offs[g + 1]])
+            pv[g] = perm_pvalue(st[g], true, nperm)[0]
+        return pv, st
+
+    run(min(G, 2 * WRTSIZE))  # warm-up: buffers and tables
+    torch.cuda.synchronize()
+    eng.profile(True)
+    before = eng.profile_read()
+    t0 = time.perf_counter()
+    pv, st = run(G)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    after = eng.profile_read()
+    eng.profile(False)
+    kern = {k: (v[0] - before.get(k, (0.0, 0))[0], v[1] - before.get(k, (0.0, 0))[1]) for k, v in after.items()}
+    kern = {k: v for k, v in kern.items() if v[1] > 0}
+    value = G / dt
+    perm = {"metric": "gene permutation p-values/s", "value": value, "unit": "genes/s", "seconds": dt,
+            "workload": "permutations: %d genes, %d subgroups sharing N=%d, cis SNPs~Pois(1000) in [500,1500], "
+                        "hard calls, %d joint permutations per gene (GSL mt19937 seed 7, one stream per %d-gene "
+                        "chunk), --pbf all --maxbf, grids 25/10" % (G, S, N, nperm, WRTSIZE),
+            "pairs": NP, "host_table_generation_s": t_tab,
+            "kernels_ms_total": {k: v[0] for k, v in kern.items()},
+            "kernels_launches": {k: v[1] for k, v in kern.items()},
+            "egenes_p_below_0.01": int(np.sum(pv < 0.01))}
+    # roofline of the contraction: 2 N P S nperm int8 multiply-adds per gene counted once
+    if "pf_gemm_kernel" in kern:
+        ops = 2.0 * N * NP * S * nperm
+        gm = kern["pf_gemm_kernel"][0] / 1e3
+        perm["roofline_gemm"] = {"bound": "mfma", "achieved": ops / gm / 1e12, "peak": 5000.0,
+                                 "unit": "TOP/s (int8)", "frac": ops / gm / 1e12 / 5000.0, "kernel": "pf_gemm_kernel",
+                                 "alg_ops": ops, "total_ms": gm * 1e3}
+    if "pf_bma_kernel" in kern:
+        terms = float(NP) * nperm * n_cfg * gS.shape[0]
+        bm = kern["pf_bma_kernel"][0] / 1e3
+        # one exp2 per (SNP, permutation, configuration, grid point): transcendental rate
+        # of 256 CUs x 16 lanes per clock x 2.4 GHz
+        peak = 256 * 16 * 2.4e9
+        perm["roofline_bma"] = {"bound": "transcendental", "achieved": terms / bm / 1e12, "peak": peak / 1e12,
+                                "unit": "Tterm/s", "frac": terms / bm / peak, "kernel": "pf_bma_kernel",
+                                "terms": terms, "total_ms": bm * 1e3}
+    log("perm: %d genes x %d permutations in %.2f s = %.1f genes/s; kernels %s" %
+        (G, nperm, dt, value, {k: round(v[0], 1) for k, v in kern.items()}))
+
+    # plain FP64 one-gene-per-launch path: a few permutations of one gene, per (gene, permutation)
+    if not args.no_plain:
+        k = 8
+        c = wl.chunk(_abi, 0, 1)
+        tab1 = torch.from_numpy(np.ascontiguousarray(tables[:1, :k])).to(dev)
+        st1 = torch.empty((1, k), dtype=torch.float64, device=dev)
+        pp = PermOpts(_abi.ANALYSIS["join"], _abi.PBF["all"], 1, -1, _abi.PREC["fp64_plain"])
+        eng.perm_stats_device(c, pp, C.c_void_p(tab1.data_ptr()), 1, np.zeros(1, np.int32), k,
+                              C.c_void_p(st1.data_ptr()), sptr)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        eng.perm_stats_device(c, pp, C.c_void_p(tab1.data_ptr()), 1, np.zeros(1, np.int32), k,
+                              C.c_void_p(st1.data_ptr()), sptr)
+        torch.cuda.synchronize()
+        dtp = time.perf_counter() - t
+        plain_rate = 1.0 / (dtp / k * nperm)
+        perm["plain_gpu"] = {"value": plain_rate, "unit": "genes/s",
+                             "sample": "gene 0 (%d SNPs), %d permutations, FP64, one launch per gene and "
+                                       "permutation; rate per gene at %d permutations" % (wl.P[0], k, nperm),
+                             "speedup_vs_plain": value / plain_rate}
+        e_plain = float(np.max(np.abs(st1.cpu().numpy()[0] - st[0, :k])))
+        perm["plain_vs_mixed_stat_abs"] = e_plain
+        log("perm plain %.3e genes/s" % plain_rate)
+    if not args.no_cpu_baseline:
+        from oracle import oracle as orc
+        hc = wl.host_chunk(_abi, 0, 1)
+        k = 2
+        t = time.perf_counter()
+        ref = orc.perm_join_stats(hc, 0, "all", True, gL, gS, tables[0, :k])
+        dtc = time.perf_counter() - t
+        k2 = max(k, min(nperm, int(args.cpu_seconds / max(dtc / k, 1e-4))))
+        if k2 > k:
+            t = time.perf_counter()
+            ref = orc.perm_join_stats(hc, 0, "all", True, gL, gS, tables[0, :k2])
+            dtc = time.perf_counter() - t
+            k = k2
+        perm["cpu_baseline"] = {"value": 1.0 / (dtc / k * nperm), "unit": "genes/s", "cores": 1, "kind": "port",
+                                "sample": "gene 0 (%d SNPs), first %d permutations, oracle restatement, single "
+                                          "thread, %.1f s; rate per gene at %d permutations" %
+                                          (wl.P[0], k, dtc, nperm)}
+        perm["error_vs_fp64_oracle"] = {"perm_stat_abs": float(np.max(np.abs(st[0, :k] - ref))), "sample_perms": k}
+        log("perm cpu oracle %.3e genes/s; stat error %.3g" % (perm["cpu_baseline"]["value"],
+                                                                 perm["error_vs_fp64_oracle"]["perm_stat_abs"]))
+    eng.close()
+    del wl, stats, tables_d, out_n, out_w
+    torch.cuda.empty_cache()
+    result["perm"] = perm
+
+
 def alg_bytes(P_g, S, N, GL, GS):
     """Algorithmic HBM bytes of the association pass for one gene: per pair
     the genotype row (4N), the raw Bayes factors in FP32 (3 GL + S GS), the
@@ -203,11 +431,18 @@ def main():
     args = parse()
     if args.gpus != 1:
         print("bench.py measures one GPU: the engine shards genes across GPUs with no "
-              "collective on the hot path (see DESIGN.md); --gpus %d is not run." % args.gpus)
+              "collective on the hot path; --gpus %d is not run." % args.gpus)
         sys.exit(2)
     import torch
     from eqtlbma_amd import _abi, synth
     from eqtlbma_amd.engine import Engine, lib
+
+    if args.only_perm:
+        result = {"metric": METRIC, "n_gpus": 1, "data": "synthetic"}
+        perm_leg(args, torch, result)
+        result.update({"value": result["perm"]["value"], "unit": "genes/s"})
+        print(json.dumps(result), flush=True)
+        return
 
     grid = synth.make_grid("general", n_aes=10)  # 50 points: 10 effect sizes x 5 homogeneities
     GL = GS = grid.shape[0]
@@ -225,7 +460,7 @@ def main():
     if args.calib_copy_mib > 0:
         src = torch.ones(args.calib_copy_mib << 18, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
-        torch.mul(src, 1.0, out=dst)  # an elementwise kernel: reads and writes every byte once
+        torch.neg(src, out=dst)  # an elementwise kernel (the only neg of the run): reads and writes every byte once
         torch.cuda.synchronize()
         log("calibration copy: %d MiB read + %d MiB written" % (args.calib_copy_mib, args.calib_copy_mib))
         del src, dst
@@ -404,6 +639,10 @@ def main():
         log("plain %.3e pairs/s" % plain_rate)
 
     eng.close()
+    del wl, chunks, out_n, out_raw, out_w
+    torch.cuda.empty_cache()
+    if not args.no_perm:
+        perm_leg(args, torch, result)
     print(json.dumps(result), flush=True)
 
 

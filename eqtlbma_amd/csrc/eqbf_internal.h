@@ -317,5 +317,10 @@ int mvlr_run(eqbf_ctx* ctx, const eqbf_chunk* c, const eqbf_mvlr_opts* o, eqbf_a
 int launch_bma(eqbf_ctx* ctx, double* w, int64_t n_pairs, int S, int bfs, hipStream_t st);
 int perm_stats_run(eqbf_ctx* ctx, const eqbf_chunk* c, const eqbf_perm_opts* po, const int32_t* tables_dev,
                    int32_t n_tables, const int32_t* gene_table, int64_t nperm, double* stats_dev, hipStream_t st);
+/* contraction path for the genes with full membership; done[g] = 1 for the
+ * genes it finished (the others go to the general path) */
+int perm_fast_run(eqbf_ctx* ctx, const eqbf_chunk* c, const eqbf_perm_opts* po, const int32_t* tables_dev,
+                  int32_t n_tables, const int32_t* gene_table, int64_t nperm, double* stats_dev, hipStream_t st,
+                  std::vector<uint8_t>& done);
 
 #endif

@@ -18,7 +18,9 @@
  *         (utils::log10_weighted_sum, gene.cpp:689-690);
  *   sep:  minimum p-value over SNPs (and subgroups for --permsep 1).
  * This path is exact for any input (missing values, uneven membership,
- * covariates); a faster contraction path for full membership is separate.
+ * covariates).  With the mixed precision, genes whose every sample is in
+ * every subgroup go through the contraction path (eqbf_perm_fast.hip) first
+ * and only the others come here.
  */
 #include <algorithm>
 #include <cstring>
@@ -150,6 +152,12 @@ int perm_stats_run(eqbf_ctx* ctx, const eqbf_chunk* cp, const eqbf_perm_opts* po
     EQBF_HIP(ctx, hipMemcpyAsync(stats_dev, nanrow.data(), nanrow.size() * sizeof(double), hipMemcpyHostToDevice, st));
     EQBF_HIP(ctx, hipStreamSynchronize(st));
   }
+  /* genes that qualify go through the contraction path first */
+  std::vector<uint8_t> done;
+  {
+    int rc = perm_fast_run(ctx, cp, po, tables_dev, n_tables, gene_table, nperm, stats_dev, st, done);
+    if (rc) return rc;
+  }
   const int bfs = join ? pbf_bfs(po->pbf) : EQBF_BFS_GEN;
   const int64_t n_cfg = join ? eqbf_n_configs(bfs, S) : 0;
   if (n_cfg < 0) return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED, "--pbf all with %d subgroups", S);
@@ -223,7 +231,7 @@ int perm_stats_run(eqbf_ctx* ctx, const eqbf_chunk* cp, const eqbf_perm_opts* po
   };
   off_v.assign(1, 0);
   for (int g = 0; g < G; ++g) {
-    if (gene_table[g] < 0) continue;
+    if (gene_table[g] < 0 || done[g]) continue;
     const int64_t np = off[g + 1] - off[g];
     if (np <= 0) continue;
     for (int64_t p = 0; p < nperm; ++p) {

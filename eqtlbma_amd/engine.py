@@ -56,6 +56,8 @@ def lib():
         L.eqbf_rng_shuffle.argtypes = [vp, vp, i64]
         L.eqbf_perm_tables.argtypes = [vp, i64, i64, i64, vp]
         L.eqbf_perm_pvalue.argtypes = [vp, C.c_double, i32, vp, i32, i64, vp, vp, vp, vp]
+        L.eqbf_profile.argtypes = [vp, C.c_int]
+        L.eqbf_profile_read.argtypes = [vp, vp, vp, vp, i32, vp]
         L.eqbf_perm_pvalue.restype = i64
         _lib = L
     return _lib
@@ -196,8 +198,31 @@ class Engine:
                                           _p(gt), nperm, _p(out)))
         return out
 
+    def perm_stats_device(self, chunk_struct, opts, tables_ptr, n_tables, gene_table, nperm, stats_ptr, stream=None):
+        """Device-pointer entry: tables_ptr [n_tables][nperm][N] int32 and
+        stats_ptr [n_genes][nperm] float64 are device addresses; gene_table
+        is a host int32 array."""
+        gt = np.ascontiguousarray(gene_table, dtype=np.int32)
+        self._check(lib().eqbf_perm_stats_device(self._ctx, C.byref(chunk_struct), C.byref(opts), tables_ptr,
+                                                 int(n_tables), _p(gt), int(nperm), stats_ptr, stream))
+
     def sync(self):
         self._check(lib().eqbf_sync(self._ctx))
+
+    def profile(self, enable=True):
+        """Turns the live per-kernel timing of this context on or off."""
+        self._check(lib().eqbf_profile(self._ctx, int(enable)))
+
+    def profile_read(self, n_max=64):
+        """{kernel name: (total ms, launches)} since profiling was turned on."""
+        names = C.create_string_buffer(64 * n_max)
+        tot = np.zeros(n_max)
+        cnt = np.zeros(n_max, np.int64)
+        nk = C.c_int32(0)
+        self._check(lib().eqbf_profile_read(self._ctx, names, _p(tot), _p(cnt), n_max, C.byref(nk)))
+        raw = names.raw
+        return {raw[64 * k:64 * (k + 1)].split(b"\0")[0].decode(): (float(tot[k]), int(cnt[k]))
+                for k in range(nk.value)}
 
     def debug_ttail(self, t, nu):
         t = np.ascontiguousarray(t, dtype=np.float64)
