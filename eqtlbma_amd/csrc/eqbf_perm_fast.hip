@@ -25,8 +25,9 @@
  * From XY the fit (FitSingleGeneWithSingleSnp, utils_math.cpp:166-209) is
  * closed form in FP64 (the residual sum of squares y'y - (x'y)^2/x'x included);
  * the standardised z (gene_snp_pair.cpp:256-290) comes from a table of
- * z(asinh(|t|/sqrt(nu))) in FP32, or the FP64 tail functions for |t| < 1e-5
- * where the reference's |z| > 1e-8 test is decided.
+ * z(asinh(|t|/sqrt(nu))) = z(atanh(R)) in FP32, R = |x'y| / sqrt(x'x y'y),
+ * or from the FP64 tail functions for |t| < 1e-5, where the reference's
+ * |z| > 1e-8 test is decided.
  *
  * Because every subgroup has the same members, the standardised variance
  * v = 1/x'x is the same in every subgroup, and the log10 ABF of a
@@ -61,7 +62,9 @@ namespace {
 typedef int v4i __attribute__((ext_vector_type(4)));
 
 constexpr int PF_NMAX = 2048;        /* samples (LDS holds the digit fragments) */
-constexpr int ZT_ROW = 4 + 2 * TZ_NODES; /* floats per zeta-table row            */
+constexpr int ZT_STRIDE = 3;                             /* every third t-tail node   */
+constexpr int ZT_NODES = (TZ_NODES - 1) / ZT_STRIDE + 1; /* nodes of the z table      */
+constexpr int ZT_ROW = 4 + 2 * ZT_NODES;                 /* floats per z-table row    */
 constexpr int PF_SB = 64;            /* SNPs per Bayes-factor work item         */
 constexpr float kLog2_10f = 3.32192809488736235f;
 constexpr double kLog2_10 = 3.32192809488736235;
@@ -188,7 +191,7 @@ __global__ void __launch_bounds__(256) pf_prep_y_kernel(const eqbf_chunk c, cons
  * x'x - (sum x)^2 / N, 1 / that, sum x and the not-estimable flag
  * (uvlr_stats: x'x - |U'x|^2 <= 1e-12 x'x). */
 struct PStat {
-  double xx, inv_xx, sx;
+  double xx, inv_xx, sx, rs_xx; /* rs_xx = 1/sqrt(xx) */
 };
 __global__ void __launch_bounds__(64) pf_prep_x_kernel(const eqbf_chunk c, const int32_t* __restrict__ bsrc,
                                                        int64_t nbp, int Npad, int8_t* __restrict__ G8,
@@ -211,38 +214,30 @@ __global__ void __launch_bounds__(64) pf_prep_x_kernel(const eqbf_chunk c, const
     row[i0 >> 2] = word;
   }
   const double xx = sxx - sx * sx / (double)N;
-  ps[b] = PStat{xx, 1.0 / xx, sx};
+  ps[b] = PStat{xx, 1.0 / xx, sx, 1.0 / sqrt(xx)};
   pflag[b] = !(xx > 1e-12 * sxx);
 }
 
 /* ---- z as a function of s = asinh(|t| / sqrt(nu)) for one nu -------------------- */
-/* From the t-tail table row of nu (segment A, s in [0, sA]): value and
- * derivative of |z| = -Phi^-1(exp(-w)) at the nodes, d|z|/dw =
- * exp(-w + z^2/2) sqrt(2 pi). */
+/* From every third node of the t-tail table row of nu (segment A, s in
+ * [0, sA]): value and derivative of |z| = -Phi^-1(exp(-w)), d|z|/dw =
+ * exp(-w + z^2/2) sqrt(2 pi).  11 KB, so that it sits in LDS. */
 __global__ void pf_zeta_table_kernel(const double* __restrict__ tz, int nu, float* __restrict__ out) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const double* row = tz + (size_t)nu * TZ_ROW;
-  const double hA = row[0], sA = row[1];
+  const double h = row[0] * ZT_STRIDE, sA = row[1];
   if (k == 0) {
-    out[0] = (float)hA;
+    out[0] = (float)h;
     out[1] = (float)sA;
-    out[2] = (float)(1.0 / hA);
+    out[2] = (float)(1.0 / h);
     out[3] = 0.f;
   }
-  if (k >= TZ_NODES) return;
-  const double w = row[4 + 2 * k], dw = row[4 + 2 * k + 1];
+  if (k >= ZT_NODES) return;
+  const double w = row[4 + 2 * ZT_STRIDE * k], dw = row[4 + 2 * ZT_STRIDE * k + 1];
   const double z = -dm::gauss_pinv_from_w(w);
   const double dz = exp(-w + 0.5 * z * z) * 2.5066282746310002 * dw;
   out[4 + 2 * k] = (float)z;
   out[4 + 2 * k + 1] = (float)dz;
-}
-
-__device__ inline float pf_asinh(float a) { /* a >= 0 */
-  if (a < 0.125f) {
-    const float a2 = a * a;
-    return a * (1.f + a2 * (-1.f / 6.f + a2 * (3.f / 40.f + a2 * (-15.f / 336.f + a2 * (105.f / 3456.f)))));
-  }
-  return __logf(a + sqrtf(fmaf(a, a, 1.f)));
 }
 
 /* |z| at s (< sA) by cubic Hermite interpolation of the table. */
@@ -250,7 +245,7 @@ __device__ inline float pf_zeta_abs(const float* __restrict__ zt, float s) {
   const float h = zt[0], inv_h = zt[2];
   const float u = s * inv_h;
   int k = (int)u;
-  if (k > TZ_NODES - 2) k = TZ_NODES - 2;
+  if (k > ZT_NODES - 2) k = ZT_NODES - 2;
   const float f = u - (float)k;
   const float2 a = ((const float2*)(zt + 4))[k], b = ((const float2*)(zt + 4))[k + 1];
   const float f2 = f * f, f3 = f2 * f;
@@ -300,6 +295,7 @@ __global__ void __launch_bounds__(256) pf_gemm_kernel(const PfGemmArgs a) {
   uint32_t* Ws = (uint32_t*)sm;
   v4i* Bf = (v4i*)(sm + (size_t)a.Npad * 4);
   __shared__ double redt[4][16];
+  __shared__ __align__(16) float zts[MODE == PF_SEP ? 4 : ZT_ROW];
   const int4 it = a.items[blockIdx.x];
   const int bi = it.x, pb = it.y, s = it.z;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -307,6 +303,8 @@ __global__ void __launch_bounds__(256) pf_gemm_kernel(const PfGemmArgs a) {
   {
     const uint32_t* Wg = a.W + ((int64_t)bi * a.S + s) * a.Npad;
     for (int i = tid; i < a.Npad; i += 256) Ws[i] = Wg[i];
+    if (MODE != PF_SEP)
+      for (int i = tid; i < ZT_ROW; i += 256) zts[i] = a.zt[i];
   }
   __syncthreads();
   {
@@ -317,12 +315,24 @@ __global__ void __launch_bounds__(256) pf_gemm_kernel(const PfGemmArgs a) {
       const int k0 = ks * 64 + (ln >> 4) * 16;
       const int32_t* row = tab + p * N;
       uint32_t wd[16];
+      if ((N & 3) == 0 && k0 + 16 <= N) { /* 16-byte aligned rows: four vector loads */
+        const int4* r4 = (const int4*)(row + k0);
 #pragma unroll
-      for (int b = 0; b < 16; ++b) {
-        const int k = k0 + b;
-        int idx = k < N ? row[k] : 0;
-        idx = min(max(idx, 0), N - 1);
-        wd[b] = Ws[idx];
+        for (int b = 0; b < 4; ++b) {
+          const int4 v = r4[b];
+          wd[4 * b] = Ws[min(max(v.x, 0), N - 1)];
+          wd[4 * b + 1] = Ws[min(max(v.y, 0), N - 1)];
+          wd[4 * b + 2] = Ws[min(max(v.z, 0), N - 1)];
+          wd[4 * b + 3] = Ws[min(max(v.w, 0), N - 1)];
+        }
+      } else {
+#pragma unroll
+        for (int b = 0; b < 16; ++b) {
+          const int k = k0 + b;
+          int idx = k < N ? row[k] : 0;
+          idx = min(max(idx, 0), N - 1);
+          wd[b] = Ws[idx];
+        }
       }
       uint32_t o[4][4];
 #pragma unroll
@@ -351,11 +361,11 @@ __global__ void __launch_bounds__(256) pf_gemm_kernel(const PfGemmArgs a) {
   const int64_t P0 = a.bpoff[bi], np = a.bpoff[bi + 1] - P0;
   const int ntile = (int)((np + 15) / 16);
   const YStat y = a.ys[(int64_t)bi * a.S + s];
-  const double invN = 1.0 / (double)N, nu = (double)a.nu;
-  const float rs_nu = rsqrtf((float)a.nu), sA = a.zt[1];
+  const double invN = 1.0 / (double)N, nu = (double)a.nu, inv_yy = 1.0 / y.yy, rs_yy = sqrt(inv_yy);
+  const float sA = zts[MODE == PF_SEP ? 0 : 1];
   const int64_t perm = (int64_t)pb * 16 + (lane & 15);
   const bool pvalid = perm < a.nperm;
-  double tmx = -1.0;
+  double r2mx = -1.0; /* sep: max R^2 = (x'y)^2 / (x'x y'y) over SNPs; t^2 = nu R^2 / (1 - R^2) */
   bool fb = false;
   for (int t = w; t < ntile; t += 4) {
     v4i acc[4];
@@ -363,11 +373,20 @@ __global__ void __launch_bounds__(256) pf_gemm_kernel(const PfGemmArgs a) {
     for (int d = 0; d < 4; ++d) acc[d] = v4i{0, 0, 0, 0};
     const int64_t arow = P0 + min((int64_t)t * 16 + (lane & 15), np - 1);
     const v4i* ap = (const v4i*)(a.G8 + arow * a.Npad) + (lane >> 4);
-    for (int ks = 0; ks < a.nks; ++ks) {
-      const v4i af = ap[ks * 4];
+    /* four 64-sample steps per round: their A loads are in flight together */
+    for (int ks0 = 0; ks0 < a.nks; ks0 += 4) {
+      v4i af[4];
 #pragma unroll
-      for (int d = 0; d < 4; ++d)
-        acc[d] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af, Bf[(ks * 4 + d) * 64 + lane], acc[d], 0, 0, 0);
+      for (int u = 0; u < 4; ++u) af[u] = ks0 + u < a.nks ? ap[(ks0 + u) * 4] : v4i{0, 0, 0, 0};
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (ks0 + u < a.nks) {
+#pragma unroll
+          for (int d = 0; d < 4; ++d)
+            acc[d] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[u], Bf[((ks0 + u) * 4 + d) * 64 + lane], acc[d], 0,
+                                                           0, 0);
+        }
+      }
     }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -379,36 +398,41 @@ __global__ void __launch_bounds__(256) pf_gemm_kernel(const PfGemmArgs a) {
       const double xy = (double)acc[0][r] + 256.0 * (double)acc[1][r] + 65536.0 * (double)acc[2][r] +
                         16777216.0 * (double)acc[3][r];
       const double sxy = (xy - x.sx * y.tsum * invN) * y.scale;
-      const double sxy2 = sxy * sxy;
-      const double rss = fma(-sxy2, x.inv_xx, y.yy);
       if (MODE == PF_SEP) {
-        double tt;
-        if (rss > 0)
-          tt = sxy2 * x.inv_xx * nu / rss;
-        else if (rss == 0 && sxy != 0)
-          tt = INFINITY;
-        else {
-          if (q == 0) fb = true; /* NaN first statistic: general path */
+        /* R^2 = 1 - rss / y'y; the residual sum of squares stays in FP64 */
+        const double r2 = sxy * sxy * x.inv_xx * inv_yy;
+        if (!(1.0 - r2 >= 0.0)) { /* rss < 0: NaN statistic */
+          if (q == 0) fb = true;  /* a NaN first statistic: general path */
           continue;
         }
-        tmx = fmax(tmx, tt);
+        r2mx = fmax(r2mx, sxy != 0.0 ? r2 : 0.0);
       } else {
+        /* R = |x'y| / sqrt(x'x y'y); |t| / sqrt(nu) = R / sqrt(1 - R^2), so
+         * s = asinh(|t| / sqrt(nu)) = atanh(R); 1 - R in FP64 */
+        const double R = fabs(sxy) * x.rs_xx * rs_yy;
+        const double omR = 1.0 - R;
         float zeta;
-        if (!(rss > 0)) {
-          fb = true;
+        if (!(omR > 0.0)) {
+          fb = true; /* rss <= 0 */
           zeta = 0.f;
         } else {
-          const double tt = sxy2 * x.inv_xx * nu / rss;
-          if (tt < 1e-10) {
-            zeta = pf_zeta_exact(a.tz, a.nu, tt, sxy);
+          const float Rf = (float)R;
+          if (Rf * Rf * (float)a.nu < 1e-10f) {
+            zeta = pf_zeta_exact(a.tz, a.nu, nu * R * R / (omR * (1.0 + R)), sxy);
           } else {
-            const float at = sqrtf((float)tt);
-            const float sv = pf_asinh(at * rs_nu);
+            float sv;
+            if (Rf < 0.3f) {
+              const float r2 = Rf * Rf;
+              sv = Rf * (1.f + r2 * (1.f / 3.f + r2 * (1.f / 5.f + r2 * (1.f / 7.f + r2 * (1.f / 9.f + r2 * (1.f / 11.f +
+                                                                                                          r2 * (1.f / 13.f + r2 * (1.f / 15.f))))))));
+            } else {
+              sv = 0.34657359027997264f * __log2f((1.f + Rf) * __builtin_amdgcn_rcpf((float)omR));
+            }
             if (!(sv < sA)) {
               fb = true;
               zeta = 0.f;
             } else {
-              zeta = copysignf(pf_zeta_abs(a.zt, sv), (float)sxy);
+              zeta = copysignf(pf_zeta_abs(zts, sv), (float)sxy);
             }
           }
         }
@@ -418,14 +442,14 @@ __global__ void __launch_bounds__(256) pf_gemm_kernel(const PfGemmArgs a) {
   }
   if (fb) a.fallback[bi] = 1;
   if (MODE == PF_SEP) {
-    tmx = fmax(tmx, __shfl_xor(tmx, 16, 64));
-    tmx = fmax(tmx, __shfl_xor(tmx, 32, 64));
-    if (lane < 16) redt[w][lane] = tmx;
+    r2mx = fmax(r2mx, __shfl_xor(r2mx, 16, 64));
+    r2mx = fmax(r2mx, __shfl_xor(r2mx, 32, 64));
+    if (lane < 16) redt[w][lane] = r2mx;
     __syncthreads();
     if (tid < 16) {
       const int64_t p = (int64_t)pb * 16 + tid;
       const double m = fmax(fmax(redt[0][tid], redt[1][tid]), fmax(redt[2][tid], redt[3][tid]));
-      if (p < a.nperm) a.tmax[((int64_t)bi * a.S + s) * a.NPP + p] = m;
+      if (p < a.nperm) a.tmax[((int64_t)bi * a.S + s) * a.NPP + p] = m < 0.0 ? -1.0 : nu * m / (1.0 - m);
     }
   }
 }
@@ -503,7 +527,7 @@ __device__ __noinline__ float pf_sum_all_zero(const PfZ<S> z, const float* __res
       const float* row = cf + j * NCF;
       float l = lw.v[m];
       if (me > 0) l += fmaf(row[S + me], Z1, fmaf(row[0], Z2, row[me]));
-      sum += exp2f(l);
+      sum += __builtin_amdgcn_exp2f(l);
     }
   }
   return sum;
@@ -534,7 +558,7 @@ __device__ inline float pf_sum_all(const float (&z)[S], const float* __restrict_
   }
 #pragma unroll
   for (int g = 1; g <= NC; ++g) Z1[g] = Z1[g] * Z1[g];
-  float sum = 0.f;
+  float part[4] = {0.f, 0.f, 0.f, 0.f}; /* four partial sums: independent add chains */
   {
     for (int j = 0; j < GS; ++j) {
       const float* row = cf + j * NCF;
@@ -548,11 +572,11 @@ __device__ inline float pf_sum_all(const float (&z)[S], const float* __restrict_
 #pragma unroll
       for (int g = 1; g <= NC; ++g) {
         const int m = __builtin_popcount(g);
-        sum += exp2f(fmaf(D[m], Z1[g], fmaf(q, Z2[g], C[m])));
+        part[g & 3] += __builtin_amdgcn_exp2f(fmaf(D[m], Z1[g], fmaf(q, Z2[g], C[m])));
       }
     }
   }
-  return sum;
+  return (part[0] + part[1]) + (part[2] + part[3]);
 }
 
 template <int S>
@@ -567,10 +591,10 @@ __device__ inline float pf_sum_gen(const float (&z)[S], const float* __restrict_
   Z1 *= Z1;
   const int me = __popc(nzm);
   float sum = 0.f;
-  if (me == 0) return (float)G * exp2f(lw);
+  if (me == 0) return (float)G * __builtin_amdgcn_exp2f(lw);
   for (int j = 0; j < G; ++j) {
     const float* row = cf + j * NCF;
-    sum += exp2f(fmaf(row[S + me], Z1, fmaf(row[0], Z2, row[me] + lw)));
+    sum += __builtin_amdgcn_exp2f(fmaf(row[S + me], Z1, fmaf(row[0], Z2, row[me] + lw)));
   }
   return sum;
 }
@@ -583,7 +607,7 @@ __device__ inline float pf_sum_sin(const float (&z)[S], const float* __restrict_
     const float* row = cf + j * NCF;
     const float c1 = row[1] + lw, k1 = row[0] + row[S + 1];
 #pragma unroll
-    for (int s = 0; s < S; ++s) sum += exp2f(z[s] != 0.f ? fmaf(k1, z[s] * z[s], c1) : lw);
+    for (int s = 0; s < S; ++s) sum += __builtin_amdgcn_exp2f(z[s] != 0.f ? fmaf(k1, z[s] * z[s], c1) : lw);
   }
   return sum;
 }
@@ -776,7 +800,7 @@ int perm_fast_run(eqbf_ctx* ctx, const eqbf_chunk* cp, const eqbf_perm_opts* po,
   if (join) {
     d_zt = (float*)ctx_buf(ctx, "pf_zt", (size_t)ZT_ROW * sizeof(float));
     if (!d_zt) return ctx_fail(ctx, EQBF_ERR_HIP, "allocating the permutation buffers failed");
-    hipLaunchKernelGGL(pf_zeta_table_kernel, dim3((TZ_NODES + 255) / 256), dim3(256), 0, st, ctx_tz_table(ctx), nu,
+    hipLaunchKernelGGL(pf_zeta_table_kernel, dim3((ZT_NODES + 255) / 256), dim3(256), 0, st, ctx_tz_table(ctx), nu,
                        d_zt);
     EQBF_HIP(ctx, hipGetLastError());
   }
