@@ -58,6 +58,18 @@ def lib():
         L.eqbf_perm_pvalue.argtypes = [vp, C.c_double, i32, vp, i32, i64, vp, vp, vp, vp]
         L.eqbf_profile.argtypes = [vp, C.c_int]
         L.eqbf_profile_read.argtypes = [vp, vp, vp, vp, i32, vp]
+        L.eqbf_mlp_create.argtypes = [vp, i32, i32, i32, i32, C.c_uint64, C.POINTER(vp)]
+        L.eqbf_mlp_destroy.argtypes = [vp]
+        L.eqbf_mlp_n_params.argtypes = [vp]
+        L.eqbf_mlp_n_params.restype = i64
+        L.eqbf_mlp_set_params.argtypes = [vp, vp, vp]
+        L.eqbf_mlp_get_params.argtypes = [vp, vp, vp]
+        L.eqbf_mlp_forward.argtypes = [vp, vp, vp, i64, vp]
+        L.eqbf_mlp_loss_grad.argtypes = [vp, vp, vp, vp, vp, i64, vp, vp]
+        L.eqbf_mlp_train.argtypes = [vp, vp, vp, vp, vp, i64, i32, i32, C.c_float, C.c_uint64, vp]
+        L.eqbf_ld_eigenvalues.argtypes = [vp, vp, i32, vp]
+        L.eqbf_ld_eigenvalues_device.argtypes = [vp, vp, i32, vp]
+        L.eqbf_gev_fit.argtypes = [vp, vp, i64, i64, vp]
         L.eqbf_perm_pvalue.restype = i64
         _lib = L
     return _lib
@@ -206,6 +218,25 @@ class Engine:
         self._check(lib().eqbf_perm_stats_device(self._ctx, C.byref(chunk_struct), C.byref(opts), tables_ptr,
                                                  int(n_tables), _p(gt), int(nperm), stats_ptr, stream))
 
+    def ld_eigenvalues(self, hc, n_eig=20):
+        """[G][n_eig] largest eigenvalues of each gene's cis LD matrix."""
+        out = np.zeros((hc.G, n_eig), np.float32)
+        st = hc.struct()
+        self._check(lib().eqbf_ld_eigenvalues(self._ctx, C.byref(st), n_eig, _p(out)))
+        return out
+
+    def ld_eigenvalues_device(self, chunk_struct, n_genes, n_eig=20):
+        out = np.zeros((n_genes, n_eig), np.float32)
+        self._check(lib().eqbf_ld_eigenvalues_device(self._ctx, C.byref(chunk_struct), n_eig, _p(out)))
+        return out
+
+    def gev_fit(self, stats):
+        """[G][3] (mu, sigma, xi) of each row of permutation statistics."""
+        st = np.ascontiguousarray(stats, dtype=np.float64)
+        out = np.zeros((st.shape[0], 3))
+        self._check(lib().eqbf_gev_fit(self._ctx, _p(st), st.shape[0], st.shape[1], _p(out)))
+        return out
+
     def sync(self):
         self._check(lib().eqbf_sync(self._ctx))
 
@@ -231,3 +262,62 @@ class Engine:
         outs = [np.zeros(n) for _ in range(4)]
         self._check(lib().eqbf_debug_ttail(self._ctx, _p(t), _p(nu), n, *[_p(o) for o in outs]))
         return outs
+
+
+class Mlp:
+    """The allocator's network on the device (eqbf_mlp_* in include/eqbf.h):
+    n_in -> n_hidden x n_layers (GELU) -> n_out, BF16 weights, FP32 master
+    copy and accumulation."""
+
+    def __init__(self, engine, n_in, n_hidden=256, n_layers=3, n_out=3, seed=1):
+        self._eng = engine
+        self._m = C.c_void_p()
+        self.n_in, self.n_out = n_in, n_out
+        engine._check(lib().eqbf_mlp_create(engine._ctx, n_in, n_hidden, n_layers, n_out, seed, C.byref(self._m)))
+        self.n_params = lib().eqbf_mlp_n_params(self._m)
+
+    def close(self):
+        if self._m:
+            lib().eqbf_mlp_destroy(self._m)
+            self._m = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        p = np.zeros(self.n_params, np.float32)
+        self._eng._check(lib().eqbf_mlp_get_params(self._eng._ctx, self._m, _p(p)))
+        return p
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        assert p.shape == (self.n_params,)
+        self._eng._check(lib().eqbf_mlp_set_params(self._eng._ctx, self._m, _p(p)))
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros((x.shape[0], self.n_out), np.float32)
+        self._eng._check(lib().eqbf_mlp_forward(self._eng._ctx, self._m, _p(x), x.shape[0], _p(y)))
+        return y
+
+    def loss_grad(self, x, t, w):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        g = np.zeros(self.n_params, np.float32)
+        loss = C.c_double(0)
+        self._eng._check(lib().eqbf_mlp_loss_grad(self._eng._ctx, self._m, _p(x), _p(t), _p(w), x.shape[0],
+                                                   C.byref(loss), _p(g)))
+        return loss.value, g
+
+    def train(self, x, t, w, epochs, batch=256, lr=1e-3, seed=1):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        curve = np.zeros(max(epochs, 1), np.float32)
+        self._eng._check(lib().eqbf_mlp_train(self._eng._ctx, self._m, _p(x), _p(t), _p(w), x.shape[0], epochs,
+                                               batch, float(lr), seed, _p(curve)))
+        return curve[:epochs]

@@ -262,6 +262,47 @@ int eqbf_profile_read(eqbf_ctx* ctx, char* names, double* total_ms, int64_t* lau
 int eqbf_debug_ttail(eqbf_ctx* ctx, const double* t, const int32_t* nu, int32_t n,
                      double* w_tab, double* w_cf, double* z_tab, double* z_cf);
 
+/* ---- learned permutation allocator: its network ------------------------------
+ * No counterpart in the reference (eqtlbma_bf gives every gene the same
+ * --nperm).  A multilayer perceptron n_in -> n_hidden (n_layers, GELU) ->
+ * n_out with BF16 weights, an FP32 master copy and FP32 accumulation,
+ * trained with Adam on the device.  It predicts the null distribution of a
+ * gene's permutation statistic (GEV location, log scale, shape) from the
+ * gene's features; the host decides how many permutations each gene gets.
+ * Parameters are one flat FP32 vector: per layer W[out][in] then b[out].
+ * All arrays are host arrays; the calls synchronise. */
+typedef struct eqbf_mlp eqbf_mlp;
+int eqbf_mlp_create(eqbf_ctx* ctx, int32_t n_in, int32_t n_hidden, int32_t n_layers, int32_t n_out,
+                    uint64_t seed, eqbf_mlp** out);
+void eqbf_mlp_destroy(eqbf_mlp* mlp);
+int64_t eqbf_mlp_n_params(const eqbf_mlp* mlp);
+int eqbf_mlp_set_params(eqbf_ctx* ctx, eqbf_mlp* mlp, const float* params);
+int eqbf_mlp_get_params(eqbf_ctx* ctx, eqbf_mlp* mlp, float* params);
+/* y[B][n_out] = network(x[B][n_in]) */
+int eqbf_mlp_forward(eqbf_ctx* ctx, eqbf_mlp* mlp, const float* x, int64_t B, float* y);
+/* loss = sum_b sum_j w_j (y_bj - target_bj)^2 / B and its gradient with
+ * respect to every parameter (grad: n_params floats, may be NULL) */
+int eqbf_mlp_loss_grad(eqbf_ctx* ctx, eqbf_mlp* mlp, const float* x, const float* target,
+                       const float* out_weight, int64_t B, double* loss, float* grad);
+/* epochs of minibatch Adam over the n_rows examples (shuffled per epoch with
+ * the library's mt19937 seeded by seed); epoch_loss: mean minibatch loss of
+ * each epoch (may be NULL) */
+int eqbf_mlp_train(eqbf_ctx* ctx, eqbf_mlp* mlp, const float* x, const float* target, const float* out_weight,
+                   int64_t n_rows, int32_t epochs, int32_t batch, float lr, uint64_t seed, float* epoch_loss);
+
+/* Inputs and targets of the network (no counterpart in the reference).
+ * eqbf_ld_eigenvalues: the n_eig (<= 28) largest eigenvalues of the
+ * correlation matrix of each gene's cis genotypes over the chunk's samples
+ * (N <= 512), descending; eig: host [n_genes][n_eig].  The _device variant
+ * takes a chunk whose pointers are device addresses.
+ * eqbf_gev_fit: location, scale and shape (xi; GEV tail
+ * 1 - exp(-(1 + xi (x - mu) / sigma)^(-1/xi))) of each gene's permutation
+ * statistics stats[n_genes][nperm] (host, NaN draws ignored, nperm <= 16384)
+ * by probability-weighted moments; params: host [n_genes][3]. */
+int eqbf_ld_eigenvalues(eqbf_ctx* ctx, const eqbf_chunk* chunk, int32_t n_eig, float* eig);
+int eqbf_ld_eigenvalues_device(eqbf_ctx* ctx, const eqbf_chunk* chunk_dev, int32_t n_eig, float* eig);
+int eqbf_gev_fit(eqbf_ctx* ctx, const double* stats, int64_t n_genes, int64_t nperm, double* params);
+
 #ifdef __cplusplus
 }
 #endif
