@@ -67,6 +67,11 @@ def parse():
     ap.add_argument("--perm-genes", type=int, default=5000)
     ap.add_argument("--perm-nperm", type=int, default=10000)
     ap.add_argument("--no-perm", action="store_true", help="skip the permutation workload")
+    ap.add_argument("--no-alloc", action="store_true", help="skip the learned-allocation workload")
+    ap.add_argument("--alloc-train", type=int, default=2000)
+    ap.add_argument("--alloc-genes", type=int, default=20000)
+    ap.add_argument("--alloc-holdout", type=int, default=1000)
+    ap.add_argument("--alloc-epochs", type=int, default=200)
     ap.add_argument("--only-perm", action="store_true", help="run only the permutation workload")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a kernel of known "
@@ -229,6 +234,7 @@ class PermWorkload:
         self.snp_in = torch.ones((n_snps, S), dtype=torch.uint8, device=dev)
         self.pair_snp = torch.arange(n_snps, dtype=torch.int32, device=dev)
         self.rel = torch.from_numpy(self.offs.copy()).to(dev)
+        self._rels = {}
         torch.cuda.synchronize()
         log("perm workload: %d genes, %d pairs, %.1f GB genotypes in %.1f s" %
             (G, n_snps, self.geno.numel() * 4 / 1e9, time.time() - t0))
@@ -252,10 +258,12 @@ class PermWorkload:
         c.geno_ld = self.n_snps
         c.geno_set_stride = N * self.n_snps
         c.snp_in_sbgrp = self.snp_in.data_ptr()
-        # pair offsets relative to gene g0: a shifted copy
-        rel = self.rel[g0:g1 + 1] - int(self.offs[g0])
-        self._keep = rel
-        c.pair_offset = rel.data_ptr()
+        # pair offsets relative to gene g0: a shifted copy, kept alive as long
+        # as the workload (the chunk holds only its address)
+        key = (g0, g1)
+        if key not in self._rels:
+            self._rels[key] = self.rel[g0:g1 + 1] - int(self.offs[g0])
+        c.pair_offset = self._rels[key].data_ptr()
         c.pair_snp = self.pair_snp.data_ptr() + int(self.offs[g0]) * 4
         return c
 
@@ -311,12 +319,11 @@ def perm_leg(args, torch, result):
         eng.assoc_uvlr_device(c, aopts, o, sptr)
         eng.perm_stats_device(c, popts, C.c_void_p(tables_d.data_ptr()), WRTSIZE, gene_table[:g1], nperm,
                               C.c_void_p(stats.data_ptr()), sptr)
-        w = out_w[w_all_row, :int(wl.offs[g1])].cpu().numpy()
+        npc = int(wl.offs[g1])  # the outputs' leading dimension is this chunk's pair count
+        w = out_w.view(-1)[w_all_row * npc:(w_all_row + 1) * npc].cpu().numpy()
+        true = np.maximum.reduceat(w, wl.offs[:g1].astype(np.int64))
         st = stats[:g1].cpu().numpy()
-        pv = np.empty(g1)
-        for g in range(g1):
-            true = np.max(w[wl.offs[g]:wl.offs[g + 1]])
-            pv[g] = perm_pvalue(st[g], true, nperm)[0]
+        pv = np.array([perm_pvalue(st[g], true[g], nperm)[0] for g in range(g1)])
         return pv, st
 
     run(min(G, 2 * WRTSIZE))  # warm-up: buffers and tables
@@ -408,6 +415,141 @@ def perm_leg(args, torch, result):
     result["perm"] = perm
 
 
+def perm_alloc_leg(args, torch, result):
+    """Learned permutation allocator at scale: trained on null draws of 2,000
+    genes x 10,000 permutations from the permutation engine, then applied to
+    20,000 genes (predicted p > 0.2: 1,000 permutations, else 10,000), with
+    1,000 of them also permuted fully to judge the p-values and eGene calls."""
+    from eqtlbma_amd import _abi, allocator, synth
+    from eqtlbma_amd.engine import Engine, PermOpts, Rng, perm_pvalue
+    gL, gS = synth.make_grid("general"), synth.make_grid("small")
+    n_train, n_inf, n_hold = args.alloc_train, args.alloc_genes, args.alloc_holdout
+    nfull, nlow = args.perm_nperm, max(1, args.perm_nperm // 10)
+    G = n_train + n_inf
+    wl = PermWorkload(torch, G, 1004)
+    S, N, dev = wl.S, wl.N, wl.dev
+    eng = Engine(0)
+    eng.set_grids(gL, gS)
+    n_cfg = 2 ** S - 1
+    NP = int(wl.offs[-1])
+    out_n = torch.empty((S, NP), dtype=torch.int32, device=dev)
+    out_w = torch.empty((3 + n_cfg + 2, NP), dtype=torch.float64, device=dev)
+    o = _abi.AssocOut()
+    o.n, o.sstats, o.stdsstats, o.raw_abf, o.w_abf = out_n.data_ptr(), None, None, None, out_w.data_ptr()
+    aopts = _abi.AssocOpts(_abi.ANALYSIS["join"], _abi.BFS["all"], _abi.PREC["mixed"], _abi.F32)
+    popts = PermOpts(_abi.ANALYSIS["join"], _abi.PBF["all"], 1, -1, _abi.PREC["mixed"])
+    tables = Rng(7).tables(N, WRTSIZE, nfull)
+    tab_full = torch.from_numpy(tables).to(dev)
+    tab_low = torch.from_numpy(np.ascontiguousarray(tables[:, :nlow])).to(dev)
+    stats = torch.empty((G, nfull), dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream()
+    sptr = C.c_void_p(stream.cuda_stream)
+    c_all = wl.chunk(_abi, 0, G)
+    slot = (np.arange(G) % WRTSIZE).astype(np.int32)
+
+    def perm(genes, k):
+        gt = np.full(G, -1, np.int32)
+        gt[genes] = slot[genes]
+        tab = tab_full if k == nfull else tab_low
+        eng.perm_stats_device(c_all, popts, C.c_void_p(tab.data_ptr()), WRTSIZE, gt, k,
+                              C.c_void_p(stats.data_ptr()), sptr)
+        return stats[:, :k]  # statistics of gene g in row g (leading dimension k)
+
+    def stats_rows(genes, k):
+        flat = stats.view(-1)[:G * k].view(G, k)
+        return flat[torch.from_numpy(np.asarray(genes)).to(dev)].cpu().numpy()
+
+    t0 = time.perf_counter()
+    eng.assoc_uvlr_device(c_all, aopts, o, sptr)
+    w_all = out_w[3 + n_cfg + 1].cpu().numpy()
+    true = np.maximum.reduceat(w_all, wl.offs[:G].astype(np.int64))
+    t_true = time.perf_counter() - t0
+    # ---- training: null draws of the training genes, GEV fits, network ----
+    t0 = time.perf_counter()
+    tr = np.arange(n_train)
+    perm(tr, nfull)
+    null = stats_rows(tr, nfull)
+    gev = eng.gev_fit(null)
+    eig_tr = eng.ld_eigenvalues_device(wl.chunk(_abi, 0, n_train), n_train, allocator.N_EIG)
+    x_tr = allocator.features(wl.P[:n_train], N, S, eig_tr, true[:n_train])
+    al = allocator.Allocator(eng, seed=3)
+    curve = al.fit(x_tr, gev, epochs=args.alloc_epochs, batch=256, lr=1e-3)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    # ---- inference on the other genes (timed end to end) ----
+    inf = np.arange(n_train, G)
+    eng.profile(True)
+    before = eng.profile_read()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    phase = {}
+    eng.assoc_uvlr_device(wl.chunk(_abi, n_train, G), aopts, o, sptr)
+    npc = int(wl.offs[G] - wl.offs[n_train])  # the outputs' leading dimension is this chunk's pair count
+    w_inf = out_w.view(-1)[(3 + n_cfg + 1) * npc:(3 + n_cfg + 2) * npc].cpu().numpy()
+    base = int(wl.offs[n_train])
+    true_inf = np.maximum.reduceat(w_inf, (wl.offs[n_train:G] - base).astype(np.int64))
+    phase["true_statistics"] = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    eig = eng.ld_eigenvalues_device(wl.chunk(_abi, n_train, G), n_inf, allocator.N_EIG)
+    x = allocator.features(wl.P[n_train:], N, S, eig, true_inf)
+    p_hat = al.p_hat(x, true_inf)
+    k_gene = allocator.Allocator.allocate(p_hat, 0.2, nlow, nfull)
+    phase["features_and_network"] = time.perf_counter() - t1
+    t1 = time.perf_counter()
+
+    def run_perm(k, local):
+        perm(inf[local], k)
+        return stats_rows(inf[local], k)
+
+    pv, used = allocator.allocated_pvalues(eng, run_perm, true_inf, k_gene)
+    torch.cuda.synchronize()
+    phase["permutations_and_pvalues"] = time.perf_counter() - t1
+    dt = time.perf_counter() - t0
+    after = eng.profile_read()
+    eng.profile(False)
+    kern = {k: round(v[0] - before.get(k, (0.0, 0))[0], 2) for k, v in after.items() if v[1] > before.get(k, (0, 0))[1]}
+    value = n_inf / dt
+    # ---- held-out genes: full permutation of the first n_hold inference genes ----
+    hold = np.arange(n_hold)
+    perm(inf[hold], nfull)
+    full_rows = stats_rows(inf[hold], nfull)
+    pv_full = np.array([perm_pvalue(full_rows[r], true_inf[h], nfull)[0] for r, h in enumerate(hold)])
+    calls_a = allocator.bh_reject(pv[hold], 0.05)
+    calls_f = allocator.bh_reject(pv_full, 0.05)
+    lowk = k_gene[hold] == nlow
+    res = {"metric": "gene permutation p-values/s, learned allocation", "value": value, "unit": "genes/s",
+           "seconds": dt, "phase_seconds": phase, "genes": int(n_inf), "train_genes": int(n_train),
+           "train_seconds": t_train,
+           "true_stat_seconds_train_set": t_true,
+           "workload": "as the permutation workload (5 subgroups sharing N=300, hard calls, cis SNPs~Pois(1000), "
+                       "--pbf all --maxbf), %d genes; network 24 -> 256x3 GELU -> 3 (BF16 weights) trained on "
+                       "%d genes x %d permutations; predicted p > 0.2: %d permutations, else %d" %
+                       (n_inf, n_train, nfull, nlow, nfull),
+           "epochs": args.alloc_epochs, "final_train_loss": float(curve[-1]) if len(curve) else None,
+           "permutations_used_fraction_of_full": float(used.sum() / (nfull * n_inf)),
+           "genes_given_short_run": int(np.sum(k_gene == nlow)),
+           "speedup_vs_full_permutation_same_engine": None,
+           "heldout": {"genes": int(n_hold), "egenes_bh5_full": int(calls_f.sum()),
+                       "egenes_bh5_allocated": int(calls_a.sum()),
+                       "call_agreement": float(np.mean(calls_a == calls_f)),
+                       "max_abs_p_error_short_runs": float(np.max(np.abs(pv[hold] - pv_full)[lowk])) if lowk.any()
+                       else 0.0,
+                       "max_abs_p_error_full_runs": float(np.max(np.abs(pv[hold] - pv_full)[~lowk])) if (~lowk).any()
+                       else 0.0},
+           "kernels_ms_total": kern}
+    if "perm" in result and result["perm"].get("value"):
+        res["speedup_vs_full_permutation_same_engine"] = value / result["perm"]["value"]
+        if "plain_gpu" in result["perm"]:
+            res["speedup_vs_plain_gpu"] = value / result["perm"]["plain_gpu"]["value"]
+    log("allocated: %d genes in %.2f s = %.1f genes/s; %.2f of full permutations; held-out calls %d vs %d" %
+        (n_inf, dt, value, res["permutations_used_fraction_of_full"], calls_a.sum(), calls_f.sum()))
+    al.close()
+    eng.close()
+    del wl, stats, tab_full, tab_low, out_n, out_w
+    torch.cuda.empty_cache()
+    result["perm_allocated"] = res
+
+
 def alg_bytes(P_g, S, N, GL, GS):
     """Algorithmic HBM bytes of the association pass for one gene: per pair
     the genotype row (4N), the raw Bayes factors in FP32 (3 GL + S GS), the
@@ -440,6 +582,8 @@ def main():
     if args.only_perm:
         result = {"metric": METRIC, "n_gpus": 1, "data": "synthetic"}
         perm_leg(args, torch, result)
+        if not args.no_alloc:
+            perm_alloc_leg(args, torch, result)
         result.update({"value": result["perm"]["value"], "unit": "genes/s"})
         print(json.dumps(result), flush=True)
         return
@@ -643,6 +787,8 @@ def main():
     torch.cuda.empty_cache()
     if not args.no_perm:
         perm_leg(args, torch, result)
+        if not args.no_alloc:
+            perm_alloc_leg(args, torch, result)
     print(json.dumps(result), flush=True)
 
 

@@ -24,7 +24,7 @@ namespace {
 
 constexpr int EIG_NMAX = 512;  /* samples for the eigenvalue kernel (LDS) */
 constexpr int EIG_P = 32;      /* subspace vectors                        */
-constexpr int EIG_ITERS = 30;
+constexpr int EIG_ITERS = 20;
 constexpr int GEV_NMAX = 16384; /* permutations per gene for the GEV fit    */
 
 __device__ inline double al_geno(const eqbf_chunk& c, int64_t i, int64_t j) {
@@ -159,10 +159,24 @@ __global__ void __launch_bounds__(256) al_eig_kernel(const float* __restrict__ K
       float acc[p];
 #pragma unroll
       for (int c2 = 0; c2 < p; ++c2) acc[c2] = 0.f;
-      for (int j = 0; j < N; ++j) {
-        const float kij = K[(int64_t)j * N + i];
+      for (int j0 = 0; j0 < N; j0 += 16) {
+        float kv[16]; /* sixteen loads in flight before the products */
 #pragma unroll
-        for (int c2 = 0; c2 < p; ++c2) acc[c2] = fmaf(kij, V[j * p + c2], acc[c2]);
+        for (int u = 0; u < 16; ++u) kv[u] = j0 + u < N ? K[(int64_t)(j0 + u) * N + i] : 0.f;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+          if (j0 + u < N) {
+            const float4* v4 = (const float4*)(V + (j0 + u) * p);
+#pragma unroll
+            for (int c4 = 0; c4 < p / 4; ++c4) {
+              const float4 v = v4[c4];
+              acc[4 * c4] = fmaf(kv[u], v.x, acc[4 * c4]);
+              acc[4 * c4 + 1] = fmaf(kv[u], v.y, acc[4 * c4 + 1]);
+              acc[4 * c4 + 2] = fmaf(kv[u], v.z, acc[4 * c4 + 2]);
+              acc[4 * c4 + 3] = fmaf(kv[u], v.w, acc[4 * c4 + 3]);
+            }
+          }
+        }
       }
 #pragma unroll
       for (int c2 = 0; c2 < p; ++c2) W[i * p + c2] = acc[c2];
@@ -337,7 +351,7 @@ static int ld_eig_run(eqbf_ctx* ctx, const eqbf_chunk& d, const std::vector<int6
   const int gb = std::max(1, (int)std::min<int64_t>(G, ((int64_t)1 << 30) / ((int64_t)N * N * 4)));
   float* dK = (float*)ctx_buf(ctx, "al_gram", (size_t)gb * N * N * sizeof(float));
   float* deig = (float*)ctx_buf(ctx, "al_eig", (size_t)gb * n_eig * sizeof(float));
-  if (!dK || !deig) return ctx_fail(ctx, EQBF_ERR_HIP, "allocating the Gram matrices failed");
+  if (!dK || !deig) return EQBF_ERR_HIP; /* ctx_buf has set the message */
   std::vector<int64_t> src, boff;
   for (int g0 = 0; g0 < G; g0 += gb) {
     const int g1 = std::min(G, g0 + gb), nb = g1 - g0;
@@ -351,7 +365,7 @@ static int ld_eig_run(eqbf_ctx* ctx, const eqbf_chunk& d, const std::vector<int6
     float2* dms = (float2*)ctx_buf(ctx, "al_ms", nbp * sizeof(float2));
     int64_t* dsrc = (int64_t*)ctx_buf(ctx, "al_src", nbp * sizeof(int64_t));
     int64_t* doff = (int64_t*)ctx_buf(ctx, "al_off", (nb + 1) * sizeof(int64_t));
-    if (!dms || !dsrc || !doff) return ctx_fail(ctx, EQBF_ERR_HIP, "allocating the SNP statistics failed");
+    if (!dms || !dsrc || !doff) return EQBF_ERR_HIP;
     if (!src.empty())
       EQBF_HIP(ctx, hipMemcpyAsync(dsrc, src.data(), src.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
     EQBF_HIP(ctx, hipMemcpyAsync(doff, boff.data(), boff.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
@@ -403,7 +417,7 @@ extern "C" int eqbf_ld_eigenvalues(eqbf_ctx* ctx, const eqbf_chunk* chunk, int32
   const size_t gbytes = (size_t)c.n_samples * c.geno_ld * (c.geno_dtype == EQBF_F32 ? 4 : 8);
   void* g = ctx_buf(ctx, "al_geno", gbytes);
   void* ps = ctx_buf(ctx, "al_psnp", (size_t)std::max<int64_t>(c.n_pairs, 1) * sizeof(int32_t));
-  if (!g || !ps) return ctx_fail(ctx, EQBF_ERR_HIP, "staging the genotypes failed");
+  if (!g || !ps) return EQBF_ERR_HIP;
   EQBF_HIP(ctx, hipMemcpyAsync(g, c.genotypes, gbytes, hipMemcpyHostToDevice, st));
   EQBF_HIP(ctx, hipMemcpyAsync(ps, c.pair_snp, (size_t)c.n_pairs * sizeof(int32_t), hipMemcpyHostToDevice, st));
   d.genotypes = g;
@@ -422,7 +436,7 @@ extern "C" int eqbf_gev_fit(eqbf_ctx* ctx, const double* stats, int64_t n_genes,
   const int64_t gb = std::min<int64_t>(n_genes, 65536);
   double* ds = (double*)ctx_buf(ctx, "gev_in", (size_t)gb * nperm * sizeof(double));
   double* dp = (double*)ctx_buf(ctx, "gev_out", (size_t)gb * 3 * sizeof(double));
-  if (!ds || !dp) return ctx_fail(ctx, EQBF_ERR_HIP, "allocating the GEV buffers failed");
+  if (!ds || !dp) return EQBF_ERR_HIP;
   for (int64_t g0 = 0; g0 < n_genes; g0 += gb) {
     const int64_t nb = std::min(gb, n_genes - g0);
     EQBF_HIP(ctx, hipMemcpyAsync(ds, stats + g0 * nperm, (size_t)nb * nperm * sizeof(double), hipMemcpyHostToDevice,

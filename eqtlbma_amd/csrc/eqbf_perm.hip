@@ -53,6 +53,11 @@ __global__ void perm_pairs_kernel(const int64_t* __restrict__ pair_offset, const
   for (int64_t k = threadIdx.x; k < n; k += blockDim.x) pair_snp_v[d + k] = pair_snp[a + k];
 }
 
+__global__ void fill_nan_kernel(double* __restrict__ p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = NAN;
+}
+
 __device__ inline double wmax(double v) {
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
   return v;
@@ -148,9 +153,10 @@ int perm_stats_run(eqbf_ctx* ctx, const eqbf_chunk* cp, const eqbf_perm_opts* po
   EQBF_HIP(ctx, hipStreamSynchronize(st));
   /* every statistic starts as NaN (skipped genes keep it) */
   {
-    std::vector<double> nanrow((size_t)G * nperm, NAN);
-    EQBF_HIP(ctx, hipMemcpyAsync(stats_dev, nanrow.data(), nanrow.size() * sizeof(double), hipMemcpyHostToDevice, st));
-    EQBF_HIP(ctx, hipStreamSynchronize(st));
+    const int64_t n = (int64_t)G * nperm;
+    hipLaunchKernelGGL(fill_nan_kernel, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 65536)), dim3(256), 0, st,
+                       stats_dev, n);
+    EQBF_HIP(ctx, hipGetLastError());
   }
   /* genes that qualify go through the contraction path first */
   std::vector<uint8_t> done;

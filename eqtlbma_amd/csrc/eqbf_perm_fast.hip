@@ -780,7 +780,7 @@ int perm_fast_run(eqbf_ctx* ctx, const eqbf_chunk* cp, const eqbf_perm_opts* po,
   }
   /* eligibility per gene */
   uint8_t* d_ok = (uint8_t*)ctx_buf(ctx, "pf_ok", (size_t)G + 64);
-  if (!d_ok) return ctx_fail(ctx, EQBF_ERR_HIP, "allocating the permutation buffers failed");
+  if (!d_ok) return EQBF_ERR_HIP; /* ctx_buf has set the message */
   hipLaunchKernelGGL(pf_check_kernel, dim3(G), dim3(256), 0, st, c, d_ok);
   EQBF_HIP(ctx, hipGetLastError());
   std::vector<uint8_t> ok(G);
@@ -799,7 +799,7 @@ int perm_fast_run(eqbf_ctx* ctx, const eqbf_chunk* cp, const eqbf_perm_opts* po,
   float* d_zt = nullptr;
   if (join) {
     d_zt = (float*)ctx_buf(ctx, "pf_zt", (size_t)ZT_ROW * sizeof(float));
-    if (!d_zt) return ctx_fail(ctx, EQBF_ERR_HIP, "allocating the permutation buffers failed");
+    if (!d_zt) return EQBF_ERR_HIP; /* ctx_buf has set the message */
     hipLaunchKernelGGL(pf_zeta_table_kernel, dim3((ZT_NODES + 255) / 256), dim3(256), 0, st, ctx_tz_table(ctx), nu,
                        d_zt);
     EQBF_HIP(ctx, hipGetLastError());
@@ -810,7 +810,12 @@ int perm_fast_run(eqbf_ctx* ctx, const eqbf_chunk* cp, const eqbf_perm_opts* po,
   const int ngrid = mode == PF_JOIN_ALL ? GS : (mode == PF_JOIN_GEN ? GL : GL + GS);
   const int NCF = 1 + 2 * S;
   /* batch: z values of at most 16 GB */
-  const size_t zeta_budget = (size_t)16 << 30;
+  size_t zeta_budget = (size_t)16 << 30;
+  {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) == hipSuccess) zeta_budget = std::min(zeta_budget, fr / 4);
+    zeta_budget = std::max(zeta_budget, (size_t)256 << 20);
+  }
   const int s_lo = (!join && po->sep_sbgrp >= 0) ? po->sep_sbgrp : 0;
   const int s_hi = (!join && po->sep_sbgrp >= 0) ? po->sep_sbgrp + 1 : S;
 
@@ -877,7 +882,7 @@ int perm_fast_run(eqbf_ctx* ctx, const eqbf_chunk* cp, const eqbf_perm_opts* po,
     uint8_t* d_pf = (uint8_t*)ctx_buf(ctx, "pf_pflag", (size_t)nbp + 64);
     int* d_fb = (int*)ctx_buf(ctx, "pf_fb", (size_t)nb * sizeof(int) + 64);
     if (!meta || !d_G8 || !d_W || !d_ys || !d_ps || !d_pf || !d_fb)
-      return ctx_fail(ctx, EQBF_ERR_HIP, "allocating the permutation buffers failed");
+      return EQBF_ERR_HIP; /* ctx_buf has set the message */
     float* d_zeta = nullptr;
     float* d_coef = nullptr;
     float* d_pmax = nullptr;
@@ -889,10 +894,10 @@ int perm_fast_run(eqbf_ctx* ctx, const eqbf_chunk* cp, const eqbf_perm_opts* po,
       d_pmax = (float*)ctx_buf(ctx, "pf_pmax", (size_t)nslots * NPP * sizeof(float) + 64);
       d_psum = (double*)ctx_buf(ctx, "pf_psum", (size_t)nslots * NPP * sizeof(double) + 64);
       if (!d_zeta || !d_coef || !d_pmax || !d_psum)
-        return ctx_fail(ctx, EQBF_ERR_HIP, "allocating the permutation buffers failed");
+        return EQBF_ERR_HIP; /* ctx_buf has set the message */
     } else {
       d_tmax = (double*)ctx_buf(ctx, "pf_tmax", (size_t)nb * S * NPP * sizeof(double) + 64);
-      if (!d_tmax) return ctx_fail(ctx, EQBF_ERR_HIP, "allocating the permutation buffers failed");
+      if (!d_tmax) return EQBF_ERR_HIP; /* ctx_buf has set the message */
       /* subgroups that are not computed (--permsep 2) are never read */
     }
     int32_t* d_bgene = (int32_t*)meta;
@@ -930,7 +935,7 @@ int perm_fast_run(eqbf_ctx* ctx, const eqbf_chunk* cp, const eqbf_perm_opts* po,
         goma.insert(goma.end(), ctx->omaS.begin(), ctx->omaS.end());
       }
       double* d_g = (double*)ctx_buf(ctx, "pf_grid", 2 * gphi.size() * sizeof(double) + 64);
-      if (!d_g) return ctx_fail(ctx, EQBF_ERR_HIP, "allocating the permutation buffers failed");
+      if (!d_g) return EQBF_ERR_HIP; /* ctx_buf has set the message */
       EQBF_HIP(ctx, hipMemcpyAsync(d_g, gphi.data(), gphi.size() * sizeof(double), hipMemcpyHostToDevice, st));
       EQBF_HIP(ctx, hipMemcpyAsync(d_g + gphi.size(), goma.data(), goma.size() * sizeof(double), hipMemcpyHostToDevice,
                                    st));
