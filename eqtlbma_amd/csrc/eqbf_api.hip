@@ -4,6 +4,7 @@
  * the chunk into context-owned device buffers, runs, and copies back.
  */
 #include <cstring>
+#include <string>
 
 #include "eqbf_internal.h"
 
@@ -21,12 +22,114 @@ static int check_flag(eqbf_ctx* ctx, hipStream_t st) {
   return EQBF_OK;
 }
 
+/* hipGraph replay of the batched association.  A call is identified by
+ * its argument bytes (chunk, options, output pointers, stream).  The first
+ * time it runs eagerly (sizing every buffer and uploading the per-grid
+ * tables); the second time, if nothing was reallocated in between, the
+ * enqueued kernels are captured into a graph and launched; later calls
+ * launch the graph.  A reallocation or new grids (buf_epoch) retire it. */
+static int assoc_graph_run(eqbf_ctx* ctx, const eqbf_chunk* c, const eqbf_assoc_opts* o, eqbf_assoc_out* out,
+                           hipStream_t st) {
+  std::string key((const char*)c, sizeof *c);
+  key.append((const char*)o, sizeof *o);
+  key.append((const char*)out, sizeof *out);
+  key.append((const char*)&st, sizeof st);
+  eqbf_ctx::GraphEntry& e = ctx->graphs[key];
+  if (e.exec && e.epoch == ctx->buf_epoch) {
+    EQBF_HIP(ctx, hipGraphLaunch(e.exec, st));
+    ++ctx->graph_launches;
+    return EQBF_OK;
+  }
+  if (e.exec) {
+    (void)hipGraphExecDestroy(e.exec);
+    e.exec = nullptr;
+    e.seen = 0;
+  }
+  if (!e.seen || e.epoch != ctx->buf_epoch) {
+    const int rc = assoc_uvlr_run(ctx, c, o, out, st);
+    e.seen = 1;
+    e.epoch = ctx->buf_epoch;
+    return rc;
+  }
+  hipGraph_t g = nullptr;
+  EQBF_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  const uint64_t ep = ctx->buf_epoch;
+  ctx->capturing = true;
+  const int rc = assoc_uvlr_run(ctx, c, o, out, st);
+  ctx->capturing = false;
+  const hipError_t ec = hipStreamEndCapture(st, &g);
+  if (rc == EQBF_OK && ec == hipSuccess && g && ctx->buf_epoch == ep) {
+    hipGraphExec_t ex = nullptr;
+    if (hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) == hipSuccess) {
+      (void)hipGraphDestroy(g);
+      e.exec = ex;
+      e.epoch = ep;
+      EQBF_HIP(ctx, hipGraphLaunch(ex, st));
+      ++ctx->graph_launches;
+      return EQBF_OK;
+    }
+  }
+  if (g) (void)hipGraphDestroy(g);
+  (void)hipGetLastError();
+  /* nothing ran during the capture: run the call eagerly */
+  e.seen = 0;
+  return assoc_uvlr_run(ctx, c, o, out, st);
+}
+
 extern "C" int eqbf_assoc_uvlr_device(eqbf_ctx* ctx, const eqbf_chunk* c, const eqbf_assoc_opts* o,
                                       eqbf_assoc_out* out, void* stream) {
   if (!ctx || !c || !o || !out) return ctx_fail(ctx, EQBF_ERR_ARG, "null argument");
   EQBF_HIP(ctx, hipSetDevice(ctx->device));
   hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  if (ctx->graph_on && !ctx->prof_on && o->precision == EQBF_PREC_MIXED) return assoc_graph_run(ctx, c, o, out, st);
   return assoc_uvlr_run(ctx, c, o, out, st);
+}
+
+extern "C" int eqbf_set_graphs(eqbf_ctx* ctx, int enable) {
+  if (!ctx) return EQBF_ERR_ARG;
+  ctx->graph_on = enable != 0;
+  if (!ctx->graph_on) {
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& kv : ctx->graphs)
+      if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+    ctx->graphs.clear();
+  }
+  return EQBF_OK;
+}
+
+extern "C" int eqbf_graph_stats(eqbf_ctx* ctx, int64_t* n_graphs, int64_t* n_launches) {
+  if (!ctx) return EQBF_ERR_ARG;
+  int64_t n = 0;
+  for (auto& kv : ctx->graphs) n += kv.second.exec != nullptr;
+  if (n_graphs) *n_graphs = n;
+  if (n_launches) *n_launches = ctx->graph_launches;
+  return EQBF_OK;
+}
+
+extern "C" int eqbf_set_autotune(eqbf_ctx* ctx, int enable) {
+  if (!ctx) return EQBF_ERR_ARG;
+  ctx->autotune_on = enable != 0;
+  return EQBF_OK;
+}
+
+extern "C" int eqbf_autotune_read(eqbf_ctx* ctx, char* keys, int32_t* kch, int32_t* spp, int32_t* grid_mult,
+                                  float* ms, int32_t n_max, int32_t* n_out) {
+  if (!ctx || !n_out) return EQBF_ERR_ARG;
+  int32_t k = 0;
+  for (auto& kv : ctx->tuned) {
+    if (k >= n_max) break;
+    if (keys) {
+      strncpy(keys + 64 * k, kv.first.c_str(), 63);
+      keys[64 * k + 63] = 0;
+    }
+    if (kch) kch[k] = kv.second.kch;
+    if (spp) spp[k] = kv.second.spp;
+    if (grid_mult) grid_mult[k] = kv.second.grid_mult;
+    if (ms) ms[k] = kv.second.ms;
+    ++k;
+  }
+  *n_out = k;
+  return EQBF_OK;
 }
 
 namespace {

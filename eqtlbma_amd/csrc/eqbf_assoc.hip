@@ -1113,7 +1113,12 @@ int assoc_uvlr_run(eqbf_ctx* ctx, const eqbf_chunk* cp, const eqbf_assoc_opts* o
   uint64_t* d_masks = (uint64_t*)ctx_buf(ctx, "masks", (size_t)std::max<int64_t>(n_cfg, 1) * sizeof(uint64_t) + 65 * sizeof(double));
   if (!rec || !B || !memb || !idx || !tiles || !n_tiles || !d_masks) return EQBF_ERR_HIP;
   double* d_inv_choose = (double*)(d_masks + std::max<int64_t>(n_cfg, 1));
-  if (n_cfg > 0) {
+  /* the masks stay on the device until (bfs, S) or the buffer changes, so
+   * that a repeated call enqueues kernels only (hipGraph capture) */
+  char mkey[96];
+  snprintf(mkey, sizeof mkey, "%d/%d/%p/%llu", (int)o->bfs, S, (void*)d_masks, (unsigned long long)ctx->buf_epoch);
+  if (n_cfg > 0 && ctx->masks_key != mkey) {
+    ctx->masks_key = mkey;
     std::vector<uint64_t> hm(n_cfg);
     eqbf_config_masks(o->bfs, S, hm.data());
     std::vector<double> ic(65, 0.0);

@@ -27,6 +27,7 @@ void* ctx_buf(eqbf_ctx* ctx, const std::string& name, size_t bytes) {
     b.ptr = nullptr;
     b.bytes = 0;
     size_t want = bytes + bytes / 8;
+    ++ctx->buf_epoch;
     if (hipMalloc(&b.ptr, want) != hipSuccess) {
       b.ptr = nullptr;
       ctx_fail(ctx, EQBF_ERR_HIP, "hipMalloc of %zu bytes for %s failed", want, name.c_str());
@@ -71,6 +72,8 @@ extern "C" void eqbf_destroy(eqbf_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (auto& kv : ctx->graphs)
+    if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
   for (auto& kv : ctx->bufs)
     if (kv.second.ptr) (void)hipFree(kv.second.ptr);
   if (ctx->d_grid.ptr) (void)hipFree(ctx->d_grid.ptr);
@@ -107,6 +110,7 @@ extern "C" int eqbf_set_grids(eqbf_ctx* ctx, const double* phiL, const double* o
     ctx->d_grid.ptr = nullptr;
   }
   size_t bytes = std::max<size_t>(packed.size(), 1) * sizeof(double);
+  ++ctx->buf_epoch; /* graphs and cached Bayes-factor jobs refer to the old grids */
   EQBF_HIP(ctx, hipMalloc(&ctx->d_grid.ptr, bytes));
   ctx->d_grid.bytes = bytes;
   if (!packed.empty())

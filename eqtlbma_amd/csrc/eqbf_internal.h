@@ -35,6 +35,31 @@ struct eqbf_ctx {
   int tz_nu_max = 0;
   /* device error word written by kernels (EQBF_ERR_DATA conditions) */
   DevBuf d_flag;
+  /* bumped whenever a buffer that a kernel argument points to is replaced
+   * (ctx_buf growth, new grids): captured graphs older than it are stale */
+  uint64_t buf_epoch = 1;
+  /* per-call tables kept on the device between calls */
+  std::string masks_key;   /* (bfs, S) of the configuration masks uploaded */
+  uint64_t jobs_epoch = 0; /* buf_epoch of the uploaded Bayes-factor jobs   */
+  size_t jobs_bytes = 0;   /* bytes of the job records before the entries  */
+  int jobs_n = 0;
+  /* hipGraph replay of the batched association (eqbf_set_graphs) */
+  bool graph_on = false;
+  bool capturing = false; /* a stream capture is open: enqueue work only */
+  struct GraphEntry {
+    uint64_t epoch = 0;
+    int seen = 0;
+    hipGraphExec_t exec = nullptr;
+  };
+  std::map<std::string, GraphEntry> graphs;
+  int64_t graph_launches = 0;
+  /* launch-shape autotuner of the batched sums kernel (eqbf_set_autotune) */
+  bool autotune_on = false;
+  struct TuneChoice {
+    int kch = 0, spp = 0, grid_mult = 0;
+    float ms = 0.f;
+  };
+  std::map<std::string, TuneChoice> tuned;
   /* live kernel timing (eqbf_profile) */
   bool prof_on = false;
   struct ProfEntry {

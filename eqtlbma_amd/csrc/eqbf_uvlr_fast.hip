@@ -532,33 +532,91 @@ int uvlr_fast_run(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_assoc_opts* o, 
   const int nxw = sums_nxw((sp.NXT + 1) / 2), nmw = ((S + 15) / 16 + 1) / 2;
   if ((sp.NXT + 1) / 2 > 18) return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED, "too many subgroup columns for the batched path");
   const int CXPP = 2 * nxw * 16;
-  /* LDS: two staging buffers of KCH samples within 80 KB */
-  int KCH = (int)((80 * 1024) / (2 * 8 * (size_t)(65 + CXPP)));
-  KCH = std::min(16, KCH & ~3);
-  if (KCH < 4) return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED, "too many subgroup columns for the batched path");
-  sp.KCH = KCH;
-  /* the epilogue takes every subgroup at once when 160 KB allow it (the
-   * accumulators are then dead while the statistics are computed) */
-  const size_t lds_stage = (size_t)2 * 8 * KCH * (65 + CXPP);
-  const size_t lds_epi = (size_t)64 * 8 * S * (sp.KC + 1);
-  const size_t lds = std::max(lds_stage, std::min<size_t>(lds_epi, 160 * 1024));
-  sp.SPP = (int)std::min<size_t>(S, lds / (64 * 8 * (size_t)(sp.KC + 1)));
-  int grid = ctx->n_cu * 2;
-  grid = (grid + 7) / 8 * 8;
+  /* Launch shape: samples per LDS stage (KCH, two stages within 80 KB),
+   * subgroups per epilogue pass (SPP; all at once needs 512 (KC + 1) bytes
+   * per subgroup and may leave room for one workgroup per CU), workgroups per
+   * CU.  With the autotuner on, the first call of each shape times the
+   * candidates on the chunk itself (every candidate makes the same sums in
+   * the same order) and keeps the fastest for the context's lifetime. */
+  int kch_max = (int)((80 * 1024) / (2 * 8 * (size_t)(65 + CXPP)));
+  kch_max = std::min(16, kch_max & ~3);
+  if (kch_max < 4) return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED, "too many subgroup columns for the batched path");
+  const size_t epi_unit = (size_t)64 * 8 * (sp.KC + 1);
+  auto shape_lds = [&](int kch, int spp) {
+    return std::max((size_t)2 * 8 * kch * (65 + CXPP), epi_unit * spp);
+  };
+  eqbf_ctx::TuneChoice choice;
+  choice.kch = kch_max;
+  choice.spp = (int)std::min<size_t>(S, std::max(shape_lds(kch_max, 0), std::min<size_t>(epi_unit * S, 160 * 1024)) /
+                                            epi_unit);
+  choice.grid_mult = 2;
+  auto launch = [&](const eqbf_ctx::TuneChoice& ch) -> int {
+    SumsParams q = sp;
+    q.KCH = ch.kch;
+    q.SPP = ch.spp;
+    const size_t lds = shape_lds(ch.kch, ch.spp);
+    const int grid = (ctx->n_cu * ch.grid_mult + 7) / 8 * 8;
+    return nmw <= 1 ? dispatch_sums<1>(nxw, c, q, grid, lds, st) : dispatch_sums<2>(nxw, c, q, grid, lds, st);
+  };
+  char key[96];
+  snprintf(key, sizeof key, "sums/%d/%d/%d/%d/%d", S, sp.KC, nxw, nmw, c.n_samples);
+  auto tk = ctx->tuned.find(key);
+  if (tk != ctx->tuned.end()) {
+    choice = tk->second;
+  } else if (ctx->autotune_on && !ctx->capturing) {
+    std::vector<eqbf_ctx::TuneChoice> cands;
+    for (int kch : {kch_max, std::max(4, (kch_max / 2) & ~3)})
+      for (int spp : {choice.spp, (S + 1) / 2, (S + 3) / 4})
+        for (int gm : {1, 2}) {
+          eqbf_ctx::TuneChoice ch;
+          ch.kch = kch;
+          ch.spp = std::max(1, std::min(S, spp));
+          ch.grid_mult = gm;
+          if (shape_lds(ch.kch, ch.spp) > 160 * 1024) continue;
+          bool dup = false;
+          for (auto& d : cands) dup |= d.kch == ch.kch && d.spp == ch.spp && d.grid_mult == ch.grid_mult;
+          if (!dup) cands.push_back(ch);
+        }
+    hipEvent_t e0, e1;
+    EQBF_HIP(ctx, hipEventCreate(&e0));
+    EQBF_HIP(ctx, hipEventCreate(&e1));
+    float best = 1e30f;
+    for (auto& ch : cands) {
+      if (launch(ch)) continue; /* warm-up */
+      EQBF_HIP(ctx, hipEventRecord(e0, st));
+      launch(ch);
+      EQBF_HIP(ctx, hipEventRecord(e1, st));
+      EQBF_HIP(ctx, hipEventSynchronize(e1));
+      float ms = 0.f;
+      EQBF_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) {
+        best = ms;
+        choice = ch;
+        choice.ms = ms;
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ctx->tuned[key] = choice;
+  }
   {
     ProfScope ps(ctx, "uvlr_sums_kernel", st);
-    const int bad = nmw <= 1 ? dispatch_sums<1>(nxw, c, sp, grid, lds, st) : dispatch_sums<2>(nxw, c, sp, grid, lds, st);
-    if (bad) return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED, "too many subgroup columns for the batched path");
+    if (launch(choice)) return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED, "too many subgroup columns for the batched path");
   }
   EQBF_HIP(ctx, hipGetLastError());
   if (!join) return EQBF_OK;
 
   /* jobs of the consistent configuration: distinct phi2 of gen, gen-fix,
-   * gen-maxh with the grid points each one finishes */
+   * gen-maxh with the grid points each one finishes.  They depend on the
+   * large grid only and stay on the device until a buffer or the grids
+   * change, so that a repeated call enqueues kernels only. */
   const int GL = (int)ctx->phiL.size(), GS = (int)ctx->phiS.size();
-  std::vector<PhiJob> jobs;
-  std::vector<PhiEntry> ents;
-  {
+  const size_t jobs_cap = (size_t)3 * GL * (sizeof(PhiJob) + sizeof(PhiEntry)) + 64;
+  uint8_t* jt = (uint8_t*)ctx_buf(ctx, "abf_jobs", jobs_cap);
+  if (!jt) return EQBF_ERR_HIP;
+  if (ctx->jobs_epoch != ctx->buf_epoch) {
+    std::vector<PhiJob> jobs;
+    std::vector<PhiEntry> ents;
     std::vector<std::pair<float, PhiEntry>> all;
     for (int gi = 0; gi < GL; ++gi) {
       const double p = ctx->phiL[gi], om = ctx->omaL[gi];
@@ -573,13 +631,14 @@ int uvlr_fast_run(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_assoc_opts* o, 
       ents.push_back(all[k].second);
       jobs.back().e1 = (int)k + 1;
     }
+    const size_t jb_bytes = jobs.size() * sizeof(PhiJob), en_bytes = ents.size() * sizeof(PhiEntry);
+    EQBF_HIP(ctx, hipMemcpyAsync(jt, jobs.data(), jb_bytes, hipMemcpyHostToDevice, st));
+    EQBF_HIP(ctx, hipMemcpyAsync(jt + jb_bytes, ents.data(), en_bytes, hipMemcpyHostToDevice, st));
+    EQBF_HIP(ctx, hipStreamSynchronize(st)); /* host vectors go out of scope */
+    ctx->jobs_bytes = jb_bytes;
+    ctx->jobs_n = (int)jobs.size();
+    ctx->jobs_epoch = ctx->buf_epoch;
   }
-  const size_t jb_bytes = jobs.size() * sizeof(PhiJob), en_bytes = ents.size() * sizeof(PhiEntry);
-  uint8_t* jt = (uint8_t*)ctx_buf(ctx, "abf_jobs", jb_bytes + en_bytes + 64);
-  if (!jt) return ctx_fail(ctx, EQBF_ERR_HIP, "allocating the Bayes-factor jobs failed");
-  EQBF_HIP(ctx, hipMemcpyAsync(jt, jobs.data(), jb_bytes, hipMemcpyHostToDevice, st));
-  EQBF_HIP(ctx, hipMemcpyAsync(jt + jb_bytes, ents.data(), en_bytes, hipMemcpyHostToDevice, st));
-  EQBF_HIP(ctx, hipStreamSynchronize(st)); /* host vectors go out of scope */
   AbfParams ap;
   memset(&ap, 0, sizeof ap);
   ap.st_v = sp.st_v;
@@ -587,12 +646,12 @@ int uvlr_fast_run(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_assoc_opts* o, 
   ap.st_z2 = sp.st_z2;
   ap.st_f = sp.st_f;
   ap.jobs = (const PhiJob*)jt;
-  ap.ents = (const PhiEntry*)(jt + jb_bytes);
+  ap.ents = (const PhiEntry*)(jt + ctx->jobs_bytes);
   ap.masks = d_masks;
   ap.grid = (const double*)ctx->d_grid.ptr;
   ap.raw = out->raw_abf;
   ap.w = out->w_abf;
-  ap.n_jobs = (int)jobs.size();
+  ap.n_jobs = ctx->jobs_n;
   ap.n_cfg = n_cfg;
   ap.GL = GL;
   ap.GS = GS;

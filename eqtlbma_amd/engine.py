@@ -67,6 +67,10 @@ def lib():
         L.eqbf_mlp_forward.argtypes = [vp, vp, vp, i64, vp]
         L.eqbf_mlp_loss_grad.argtypes = [vp, vp, vp, vp, vp, i64, vp, vp]
         L.eqbf_mlp_train.argtypes = [vp, vp, vp, vp, vp, i64, i32, i32, C.c_float, C.c_uint64, vp]
+        L.eqbf_set_graphs.argtypes = [vp, C.c_int]
+        L.eqbf_set_autotune.argtypes = [vp, C.c_int]
+        L.eqbf_graph_stats.argtypes = [vp, vp, vp]
+        L.eqbf_autotune_read.argtypes = [vp, vp, vp, vp, vp, vp, i32, vp]
         L.eqbf_ld_eigenvalues.argtypes = [vp, vp, i32, vp]
         L.eqbf_ld_eigenvalues_device.argtypes = [vp, vp, i32, vp]
         L.eqbf_gev_fit.argtypes = [vp, vp, i64, i64, vp]
@@ -239,6 +243,31 @@ class Engine:
 
     def sync(self):
         self._check(lib().eqbf_sync(self._ctx))
+
+    def set_graphs(self, enable=True):
+        """hipGraph replay of repeated batched association calls."""
+        self._check(lib().eqbf_set_graphs(self._ctx, int(enable)))
+
+    def graph_stats(self):
+        """(graphs held, graph launches so far)."""
+        a, b = C.c_int64(0), C.c_int64(0)
+        self._check(lib().eqbf_graph_stats(self._ctx, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def set_autotune(self, enable=True):
+        """Per-shape timing of the sums kernel's launch shapes."""
+        self._check(lib().eqbf_set_autotune(self._ctx, int(enable)))
+
+    def autotune_read(self, n_max=32):
+        keys = C.create_string_buffer(64 * n_max)
+        kch, spp, gm = (np.zeros(n_max, np.int32) for _ in range(3))
+        ms = np.zeros(n_max, np.float32)
+        n = C.c_int32(0)
+        self._check(lib().eqbf_autotune_read(self._ctx, keys, _p(kch), _p(spp), _p(gm), _p(ms), n_max, C.byref(n)))
+        raw = keys.raw
+        return {raw[64 * k:64 * (k + 1)].split(b"\0")[0].decode(): {"kch": int(kch[k]), "spp": int(spp[k]),
+                                                                    "grid_mult": int(gm[k]), "ms": float(ms[k])}
+                for k in range(n.value)}
 
     def profile(self, enable=True):
         """Turns the live per-kernel timing of this context on or off."""

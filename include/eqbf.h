@@ -253,6 +253,26 @@ int eqbf_profile(eqbf_ctx* ctx, int enable);
 int eqbf_profile_read(eqbf_ctx* ctx, char* names, double* total_ms, int64_t* launches, int32_t n_max,
                       int32_t* n_out);
 
+/* ---- launch graphs and launch-shape tuning -------------------------------------
+ * No counterpart in the reference.
+ * eqbf_set_graphs: when enabled, eqbf_assoc_uvlr_device calls with the mixed
+ * precision are replayed from a hipGraph: the first call with a given set of
+ * arguments (chunk, options, output pointers, stream) runs normally, the
+ * second is captured and launched, later ones launch the graph.  The inputs'
+ * contents may change between calls; their addresses and sizes may not
+ * (other arguments are another graph).  Ignored while eqbf_profile is on.
+ * eqbf_set_autotune: when enabled, the first batched association of each
+ * shape (subgroups, coefficient columns, samples) times the candidate launch
+ * shapes of the sums kernel on that call's own data and keeps the fastest;
+ * results do not depend on the shape.  eqbf_autotune_read lists the choices
+ * (keys: n_max entries of 64 chars). */
+int eqbf_set_graphs(eqbf_ctx* ctx, int enable);
+/* graphs currently held and graph launches so far */
+int eqbf_graph_stats(eqbf_ctx* ctx, int64_t* n_graphs, int64_t* n_launches);
+int eqbf_set_autotune(eqbf_ctx* ctx, int enable);
+int eqbf_autotune_read(eqbf_ctx* ctx, char* keys, int32_t* kch, int32_t* spp, int32_t* grid_mult, float* ms,
+                       int32_t n_max, int32_t* n_out);
+
 /* ---- diagnostics ------------------------------------------------------------
  * No counterpart in the reference.  Evaluates, on the device, the Student-t
  * tail w = -ln P(T_nu <= -|t|) from the tabulated interpolant (w_tab) and
