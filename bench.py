@@ -601,6 +601,7 @@ def main():
     out_w = torch.empty((n_w, max_np), dtype=torch.float64, device=dev)
     eng = Engine(0)
     eng.set_grids(grid, grid)
+    eng.set_autotune(True)  # the first call of the warm-up picks the sums kernel's launch shape
     if args.calib_copy_mib > 0:
         src = torch.ones(args.calib_copy_mib << 18, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
@@ -672,11 +673,28 @@ def main():
         a_bytes = sum(abf_bytes(int(p), S, GL, GS) for p in wl.P) / launches_per_step
         others["uvlr_abf_kernel"] = {"bound": "hbm", "achieved": a_bytes / (a_ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
                                      "unit": "GB/s", "frac": a_bytes / (a_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
-                                     "avg_launch_ms": a_ms, "alg_bytes_per_launch": a_bytes}
+                                     "avg_launch_ms": a_ms, "alg_bytes_per_launch": a_bytes,
+                                     "traffic": _load_traffic("uvlr_abf_kernel", args)}
     traffic = _load_traffic(dominant, args)
     if traffic is not None:
         roof["traffic"] = traffic
     log("step %.2f ms, %.3e pairs/s; kernels: %s" % (ms_step, value, {k: (round(v[0] / max(v[1], 1), 3), v[1]) for k, v in kern.items()}))
+
+    # the same steps replayed from hipGraphs (first pass eager, second captured)
+    graph_ms = None
+    if args.steps > 0:
+        eng.set_graphs(True)
+        for _ in range(2):
+            step()
+        torch.cuda.synchronize()
+        tg = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        graph_ms = (time.perf_counter() - tg) * 1e3 / args.steps
+        n_graphs, n_launch = eng.graph_stats()
+        eng.set_graphs(False)
+        log("graph replay: %.2f ms per step (%d graphs, %d launches)" % (graph_ms, n_graphs, n_launch))
 
     result = {
         "metric": METRIC, "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
@@ -690,6 +708,8 @@ def main():
         "roofline": roof,
         "roofline_other_kernels": others,
         "kernels_ms_per_launch": {k: v[0] / max(v[1], 1) for k, v in kern.items()},
+        "autotune": eng.autotune_read(),
+        "graph_replay_ms_per_step": graph_ms,
     }
 
     # ---- per-phase split of the dominant kernel on one chunk ----
