@@ -6,8 +6,9 @@
  *    a gene's cis genotypes (P x P).  They are the k largest eigenvalues of
  *    Z Z^T / N (N x N), Z the N x P standardised genotypes, so the kernels
  *    form that Gram matrix (64 x 64 tiles, FP32) and run a blocked subspace
- *    iteration on it (k + 8 vectors, FP32 products, FP64 Cholesky-QR, then
- *    the Rayleigh-Ritz matrix diagonalised by cyclic Jacobi in FP64).
+ *    iteration on it (32 vectors, FP32 products, FP64 Cholesky-QR, then the
+ *    Rayleigh-Ritz matrix diagonalised by Jacobi rotations in FP64, p/2
+ *    disjoint pairs per round).
  *  - GEV fit: location, scale and shape of each gene's permutation
  *    statistics by probability-weighted moments (Hosking, Wallis & Wood
  *    1985, Technometrics 27:251-261): the draws are sorted in LDS (bitonic,
@@ -112,27 +113,6 @@ __global__ void __launch_bounds__(256) al_gram_kernel(const eqbf_chunk c, const 
     }
 }
 
-/* Cholesky of the p x p matrix G (FP64, in place, lower), one thread.  A
- * pivot below 1e-14 of the first (a rank-deficient block: fewer SNPs or
- * samples than vectors) is raised to that floor. */
-__device__ inline bool chol(double* G, int p) {
-  const double floor_ = 1e-14 * G[0];
-  if (!(G[0] > 0)) return false;
-  for (int j = 0; j < p; ++j) {
-    double d = G[j * p + j];
-    for (int k = 0; k < j; ++k) d -= G[j * p + k] * G[j * p + k];
-    if (!(d > floor_)) d = floor_;
-    d = sqrt(d);
-    G[j * p + j] = d;
-    for (int i = j + 1; i < p; ++i) {
-      double s = G[i * p + j];
-      for (int k = 0; k < j; ++k) s -= G[i * p + k] * G[j * p + k];
-      G[i * p + j] = s / d;
-    }
-  }
-  return true;
-}
-
 /* k largest eigenvalues of one gene's K (one workgroup per gene). */
 __global__ void __launch_bounds__(256) al_eig_kernel(const float* __restrict__ Kall, int N, int k,
                                                      float* __restrict__ eig) {
@@ -193,13 +173,26 @@ __global__ void __launch_bounds__(256) al_eig_kernel(const float* __restrict__ K
     }
     __syncthreads();
     if (it == EIG_ITERS) break;
-    if (tid == 0) {
-      for (int a = 0; a < p; ++a)
-        for (int b = a + 1; b < p; ++b) G[a * p + b] = G[b * p + a];
-      ok = chol(G, p);
-    }
+    /* Cholesky G = L L^T, right-looking, the trailing updates in parallel;
+     * a pivot below 1e-14 of the first (rank-deficient block) is raised */
+    if (tid == 0) ok = G[0] > 0;
     __syncthreads();
     if (!ok) break;
+    const double floor_ = 1e-14 * G[0];
+    for (int j = 0; j < p; ++j) {
+      if (tid == 0) G[j * p + j] = sqrt(fmax(G[j * p + j], floor_));
+      __syncthreads();
+      const double d = G[j * p + j];
+      for (int i = j + 1 + tid; i < p; i += 256) G[i * p + j] /= d;
+      __syncthreads();
+      /* G[i][c] -= L[i][j] L[c][j] for j < c <= i */
+      const int m = p - 1 - j;
+      for (int e = tid; e < m * m; e += 256) {
+        const int i = j + 1 + e / m, c2 = j + 1 + e % m;
+        if (c2 <= i) G[i * p + c2] -= G[i * p + j] * G[c2 * p + j];
+      }
+      __syncthreads();
+    }
     /* V = W L^-T (rows are independent: forward substitution per row) */
     for (int i = tid; i < N; i += 256) {
       float* w = W + i * p;
@@ -212,37 +205,66 @@ __global__ void __launch_bounds__(256) al_eig_kernel(const float* __restrict__ K
     }
     __syncthreads();
   }
-  if (tid == 0) {
-    /* cyclic Jacobi on the symmetrised p x p matrix */
-    for (int a = 0; a < p; ++a)
-      for (int b = a + 1; b < p; ++b) {
-        const double m = 0.5 * (G[a * p + b] + G[b * p + a]);
-        G[a * p + b] = G[b * p + a] = m;
-      }
-    for (int sweep = 0; sweep < 30; ++sweep) {
-      double off = 0;
-      for (int a = 0; a < p; ++a)
-        for (int b = a + 1; b < p; ++b) off += G[a * p + b] * G[a * p + b];
-      if (off < 1e-24) break;
-      for (int a = 0; a < p; ++a)
-        for (int b = a + 1; b < p; ++b) {
-          const double apq = G[a * p + b];
-          if (fabs(apq) < 1e-300) continue;
+  /* Jacobi on the symmetrised p x p Rayleigh-Ritz matrix, parallel
+   * (tournament) ordering: each round rotates p/2 disjoint pairs, so their
+   * row updates and then their column updates are independent. */
+  __shared__ double cs_[EIG_P / 2], sn_[EIG_P / 2];
+  __shared__ int pa_[EIG_P / 2], pb_[EIG_P / 2];
+  for (int e = tid; e < p * p; e += 256) {
+    const int a = e / p, b = e % p;
+    if (b > a) {
+      const double m = 0.5 * (G[a * p + b] + G[b * p + a]);
+      G[a * p + b] = G[b * p + a] = m;
+    }
+  }
+  __syncthreads();
+  for (int sweep = 0; sweep < 12; ++sweep) {
+    for (int r = 0; r < p - 1; ++r) {
+      if (tid < p / 2) {
+        int a, b;
+        if (tid == 0) {
+          a = r;
+          b = p - 1;
+        } else {
+          a = (r + tid) % (p - 1);
+          b = (r - tid + (p - 1)) % (p - 1);
+        }
+        if (a > b) {
+          const int t2 = a;
+          a = b;
+          b = t2;
+        }
+        const double apq = G[a * p + b];
+        double c2 = 1.0, s2 = 0.0;
+        if (fabs(apq) > 1e-300) {
           const double th = (G[b * p + b] - G[a * p + a]) / (2 * apq);
           const double tt = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1));
-          const double cs = 1 / sqrt(tt * tt + 1), sn = tt * cs;
-          for (int r = 0; r < p; ++r) {
-            const double gra = G[r * p + a], grb = G[r * p + b];
-            G[r * p + a] = cs * gra - sn * grb;
-            G[r * p + b] = sn * gra + cs * grb;
-          }
-          for (int r = 0; r < p; ++r) {
-            const double gar = G[a * p + r], gbr = G[b * p + r];
-            G[a * p + r] = cs * gar - sn * gbr;
-            G[b * p + r] = sn * gar + cs * gbr;
-          }
+          c2 = 1 / sqrt(tt * tt + 1);
+          s2 = tt * c2;
         }
+        pa_[tid] = a;
+        pb_[tid] = b;
+        cs_[tid] = c2;
+        sn_[tid] = s2;
+      }
+      __syncthreads();
+      for (int e = tid; e < (p / 2) * p; e += 256) { /* columns a, b of every row */
+        const int q = e / p, rr = e % p, a = pa_[q], b = pb_[q];
+        const double gra = G[rr * p + a], grb = G[rr * p + b];
+        G[rr * p + a] = cs_[q] * gra - sn_[q] * grb;
+        G[rr * p + b] = sn_[q] * gra + cs_[q] * grb;
+      }
+      __syncthreads();
+      for (int e = tid; e < (p / 2) * p; e += 256) { /* rows a, b of every column */
+        const int q = e / p, rr = e % p, a = pa_[q], b = pb_[q];
+        const double gar = G[a * p + rr], gbr = G[b * p + rr];
+        G[a * p + rr] = cs_[q] * gar - sn_[q] * gbr;
+        G[b * p + rr] = sn_[q] * gar + cs_[q] * gbr;
+      }
+      __syncthreads();
     }
+  }
+  if (tid == 0) {
     /* the k largest diagonal entries, descending (NaN when the iteration failed) */
     float* out = eig + (int64_t)blockIdx.x * k;
     for (int r = 0; r < k; ++r) {
