@@ -284,13 +284,13 @@ __device__ inline float lg10f(float x) { return __log2f(x) * 0.30102999566398120
  * Bs = sum 1/(v+phi2) (CalcLog10AbfUvlr, gene_snp_pair.cpp:336-354). */
 __device__ inline float abf_fin(float L, float A, float Bs, float oma2) {
   if (!(Bs > 0.f)) return 0.f;
-  const float bbar = A / Bs, V = 1.f / Bs;
+  const float V = __builtin_amdgcn_rcpf(Bs), bbar = A * V;
   if (bbar == 0.f || !(V < INFINITY)) return 0.f;
-  const float T2 = bbar * bbar / V;
+  const float T2 = A * bbar; /* bbar^2 / V */
   float lbar = 0.f;
   if (T2 != 0.f) {
-    const float Vo = V + oma2;
-    lbar = 0.5f * lg10f(V / Vo) + kHalfInvLn10f * T2 * oma2 / Vo;
+    const float rVo = __builtin_amdgcn_rcpf(V + oma2);
+    lbar = 0.5f * lg10f(V * rVo) + kHalfInvLn10f * T2 * oma2 * rVo;
   }
   return lbar + L;
 }
@@ -343,6 +343,7 @@ __global__ void __launch_bounds__(NT_ABF) uvlr_abf_kernel(const int S, const int
   float* lz = lb + S * 64;        /* [S][64] z^2                        */
   uint8_t* lf = (uint8_t*)(lz + S * 64);
   double* part = (double*)(sm2 + ((S * 64 * 13 + 15) / 16) * 16); /* [4 waves][3 types][2][64] */
+  float* gsm = (float*)(part + 4 * 3 * 2 * 64); /* small grid as FP32: phi2[GS], omega2[GS] */
   const int tid = threadIdx.x, lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int64_t p0 = (int64_t)blockIdx.x * 64;
@@ -355,6 +356,7 @@ __global__ void __launch_bounds__(NT_ABF) uvlr_abf_kernel(const int S, const int
     lz[e] = ap.st_z2[o];
     lf[e] = ap.st_f[o];
   }
+  for (int e = tid; e < 2 * ap.GS; e += NT_ABF) gsm[e] = (float)ap.grid[2 * ap.GL + e];
   __syncthreads();
   const bool valid = lane < cnt;
   const int64_t p = p0 + (valid ? lane : 0);
@@ -366,8 +368,8 @@ __global__ void __launch_bounds__(NT_ABF) uvlr_abf_kernel(const int S, const int
   ta0.init();
   ta1.init();
   ta2.init();
-  const double* phiS = ap.grid + 2 * ap.GL;
-  const double* omaS = phiS + ap.GS;
+  const float* phiS = gsm; /* LDS: uniform reads, no memory latency per grid point */
+  const float* omaS = gsm + ap.GS;
   const int njobs = ap.n_jobs + ap.n_cfg;
   for (int j = w; j < njobs; j += NT_ABF / 64) {
     if (j < ap.n_jobs) {
@@ -410,8 +412,8 @@ __global__ void __launch_bounds__(NT_ABF) uvlr_abf_kernel(const int S, const int
 #pragma unroll 2
         for (int gi = 0; gi < ap.GS; ++gi) {
           float l = 0.f;
+          const float p2 = phiS[gi], o2 = omaS[gi];
           if (f == F_NORMAL) {
-            const float p2 = (float)phiS[gi], o2 = (float)omaS[gi];
             const float vp = v + p2, va = vp + o2;
             const float r1 = __builtin_amdgcn_rcpf(vp), r2 = __builtin_amdgcn_rcpf(va);
             l = 0.5f * lg10f(v * r2) + kHalfInvLn10f * z2 * (p2 * r1 + v * o2 * r1 * r2);
@@ -428,7 +430,7 @@ __global__ void __launch_bounds__(NT_ABF) uvlr_abf_kernel(const int S, const int
         for (int s = 0; s < S; ++s)
           if (((mk >> s) & 1ull) && lf[s * 64 + lane] == F_NAN) nan = true;
         for (int gi = 0; gi < ap.GS; ++gi) {
-          const float p2 = (float)phiS[gi], o2 = (float)omaS[gi];
+          const float p2 = phiS[gi], o2 = omaS[gi];
           float L = 0.f, A = 0.f, Bs = 0.f;
           uint64_t m = mk;
           while (m) {
@@ -656,7 +658,7 @@ int uvlr_fast_run(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_assoc_opts* o, 
   ap.GL = GL;
   ap.GS = GS;
   ap.raw_f64 = o->raw_dtype == EQBF_F64;
-  const size_t lds2 = ((size_t)(S * 64 * 13 + 15) / 16) * 16 + (size_t)4 * 3 * 2 * 64 * 8;
+  const size_t lds2 = ((size_t)(S * 64 * 13 + 15) / 16) * 16 + (size_t)4 * 3 * 2 * 64 * 8 + (size_t)2 * GS * 4;
   {
     ProfScope ps(ctx, "uvlr_abf_kernel", st);
     hipLaunchKernelGGL(uvlr_abf_kernel, dim3((unsigned)((NP + 63) / 64)), dim3(NT_ABF), lds2, st, S, NP, ap);
