@@ -505,6 +505,7 @@ int main(int argc, char** argv) {
   const int S = (int)d.sbgrps.size();
   const int bfs = bfs_code(o.bfs);
   Engine eng(o.device);
+  if (o.kernel_times) eng.check(eqbf_profile(eng.ctx, 1));
   eng.check(eqbf_set_grids(eng.ctx, phiL.data(), omaL.data(), (int32_t)phiL.size(), phiS.data(), omaS.data(),
                            (int32_t)phiS.size()));
   Shared sh = make_shared(d);
@@ -719,6 +720,16 @@ int main(int argc, char** argv) {
       }
     }
     write_all(&ch, &r, &pj, &ps);
+  }
+  if (o.kernel_times) {
+    const int32_t kmax = 256;
+    std::vector<char> names((size_t)kmax * 64);
+    std::vector<double> ms(kmax);
+    std::vector<int64_t> launches(kmax);
+    int32_t nk = 0;
+    eng.check(eqbf_profile_read(eng.ctx, names.data(), ms.data(), launches.data(), kmax, &nk));
+    for (int32_t k = 0; k < std::min(nk, kmax); ++k)
+      std::cerr << "kernel\t" << &names[(size_t)k * 64] << "\t" << launches[k] << "\t" << ms[k] << " ms\n";
   }
   if (verbose > 0) {
     std::cout << "nb of analyzed gene-SNP pairs: " << n_pairs_done << " (" << n_genes_done << " genes)" << std::endl;

@@ -55,6 +55,7 @@ void help(const char* prog) {
       << "      --wrtsize\tnumber of genes processed and written at once (default=10)\n"
       << "      --precision\tdevice path: mixed (default) or fp64\n"
       << "      --device\tHIP device index (default=0)\n"
+      << "      --ktimes\tprint the GPU time of every kernel to stderr at the end\n"
       << std::endl;
 }
 
@@ -94,7 +95,8 @@ Options parse_options(int argc, char** argv) {
       {"maxbf", no_argument, 0, 0},           {"thread", required_argument, 0, 0},
       {"snp", required_argument, 0, 0},       {"sbgrp", required_argument, 0, 0},
       {"wrtsize", required_argument, 0, 0},   {"precision", required_argument, 0, 0},
-      {"device", required_argument, 0, 0},    {0, 0, 0, 0}};
+      {"device", required_argument, 0, 0},    {"ktimes", no_argument, 0, 0},
+      {0, 0, 0, 0}};
   while (true) {
     int idx = 0;
     const int c = getopt_long(argc, argv, "hVv:", longopts, &idx);
@@ -151,6 +153,7 @@ Options parse_options(int argc, char** argv) {
     else if (n == "wrtsize") o.write_group_size = std::atoi(a);
     else if (n == "precision") o.precision = a;
     else if (n == "device") o.device = std::atoi(a);
+    else if (n == "ktimes") o.kernel_times = true;
   }
   const char* prog = argv[0];
   if (!o.inss.empty()) bad(o, prog, "--inss (summary statistics as input) is not supported by this build");

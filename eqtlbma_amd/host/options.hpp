@@ -1,7 +1,7 @@
 /*
  * Command line of the eqtlbma_bf drop-in: the reference's flags, defaults
  * and validation rules (src/eqtlbma_bf.cpp:262-693, defaults :1586-1597),
- * plus --precision to choose the device path.
+ * plus --precision to choose the device path, --device and --ktimes.
  */
 #ifndef EQBF_HOST_OPTIONS_HPP
 #define EQBF_HOST_OPTIONS_HPP
@@ -24,6 +24,9 @@ struct Options {
    * or "fp64" (every quantity in FP64) -- no counterpart in the reference */
   std::string precision = "mixed";
   int device = 0;
+  /* print the GPU time of every kernel to stderr at the end (no counterpart
+   * in the reference) */
+  bool kernel_times = false;
   std::string cmdline;
 };
 

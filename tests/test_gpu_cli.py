@@ -207,3 +207,46 @@ def test_cli_mvlr(tmp_path, orc):
         assert got[:3] == want[:3]
         assert all(_close(_num(a), b) for a, b in zip(got[3:], want[3:])), (got, want)
     assert not os.path.exists(out + "_sumstats_s1.txt.gz")  # not written with --error mvlr
+
+
+def test_cli_join_permutations_contraction_path(tmp_path, orc):
+    # every sample in every subgroup, one genotype file of hard calls, no
+    # covariates: the permutation statistics take the contraction path
+    root = str(tmp_path)
+    c = cli_case.make_case(root, seed=12, S=3, N=40, mvlr=True, covariates=False)
+    inp = cli_case.load_expected_inputs(c)
+    out = os.path.join(root, "jc")
+    nperm, seed = 30, 21
+    r = _run(_common_args(root, out, "join") + ["--bfs", "all", "--nperm", str(nperm), "--seed", str(seed),
+                                                "--pbf", "all", "--maxbf", "--precision", "mixed", "--ktimes"], root)
+    kernels = {l.split("\t")[1] for l in r.stderr.splitlines() if l.startswith("kernel\t")}
+    _say("[gpu]   kernels: %s" % sorted(kernels))
+    assert "pf_gemm_kernel" in kernels and "pf_bma_kernel" in kernels
+    assert "perm_gather_kernel" not in kernels  # no gene fell back to the general path
+    S = len(inp["sbgrps"])
+    labels = cli_case.config_labels("all", S)
+    gL, gS = c["gL"], c["gS"]
+    exp_perm = []
+    for names, hc, psn in cli_case.expected_chunks(inp, 2500, 2):
+        ref = orc.assoc_uvlr(hc, "join", "all", gL, gS)
+        tables = orc.perm_tables(seed, hc.N, sum(hc.pair_offset[g + 1] > hc.pair_offset[g] for g in range(hc.G)),
+                                 nperm)
+        t = 0
+        for g, gname in enumerate(names):
+            p0, p1 = hc.pair_offset[g], hc.pair_offset[g + 1]
+            if p1 > p0:
+                true = orc.true_stat(ref.w[3 + len(labels) + 1, p0:p1], True)
+                st = orc.perm_join_stats(hc, g, "all", True, gL, gS, tables[t])
+                pv, used, med, _nb, _dr = orc.perm_count(st, true, nperm)
+                near = int(np.sum(np.abs(st - true) <= 1e-4))
+                exp_perm.append([gname, p1 - p0, pv, used, true, med, near])
+                t += 1
+    rows = cli_case.read_table(out + "_joinPermPvals.txt.gz")
+    assert rows[0] == ["# perm.bf=all seed=%d" % seed]
+    assert len(rows) - 2 == len(exp_perm)
+    for got, want in zip(rows[2:], exp_perm):
+        assert got[0] == want[0] and int(got[1]) == want[1]
+        if want[6] == 0:  # no permutation statistic within the tolerance of the true one
+            assert int(got[3]) == want[3]
+            assert _num(got[2]) == pytest.approx(want[2], rel=1e-5)
+        assert abs(_num(got[4]) - want[4]) <= 1e-4 and abs(_num(got[5]) - want[5]) <= 1e-4
