@@ -301,3 +301,50 @@ extern "C" int eqbf_assoc_uvlr(eqbf_ctx* ctx, const eqbf_chunk* c, const eqbf_as
   EQBF_HIP(ctx, hipStreamSynchronize(st));
   return EQBF_OK;
 }
+
+extern "C" int eqbf_abf_from_sstats_device(eqbf_ctx* ctx, const eqbf_sstats_in* in, int32_t bfs, int32_t raw_dtype,
+                                           eqbf_assoc_out* out, void* stream) {
+  if (!ctx || !in || !out) return ctx_fail(ctx, EQBF_ERR_ARG, "null argument");
+  EQBF_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  return abf_from_sstats_run(ctx, in, bfs, raw_dtype, out, st);
+}
+
+extern "C" int eqbf_abf_from_sstats(eqbf_ctx* ctx, const eqbf_sstats_in* in, int32_t bfs, int32_t raw_dtype,
+                                    eqbf_assoc_out* out) {
+  if (!ctx || !in || !out || !out->w_abf) return ctx_fail(ctx, EQBF_ERR_ARG, "null argument");
+  EQBF_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const int64_t S = in->n_subgroups, NP = in->n_pairs;
+  if (S < 1 || NP < 0) return ctx_fail(ctx, EQBF_ERR_ARG, "invalid summary statistics sizes");
+  if (NP == 0) return EQBF_OK;
+  if (!in->n || !in->sigmahat || !in->betahat || !in->sebetahat) return ctx_fail(ctx, EQBF_ERR_ARG, "null input");
+  const int GL = (int)ctx->phiL.size(), GS = (int)ctx->phiS.size();
+  const int64_t n_raw = eqbf_n_raw(bfs, (int)S, GL, GS), n_w = eqbf_n_weighted(bfs, (int)S);
+  if (eqbf_n_configs(bfs, (int)S) < 0 || n_raw < 0)
+    return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED, "--bfs %d with %d subgroups", bfs, (int)S);
+  const size_t rsz = raw_dtype == EQBF_F64 ? 8 : 4;
+  Stager sg{ctx, st};
+  eqbf_sstats_in d = *in;
+  d.n = sg.put("h_inss_n", in->n, (size_t)(S * NP));
+  d.sigmahat = sg.put("h_inss_sig", in->sigmahat, (size_t)(S * NP));
+  d.betahat = sg.put("h_inss_beta", in->betahat, (size_t)(S * NP));
+  d.sebetahat = sg.put("h_inss_se", in->sebetahat, (size_t)(S * NP));
+  eqbf_assoc_out dout;
+  memset(&dout, 0, sizeof dout);
+  dout.n = sg.alloc<int32_t>("o_n", (size_t)(S * NP), out->n != nullptr);
+  dout.stdsstats = sg.alloc<double>("o_std", (size_t)(3 * S * NP), out->stdsstats != nullptr);
+  dout.raw_abf = sg.alloc<uint8_t>("o_raw", (size_t)(n_raw * NP) * rsz, out->raw_abf != nullptr);
+  dout.w_abf = sg.alloc<double>("o_w", (size_t)(n_w * NP), true);
+  if (sg.rc) return ctx_fail(ctx, sg.rc, "staging the summary statistics failed");
+  int rc = abf_from_sstats_run(ctx, &d, bfs, raw_dtype, &dout, st);
+  if (rc) return rc;
+  if (out->n) EQBF_HIP(ctx, hipMemcpyAsync(out->n, dout.n, S * NP * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  if (out->stdsstats)
+    EQBF_HIP(ctx, hipMemcpyAsync(out->stdsstats, dout.stdsstats, 3 * S * NP * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (out->raw_abf)
+    EQBF_HIP(ctx, hipMemcpyAsync(out->raw_abf, dout.raw_abf, n_raw * NP * rsz, hipMemcpyDeviceToHost, st));
+  EQBF_HIP(ctx, hipMemcpyAsync(out->w_abf, dout.w_abf, n_w * NP * sizeof(double), hipMemcpyDeviceToHost, st));
+  EQBF_HIP(ctx, hipStreamSynchronize(st));
+  return EQBF_OK;
+}

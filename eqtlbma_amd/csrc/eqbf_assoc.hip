@@ -916,6 +916,53 @@ __device__ double l10ws_plain(const double* v, int n) {
 #define PLAIN_SMAX 64
 #define PLAIN_GMAX 128
 
+/* The consistent configuration over the large grid (gen, gen-fix, gen-maxh)
+ * and the configurations over the small grid of pair p, FP64
+ * (CalcAbfsUvlrForConsistentConfiguration / ForSingletons /
+ * ForEachConfiguration, gene_snp_pair.cpp:364-550); gen-sin and all follow in
+ * bma_kernel. */
+__device__ __forceinline__ void plain_pair_abfs(const AssocParams& kp, int S, int64_t NP, int64_t p,
+                                                const double* st_b, const double* st_se, const double* st_t,
+                                                const uint8_t* has) {
+  const double* phiL = kp.grid;
+  const double* omaL = phiL + kp.GL;
+  const double* phiS = phiL + 2 * kp.GL;
+  const double* omaS = phiS + kp.GS;
+  const uint64_t all = (S >= 64) ? ~0ull : ((1ull << S) - 1ull);
+  double buf[PLAIN_GMAX];
+  for (int type = 0; type < 3; ++type) {
+    for (int gi = 0; gi < kp.GL; ++gi) {
+      double p2 = type == 0 ? phiL[gi] : (type == 1 ? 0.0 : phiL[gi] + omaL[gi]);
+      double o2 = type == 0 ? omaL[gi] : (type == 1 ? phiL[gi] + omaL[gi] : 0.0);
+      buf[gi] = abf_plain(all, S, st_b, st_se, st_t, has, p2, o2);
+      if (kp.raw) {
+        int64_t ro = (int64_t)(type * kp.GL + gi) * NP + p;
+        if (kp.raw_f64)
+          ((double*)kp.raw)[ro] = buf[gi];
+        else
+          ((float*)kp.raw)[ro] = (float)buf[gi];
+      }
+    }
+    kp.w[(int64_t)type * NP + p] = l10ws_plain(buf, kp.GL);
+  }
+  for (int cfg = 0; cfg < kp.n_cfg; ++cfg) {
+    const uint64_t mk = kp.masks[cfg];
+    bool any = false;
+    for (int s = 0; s < S; ++s) any |= ((mk >> s) & 1ull) && has[s];
+    for (int gi = 0; gi < kp.GS; ++gi) {
+      buf[gi] = any ? abf_plain(mk, S, st_b, st_se, st_t, has, phiS[gi], omaS[gi]) : 0.0;
+      if (kp.raw) {
+        int64_t ro = (int64_t)(3 * kp.GL + cfg * kp.GS + gi) * NP + p;
+        if (kp.raw_f64)
+          ((double*)kp.raw)[ro] = buf[gi];
+        else
+          ((float*)kp.raw)[ro] = (float)buf[gi];
+      }
+    }
+    kp.w[(int64_t)(3 + cfg) * NP + p] = l10ws_plain(buf, kp.GS);
+  }
+}
+
 __global__ void __launch_bounds__(64) assoc_plain_kernel(const eqbf_chunk c, const AssocParams kp, int g) {
   const int S = c.n_subgroups, N = c.n_samples;
   const int64_t NP = c.n_pairs;
@@ -990,43 +1037,54 @@ __global__ void __launch_bounds__(64) assoc_plain_kernel(const eqbf_chunk c, con
     }
   }
   if (!kp.join) return;
-  const double* phiL = kp.grid;
-  const double* omaL = phiL + kp.GL;
-  const double* phiS = phiL + 2 * kp.GL;
-  const double* omaS = phiS + kp.GS;
-  const uint64_t all = (S >= 64) ? ~0ull : ((1ull << S) - 1ull);
-  double buf[PLAIN_GMAX];
-  for (int type = 0; type < 3; ++type) {
-    for (int gi = 0; gi < kp.GL; ++gi) {
-      double p2 = type == 0 ? phiL[gi] : (type == 1 ? 0.0 : phiL[gi] + omaL[gi]);
-      double o2 = type == 0 ? omaL[gi] : (type == 1 ? phiL[gi] + omaL[gi] : 0.0);
-      buf[gi] = abf_plain(all, S, st_b, st_se, st_t, has, p2, o2);
-      if (kp.raw) {
-        int64_t ro = (int64_t)(type * kp.GL + gi) * NP + p;
-        if (kp.raw_f64)
-          ((double*)kp.raw)[ro] = buf[gi];
-        else
-          ((float*)kp.raw)[ro] = (float)buf[gi];
+  plain_pair_abfs(kp, S, NP, p, st_b, st_se, st_t, has);
+}
+
+/* --inss: summary statistics given per (pair, subgroup) instead of computed
+ * (GeneSnpPair::SetSstats, gene_snp_pair.cpp:241-254), standardised with
+ * nu = n - 2 since no covariate count is known for them (:273, the map entry
+ * defaults to 0), then the Bayes factors as on the plain path; one thread per
+ * pair, FP64.  n <= 0: no result in that subgroup (HasResults, :61-68). */
+__global__ void __launch_bounds__(64) inss_abf_kernel(const eqbf_sstats_in in, const AssocParams kp) {
+  const int S = in.n_subgroups;
+  const int64_t NP = in.n_pairs;
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= NP) return;
+  double st_b[PLAIN_SMAX], st_se[PLAIN_SMAX], st_t[PLAIN_SMAX];
+  uint8_t has[PLAIN_SMAX];
+  for (int s = 0; s < S; ++s) {
+    const int64_t e = (int64_t)s * NP + p;
+    const int64_t n = in.n[e];
+    has[s] = n > 0;
+    st_b[s] = st_se[s] = st_t[s] = NAN;
+    if (has[s]) {
+      /* StandardizeSstatsAndCorrectSmallSampleSize (:256-290) */
+      const double beta = in.betahat[e];
+      double bhat = beta / in.sigmahat[e], sebhat = in.sebetahat[e] / in.sigmahat[e], t = bhat / sebhat;
+      if (!isnan(t)) {
+        const double nu = (double)n - 2;
+        t = nu > 0 ? dm::gauss_pinv(dm::t_lower_neg(fabs(bhat / sebhat), nu)) : NAN;
+        if (fabs(t) > 1e-8) {
+          const double sig2 = fabs(beta) / (fabs(t) * sebhat);
+          bhat = beta / sig2;
+          sebhat = fabs(bhat / t);
+        } else {
+          bhat = 0;
+          sebhat = INFINITY;
+        }
+        st_b[s] = bhat;
+        st_se[s] = sebhat;
+        st_t[s] = t;
       }
     }
-    kp.w[(int64_t)type * NP + p] = l10ws_plain(buf, kp.GL);
-  }
-  for (int cfg = 0; cfg < kp.n_cfg; ++cfg) {
-    const uint64_t mk = kp.masks[cfg];
-    bool any = false;
-    for (int s = 0; s < S; ++s) any |= ((mk >> s) & 1ull) && has[s];
-    for (int gi = 0; gi < kp.GS; ++gi) {
-      buf[gi] = any ? abf_plain(mk, S, st_b, st_se, st_t, has, phiS[gi], omaS[gi]) : 0.0;
-      if (kp.raw) {
-        int64_t ro = (int64_t)(3 * kp.GL + cfg * kp.GS + gi) * NP + p;
-        if (kp.raw_f64)
-          ((double*)kp.raw)[ro] = buf[gi];
-        else
-          ((float*)kp.raw)[ro] = (float)buf[gi];
-      }
+    if (kp.n_out) kp.n_out[e] = has[s] ? (int32_t)(n < 2147483647 ? n : 2147483647) : 0;
+    if (kp.stdsstats) {
+      kp.stdsstats[e] = st_b[s];
+      kp.stdsstats[(int64_t)S * NP + e] = st_se[s];
+      kp.stdsstats[(int64_t)2 * S * NP + e] = st_t[s];
     }
-    kp.w[(int64_t)(3 + cfg) * NP + p] = l10ws_plain(buf, kp.GS);
   }
+  plain_pair_abfs(kp, S, NP, p, st_b, st_se, st_t, has);
 }
 
 template <int NWV, int SPW, int KU>
@@ -1244,5 +1302,72 @@ int assoc_uvlr_run(eqbf_ctx* ctx, const eqbf_chunk* cp, const eqbf_assoc_opts* o
                        c.n_pairs, S, (int)n_cfg, n_cfg > 0 ? o->bfs : EQBF_BFS_GEN, d_masks, d_inv_choose);
     EQBF_HIP(ctx, hipGetLastError());
   }
+  return EQBF_OK;
+}
+
+/* --inss: Bayes factors from given summary statistics (inss_abf_kernel, then
+ * gen-sin / all in bma_kernel).  in and out are device pointers. */
+int abf_from_sstats_run(eqbf_ctx* ctx, const eqbf_sstats_in* in, int bfs, int raw_dtype, eqbf_assoc_out* out,
+                        hipStream_t st) {
+  const int S = in->n_subgroups;
+  const int64_t NP = in->n_pairs;
+  if (S < 1 || NP < 0) return ctx_fail(ctx, EQBF_ERR_ARG, "invalid summary statistics sizes");
+  if (S > PLAIN_SMAX) return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED, "at most %d subgroups", PLAIN_SMAX);
+  if (bfs != EQBF_BFS_GEN && bfs != EQBF_BFS_SIN && bfs != EQBF_BFS_ALL)
+    return ctx_fail(ctx, EQBF_ERR_ARG, "invalid bfs %d", bfs);
+  if (raw_dtype != EQBF_F32 && raw_dtype != EQBF_F64) return ctx_fail(ctx, EQBF_ERR_ARG, "invalid raw_dtype");
+  const int GL = (int)ctx->phiL.size(), GS = (int)ctx->phiS.size();
+  if (GL < 1) return ctx_fail(ctx, EQBF_ERR_ARG, "the large grid is empty (eqbf_set_grids)");
+  if (bfs != EQBF_BFS_GEN && GS < 1) return ctx_fail(ctx, EQBF_ERR_ARG, "the small grid is empty (eqbf_set_grids)");
+  if (GL > PLAIN_GMAX || GS > PLAIN_GMAX)
+    return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED, "grids of at most %d points", PLAIN_GMAX);
+  const int64_t n_cfg = eqbf_n_configs(bfs, S);
+  if (n_cfg < 0) return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED, "--bfs all with %d subgroups", S);
+  if (!out->w_abf) return ctx_fail(ctx, EQBF_ERR_ARG, "w_abf is required");
+  if (NP == 0) return EQBF_OK;
+  if (!in->n || !in->sigmahat || !in->betahat || !in->sebetahat) return ctx_fail(ctx, EQBF_ERR_ARG, "null input");
+  const int64_t nm = std::max<int64_t>(n_cfg, 1);
+  uint64_t* d_masks = (uint64_t*)ctx_buf(ctx, "inss_tab", (size_t)nm * sizeof(uint64_t) + 65 * sizeof(double));
+  if (!d_masks) return EQBF_ERR_HIP; /* ctx_buf has set the message */
+  double* d_inv_choose = (double*)(d_masks + nm);
+  {
+    std::vector<uint64_t> hm(nm, 0);
+    if (n_cfg > 0) eqbf_config_masks(bfs, S, hm.data());
+    std::vector<double> ic(65, 0.0);
+    for (int k = 1; k <= S; ++k) {
+      double ch = 1;
+      for (int i = 1; i <= k; ++i) ch = ch * (S - k + i) / i;
+      ic[k] = 1.0 / ch;
+    }
+    EQBF_HIP(ctx, hipMemcpyAsync(d_masks, hm.data(), nm * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+    EQBF_HIP(ctx, hipMemcpyAsync(d_inv_choose, ic.data(), 65 * sizeof(double), hipMemcpyHostToDevice, st));
+    EQBF_HIP(ctx, hipStreamSynchronize(st)); /* host vectors go out of scope */
+  }
+  AssocParams kp;
+  memset(&kp, 0, sizeof kp);
+  kp.grid = (const double*)ctx->d_grid.ptr;
+  kp.masks = d_masks;
+  kp.n_out = out->n;
+  kp.stdsstats = out->stdsstats;
+  kp.raw = out->raw_abf;
+  kp.w = out->w_abf;
+  kp.GL = GL;
+  kp.GS = GS;
+  kp.n_cfg = (int)n_cfg;
+  kp.n_w = (int)(3 + n_cfg + 2);
+  kp.bfs = bfs;
+  kp.join = 1;
+  kp.raw_f64 = raw_dtype == EQBF_F64;
+  {
+    ProfScope ps(ctx, "inss_abf_kernel", st);
+    hipLaunchKernelGGL(inss_abf_kernel, dim3((unsigned)((NP + 63) / 64)), dim3(64), 0, st, *in, kp);
+  }
+  EQBF_HIP(ctx, hipGetLastError());
+  {
+    ProfScope ps(ctx, "bma_kernel", st);
+    hipLaunchKernelGGL(bma_kernel, dim3((unsigned)((NP + 255) / 256)), dim3(256), 0, st, out->w_abf, NP, S,
+                       (int)n_cfg, n_cfg > 0 ? bfs : EQBF_BFS_GEN, d_masks, d_inv_choose);
+  }
+  EQBF_HIP(ctx, hipGetLastError());
   return EQBF_OK;
 }

@@ -338,6 +338,9 @@ int uvlr_fast_run(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_assoc_opts* o, 
                   const PrepRec* rec, const double* B, const uint64_t* memb, const int4* tiles, const int* n_tiles,
                   int KU, const uint64_t* d_masks, int n_cfg);
 int mvlr_run(eqbf_ctx* ctx, const eqbf_chunk* c, const eqbf_mvlr_opts* o, eqbf_assoc_out* out, hipStream_t st);
+/* --inss: Bayes factors from given summary statistics (device pointers) */
+int abf_from_sstats_run(eqbf_ctx* ctx, const eqbf_sstats_in* in, int bfs, int raw_dtype, eqbf_assoc_out* out,
+                        hipStream_t st);
 /* gen-sin and all rows of out->w_abf from the grid-averaged rows. */
 int launch_bma(eqbf_ctx* ctx, double* w, int64_t n_pairs, int S, int bfs, hipStream_t st);
 int perm_stats_run(eqbf_ctx* ctx, const eqbf_chunk* c, const eqbf_perm_opts* po, const int32_t* tables_dev,

@@ -43,6 +43,8 @@ def lib():
         L.eqbf_assoc_uvlr_device.argtypes = [vp, vp, vp, vp, vp]
         L.eqbf_assoc_mvlr.argtypes = [vp, vp, vp, vp]
         L.eqbf_assoc_mvlr_device.argtypes = [vp, vp, vp, vp, vp]
+        L.eqbf_abf_from_sstats.argtypes = [vp, vp, i32, i32, vp]
+        L.eqbf_abf_from_sstats_device.argtypes = [vp, vp, i32, i32, vp, vp]
         L.eqbf_debug_ttail.argtypes = [vp, vp, vp, i32, vp, vp, vp, vp]
         L.eqbf_perm_stats.argtypes = [vp, vp, vp, vp, i32, vp, i64, vp]
         L.eqbf_perm_stats_device.argtypes = [vp, vp, vp, vp, i32, vp, i64, vp, vp]
@@ -86,6 +88,12 @@ class PermOpts(C.Structure):
 
 class MvlrOpts(C.Structure):
     _fields_ = [("bfs", C.c_int32), ("fiterr", C.c_double)]
+
+
+class SstatsIn(C.Structure):
+    """eqbf_sstats_in: given summary statistics, arrays [S][n_pairs]."""
+    _fields_ = [("n_pairs", C.c_int64), ("n_subgroups", C.c_int32), ("n", C.c_void_p), ("sigmahat", C.c_void_p),
+                ("betahat", C.c_void_p), ("sebetahat", C.c_void_p)]
 
 
 class Rng:
@@ -199,6 +207,30 @@ class Engine:
     def assoc_mvlr_device(self, chunk_struct, opts, out_struct, stream=None):
         self._check(lib().eqbf_assoc_mvlr_device(self._ctx, C.byref(chunk_struct), C.byref(opts),
                                                  C.byref(out_struct), stream))
+
+    def abf_from_sstats(self, n, sigmahat, betahat, sebetahat, bfs="gen", raw_dtype=np.float64, want_std=True,
+                        want_raw=True):
+        """--inss: Bayes factors from given summary statistics, arrays
+        [S][n_pairs] (n <= 0: no result in that subgroup); FP64 throughout.
+        Returns a HostAssocOut (sstats not filled)."""
+        n = np.ascontiguousarray(n, dtype=np.int64)
+        S, NP = n.shape
+        sig, beta, se = (np.ascontiguousarray(a, dtype=np.float64) for a in (sigmahat, betahat, sebetahat))
+        if not (sig.shape == beta.shape == se.shape == (S, NP)):
+            raise ValueError("summary statistics must all be [S][n_pairs]")
+        out = _abi.HostAssocOut(S, NP, bfs, self.GL, self.GS, raw_dtype=raw_dtype, want_sstats=False,
+                                want_std=want_std, want_raw=want_raw)
+        si = SstatsIn(NP, S, _p(n), _p(sig), _p(beta), _p(se))
+        o = out.struct()
+        self._check(lib().eqbf_abf_from_sstats(self._ctx, C.byref(si), _abi.BFS[bfs],
+                                               _abi.F64 if raw_dtype == np.float64 else _abi.F32, C.byref(o)))
+        return out
+
+    def abf_from_sstats_device(self, sstats_struct, bfs, raw_dtype, out_struct, stream=None):
+        """Device-pointer entry: sstats_struct (SstatsIn) and out_struct
+        (_abi.AssocOut) hold device addresses."""
+        self._check(lib().eqbf_abf_from_sstats_device(self._ctx, C.byref(sstats_struct), _abi.BFS[bfs], raw_dtype,
+                                                      C.byref(out_struct), stream))
 
     def perm_stats(self, hc, tables, gene_table, analysis="join", pbf="all", use_max_bf=True, sep_sbgrp=-1,
                    precision="mixed"):

@@ -478,4 +478,82 @@ bool load_data(const LoadOptions& o, Data& d) {
   return !d.snps.empty();
 }
 
+
+/* loadListSstatsFile + fillGeneSnpPairsWithSstats (data_loader.cpp:1212-1327):
+ * one file per subgroup (subgroups in name order), header columns found by
+ * name among tab-separated tokens, genes and pairs created in order of first
+ * appearance, the first line of a (gene, SNP, subgroup) kept
+ * (GeneSnpPair::SetSstats inserts into maps, gene_snp_pair.cpp:241-254). */
+bool load_inss(const std::string& list, int verbose, InssData& d) {
+  std::map<std::string, std::string> files;
+  {
+    LineReader r(list);
+    if (verbose > 0) std::cout << "load file " << list << " ..." << std::endl;
+    std::string line;
+    std::vector<std::string> tok;
+    size_t nl = 0;
+    while (r.next(line)) {
+      ++nl;
+      tokenize(line, " \t,", tok);
+      if (tok.size() != 2) fail("file " + list + " should have only two columns at line " + std::to_string(nl));
+      if (tok[0][0] == '#') continue;
+      files.emplace(tok[0], tok[1]);
+    }
+    if (!r.complete()) fail("can't read successfully file " + list + " up to the end");
+    if (verbose > 0) std::cout << "items loaded: " << files.size() << " files" << std::endl;
+  }
+  d.sbgrps.clear();
+  for (auto& kv : files) d.sbgrps.push_back(kv.first);
+  const size_t S = d.sbgrps.size();
+  static const char* kCols[6] = {"gene", "snp", "n", "sigmahat", "betahat.geno", "sebetahat.geno"};
+  std::set<std::string> snp_names;
+  size_t s = 0;
+  for (auto& kv : files) {
+    if (verbose > 0) std::cout << "load summary statistics for subgroup " << kv.first << " ..." << std::endl;
+    LineReader r(kv.second);
+    std::string line;
+    std::vector<std::string> tok;
+    size_t col[6];
+    size_t need = 0;
+    if (!r.next(line)) line.clear();
+    tokenize(line, "\t", tok);
+    for (int k = 0; k < 6; ++k) {
+      col[k] = std::string::npos;
+      for (size_t c = 0; c < tok.size(); ++c)
+        if (tok[c] == kCols[k]) col[k] = c;
+      /* the reference prints the missing index, npos, rather than the name */
+      if (col[k] == std::string::npos)
+        fail("missing " + std::to_string((unsigned long long)std::string::npos) + " in header of " + kv.second);
+      need = std::max(need, col[k] + 1);
+    }
+    size_t nl = 1;
+    while (r.next(line)) {
+      ++nl;
+      tokenize(line, "\t", tok);
+      if (tok.size() < need)
+        fail("line " + std::to_string(nl) + " of " + kv.second + " has fewer than " + std::to_string(need) +
+             " columns");
+      InssGene& g = d.genes[tok[col[0]]];
+      const std::string& snp = tok[col[1]];
+      auto it = g.idx.find(snp);
+      if (it == g.idx.end()) {
+        it = g.idx.emplace(snp, g.snps.size()).first;
+        g.snps.push_back(snp);
+        g.st.emplace_back(S);
+        snp_names.insert(snp);
+      }
+      SstatsEntry& e = g.st[it->second][s];
+      if (e.set) continue;
+      e.set = true;
+      e.n = std::atol(tok[col[2]].c_str());
+      e.sigmahat = std::atof(tok[col[3]].c_str());
+      e.betahat = std::atof(tok[col[4]].c_str());
+      e.sebetahat = std::atof(tok[col[5]].c_str());
+    }
+    ++s;
+  }
+  d.n_snps = snp_names.size();
+  return !d.genes.empty() && d.n_snps > 0;
+}
+
 }  // namespace host

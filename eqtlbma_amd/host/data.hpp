@@ -73,6 +73,26 @@ struct Data {
  * Returns false when there is nothing to analyse (no gene or no SNP). */
 bool load_data(const LoadOptions& o, Data& d);
 
+/* --inss: summary statistics per (gene, SNP, subgroup) given as input
+ * (loadSummaryStats, data_loader.cpp:1212-1343). */
+struct SstatsEntry {
+  bool set = false; /* a line was read (GeneSnpPair::GetNbSubgroups counts it) */
+  int64_t n = 0;
+  double sigmahat = 0, betahat = 0, sebetahat = 0;
+};
+struct InssGene {
+  std::vector<std::string> snps;           /* pairs, in order of first appearance */
+  std::map<std::string, size_t> idx;
+  std::vector<std::vector<SstatsEntry>> st; /* [pair][subgroup]                   */
+};
+struct InssData {
+  std::vector<std::string> sbgrps;          /* sorted by name                     */
+  std::map<std::string, InssGene> genes;
+  size_t n_snps = 0;                        /* distinct SNP names                 */
+};
+/* Returns false when there is nothing to analyse (no gene or no SNP). */
+bool load_inss(const std::string& list, int verbose, InssData& d);
+
 }  // namespace host
 
 #endif

@@ -289,6 +289,9 @@ struct Results {
   std::vector<int32_t> n;
   std::vector<double> sstats, raw, w;
   bool has_abf = false;
+  /* --inss: subgroups with a line per pair (GeneSnpPair::GetNbSubgroups,
+   * gene_snp_pair.hpp:251, counts n = 0 lines too); empty: count n > 0 */
+  std::vector<int32_t> nsub;
 };
 
 /* ---- writers (eqtlbma_bf.cpp:919-1447) ---------------------------------------- */
@@ -380,7 +383,10 @@ struct Writer {
     for (size_t g = 0; g < ch->genes.size(); ++g)
       for (int64_t p = ch->pair_offset[g]; p < ch->pair_offset[g + 1]; ++p) {
         int nsub = 0;
-        for (int s = 0; s < r->S; ++s) nsub += r->n[(size_t)s * r->NP + p] > 0;
+        if (!r->nsub.empty())
+          nsub = r->nsub[p];
+        else
+          for (int s = 0; s < r->S; ++s) nsub += r->n[(size_t)s * r->NP + p] > 0;
         t += ch->genes[g]->name + "\t" + ch->snps[ch->pair_snp[p]]->name + "\t";
         put_int(t, nsub);
         auto w = [&](size_t row) { return r->w[row * r->NP + p]; };
@@ -405,6 +411,18 @@ struct Writer {
     f.put(t);
   }
 };
+
+/* --ktimes: GPU time of every kernel of the run, on stderr. */
+void print_kernel_times(Engine& eng) {
+  const int32_t kmax = 256;
+  std::vector<char> names((size_t)kmax * 64);
+  std::vector<double> ms(kmax);
+  std::vector<int64_t> launches(kmax);
+  int32_t nk = 0;
+  eng.check(eqbf_profile_read(eng.ctx, names.data(), ms.data(), launches.data(), kmax, &nk));
+  for (int32_t k = 0; k < std::min(nk, kmax); ++k)
+    std::cerr << "kernel\t" << &names[(size_t)k * 64] << "\t" << launches[k] << "\t" << ms[k] << " ms\n";
+}
 
 /* Per-gene permutation results. */
 struct PermRes {
@@ -475,6 +493,120 @@ void run_perms(Engine& eng, const Chunk& ch, const eqbf_perm_opts& po, const std
   }
 }
 
+/* --inss (run(), eqtlbma_bf.cpp:1496-1503 with loadSummaryStats): the pairs
+ * of the summary-statistics files, genes in name order, chunks of --wrtsize
+ * genes through eqbf_abf_from_sstats, then the same writers.  The sumstats
+ * files get their header only: the reference writes a row only for a gene
+ * with expression levels and a SNP with genotypes in the subgroup
+ * (Gene::HasAtLeastOneCisSnp, gene.cpp:218-232), which given statistics never
+ * provide. */
+int run_inss(const Options& o, time_t t_start, const char* prog) {
+  const int verbose = o.verbose;
+  InssData in;
+  if (!load_inss(o.inss, verbose, in)) return EXIT_SUCCESS;
+  std::vector<double> phiL, omaL, phiS, omaS;
+  load_grid(o.gridL, phiL, omaL, verbose);
+  load_grid(o.gridS, phiS, omaS, verbose);
+  const int S = (int)in.sbgrps.size();
+  const int bfs = bfs_code(o.bfs);
+  Engine eng(o.device);
+  if (o.kernel_times) eng.check(eqbf_profile(eng.ctx, 1));
+  eng.check(eqbf_set_grids(eng.ctx, phiL.data(), omaL.data(), (int32_t)phiL.size(), phiS.data(), omaS.data(),
+                           (int32_t)phiS.size()));
+  Data d; /* subgroup names for the writers */
+  for (auto& n : in.sbgrps) {
+    d.sbgrps.emplace_back();
+    d.sbgrps.back().name = n;
+  }
+  Writer wr{o, d, {}, (int)phiL.size(), (int)phiS.size(), "wb"};
+  if (bfs != EQBF_BFS_GEN) {
+    const int64_t nc = eqbf_n_configs(bfs, S);
+    if (nc < 0) fail("--bfs all with " + std::to_string(S) + " subgroups is not supported");
+    std::vector<uint64_t> masks(nc);
+    eqbf_config_masks(bfs, S, masks.data());
+    for (uint64_t m : masks) {
+      std::string l;
+      for (int s = 0; s < S; ++s)
+        if ((m >> s) & 1ull) l += (l.empty() ? "" : "-") + std::to_string(s + 1);
+      wr.cfg_labels.push_back(l);
+    }
+  }
+  if (o.save_sstats) wr.sumstats(nullptr, nullptr);
+  wr.raw(nullptr, nullptr);
+  if (o.save_weighted_abfs) wr.avg(nullptr, nullptr);
+  wr.mode = "ab";
+  if (verbose > 0)
+    std::cout << "test for association between each pair gene-SNP ..." << std::endl
+              << "analysis=" << o.analysis << " likelihood=" << o.likelihood << " error_model=" << o.error_model
+              << " precision=fp64" << std::endl;
+  std::vector<const std::pair<const std::string, InssGene>*> all;
+  for (auto& kv : in.genes) all.push_back(&kv);
+  size_t n_genes_done = 0, n_pairs_done = 0;
+  for (size_t g0 = 0; g0 < all.size(); g0 += (size_t)o.write_group_size) {
+    const size_t g1 = std::min(all.size(), g0 + (size_t)o.write_group_size);
+    const int G = (int)(g1 - g0);
+    std::vector<Gene> genes(G);
+    std::vector<Snp> snps;
+    Chunk ch;
+    ch.pair_offset.assign(G + 1, 0);
+    for (int g = 0; g < G; ++g) {
+      genes[g].name = all[g0 + g]->first;
+      ch.pair_offset[g + 1] = ch.pair_offset[g] + (int64_t)all[g0 + g]->second.snps.size();
+    }
+    const int64_t NP = ch.pair_offset[G];
+    snps.resize(NP);
+    Results r;
+    r.S = S;
+    r.NP = NP;
+    r.n_raw = eqbf_n_raw(bfs, S, (int32_t)phiL.size(), (int32_t)phiS.size());
+    r.n_w = eqbf_n_weighted(bfs, S);
+    r.n.assign((size_t)S * NP, 0);
+    r.nsub.assign(NP, 0);
+    r.raw.assign((size_t)r.n_raw * NP, kNaN);
+    r.w.assign((size_t)r.n_w * NP, kNaN);
+    std::vector<int64_t> n((size_t)S * NP, 0);
+    std::vector<double> sig((size_t)S * NP, kNaN), beta((size_t)S * NP, kNaN), se((size_t)S * NP, kNaN);
+    for (int g = 0; g < G; ++g) {
+      ch.genes.push_back(&genes[g]);
+      const InssGene& ig = all[g0 + g]->second;
+      for (size_t k = 0; k < ig.snps.size(); ++k) {
+        const int64_t p = ch.pair_offset[g] + (int64_t)k;
+        snps[p].name = ig.snps[k];
+        ch.snps.push_back(&snps[p]);
+        ch.pair_snp.push_back((int32_t)p);
+        for (int s = 0; s < S; ++s) {
+          const SstatsEntry& e = ig.st[k][s];
+          if (!e.set) continue;
+          ++r.nsub[p];
+          const size_t i = (size_t)s * NP + p;
+          n[i] = e.n;
+          sig[i] = e.sigmahat;
+          beta[i] = e.betahat;
+          se[i] = e.sebetahat;
+        }
+      }
+      ++n_genes_done;
+      n_pairs_done += ig.snps.size();
+    }
+    if (NP > 0) {
+      eqbf_sstats_in si{NP, S, n.data(), sig.data(), beta.data(), se.data()};
+      eqbf_assoc_out out{};
+      out.n = r.n.data();
+      out.raw_abf = r.raw.data();
+      out.w_abf = r.w.data();
+      eng.check(eqbf_abf_from_sstats(eng.ctx, &si, bfs, EQBF_F64, &out));
+    }
+    wr.raw(&ch, &r);
+    if (o.save_weighted_abfs) wr.avg(&ch, &r);
+  }
+  if (o.kernel_times) print_kernel_times(eng);
+  if (verbose > 0) {
+    std::cout << "nb of analyzed gene-SNP pairs: " << n_pairs_done << " (" << n_genes_done << " genes)" << std::endl;
+    std::cout << "END " << prog << "\nelapsed -> " << (long)(time(nullptr) - t_start) << " sec" << std::endl;
+  }
+  return EXIT_SUCCESS;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -482,6 +614,8 @@ int main(int argc, char** argv) {
   const int verbose = o.verbose;
   time_t t_start = time(nullptr);
   if (verbose > 0) std::cout << "START " << argv[0] << "\ncmd-line: " << o.cmdline << std::endl;
+
+  if (!o.inss.empty()) return run_inss(o, t_start, argv[0]);
 
   LoadOptions lo;
   lo.geno_list = o.geno;
@@ -721,16 +855,7 @@ int main(int argc, char** argv) {
     }
     write_all(&ch, &r, &pj, &ps);
   }
-  if (o.kernel_times) {
-    const int32_t kmax = 256;
-    std::vector<char> names((size_t)kmax * 64);
-    std::vector<double> ms(kmax);
-    std::vector<int64_t> launches(kmax);
-    int32_t nk = 0;
-    eng.check(eqbf_profile_read(eng.ctx, names.data(), ms.data(), launches.data(), kmax, &nk));
-    for (int32_t k = 0; k < std::min(nk, kmax); ++k)
-      std::cerr << "kernel\t" << &names[(size_t)k * 64] << "\t" << launches[k] << "\t" << ms[k] << " ms\n";
-  }
+  if (o.kernel_times) print_kernel_times(eng);
   if (verbose > 0) {
     std::cout << "nb of analyzed gene-SNP pairs: " << n_pairs_done << " (" << n_genes_done << " genes)" << std::endl;
     std::cout << "END " << argv[0] << "\nelapsed -> " << (long)(time(nullptr) - t_start) << " sec" << std::endl;

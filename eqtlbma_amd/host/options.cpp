@@ -156,16 +156,24 @@ Options parse_options(int argc, char** argv) {
     else if (n == "ktimes") o.kernel_times = true;
   }
   const char* prog = argv[0];
-  if (!o.inss.empty()) bad(o, prog, "--inss (summary statistics as input) is not supported by this build");
-  if (o.geno.empty()) bad(o, prog, "missing compulsory option --geno");
-  if (!file_exists(o.geno)) bad(o, prog, "can't find " + o.geno);
-  if (!o.scoord.empty() && !file_exists(o.scoord)) bad(o, prog, "can't find " + o.scoord);
-  if (o.exp.empty()) bad(o, prog, "missing compulsory option --exp");
-  if (!file_exists(o.exp)) bad(o, prog, "can't find " + o.exp);
-  if (o.gcoord.empty()) bad(o, prog, "missing compulsory option --gcoord");
-  if (!file_exists(o.gcoord)) bad(o, prog, "can't find " + o.gcoord);
-  if (o.anchor.empty()) bad(o, prog, "SNPs in trans not yet implemented, see --anchor and --cis");
-  if (o.anchor != "TSS" && o.anchor != "TSS+TES") bad(o, prog, "--anchor should be TSS or TSS+TES");
+  if (o.inss.empty()) {
+    if (o.geno.empty()) bad(o, prog, "missing compulsory option --geno");
+    if (!file_exists(o.geno)) bad(o, prog, "can't find " + o.geno);
+    if (!o.scoord.empty() && !file_exists(o.scoord)) bad(o, prog, "can't find " + o.scoord);
+    if (o.exp.empty()) bad(o, prog, "missing compulsory option --exp");
+    if (!file_exists(o.exp)) bad(o, prog, "can't find " + o.exp);
+    if (o.gcoord.empty()) bad(o, prog, "missing compulsory option --gcoord");
+    if (!file_exists(o.gcoord)) bad(o, prog, "can't find " + o.gcoord);
+    if (o.anchor.empty()) bad(o, prog, "SNPs in trans not yet implemented, see --anchor and --cis");
+    if (o.anchor != "TSS" && o.anchor != "TSS+TES") bad(o, prog, "--anchor should be TSS or TSS+TES");
+  } else {
+    if (!file_exists(o.inss)) bad(o, prog, "can't find " + o.inss);
+    if (o.analysis != "join") bad(o, prog, "--inss requires --analys join");
+    if (o.error_model != "uvlr") bad(o, prog, "--inss requires --error uvlr");
+    /* the reference permutes no gene loaded from summary statistics and then
+     * prints the genes' uninitialised permutation results */
+    if (o.nb_permutations > 0) bad(o, prog, "--nperm with --inss is not supported by this build");
+  }
   if (o.out.empty()) bad(o, prog, "missing compulsory option --out");
   if (o.write_group_size < 1) bad(o, prog, "--wrtsize should be greater than 1");
   {

@@ -178,6 +178,32 @@ int eqbf_assoc_mvlr(eqbf_ctx* ctx, const eqbf_chunk* chunk, const eqbf_mvlr_opts
 int eqbf_assoc_mvlr_device(eqbf_ctx* ctx, const eqbf_chunk* chunk_dev, const eqbf_mvlr_opts* opts,
                            eqbf_assoc_out* out_dev, void* hip_stream);
 
+/* ---- summary statistics as input (--inss) ----------------------------------------
+ * Bayes factors of pairs whose summary statistics are given rather than
+ * computed: replaces, for the pairs loadSummaryStats / fillGeneSnpPairsWith-
+ * Sstats build (data_loader.cpp:1250-1343), GeneSnpPair::SetSstats followed
+ * by CalcAbfsUvlr (gene_snp_pair.cpp:241-254, 604-622) inside
+ * Gene::TestForAssociations (gene.cpp:263-332).  Inputs are structure of
+ * arrays [S][n_pairs]; n <= 0 means no result in that subgroup (HasResults,
+ * gene_snp_pair.cpp:61-68).  The small-sample correction uses nu = n - 2: no
+ * covariate count is known for given statistics (:273).  n <= 2 (nu <= 0)
+ * gives a NaN z.  FP64 throughout, one thread per pair.  Outputs as in
+ * eqbf_assoc_uvlr: n (optional, int32), stdsstats (optional), raw_abf (FP64
+ * or FP32 as raw_dtype, optional), w_abf (required); sstats is not written.
+ * At most 64 subgroups and grids of at most 128 points. */
+typedef struct eqbf_sstats_in {
+  int64_t n_pairs;
+  int32_t n_subgroups;
+  const int64_t* n;         /* [S][n_pairs] sample size ("n" column)          */
+  const double*  sigmahat;  /* [S][n_pairs] "sigmahat"                        */
+  const double*  betahat;   /* [S][n_pairs] "betahat.geno"                    */
+  const double*  sebetahat; /* [S][n_pairs] "sebetahat.geno"                  */
+} eqbf_sstats_in;
+int eqbf_abf_from_sstats(eqbf_ctx* ctx, const eqbf_sstats_in* in, int32_t bfs, int32_t raw_dtype,
+                         eqbf_assoc_out* out);
+int eqbf_abf_from_sstats_device(eqbf_ctx* ctx, const eqbf_sstats_in* in_dev, int32_t bfs, int32_t raw_dtype,
+                                eqbf_assoc_out* out_dev, void* hip_stream);
+
 /* ---- permutations ------------------------------------------------------------
  * Replaces makePermutations (eqtlbma_bf.cpp:867-917) and the permutation
  * loops of Gene::MakePermutationsJoin (gene.cpp:598-717),

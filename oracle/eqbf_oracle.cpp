@@ -678,3 +678,47 @@ extern "C" int orc_assoc_uvlr(const eqbf_chunk* c, int32_t analysis, int32_t bfs
   }
   return EQBF_OK;
 }
+
+/* ======================================================================== */
+/* 6. Summary statistics as input (--inss): GeneSnpPair::SetSstats          */
+/*    (gene_snp_pair.cpp:241-254) then CalcAbfsUvlr (:604-622) for pairs   */
+/*    built by fillGeneSnpPairsWithSstats (data_loader.cpp:1250-1327).     */
+/*    Arrays [S][NP]; n <= 0: no result (HasResults, :61-68).  The         */
+/*    covariate count of given statistics is the map default, 0 (:273).    */
+/*    nu = n - 2 <= 0 (n <= 2) is given a NaN z (GSL's t distribution at   */
+/*    nu <= 0 is not restated).                                            */
+/* ======================================================================== */
+extern "C" int orc_abf_from_sstats(int32_t S, int64_t NP, const int64_t* n, const double* sigmahat,
+                                   const double* betahat, const double* sebetahat, int32_t bfs,
+                                   const double* phiL, const double* omaL, int32_t GL, const double* phiS,
+                                   const double* omaS, int32_t GS, double* stdsstats, double* raw, double* w_abf) {
+  std::vector<uint64_t> masks;
+  config_masks(bfs, S, masks);
+  const int64_t nc = (int64_t)masks.size();
+  const int64_t n_raw = 3 * (int64_t)GL + nc * GS, n_w = 3 + nc + 2;
+  std::vector<double> raw1(n_raw), w1(n_w), std3(3 * S);
+  std::vector<int> has(S);
+  for (int64_t p = 0; p < NP; ++p) {
+    for (int32_t s = 0; s < S; ++s) {
+      const int64_t e = (int64_t)s * NP + p;
+      has[s] = n[e] > 0;
+      for (int k = 0; k < 3; ++k) std3[3 * s + k] = kNaN;
+      if (has[s]) {
+        if (n[e] > 2) {
+          orc_standardize((double)n[e], 0.0, sigmahat[e], betahat[e], sebetahat[e], &std3[3 * s]);
+        } else if (!std::isnan((betahat[e] / sigmahat[e]) / (sebetahat[e] / sigmahat[e]))) {
+          std3[3 * s + 0] = 0.0; /* |z| > 1e-8 fails for a NaN z (:278-286) */
+          std3[3 * s + 1] = kInf;
+        }
+      }
+      if (stdsstats)
+        for (int k = 0; k < 3; ++k) stdsstats[(int64_t)k * S * NP + e] = std3[3 * s + k];
+    }
+    pair_abfs(S, bfs, has, std3, phiL, omaL, GL, phiS, omaS, GS, raw1.data(), w1.data());
+    if (raw)
+      for (int64_t k = 0; k < n_raw; ++k) raw[k * NP + p] = raw1[k];
+    if (w_abf)
+      for (int64_t k = 0; k < n_w; ++k) w_abf[k * NP + p] = w1[k];
+  }
+  return 0;
+}

@@ -71,6 +71,8 @@ def lib():
         L.orc_mvlr_sigma.argtypes = [i32, i32, i32, vp, vp, vp, d, vp, vp, vp]
         L.orc_assoc_mvlr.argtypes = [vp, i32, d, vp, vp, i32, vp, vp, i32, vp, vp, vp]
         L.orc_assoc_mvlr.restype = C.c_int
+        L.orc_abf_from_sstats.argtypes = [i32, i64, vp, vp, vp, vp, i32, vp, vp, i32, vp, vp, i32, vp, vp, vp]
+        L.orc_abf_from_sstats.restype = C.c_int
         L.orc_last_error.restype = C.c_char_p
         _lib = L
     return _lib
@@ -176,6 +178,29 @@ def assoc_uvlr(hc, analysis, bfs, gridL, gridS):
     if rc != 0:
         raise RuntimeError("oracle: %s" % lib().orc_last_error().decode())
     return out
+
+
+def abf_from_sstats(n, sigmahat, betahat, sebetahat, bfs, gridL, gridS):
+    """--inss: Bayes factors from given summary statistics, arrays [S][NP]
+    (n <= 0: no result).  Returns (stdsstats [3][S][NP], raw [n_raw][NP],
+    w [n_w][NP]) in FP64."""
+    n = np.ascontiguousarray(n, dtype=np.int64)
+    S, NP = n.shape
+    sig, beta, se = (np.ascontiguousarray(a, dtype=np.float64) for a in (sigmahat, betahat, sebetahat))
+    assert sig.shape == beta.shape == se.shape == (S, NP)
+    gL = np.ascontiguousarray(gridL, dtype=np.float64)
+    gS = np.ascontiguousarray(gridS if gridS is not None else gridL, dtype=np.float64)
+    phiL, omaL = np.ascontiguousarray(gL[:, 0]), np.ascontiguousarray(gL[:, 1])
+    phiS, omaS = np.ascontiguousarray(gS[:, 0]), np.ascontiguousarray(gS[:, 1])
+    nc = lib().orc_config_masks(_abi.BFS[bfs], S, None)
+    std = np.full((3, S, NP), np.nan)
+    raw = np.full((3 * gL.shape[0] + nc * gS.shape[0], NP), np.nan)
+    w = np.full((3 + nc + 2, NP), np.nan)
+    rc = lib().orc_abf_from_sstats(S, NP, _p(n), _p(sig), _p(beta), _p(se), _abi.BFS[bfs], _p(phiL), _p(omaL),
+                                   gL.shape[0], _p(phiS), _p(omaS), gS.shape[0], _p(std), _p(raw), _p(w))
+    if rc != 0:
+        raise RuntimeError("oracle: %s" % lib().orc_last_error().decode())
+    return std, raw, w
 
 
 def assoc_mvlr(hc, bfs, fiterr, gridL, gridS):

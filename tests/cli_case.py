@@ -270,3 +270,104 @@ def expected_chunks(inp, radius, wrtsize, mvlr=False):
 
 def config_labels(bfs, S):
     return ["-".join(str(s + 1) for s in range(S) if (m >> s) & 1) for m in config_masks(bfs, S)]
+
+
+def make_inss_case(root, seed=7):
+    """Summary-statistics inputs for --inss: a list file naming one file per
+    subgroup (one gzipped, one with extra columns in another order), pairs
+    that appear in only some subgroups, a repeated line (the first is kept),
+    n = 0, n = 2, a 'nan' effect and a zero effect.  Returns the list path."""
+    rng = np.random.default_rng(seed)
+    sbgrps = ["tissueB", "tissueA", "tissueC"]  # sorted: tissueA, tissueB, tissueC
+    genes = {"gZ": ["rs%d" % k for k in range(1, 6)], "gA": ["rs%d" % k for k in range(3, 11)],
+             "gM": ["rs%d" % k for k in range(20, 23)]}
+    paths = {}
+    for si, sg in enumerate(sbgrps):
+        lines = []
+        for gname, snps in genes.items():
+            for j, snp in enumerate(snps):
+                if (j + si) % 4 == 3:
+                    continue  # this pair has no line in this subgroup
+                n = int(rng.integers(20, 300))
+                sig = rng.uniform(0.5, 2.0)
+                beta = rng.normal(scale=0.3) * (3.0 if j == 1 else 1.0)
+                se = rng.uniform(0.03, 0.3)
+                lines.append((gname, snp, n, sig, beta, se))
+        rng.shuffle(lines)
+        if si == 0:
+            g, sn, _n, sig, beta, se = lines[0]
+            lines.append((g, sn, 999, sig * 2, -beta, se))  # repeated pair: ignored
+            lines[1] = lines[1][:2] + (0,) + lines[1][3:]
+            lines[2] = lines[2][:2] + (2,) + lines[2][3:]
+        if si == 1:
+            lines[0] = lines[0][:4] + ("nan",) + lines[0][5:]
+            lines[1] = lines[1][:4] + (0.0,) + lines[1][5:]
+        if si == 2:
+            hdr = ["maf", "sebetahat.geno", "snp", "gene", "betahat.geno", "pve", "n", "sigmahat", "betapval.geno"]
+            rows = ["\t".join(hdr)]
+            for g, sn, n, sig, beta, se in lines:
+                d = {"maf": "0.2", "sebetahat.geno": "%.10g" % se, "snp": sn, "gene": g,
+                     "betahat.geno": beta if isinstance(beta, str) else "%.10g" % beta, "pve": "0.01",
+                     "n": str(n), "sigmahat": "%.10g" % sig, "betapval.geno": "0.5"}
+                rows.append("\t".join(d[h] for h in hdr))
+        else:
+            rows = ["gene\tsnp\tn\tsigmahat\tbetahat.geno\tsebetahat.geno"]
+            for g, sn, n, sig, beta, se in lines:
+                rows.append("%s\t%s\t%d\t%.10g\t%s\t%.10g" % (g, sn, n, sig, beta if isinstance(beta, str)
+                                                                else "%.10g" % beta, se))
+        p = os.path.join(root, "ss_%s.txt" % sg) + (".gz" if si == 1 else "")
+        if si == 1:
+            with gzip.open(p, "wt") as f:
+                f.write("\n".join(rows) + "\n")
+        else:
+            _w(p, rows)
+        paths[sg] = p
+    lst = os.path.join(root, "list_ss.txt")
+    _w(lst, ["#subgroup path"] + ["%s %s" % (sg, paths[sg]) for sg in sbgrps])
+    return lst
+
+
+def expected_inss(list_path):
+    """The pairs --inss builds, in the reference's order: genes by name, pairs
+    by first appearance over the subgroups' files (subgroups by name), the
+    first line of a (gene, SNP, subgroup) kept.  Returns (subgroups,
+    [(gene, [snp, ...])], n, sigmahat, betahat, sebetahat [S][NP], present
+    [S][NP])."""
+    files = {}
+    with open(list_path) as f:
+        for l in f:
+            t = l.split()
+            if t and not t[0].startswith("#"):
+                files.setdefault(t[0], t[1])
+    sbgrps = sorted(files)
+    S = len(sbgrps)
+    genes, order = {}, []
+    for s, sg in enumerate(sbgrps):
+        opener = gzip.open if files[sg].endswith(".gz") else open
+        with opener(files[sg], "rt") as f:
+            lines = [l.rstrip("\n") for l in f if l.strip()]
+        hdr = lines[0].split("\t")
+        c = {k: hdr.index(k) for k in ("gene", "snp", "n", "sigmahat", "betahat.geno", "sebetahat.geno")}
+        for l in lines[1:]:
+            t = l.split("\t")
+            g = genes.setdefault(t[c["gene"]], {})
+            if t[c["snp"]] not in g:
+                g[t[c["snp"]]] = [None] * S
+            if g[t[c["snp"]]][s] is None:
+                g[t[c["snp"]]][s] = (int(t[c["n"]]), float(t[c["sigmahat"]]), float(t[c["betahat.geno"]]),
+                                     float(t[c["sebetahat.geno"]]))
+    pairs = [(gname, list(genes[gname].keys())) for gname in sorted(genes)]
+    NP = sum(len(p) for _g, p in pairs)
+    n = np.zeros((S, NP), np.int64)
+    sig, beta, se = (np.full((S, NP), np.nan) for _ in range(3))
+    present = np.zeros((S, NP), bool)
+    p = 0
+    for gname, snps in pairs:
+        for snp in snps:
+            for s in range(S):
+                e = genes[gname][snp][s]
+                if e is not None:
+                    present[s, p] = True
+                    n[s, p], sig[s, p], beta[s, p], se[s, p] = e
+            p += 1
+    return sbgrps, pairs, n, sig, beta, se, present

@@ -64,3 +64,37 @@ def test_no_subgroup_left_writes_nothing(root):
     r = _run(_base(root, out) + ["--analys", "sep", "--sbgrp", "nosuch"], root)
     assert r.returncode == 0, r.stderr
     assert not [f for f in os.listdir(root) if f.startswith("none_")]
+
+
+@pytest.mark.parametrize("extra,msg", [
+    (["--analys", "sep"], "--inss requires --analys join"),
+    (["--analys", "join", "--gridL", "GL", "--error", "mvlr"], "--inss requires --error uvlr"),
+    (["--analys", "join", "--gridL", "GL", "--nperm", "5", "--pbf", "gen"],
+     "--nperm with --inss is not supported by this build"),
+])
+def test_inss_option_checks(root, extra, msg):
+    lst = cli_case.make_inss_case(root)
+    extra = [os.path.join(root, "gridL.txt") if a == "GL" else a for a in extra]
+    r = _run(["--inss", lst, "--out", os.path.join(root, "i"), "-v", "0"] + extra, root)
+    assert r.returncode == 1
+    assert msg in r.stderr
+
+
+def test_inss_missing_column_and_empty(root):
+    bad = os.path.join(root, "ss_bad.txt")
+    with open(bad, "w") as f:
+        f.write("gene\tsnp\tn\tsigmahat\tbetahat.geno\nga\trs1\t10\t1\t0.1\n")
+    lst = os.path.join(root, "list_bad.txt")
+    with open(lst, "w") as f:
+        f.write("s1 %s\n" % bad)
+    args = ["--analys", "join", "--gridL", os.path.join(root, "gridL.txt"), "-v", "0"]
+    r = _run(["--inss", lst, "--out", os.path.join(root, "b")] + args, root)
+    assert r.returncode == 1
+    # the reference prints the missing column's index, std::string::npos
+    assert "missing 18446744073709551615 in header of " + bad in r.stderr
+    # a file with its header only: nothing to analyse, nothing written
+    with open(bad, "w") as f:
+        f.write("gene\tsnp\tn\tsigmahat\tbetahat.geno\tsebetahat.geno\n")
+    r = _run(["--inss", lst, "--out", os.path.join(root, "empty")] + args, root)
+    assert r.returncode == 0, r.stderr
+    assert not [f for f in os.listdir(root) if f.startswith("empty_")]

@@ -250,3 +250,52 @@ def test_cli_join_permutations_contraction_path(tmp_path, orc):
             assert int(got[3]) == want[3]
             assert _num(got[2]) == pytest.approx(want[2], rel=1e-5)
         assert abs(_num(got[4]) - want[4]) <= 1e-4 and abs(_num(got[5]) - want[5]) <= 1e-4
+
+
+def test_cli_inss(tmp_path, orc):
+    # --inss: Bayes factors from summary-statistics files (no genotypes or
+    # expression levels); files read like the reference, values against the
+    # oracle's restatement of SetSstats + CalcAbfsUvlr
+    root = str(tmp_path)
+    lst = cli_case.make_inss_case(root)
+    c = cli_case.make_case(root)  # for the grid files only
+    gL, gS = c["gL"], c["gS"]
+    out = os.path.join(root, "ss")
+    r = _run(["--inss", lst, "--out", out, "--analys", "join", "--gridL", os.path.join(root, "gridL.txt"),
+              "--gridS", os.path.join(root, "gridS.txt"), "--bfs", "all", "--outw", "--outss", "--wrtsize", "2",
+              "-v", "0", "--ktimes"], root)
+    assert "inss_abf_kernel" in r.stderr
+    sbgrps, pairs, n, sig, beta, se, present = cli_case.expected_inss(lst)
+    S = len(sbgrps)
+    labels = cli_case.config_labels("all", S)
+    _std, raw, w = orc.abf_from_sstats(np.where(present, n, 0), sig, beta, se, "all", gL, gS)
+    nc = len(labels)
+    exp_raw, exp_avg = [], []
+    p = 0
+    for gname, snps in pairs:
+        for snp in snps:
+            for k, typ in enumerate(("gen", "gen-fix", "gen-maxh")):
+                exp_raw.append([gname, snp, typ] + list(raw[k * len(gL):(k + 1) * len(gL), p]))
+            for ci, lab in enumerate(labels):
+                r0 = 3 * len(gL) + ci * len(gS)
+                exp_raw.append([gname, snp, lab] + list(raw[r0:r0 + len(gS), p]) + [float("nan")] * (len(gL) - len(gS)))
+            exp_avg.append([gname, snp, int(present[:, p].sum()), w[0, p], w[1, p], w[2, p], w[3 + nc, p],
+                            w[3 + nc + 1, p]] + [w[3 + k, p] for k in range(nc)])
+            p += 1
+    rows = cli_case.read_table(out + "_l10abfs_raw.txt.gz")
+    assert rows[0] == ["gene", "snp", "config"] + ["l10abf.grid%d" % (i + 1) for i in range(len(gL))]
+    assert len(rows) - 1 == len(exp_raw)
+    for got, want in zip(rows[1:], exp_raw):
+        assert got[:3] == want[:3]
+        assert all(_close(_num(a), b) for a, b in zip(got[3:], want[3:])), (got, want)
+    rows = cli_case.read_table(out + "_l10abfs_avg-grids.txt.gz")
+    assert rows[0][8:] == ["l10abf." + l for l in labels]
+    assert len(rows) - 1 == len(exp_avg)
+    for got, want in zip(rows[1:], exp_avg):
+        assert got[:2] == want[:2] and int(got[2]) == want[2]
+        assert all(_close(_num(a), b) for a, b in zip(got[3:], want[3:])), (got, want)
+    # --outss: header only (no gene has expression levels in any subgroup)
+    for sg in sbgrps:
+        rows = cli_case.read_table(out + "_sumstats_%s.txt.gz" % sg)
+        assert rows == [["gene", "snp", "maf", "n", "pve", "sigmahat", "betahat.geno", "sebetahat.geno",
+                         "betapval.geno"]]
