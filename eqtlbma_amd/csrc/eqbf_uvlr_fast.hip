@@ -32,6 +32,7 @@
  */
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "eqbf_internal.h"
@@ -65,6 +66,69 @@ __device__ inline double geno_at(const eqbf_chunk& c, int64_t i, int64_t j) {
   const int64_t off = i * c.geno_ld + j;
   if (c.geno_dtype == EQBF_F32) return (double)((const float*)c.genotypes)[off];
   return ((const double*)c.genotypes)[off];
+}
+
+/* Epilogue of one tile: the sums of SPP subgroups at a time (one pass when
+ * MULTI is false) move through LDS, then one (pair, subgroup) per thread
+ * gets its summary statistics (FitSingleGeneWithSingleSnp,
+ * utils_math.cpp:166-209, and the standardisation of gene_snp_pair.cpp:
+ * 256-290) and the Bayes-factor inputs. */
+template <int NXW, int NMW, bool MULTI>
+__device__ __forceinline__ void sums_epilogue(const eqbf_chunk& c, const SumsParams& sp, double* E, const d4 (&ax)[NXW],
+                                              const d4 (&am)[NMW], int g, int64_t p0, int cnt) {
+  const int S = c.n_subgroups, KC = sp.KC, CX = sp.CX;
+  const int64_t NP = c.n_pairs;
+  const int tid = threadIdx.x, lane = tid & 63, l16 = lane & 15, kq = lane >> 4;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int rb = w & 3, half = w >> 2;
+  const int tx0 = half * NXW, tm0 = half * NMW;
+  /* lane holds D[(lane >> 4) + 4 r][lane & 15] */
+  for (int sa = 0; sa < S; sa += MULTI ? sp.SPP : S) {
+    const int sb = MULTI ? min(S, sa + sp.SPP) : S, ns = sb - sa, NL = ns * KC, ecols = NL + ns;
+#pragma unroll
+    for (int k = 0; k < NXW; ++k) {
+      const int gc = (tx0 + k) * 16 + l16;
+      const int s = gc / KC;
+      if (gc < CX && s >= sa && s < sb) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) E[(rb * 16 + kq + 4 * r) * ecols + (s - sa) * KC + (gc - s * KC)] = ax[k][r];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < NMW; ++k) {
+      const int s = (tm0 + k) * 16 + l16;
+      if (s >= sa && s < sb) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) E[(rb * 16 + kq + 4 * r) * ecols + NL + (s - sa)] = am[k][r];
+      }
+    }
+    __syncthreads();
+    for (int it = tid; it < 64 * ns; it += NT_SUMS) {
+      const int j = it & 63, s = sa + (it >> 6);
+      if (j >= cnt) continue;
+      const double* e = E + j * ecols + (s - sa) * KC;
+      double proj = 0;
+      for (int q = 0; q < sp.KU; ++q) proj += e[1 + q] * e[1 + q];
+      const int64_t p = p0 + j, o = (int64_t)s * NP + p;
+      const int64_t snp = c.pair_snp[p];
+      const PrepRec r = sp.rec[(int64_t)g * S + s];
+      const bool has = r.n > 0 && c.snp_in_sbgrp[snp * S + s];
+      double o5[5], st3[3];
+      const int f = uvlr_stats(r, has, E[j * ecols + NL + (s - sa)], e[0], proj, sp.tz, o5, st3);
+      sp.n_out[o] = has ? r.n : 0;
+      if (sp.sstats)
+        for (int k = 0; k < 5; ++k) sp.sstats[(int64_t)k * S * NP + o] = o5[k];
+      if (sp.stdsstats)
+        for (int k = 0; k < 3; ++k) sp.stdsstats[(int64_t)k * S * NP + o] = st3[k];
+      if (sp.st_f) {
+        sp.st_v[o] = (float)(st3[1] * st3[1]);
+        sp.st_b[o] = (float)st3[0];
+        sp.st_z2[o] = (float)(st3[2] * st3[2]);
+        sp.st_f[o] = (uint8_t)f;
+      }
+    }
+    __syncthreads();
+  }
 }
 
 /* Each wave owns NXW x-tiles (columns y~, U) and NMW m-tiles (membership)
@@ -202,55 +266,119 @@ __global__ void __launch_bounds__(NT_SUMS) uvlr_sums_kernel(const eqbf_chunk c, 
       __syncthreads();
     }
 
-    /* ---- epilogue: sums of SPP subgroups at a time through LDS (one pass
-     * when MULTI is false); lane holds D[(lane >> 4) + 4 r][lane & 15] ---- */
-    double* E = buf;
-    for (int sa = 0; sa < S; sa += MULTI ? sp.SPP : S) {
-      const int sb = MULTI ? min(S, sa + sp.SPP) : S, ns = sb - sa, NL = ns * KC, ecols = NL + ns;
+    sums_epilogue<NXW, NMW, MULTI>(c, sp, buf, ax, am, g, p0, cnt);
+  }
+}
+
+/* The same product with LDS-direct staging (global_load_lds): a chunk of
+ * KCH samples is copied from HBM straight into LDS by the load unit
+ * (B rows as 16-byte units of a contiguous block, genotypes gathered one
+ * sample row per wave instruction, the membership words) while the waves
+ * multiply the previous chunk; one barrier per chunk, no staging registers.
+ * Needs CX even (16-byte units never straddle the gene's block).  Rows past
+ * N in the last chunk are zero-filled so the padded products add nothing. */
+typedef __attribute__((address_space(3))) void lds_void_t;
+
+template <int NXW, int NMW, bool MULTI, bool XF64>
+__global__ void __launch_bounds__(NT_SUMS) uvlr_sums_dma_kernel(const eqbf_chunk c, const SumsParams sp) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  typedef typename std::conditional<XF64, double, float>::type xt;
+  const int N = c.n_samples;
+  const int tid = threadIdx.x, lane = tid & 63, l16 = lane & 15, kq = lane >> 4;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int rb = w & 3, half = w >> 2;
+  const int KCH = sp.KCH, CX = sp.CX, S = c.n_subgroups;
+  const int bbytes = KCH * CX * 8, xbytes = KCH * 64 * (int)sizeof(xt);
+  const int stage = bbytes + xbytes + KCH * 8;
+  const int nchunk = (N + KCH - 1) / KCH;
+  const int tx0 = half * NXW, tm0 = half * NMW;
+
+  const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3, per_xcd = gridDim.x >> 3;
+  const int T = *sp.n_tiles;
+  const int span = (T + 7) / 8;
+  const int t0 = xcd * span, t1 = min(T, t0 + span);
+  for (int t = t0 + slot; t < t1; t += per_xcd) {
+    const int4 tile = sp.tiles[t];
+    const int g = tile.x;
+    const int64_t p0 = tile.y;
+    const int cnt = tile.z;
+    /* genotype column of this lane: SNP lane (FP32), or SNP lane / 2 + 32 h
+     * and 4-byte half lane & 1 (FP64, two instructions per sample row) */
+    const int64_t xs0 = c.pair_snp[p0 + min(XF64 ? (lane >> 1) : lane, cnt - 1)];
+    const int64_t xs1 = XF64 ? c.pair_snp[p0 + min(32 + (lane >> 1), cnt - 1)] : 0;
+    const char* Bg = (const char*)(sp.B + (int64_t)g * N * CX);
+    const char* mg = (const char*)(sp.memb + (int64_t)g * N);
+    d4 ax[NXW], am[NMW];
 #pragma unroll
-      for (int k = 0; k < NXW; ++k) {
-        const int gc = (tx0 + k) * 16 + l16;
-        const int s = gc / KC;
-        if (gc < CX && s >= sa && s < sb) {
+    for (int k = 0; k < NXW; ++k) ax[k] = d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-          for (int r = 0; r < 4; ++r) E[(rb * 16 + kq + 4 * r) * ecols + (s - sa) * KC + (gc - s * KC)] = ax[k][r];
+    for (int k = 0; k < NMW; ++k) am[k] = d4{0.0, 0.0, 0.0, 0.0};
+
+    auto issue = [&](int ck, unsigned char* st) {
+      const int i1 = ck * KCH, rows = min(KCH, N - i1);
+      const int n16 = rows * CX / 2;
+      const char* src = Bg + (int64_t)i1 * CX * 8;
+      for (int u0 = w * 64; u0 < n16; u0 += NT_SUMS)
+        if (u0 + lane < n16)
+          __builtin_amdgcn_global_load_lds((const void*)(src + (int64_t)(u0 + lane) * 16), (lds_void_t*)(st + u0 * 16),
+                                           16, 0, 0);
+      unsigned char* xs = st + bbytes;
+      for (int r = w; r < rows; r += NT_SUMS / 64) {
+        const int64_t row = (int64_t)(i1 + r) * c.geno_ld;
+        if (!XF64) {
+          __builtin_amdgcn_global_load_lds((const void*)((const float*)c.genotypes + row + xs0),
+                                           (lds_void_t*)(xs + r * 256), 4, 0, 0);
+        } else {
+          const char* gp = (const char*)((const double*)c.genotypes + row);
+          __builtin_amdgcn_global_load_lds((const void*)(gp + xs0 * 8 + (lane & 1) * 4),
+                                           (lds_void_t*)(xs + r * 512), 4, 0, 0);
+          __builtin_amdgcn_global_load_lds((const void*)(gp + xs1 * 8 + (lane & 1) * 4),
+                                           (lds_void_t*)(xs + r * 512 + 256), 4, 0, 0);
         }
       }
+      if (w == NT_SUMS / 64 - 1 && lane < 2 * rows)
+        __builtin_amdgcn_global_load_lds((const void*)(mg + (int64_t)i1 * 8 + lane * 4),
+                                         (lds_void_t*)(st + bbytes + xbytes), 4, 0, 0);
+      if (rows < KCH) { /* last chunk: zero rows past N */
+        double* bz = (double*)st;
+        for (int e = rows * CX + tid; e < KCH * CX; e += NT_SUMS) bz[e] = 0.0;
+        xt* xz = (xt*)(st + bbytes);
+        for (int e = rows * 64 + tid; e < KCH * 64; e += NT_SUMS) xz[e] = (xt)0;
+        uint64_t* mz = (uint64_t*)(st + bbytes + xbytes);
+        for (int e = rows + tid; e < KCH; e += NT_SUMS) mz[e] = 0ull;
+      }
+    };
+
+    issue(0, smem);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    for (int ck = 0; ck < nchunk; ++ck) {
+      const unsigned char* cur = smem + (ck & 1) * stage;
+      if (ck + 1 < nchunk) issue(ck + 1, smem + ((ck + 1) & 1) * stage);
+      const double* bs = (const double*)cur;
+      const xt* xs = (const xt*)(cur + bbytes);
+      const uint64_t* mb = (const uint64_t*)(cur + bbytes + xbytes);
+      for (int kk = 0; kk < KCH; kk += 4) {
+        const int krow = kk + kq;
+        const double a = (double)xs[krow * 64 + rb * 16 + l16];
+        const double a2 = a * a;
+        const uint64_t mbits = mb[krow];
+        double bx[NXW];
 #pragma unroll
-      for (int k = 0; k < NMW; ++k) {
-        const int s = (tm0 + k) * 16 + l16;
-        if (s >= sa && s < sb) {
+        for (int k = 0; k < NXW; ++k) bx[k] = bs[krow * CX + (tx0 + k) * 16 + l16];
 #pragma unroll
-          for (int r = 0; r < 4; ++r) E[(rb * 16 + kq + 4 * r) * ecols + NL + (s - sa)] = am[k][r];
+        for (int k = 0; k < NXW; ++k) ax[k] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bx[k], ax[k], 0, 0, 0);
+#pragma unroll
+        for (int k = 0; k < NMW; ++k) {
+          const int s = (tm0 + k) * 16 + l16;
+          const double bm = (s < S && ((mbits >> (s & 63)) & 1ull)) ? 1.0 : 0.0;
+          am[k] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, bm, am[k], 0, 0, 0);
         }
       }
-      __syncthreads();
-      for (int it = tid; it < 64 * ns; it += NT_SUMS) {
-        const int j = it & 63, s = sa + (it >> 6);
-        if (j >= cnt) continue;
-        const double* e = E + j * ecols + (s - sa) * KC;
-        double proj = 0;
-        for (int q = 0; q < sp.KU; ++q) proj += e[1 + q] * e[1 + q];
-        const int64_t p = p0 + j, o = (int64_t)s * NP + p;
-        const int64_t snp = c.pair_snp[p];
-        const PrepRec r = sp.rec[(int64_t)g * S + s];
-        const bool has = r.n > 0 && c.snp_in_sbgrp[snp * S + s];
-        double o5[5], st3[3];
-        const int f = uvlr_stats(r, has, E[j * ecols + NL + (s - sa)], e[0], proj, sp.tz, o5, st3);
-        sp.n_out[o] = has ? r.n : 0;
-        if (sp.sstats)
-          for (int k = 0; k < 5; ++k) sp.sstats[(int64_t)k * S * NP + o] = o5[k];
-        if (sp.stdsstats)
-          for (int k = 0; k < 3; ++k) sp.stdsstats[(int64_t)k * S * NP + o] = st3[k];
-        if (sp.st_f) {
-          sp.st_v[o] = (float)(st3[1] * st3[1]);
-          sp.st_b[o] = (float)st3[0];
-          sp.st_z2[o] = (float)(st3[2] * st3[2]);
-          sp.st_f[o] = (uint8_t)f;
-        }
-      }
+      __builtin_amdgcn_s_waitcnt(0);
       __syncthreads();
     }
+    sums_epilogue<NXW, NMW, MULTI>(c, sp, (double*)smem, ax, am, g, p0, cnt);
   }
 }
 
@@ -472,23 +600,33 @@ __global__ void __launch_bounds__(NT_ABF) uvlr_abf_kernel(const int S, const int
   }
 }
 
-template <int NXW, int NMW>
-void launch_sums(const eqbf_chunk& c, const SumsParams& sp, int grid, size_t lds, hipStream_t st) {
-  if (sp.SPP >= c.n_subgroups)
-    hipLaunchKernelGGL((uvlr_sums_kernel<NXW, NMW, false>), dim3(grid), dim3(NT_SUMS), lds, st, c, sp);
+template <int NXW, int NMW, bool MULTI>
+void launch_sums_m(const eqbf_chunk& c, const SumsParams& sp, int grid, size_t lds, hipStream_t st, bool dma) {
+  if (!dma)
+    hipLaunchKernelGGL((uvlr_sums_kernel<NXW, NMW, MULTI>), dim3(grid), dim3(NT_SUMS), lds, st, c, sp);
+  else if (c.geno_dtype == EQBF_F32)
+    hipLaunchKernelGGL((uvlr_sums_dma_kernel<NXW, NMW, MULTI, false>), dim3(grid), dim3(NT_SUMS), lds, st, c, sp);
   else
-    hipLaunchKernelGGL((uvlr_sums_kernel<NXW, NMW, true>), dim3(grid), dim3(NT_SUMS), lds, st, c, sp);
+    hipLaunchKernelGGL((uvlr_sums_dma_kernel<NXW, NMW, MULTI, true>), dim3(grid), dim3(NT_SUMS), lds, st, c, sp);
+}
+
+template <int NXW, int NMW>
+void launch_sums(const eqbf_chunk& c, const SumsParams& sp, int grid, size_t lds, hipStream_t st, bool dma) {
+  if (sp.SPP >= c.n_subgroups)
+    launch_sums_m<NXW, NMW, false>(c, sp, grid, lds, st, dma);
+  else
+    launch_sums_m<NXW, NMW, true>(c, sp, grid, lds, st, dma);
 }
 
 /* column tiles per wave: x-tiles padded to one of a few counts */
 template <int NMW>
-int dispatch_sums(int nxw, const eqbf_chunk& c, const SumsParams& sp, int grid, size_t lds, hipStream_t st) {
-  if (nxw <= 2) launch_sums<2, NMW>(c, sp, grid, lds, st);
-  else if (nxw <= 4) launch_sums<4, NMW>(c, sp, grid, lds, st);
-  else if (nxw <= 7) launch_sums<7, NMW>(c, sp, grid, lds, st);
-  else if (nxw <= 10) launch_sums<10, NMW>(c, sp, grid, lds, st);
-  else if (nxw <= 14) launch_sums<14, NMW>(c, sp, grid, lds, st);
-  else if (nxw <= 18) launch_sums<18, NMW>(c, sp, grid, lds, st);
+int dispatch_sums(int nxw, const eqbf_chunk& c, const SumsParams& sp, int grid, size_t lds, hipStream_t st, bool dma) {
+  if (nxw <= 2) launch_sums<2, NMW>(c, sp, grid, lds, st, dma);
+  else if (nxw <= 4) launch_sums<4, NMW>(c, sp, grid, lds, st, dma);
+  else if (nxw <= 7) launch_sums<7, NMW>(c, sp, grid, lds, st, dma);
+  else if (nxw <= 10) launch_sums<10, NMW>(c, sp, grid, lds, st, dma);
+  else if (nxw <= 14) launch_sums<14, NMW>(c, sp, grid, lds, st, dma);
+  else if (nxw <= 18) launch_sums<18, NMW>(c, sp, grid, lds, st, dma);
   else return 1;
   return 0;
 }
@@ -540,13 +678,19 @@ int uvlr_fast_run(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_assoc_opts* o, 
    * CU.  With the autotuner on, the first call of each shape times the
    * candidates on the chunk itself (every candidate makes the same sums in
    * the same order) and keeps the fastest for the context's lifetime. */
-  int kch_max = (int)((80 * 1024) / (2 * 8 * (size_t)(65 + CXPP)));
-  kch_max = std::min(16, kch_max & ~3);
-  if (kch_max < 4) return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED, "too many subgroup columns for the batched path");
-  const size_t epi_unit = (size_t)64 * 8 * (sp.KC + 1);
-  auto shape_lds = [&](int kch, int spp) {
-    return std::max((size_t)2 * 8 * kch * (65 + CXPP), epi_unit * spp);
+  /* LDS-direct staging when the coefficient rows split into 16-byte units */
+  const bool dma = (sp.CX % 2) == 0;
+  const size_t xsz = c.geno_dtype == EQBF_F32 ? 4 : 8;
+  auto stage_bytes = [&](int kch) {
+    return dma ? (size_t)kch * ((size_t)sp.CX * 8 + 64 * xsz + 8) : (size_t)8 * kch * (65 + CXPP);
   };
+  int kch_max = 4;
+  while (kch_max < 32 && 2 * stage_bytes(kch_max + 4) <= (dma ? 144 : 80) * 1024) kch_max += 4;
+  if (!dma) kch_max = std::min(16, kch_max);
+  if (2 * stage_bytes(kch_max) > 160 * 1024)
+    return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED, "too many subgroup columns for the batched path");
+  const size_t epi_unit = (size_t)64 * 8 * (sp.KC + 1);
+  auto shape_lds = [&](int kch, int spp) { return std::max(2 * stage_bytes(kch), epi_unit * spp); };
   eqbf_ctx::TuneChoice choice;
   choice.kch = kch_max;
   choice.spp = (int)std::min<size_t>(S, std::max(shape_lds(kch_max, 0), std::min<size_t>(epi_unit * S, 160 * 1024)) /
@@ -558,16 +702,17 @@ int uvlr_fast_run(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_assoc_opts* o, 
     q.SPP = ch.spp;
     const size_t lds = shape_lds(ch.kch, ch.spp);
     const int grid = (ctx->n_cu * ch.grid_mult + 7) / 8 * 8;
-    return nmw <= 1 ? dispatch_sums<1>(nxw, c, q, grid, lds, st) : dispatch_sums<2>(nxw, c, q, grid, lds, st);
+    return nmw <= 1 ? dispatch_sums<1>(nxw, c, q, grid, lds, st, dma) : dispatch_sums<2>(nxw, c, q, grid, lds, st, dma);
   };
   char key[96];
-  snprintf(key, sizeof key, "sums/%d/%d/%d/%d/%d", S, sp.KC, nxw, nmw, c.n_samples);
+  snprintf(key, sizeof key, "sums/%d/%d/%d/%d/%d/%s", S, sp.KC, nxw, nmw, c.n_samples,
+           dma ? (xsz == 4 ? "dma32" : "dma64") : "regs");
   auto tk = ctx->tuned.find(key);
   if (tk != ctx->tuned.end()) {
     choice = tk->second;
   } else if (ctx->autotune_on && !ctx->capturing) {
     std::vector<eqbf_ctx::TuneChoice> cands;
-    for (int kch : {kch_max, std::max(4, (kch_max / 2) & ~3)})
+    for (int kch : {kch_max, std::max(4, (kch_max / 2) & ~3), std::max(4, (kch_max / 4) & ~3)})
       for (int spp : {choice.spp, (S + 1) / 2, (S + 3) / 4})
         for (int gm : {1, 2}) {
           eqbf_ctx::TuneChoice ch;
