@@ -283,15 +283,16 @@ __device__ inline double tz_eval(const double* __restrict__ tz, int nu, double a
  * o5: pve, sigmahat, betahat, sebetahat, p; st3: standardised b, se, z.
  * Returns the subgroup's role in the Bayes factors: */
 enum { F_ABSENT = 0, F_NORMAL = 1, F_ZERO = 2, F_NAN = 3 };
-__device__ inline int uvlr_stats(const PrepRec& r, bool has, double sxx, double sxy, double proj,
-                                 const double* __restrict__ tz, double (&o5)[5], double (&st3)[3]) {
+/* xx: x~'x~ = x'x - |U'x|^2 over the members, NaN when the genotype is
+ * collinear with [1, covariates] (not estimable). */
+__device__ inline int uvlr_stats_xx(const PrepRec& r, bool has, double xx, double sxy, const double* __restrict__ tz,
+                                    double (&o5)[5], double (&st3)[3]) {
   for (int k = 0; k < 5; ++k) o5[k] = NAN;
   for (int k = 0; k < 3; ++k) st3[k] = NAN;
   if (!has) return F_ABSENT;
   if (r.flags & PREP_TOO_FEW) return F_NAN;
-  const double xx = sxx - proj;
   const int n = r.n, P = r.q + 2;
-  if (!(xx > 1e-12 * sxx)) { /* genotype collinear with [1, covariates]: not estimable */
+  if (isnan(xx)) { /* genotype collinear with [1, covariates]: not estimable */
     o5[0] = 1.0 - r.yy / r.tss;
     o5[1] = sqrt(r.yy / (double)(n - r.rank));
     return F_NAN;
@@ -324,6 +325,15 @@ __device__ inline int uvlr_stats(const PrepRec& r, bool has, double sxx, double 
   st3[1] = INFINITY;
   st3[2] = z;
   return F_ZERO;
+}
+/* x~'x~ from the sums, NaN when not estimable (below 1e-12 of x'x). */
+__device__ inline double uvlr_xx(double sxx, double proj) {
+  const double xx = sxx - proj;
+  return xx > 1e-12 * sxx ? xx : NAN;
+}
+__device__ inline int uvlr_stats(const PrepRec& r, bool has, double sxx, double sxy, double proj,
+                                 const double* __restrict__ tz, double (&o5)[5], double (&st3)[3]) {
+  return uvlr_stats_xx(r, has, uvlr_xx(sxx, proj), sxy, tz, o5, st3);
 }
 
 /* ---- kernels launched from the C-ABI layer ------------------------------- */

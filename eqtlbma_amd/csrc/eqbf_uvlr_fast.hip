@@ -59,6 +59,8 @@ struct SumsParams {
   float* st_b;   /* [S][NP] standardised effect b'                      */
   float* st_z2;  /* [S][NP] z^2                                         */
   uint8_t* st_f; /* [S][NP] role of the subgroup (F_*)                  */
+  double* sx2;   /* [2][S][NP] x~'x~ (NaN: not estimable) and x'y~, when the
+                    statistics are a kernel of their own                  */
   int KU, KC, CX, NXT, KCH, SPP;
 };
 
@@ -73,7 +75,7 @@ __device__ inline double geno_at(const eqbf_chunk& c, int64_t i, int64_t j) {
  * gets its summary statistics (FitSingleGeneWithSingleSnp,
  * utils_math.cpp:166-209, and the standardisation of gene_snp_pair.cpp:
  * 256-290) and the Bayes-factor inputs. */
-template <int NXW, int NMW, bool MULTI>
+template <int NXW, int NMW, bool MULTI, bool SPLIT>
 __device__ __forceinline__ void sums_epilogue(const eqbf_chunk& c, const SumsParams& sp, double* E, const d4 (&ax)[NXW],
                                               const d4 (&am)[NMW], int g, int64_t p0, int cnt) {
   const int S = c.n_subgroups, KC = sp.KC, CX = sp.CX;
@@ -110,6 +112,11 @@ __device__ __forceinline__ void sums_epilogue(const eqbf_chunk& c, const SumsPar
       double proj = 0;
       for (int q = 0; q < sp.KU; ++q) proj += e[1 + q] * e[1 + q];
       const int64_t p = p0 + j, o = (int64_t)s * NP + p;
+      if (SPLIT) {
+        sp.sx2[o] = uvlr_xx(E[j * ecols + NL + (s - sa)], proj);
+        sp.sx2[(int64_t)S * NP + o] = e[0];
+        continue;
+      }
       const int64_t snp = c.pair_snp[p];
       const PrepRec r = sp.rec[(int64_t)g * S + s];
       const bool has = r.n > 0 && c.snp_in_sbgrp[snp * S + s];
@@ -266,7 +273,7 @@ __global__ void __launch_bounds__(NT_SUMS) uvlr_sums_kernel(const eqbf_chunk c, 
       __syncthreads();
     }
 
-    sums_epilogue<NXW, NMW, MULTI>(c, sp, buf, ax, am, g, p0, cnt);
+    sums_epilogue<NXW, NMW, MULTI, false>(c, sp, buf, ax, am, g, p0, cnt);
   }
 }
 
@@ -378,7 +385,46 @@ __global__ void __launch_bounds__(NT_SUMS) uvlr_sums_dma_kernel(const eqbf_chunk
       __builtin_amdgcn_s_waitcnt(0);
       __syncthreads();
     }
-    sums_epilogue<NXW, NMW, MULTI>(c, sp, (double*)smem, ax, am, g, p0, cnt);
+    sums_epilogue<NXW, NMW, MULTI, true>(c, sp, (double*)smem, ax, am, g, p0, cnt);
+  }
+}
+
+/* Summary statistics of every (pair, subgroup) of a tile from x~'x~ and
+ * x'y~ (FitSingleGeneWithSingleSnp, utils_math.cpp:166-209, and
+ * StandardizeSstatsAndCorrectSmallSampleSize, gene_snp_pair.cpp:256-290),
+ * with the outputs and the Bayes-factor inputs; one workgroup per tile. */
+constexpr int NT_STATS = 256;
+__global__ void __launch_bounds__(NT_STATS) __attribute__((amdgpu_waves_per_eu(4, 8)))
+uvlr_stats_kernel(const eqbf_chunk c, const SumsParams sp) {
+  /* XCD-aware order (blockIdx & 7 = XCD): contiguous tile ranges per XCD */
+  const int T = *sp.n_tiles, per = (T + 7) / 8;
+  const int t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  if (t >= T) return;
+  const int4 tile = sp.tiles[t];
+  const int g = tile.x, cnt = tile.z;
+  const int64_t p0 = tile.y;
+  const int S = c.n_subgroups;
+  const int64_t NP = c.n_pairs;
+  for (int it = threadIdx.x; it < 64 * S; it += NT_STATS) {
+    const int j = it & 63, s = it >> 6;
+    if (j >= cnt) continue;
+    const int64_t p = p0 + j, o = (int64_t)s * NP + p;
+    const int64_t snp = c.pair_snp[p];
+    const PrepRec r = sp.rec[(int64_t)g * S + s];
+    const bool has = r.n > 0 && c.snp_in_sbgrp[snp * S + s];
+    double o5[5], st3[3];
+    const int f = uvlr_stats_xx(r, has, sp.sx2[o], sp.sx2[(int64_t)S * NP + o], sp.tz, o5, st3);
+    sp.n_out[o] = has ? r.n : 0;
+    if (sp.sstats)
+      for (int k = 0; k < 5; ++k) sp.sstats[(int64_t)k * S * NP + o] = o5[k];
+    if (sp.stdsstats)
+      for (int k = 0; k < 3; ++k) sp.stdsstats[(int64_t)k * S * NP + o] = st3[k];
+    if (sp.st_f) {
+      sp.st_v[o] = (float)(st3[1] * st3[1]);
+      sp.st_b[o] = (float)st3[0];
+      sp.st_z2[o] = (float)(st3[2] * st3[2]);
+      sp.st_f[o] = (uint8_t)f;
+    }
   }
 }
 
@@ -392,6 +438,7 @@ struct PhiEntry {
   float oma2;
 };
 
+constexpr int ABF_GMAX = 256; /* small-grid points of the batched Bayes factors */
 struct AbfParams {
   const float* st_v;
   const float* st_b;
@@ -404,6 +451,9 @@ struct AbfParams {
   void* raw;
   double* w;
   int n_jobs, n_cfg, GL, GS, raw_f64;
+  /* small grid as FP32 in the kernel arguments (read through the scalar
+   * cache): phi2 at [0, GS), omega2 at [ABF_GMAX, ABF_GMAX + GS) */
+  float gS[2 * ABF_GMAX];
 };
 
 __device__ inline float lg10f(float x) { return __log2f(x) * 0.30102999566398120f; }
@@ -453,9 +503,9 @@ __device__ inline double acc_finish(float m, double s, double weight) {
   return res;
 }
 
+template <bool RAW64>
 __device__ inline void put_raw(const AbfParams& ap, int64_t row, int64_t NP, int64_t p, float l) {
-  if (!ap.raw) return;
-  if (ap.raw_f64)
+  if (RAW64)
     ((double*)ap.raw)[row * NP + p] = l;
   else
     ((float*)ap.raw)[row * NP + p] = l;
@@ -463,41 +513,50 @@ __device__ inline void put_raw(const AbfParams& ap, int64_t row, int64_t NP, int
 
 constexpr int NT_ABF = 256;
 
-/* Bayes factors of 64 pairs, one lane per pair (four waves share the jobs). */
+/* Bayes factors of 64 pairs, one lane per pair (four waves share the jobs).
+ * Per (subgroup, pair) the LDS holds {v, b, z^2 / (2 ln 10), c} with c = 1
+ * for a subgroup that enters the Bayes factors and {1, 0, 0, 0} otherwise,
+ * so the sums over subgroups run without branches (an absent or zero-z
+ * subgroup adds exact zeros).  Lanes past the end of the pairs repeat the
+ * last pair (same inputs, same values, same addresses).  WITH_RAW: the raw
+ * ABFs are stored (FP64 when RAW64). */
+template <bool WITH_RAW, bool RAW64>
 __global__ void __launch_bounds__(NT_ABF) uvlr_abf_kernel(const int S, const int64_t NP, const AbfParams ap) {
   extern __shared__ __align__(16) unsigned char sm2[];
-  float* lv = (float*)sm2;        /* [S][64] standardised variance s'^2 */
-  float* lb = lv + S * 64;        /* [S][64] standardised effect b'     */
-  float* lz = lb + S * 64;        /* [S][64] z^2                        */
-  uint8_t* lf = (uint8_t*)(lz + S * 64);
-  double* part = (double*)(sm2 + ((S * 64 * 13 + 15) / 16) * 16); /* [4 waves][3 types][2][64] */
-  float* gsm = (float*)(part + 4 * 3 * 2 * 64); /* small grid as FP32: phi2[GS], omega2[GS] */
+  float4* l4 = (float4*)sm2;                      /* [S][64]                         */
+  uint8_t* lf = (uint8_t*)(l4 + S * 64);          /* [S][64] role of the subgroup    */
+  double* part = (double*)sm2;                    /* [4 waves][3][2][64], after jobs */
   const int tid = threadIdx.x, lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int64_t p0 = (int64_t)blockIdx.x * 64;
+  /* XCD-aware order: the workgroups of one XCD (blockIdx & 7) take one
+   * contiguous range of 64-pair blocks, so that the raw rows each L2 writes
+   * back are contiguous runs rather than every eighth 256-byte piece */
+  const int nb = (int)((NP + 63) / 64), per = (nb + 7) / 8;
+  const int blk = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  if (blk >= nb) return;
+  const int64_t p0 = (int64_t)blk * 64;
   const int cnt = (int)min((int64_t)64, NP - p0);
   for (int e = tid; e < S * 64; e += NT_ABF) {
     const int s = e >> 6, j = e & 63;
     const int64_t o = (int64_t)s * NP + p0 + min(j, cnt - 1);
-    lv[e] = ap.st_v[o];
-    lb[e] = ap.st_b[o];
-    lz[e] = ap.st_z2[o];
-    lf[e] = ap.st_f[o];
+    const uint8_t f = ap.st_f[o];
+    lf[e] = f;
+    l4[e] = f == F_NORMAL ? make_float4(ap.st_v[o], ap.st_b[o], kHalfInvLn10f * ap.st_z2[o], 1.f)
+                          : make_float4(1.f, 0.f, 0.f, 0.f);
   }
-  for (int e = tid; e < 2 * ap.GS; e += NT_ABF) gsm[e] = (float)ap.grid[2 * ap.GL + e];
   __syncthreads();
-  const bool valid = lane < cnt;
-  const int64_t p = p0 + (valid ? lane : 0);
-  /* nan: an active subgroup with NaN statistics sets every ABF to 0 */
-  bool any_nan = false;
-  for (int s = 0; s < S; ++s) any_nan |= lf[s * 64 + lane] == F_NAN;
+  const int64_t p = p0 + min(lane, cnt - 1);
+  /* subgroups with NaN statistics: an active one sets every ABF to 0 */
+  uint64_t nanm = 0;
+  for (int s = 0; s < S; ++s)
+    if (lf[s * 64 + lane] == F_NAN) nanm |= 1ull << s;
+  const bool any_nan = nanm != 0;
+  const float4* lq = l4 + lane;
 
   Acc ta0, ta1, ta2;
   ta0.init();
   ta1.init();
   ta2.init();
-  const float* phiS = gsm; /* LDS: uniform reads, no memory latency per grid point */
-  const float* omaS = gsm + ap.GS;
   const int njobs = ap.n_jobs + ap.n_cfg;
   for (int j = w; j < njobs; j += NT_ABF / 64) {
     if (j < ap.n_jobs) {
@@ -505,18 +564,28 @@ __global__ void __launch_bounds__(NT_ABF) uvlr_abf_kernel(const int S, const int
       const PhiJob jb = ap.jobs[j];
       const float p2 = jb.phi2;
       float L = 0.f, A = 0.f, Bs = 0.f;
-      for (int s = 0; s < S; ++s) {
-        if (lf[s * 64 + lane] != F_NORMAL) continue;
-        const float v = lv[s * 64 + lane];
-        const float r = __builtin_amdgcn_rcpf(v + p2);
-        if (p2 != 0.f) L += 0.5f * lg10f(v * r) + kHalfInvLn10f * lz[s * 64 + lane] * p2 * r;
-        A += lb[s * 64 + lane] * r;
-        Bs += r;
+      if (p2 != 0.f) {
+#pragma unroll 4
+        for (int s = 0; s < S; ++s) {
+          const float4 q = lq[s * 64];
+          const float r = __builtin_amdgcn_rcpf(q.x + p2);
+          L += q.w * (0.5f * lg10f(q.x * r)) + q.z * p2 * r;
+          A += q.y * r;
+          Bs += q.w * r;
+        }
+      } else {
+#pragma unroll 4
+        for (int s = 0; s < S; ++s) {
+          const float4 q = lq[s * 64];
+          const float r = __builtin_amdgcn_rcpf(q.x);
+          A += q.y * r;
+          Bs += q.w * r;
+        }
       }
       for (int e = jb.e0; e < jb.e1; ++e) {
         const PhiEntry en = ap.ents[e];
         const float l = any_nan ? 0.f : abf_fin(L, A, Bs, en.oma2);
-        if (valid) put_raw(ap, (int64_t)en.type * ap.GL + en.gi, NP, p, l);
+        if (WITH_RAW) put_raw<RAW64>(ap, (int64_t)en.type * ap.GL + en.gi, NP, p, l);
         if (en.type == 0)
           ta0.add(l);
         else if (en.type == 1)
@@ -528,58 +597,68 @@ __global__ void __launch_bounds__(NT_ABF) uvlr_abf_kernel(const int S, const int
       const int cfg = j - ap.n_jobs;
       const uint64_t mk = ap.masks[cfg];
       const int64_t row0 = 3 * (int64_t)ap.GL + (int64_t)cfg * ap.GS;
-      Acc a;
-      a.init();
+      float am = -INFINITY, as = 0.f;
       if (__popcll(mk) == 1) {
-        /* singleton: bbar = b, V = v + phi2, and the log terms telescope */
+        /* singleton: bbar = b, V = v + phi2, and the log terms telescope;
+         * blocks of 8 grid points share one rescaling of the running sum */
         const int s = __ffsll((unsigned long long)mk) - 1;
-        const int f = lf[s * 64 + lane];
-        const float v = lv[s * 64 + lane], z2 = lz[s * 64 + lane];
-        Acc a1;
-        a1.init();
-#pragma unroll 2
-        for (int gi = 0; gi < ap.GS; ++gi) {
-          float l = 0.f;
-          const float p2 = phiS[gi], o2 = omaS[gi];
-          if (f == F_NORMAL) {
-            const float vp = v + p2, va = vp + o2;
-            const float r1 = __builtin_amdgcn_rcpf(vp), r2 = __builtin_amdgcn_rcpf(va);
-            l = 0.5f * lg10f(v * r2) + kHalfInvLn10f * z2 * (p2 * r1 + v * o2 * r1 * r2);
+        const float4 q = lq[s * 64];
+        const bool on = q.w != 0.f;
+        const float v = q.x, hz = q.z;
+        for (int g0 = 0; g0 < ap.GS; g0 += 8) {
+          float lb[8];
+          float bm = -INFINITY;
+#pragma unroll
+          for (int i = 0; i < 8; ++i) {
+            const int gi = g0 + i;
+            float l = -INFINITY;
+            if (gi < ap.GS) {
+              const float p2 = ap.gS[gi], o2 = ap.gS[ABF_GMAX + gi];
+              const float vp = v + p2, va = vp + o2;
+              const float r1 = __builtin_amdgcn_rcpf(vp), r2 = __builtin_amdgcn_rcpf(va);
+              l = on ? 0.5f * lg10f(v * r2) + hz * (p2 * r1 + v * o2 * r1 * r2) : 0.f;
+              if (WITH_RAW) put_raw<RAW64>(ap, row0 + gi, NP, p, l);
+            }
+            lb[i] = l;
+            bm = fmaxf(bm, l);
           }
-          if (valid) put_raw(ap, row0 + gi, NP, p, l);
-          if (gi & 1)
-            a1.add(l);
-          else
-            a.add(l);
+          const float M = fmaxf(am, bm);
+          float acc = as * exp2f((am - M) * kLog2_10);
+#pragma unroll
+          for (int i = 0; i < 8; ++i) acc += exp2f((lb[i] - M) * kLog2_10);
+          as = acc;
+          am = M;
         }
-        a.merge(a1);
       } else {
-        bool nan = false;
-        for (int s = 0; s < S; ++s)
-          if (((mk >> s) & 1ull) && lf[s * 64 + lane] == F_NAN) nan = true;
+        const bool nan = (mk & nanm) != 0ull;
+        Acc a;
+        a.init();
         for (int gi = 0; gi < ap.GS; ++gi) {
-          const float p2 = phiS[gi], o2 = omaS[gi];
+          const float p2 = ap.gS[gi], o2 = ap.gS[ABF_GMAX + gi];
           float L = 0.f, A = 0.f, Bs = 0.f;
           uint64_t m = mk;
           while (m) {
             const int s = __ffsll((unsigned long long)m) - 1;
             m &= m - 1ull;
-            if (lf[s * 64 + lane] != F_NORMAL) continue;
-            const float v = lv[s * 64 + lane];
-            const float r = __builtin_amdgcn_rcpf(v + p2);
-            if (p2 != 0.f) L += 0.5f * lg10f(v * r) + kHalfInvLn10f * lz[s * 64 + lane] * p2 * r;
-            A += lb[s * 64 + lane] * r;
-            Bs += r;
+            const float4 q = lq[s * 64];
+            const float r = __builtin_amdgcn_rcpf(q.x + p2);
+            if (p2 != 0.f) L += q.w * (0.5f * lg10f(q.x * r)) + q.z * p2 * r;
+            A += q.y * r;
+            Bs += q.w * r;
           }
           const float l = nan ? 0.f : abf_fin(L, A, Bs, o2);
-          if (valid) put_raw(ap, row0 + gi, NP, p, l);
+          if (WITH_RAW) put_raw<RAW64>(ap, row0 + gi, NP, p, l);
           a.add(l);
         }
+        am = a.m;
+        as = a.s;
       }
-      if (valid) ap.w[(int64_t)(3 + cfg) * NP + p] = acc_finish(a.m, (double)a.s, 1.0 / ap.GS);
+      ap.w[(int64_t)(3 + cfg) * NP + p] = acc_finish(am, (double)as, 1.0 / ap.GS);
     }
   }
-  /* combine the consistent-configuration partial sums of the four waves */
+  /* combine the consistent-configuration partial sums of the four waves
+   * (part lies over the subgroup records, which are no longer read) */
+  __syncthreads();
   part[((w * 3 + 0) * 2) * 64 + lane] = (double)ta0.m;
   part[((w * 3 + 0) * 2 + 1) * 64 + lane] = (double)ta0.s;
   part[((w * 3 + 1) * 2) * 64 + lane] = (double)ta1.m;
@@ -587,7 +666,7 @@ __global__ void __launch_bounds__(NT_ABF) uvlr_abf_kernel(const int S, const int
   part[((w * 3 + 2) * 2) * 64 + lane] = (double)ta2.m;
   part[((w * 3 + 2) * 2 + 1) * 64 + lane] = (double)ta2.s;
   __syncthreads();
-  if (w < 3 && valid) {
+  if (w < 3) {
     const int k = w;
     double M = -INFINITY;
     for (int ww = 0; ww < NT_ABF / 64; ++ww) M = fmax(M, part[((ww * 3 + k) * 2) * 64 + lane]);
@@ -678,8 +757,15 @@ int uvlr_fast_run(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_assoc_opts* o, 
    * CU.  With the autotuner on, the first call of each shape times the
    * candidates on the chunk itself (every candidate makes the same sums in
    * the same order) and keeps the fastest for the context's lifetime. */
-  /* LDS-direct staging when the coefficient rows split into 16-byte units */
+  /* LDS-direct staging when the coefficient rows split into 16-byte units;
+   * the statistics are then a kernel of their own (the sums kernel keeps
+   * its registers for the products) */
   const bool dma = (sp.CX % 2) == 0;
+  const size_t max_tiles = (size_t)(NP / 64) + (size_t)G + 1;
+  if (dma) {
+    sp.sx2 = (double*)ctx_buf(ctx, "sums_x2", (size_t)2 * S * std::max<int64_t>(NP, 1) * sizeof(double));
+    if (!sp.sx2) return EQBF_ERR_HIP; /* ctx_buf has set the message */
+  }
   const size_t xsz = c.geno_dtype == EQBF_F32 ? 4 : 8;
   auto stage_bytes = [&](int kch) {
     return dma ? (size_t)kch * ((size_t)sp.CX * 8 + 64 * xsz + 8) : (size_t)8 * kch * (65 + CXPP);
@@ -751,6 +837,11 @@ int uvlr_fast_run(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_assoc_opts* o, 
     if (launch(choice)) return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED, "too many subgroup columns for the batched path");
   }
   EQBF_HIP(ctx, hipGetLastError());
+  if (dma) {
+    ProfScope ps(ctx, "uvlr_stats_kernel", st);
+    hipLaunchKernelGGL(uvlr_stats_kernel, dim3((unsigned)((max_tiles + 7) / 8 * 8)), dim3(NT_STATS), 0, st, c, sp);
+    EQBF_HIP(ctx, hipGetLastError());
+  }
   if (!join) return EQBF_OK;
 
   /* jobs of the consistent configuration: distinct phi2 of gen, gen-fix,
@@ -786,6 +877,8 @@ int uvlr_fast_run(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_assoc_opts* o, 
     ctx->jobs_n = (int)jobs.size();
     ctx->jobs_epoch = ctx->buf_epoch;
   }
+  if (GS > ABF_GMAX)
+    return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED, "mixed path supports small grids of at most %d points", ABF_GMAX);
   AbfParams ap;
   memset(&ap, 0, sizeof ap);
   ap.st_v = sp.st_v;
@@ -803,12 +896,21 @@ int uvlr_fast_run(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_assoc_opts* o, 
   ap.GL = GL;
   ap.GS = GS;
   ap.raw_f64 = o->raw_dtype == EQBF_F64;
-  const size_t lds2 = ((size_t)(S * 64 * 13 + 15) / 16) * 16 + (size_t)4 * 3 * 2 * 64 * 8 + (size_t)2 * GS * 4;
+  for (int k = 0; k < GS; ++k) {
+    ap.gS[k] = (float)ctx->phiS[k];
+    ap.gS[ABF_GMAX + k] = (float)ctx->omaS[k];
+  }
+  const size_t lds2 = std::max((size_t)S * 64 * 17, (size_t)4 * 3 * 2 * 64 * 8);
   {
     ProfScope ps(ctx, "uvlr_abf_kernel", st);
-    hipLaunchKernelGGL(uvlr_abf_kernel, dim3((unsigned)((NP + 63) / 64)), dim3(NT_ABF), lds2, st, S, NP, ap);
+    const dim3 grid((unsigned)(((NP + 63) / 64 + 7) / 8 * 8));
+    if (!out->raw_abf)
+      hipLaunchKernelGGL((uvlr_abf_kernel<false, false>), grid, dim3(NT_ABF), lds2, st, S, NP, ap);
+    else if (ap.raw_f64)
+      hipLaunchKernelGGL((uvlr_abf_kernel<true, true>), grid, dim3(NT_ABF), lds2, st, S, NP, ap);
+    else
+      hipLaunchKernelGGL((uvlr_abf_kernel<true, false>), grid, dim3(NT_ABF), lds2, st, S, NP, ap);
   }
   EQBF_HIP(ctx, hipGetLastError());
-  (void)G;
   return EQBF_OK;
 }
