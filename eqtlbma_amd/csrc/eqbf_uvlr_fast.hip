@@ -287,7 +287,8 @@ __global__ void __launch_bounds__(NT_SUMS) uvlr_sums_kernel(const eqbf_chunk c, 
 typedef __attribute__((address_space(3))) void lds_void_t;
 
 template <int NXW, int NMW, bool MULTI, bool XF64>
-__global__ void __launch_bounds__(NT_SUMS) uvlr_sums_dma_kernel(const eqbf_chunk c, const SumsParams sp) {
+__global__ void __launch_bounds__(NT_SUMS) __attribute__((amdgpu_waves_per_eu(4)))
+uvlr_sums_dma_kernel(const eqbf_chunk c, const SumsParams sp) {
   extern __shared__ __align__(16) unsigned char smem[];
   typedef typename std::conditional<XF64, double, float>::type xt;
   const int N = c.n_samples;
@@ -394,6 +395,65 @@ __global__ void __launch_bounds__(NT_SUMS) uvlr_sums_dma_kernel(const eqbf_chunk
  * StandardizeSstatsAndCorrectSmallSampleSize, gene_snp_pair.cpp:256-290),
  * with the outputs and the Bayes-factor inputs; one workgroup per tile. */
 constexpr int NT_STATS = 256;
+
+/* AS241 rational approximations evaluated in FP32 (the coefficients of
+ * dm::as241_central / dm::as241_tail): relative error ~1e-7. */
+__device__ inline float as241_central_f(float q) {
+  const float r = 0.180625f - q * q;
+  return q * (((((((r * 2509.0809287301226727f + 33430.575583588128105f) * r + 67265.770927008700853f) * r +
+                  45921.953931549871457f) * r + 13731.693765509461125f) * r + 1971.5909503065514427f) * r +
+               133.14166789178437745f) * r + 3.387132872796366608f) /
+         (((((((r * 5226.495278852545561f + 28729.085735721942674f) * r + 39307.89580009271061f) * r +
+              21213.794301586595867f) * r + 5394.1960214247511077f) * r + 687.1870074920579083f) * r +
+           42.313330701600911252f) * r + 1.0f);
+}
+__device__ inline float as241_tail_f(float r) {
+  if (r <= 5.0f) {
+    r -= 1.6f;
+    return (((((((r * 7.7454501427834140764e-4f + 0.0227238449892691845833f) * r + 0.24178072517745061177f) * r +
+                1.27045825245236838258f) * r + 3.64784832476320460504f) * r + 5.7694972214606914055f) * r +
+             4.6303378461565452959f) * r + 1.42343711074968357734f) /
+           (((((((r * 1.05075007164441684324e-9f + 5.475938084995344946e-4f) * r + 0.0151986665636164571966f) * r +
+                0.14810397642748007459f) * r + 0.68976733498510000455f) * r + 1.6763848301838038494f) * r +
+             2.05319162663775882187f) * r + 1.0f);
+  }
+  r -= 5.0f;
+  return (((((((r * 2.01033439929228813265e-7f + 2.71155556874348757815e-5f) * r + 0.0012426609473880784386f) * r +
+              0.026532189526576123093f) * r + 0.29656057182850489123f) * r + 1.7848265399172913358f) * r +
+           5.4637849111641143699f) * r + 6.6579046435011037772f) /
+         (((((((r * 2.04426310338993978564e-15f + 1.4215117583164458887e-7f) * r + 1.8463183175100546818e-5f) * r +
+              7.868691311456132591e-4f) * r + 0.0148753612908506148525f) * r + 0.13692988092273580531f) * r +
+           0.59983220655588793769f) * r + 1.0f);
+}
+
+/* The FP32 Bayes-factor inputs without the FP64 outputs: beta and the
+ * residual sum of squares in FP64, then (exact identities of the reference's
+ * expressions, gene_snp_pair.cpp:256-290) s'^2 = se^2 / sigma^2 =
+ * (n - rank) / ((n - P) x~'x~), b' = sign(beta) |z| s', and z from the t tail
+ * table with the normal quantile in FP32.  Returns false where the FP64
+ * path must decide: residual sum of squares <= 0, |t| < 1e-3 (the |z| > 1e-8
+ * test), or a tail past the table. */
+__device__ inline bool uvlr_bf_inputs_fast(const PrepRec& r, double xx, double sxy, const double* __restrict__ tz,
+                                           float& v, float& b, float& z2) {
+  const int n = r.n, P = r.q + 2, rank = r.rank + 1;
+  const double beta = sxy / xx;
+  const double rss = r.yy - sxy * beta;
+  if (!(rss > 0.0)) return false;
+  const float at = fabsf((float)beta) * sqrtf((float)xx * (float)(n - P) / (float)rss);
+  if (!(at >= 1e-3f)) return false;
+  const double w = dm::tz_eval(tz, n - P, (double)at);
+  if (!(w < INFINITY)) return false;
+  const float wf = (float)w;
+  const float az = wf < 2.5902671654458267f ? -as241_central_f(__expf(-wf) - 0.5f) : as241_tail_f(sqrtf(wf));
+  v = (float)((double)(n - rank) / ((double)(n - P) * xx));
+  b = copysignf(az * sqrtf(v), (float)beta);
+  z2 = az * az;
+  return true;
+}
+
+/* FULL: the FP64 summary statistics are written too (sstats or stdsstats
+ * requested), so every item takes the FP64 path. */
+template <bool FULL>
 __global__ void __launch_bounds__(NT_STATS) __attribute__((amdgpu_waves_per_eu(4, 8)))
 uvlr_stats_kernel(const eqbf_chunk c, const SumsParams sp) {
   /* XCD-aware order (blockIdx & 7 = XCD): contiguous tile ranges per XCD */
@@ -412,12 +472,23 @@ uvlr_stats_kernel(const eqbf_chunk c, const SumsParams sp) {
     const int64_t snp = c.pair_snp[p];
     const PrepRec r = sp.rec[(int64_t)g * S + s];
     const bool has = r.n > 0 && c.snp_in_sbgrp[snp * S + s];
-    double o5[5], st3[3];
-    const int f = uvlr_stats_xx(r, has, sp.sx2[o], sp.sx2[(int64_t)S * NP + o], sp.tz, o5, st3);
+    const double xx = sp.sx2[o], sxy = sp.sx2[(int64_t)S * NP + o];
     sp.n_out[o] = has ? r.n : 0;
-    if (sp.sstats)
+    if (!FULL && sp.st_f && has && !(r.flags & PREP_TOO_FEW) && !isnan(xx)) {
+      float v, b, z2;
+      if (uvlr_bf_inputs_fast(r, xx, sxy, sp.tz, v, b, z2)) {
+        sp.st_v[o] = v;
+        sp.st_b[o] = b;
+        sp.st_z2[o] = z2;
+        sp.st_f[o] = (uint8_t)F_NORMAL;
+        continue;
+      }
+    }
+    double o5[5], st3[3];
+    const int f = uvlr_stats_xx(r, has, xx, sxy, sp.tz, o5, st3);
+    if (FULL && sp.sstats)
       for (int k = 0; k < 5; ++k) sp.sstats[(int64_t)k * S * NP + o] = o5[k];
-    if (sp.stdsstats)
+    if (FULL && sp.stdsstats)
       for (int k = 0; k < 3; ++k) sp.stdsstats[(int64_t)k * S * NP + o] = st3[k];
     if (sp.st_f) {
       sp.st_v[o] = (float)(st3[1] * st3[1]);
@@ -782,6 +853,17 @@ int uvlr_fast_run(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_assoc_opts* o, 
   choice.spp = (int)std::min<size_t>(S, std::max(shape_lds(kch_max, 0), std::min<size_t>(epi_unit * S, 160 * 1024)) /
                                             epi_unit);
   choice.grid_mult = 2;
+  if (dma) {
+    /* default: two workgroups per CU (the kernel is built for 4 waves per
+     * SIMD), i.e. at most 80 KB of LDS each */
+    int k2 = 4;
+    while (k2 < 32 && 2 * stage_bytes(k2 + 4) <= 80 * 1024) k2 += 4;
+    const int s2 = (int)std::min<size_t>(S, (80 * 1024) / epi_unit);
+    if (2 * stage_bytes(k2) <= 80 * 1024 && s2 >= 1) {
+      choice.kch = k2;
+      choice.spp = s2;
+    }
+  }
   auto launch = [&](const eqbf_ctx::TuneChoice& ch) -> int {
     SumsParams q = sp;
     q.KCH = ch.kch;
@@ -839,7 +921,11 @@ int uvlr_fast_run(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_assoc_opts* o, 
   EQBF_HIP(ctx, hipGetLastError());
   if (dma) {
     ProfScope ps(ctx, "uvlr_stats_kernel", st);
-    hipLaunchKernelGGL(uvlr_stats_kernel, dim3((unsigned)((max_tiles + 7) / 8 * 8)), dim3(NT_STATS), 0, st, c, sp);
+    const dim3 sgrid((unsigned)((max_tiles + 7) / 8 * 8));
+    if (sp.sstats || sp.stdsstats)
+      hipLaunchKernelGGL(uvlr_stats_kernel<true>, sgrid, dim3(NT_STATS), 0, st, c, sp);
+    else
+      hipLaunchKernelGGL(uvlr_stats_kernel<false>, sgrid, dim3(NT_STATS), 0, st, c, sp);
     EQBF_HIP(ctx, hipGetLastError());
   }
   if (!join) return EQBF_OK;

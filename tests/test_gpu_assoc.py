@@ -162,3 +162,30 @@ def test_large_sample_prep_path(eng, orc, mode):
     hc = synth.batched(seed=21, S=3, N_all=2000, n_genes=2, p_mean=12, p_lo=5, p_hi=20,
                        ns_lo=1600, ns_hi=2000, Q=3, nan_frac=0.02)
     _run(eng, orc, hc, "all", GL25, GS10, mode, tag="large-N")
+
+
+@pytest.mark.parametrize("bfs", ["sin", "all"])
+def test_mixed_bayes_factors_only(eng, orc, bfs):
+    # without FP64 statistic outputs the Bayes-factor inputs come from the
+    # FP32 identities s'^2 = (n - rank) / ((n - P) x~'x~), b' = sign(beta)
+    # |z| s' (FP64 where |t| < 1e-3, the residual sum of squares is not
+    # positive, or the tail runs past the table): the Bayes factors stay
+    # within 1e-4 of the oracle's; includes strong effects (far tails), an
+    # exactly null effect and subgroups with too few samples
+    hc = synth.batched(seed=31, S=6, N_all=150, n_genes=5, p_mean=60, p_lo=10, p_hi=120,
+                       ns_lo=4, ns_hi=140, Q=2, nan_frac=0.02)  # 6 x (1 + 4) coefficient columns: LDS-direct path
+    g = hc.genotypes[0]
+    y = hc.explevels
+    y[0, :, :] += 4.0 * g[:, hc.pair_snp[hc.pair_offset[0]]][None, :]  # |t| in the tens
+    y[1, :, :] = 0.5 + 1e-9 * np.arange(hc.N)[None, :]                # near-null effects
+    eng.set_grids(GL25, GS10)
+    _say("[gpu] assoc mixed without statistic outputs bfs=%s pairs=%d" % (bfs, hc.n_pairs))
+    gpu = eng.assoc_uvlr(hc, "join", bfs, precision="mixed", want_sstats=False, want_std=False)
+    ref = orc.assoc_uvlr(hc, "join", bfs, GL25, GS10)
+    assert np.array_equal(gpu.n, ref.n)
+    for a, b, name in [(gpu.raw.astype(np.float64), ref.raw, "raw"), (gpu.w, ref.w, "weighted")]:
+        _nan_same(a, b, name)
+        fin = np.isfinite(b)
+        err = np.abs(a[fin] - b[fin])
+        _say("[gpu]   %s max abs err %.2e" % (name, err.max(initial=0)))
+        assert err.max(initial=0) <= 1e-4, name
