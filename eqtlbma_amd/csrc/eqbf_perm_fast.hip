@@ -283,7 +283,7 @@ struct PfGemmArgs {
   float* zeta;   /* join: [nbp][S][NPP]  */
   double* tmax;  /* sep:  [nb][S][NPP] max t^2 over SNPs (-1: none) */
   int* fallback; /* [nb] */
-  int S, N, Npad, nks, nu;
+  int S, N, Npad, nks, nu, n_items;
   int64_t nperm, NPP;
 };
 
@@ -296,7 +296,12 @@ __global__ void __launch_bounds__(256) pf_gemm_kernel(const PfGemmArgs a) {
   v4i* Bf = (v4i*)(sm + (size_t)a.Npad * 4);
   __shared__ double redt[4][16];
   __shared__ __align__(16) float zts[MODE == PF_SEP ? 4 : ZT_ROW];
-  const int4 it = a.items[blockIdx.x];
+  /* XCD-aware order (blockIdx & 7 = XCD): contiguous item ranges per XCD, so
+   * that neighbouring permutation blocks write their z rows through one L2 */
+  const int nit = a.n_items, per = (nit + 7) / 8;
+  const int itx = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  if (itx >= nit) return;
+  const int4 it = a.items[itx];
   const int bi = it.x, pb = it.y, s = it.z;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int N = a.N;
@@ -373,11 +378,25 @@ __global__ void __launch_bounds__(256) pf_gemm_kernel(const PfGemmArgs a) {
     for (int d = 0; d < 4; ++d) acc[d] = v4i{0, 0, 0, 0};
     const int64_t arow = P0 + min((int64_t)t * 16 + (lane & 15), np - 1);
     const v4i* ap = (const v4i*)(a.G8 + arow * a.Npad) + (lane >> 4);
-    /* four 64-sample steps per round: their A loads are in flight together */
-    for (int ks0 = 0; ks0 < a.nks; ks0 += 4) {
-      v4i af[4];
+    /* the epilogue's per-SNP inputs of this lane's four rows, loaded ahead of
+     * the products so that their latency overlaps the matrix work */
+    uint8_t fl4[4];
+    double sx4[4], sc4[4]; /* sx, and rs_xx (join) or inv_xx (sep) */
 #pragma unroll
-      for (int u = 0; u < 4; ++u) af[u] = ks0 + u < a.nks ? ap[(ks0 + u) * 4] : v4i{0, 0, 0, 0};
+    for (int r = 0; r < 4; ++r) {
+      const int64_t pr = P0 + min((int64_t)t * 16 + 4 * (lane >> 4) + r, np - 1);
+      fl4[r] = a.pflag[pr];
+      sx4[r] = a.ps[pr].sx;
+      sc4[r] = MODE == PF_SEP ? a.ps[pr].inv_xx : a.ps[pr].rs_xx;
+    }
+    /* four 64-sample steps per round; the next round's A loads are issued
+     * before this round's products */
+    v4i af[4], an[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) af[u] = u < a.nks ? ap[u * 4] : v4i{0, 0, 0, 0};
+    for (int ks0 = 0; ks0 < a.nks; ks0 += 4) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) an[u] = ks0 + 4 + u < a.nks ? ap[(ks0 + 4 + u) * 4] : v4i{0, 0, 0, 0};
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         if (ks0 + u < a.nks) {
@@ -387,20 +406,21 @@ __global__ void __launch_bounds__(256) pf_gemm_kernel(const PfGemmArgs a) {
                                                            0, 0);
         }
       }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) af[u] = an[u];
     }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int64_t q = (int64_t)t * 16 + 4 * (lane >> 4) + r;
       if (q >= np || !pvalid) continue;
       const int64_t pr = P0 + q;
-      if (a.pflag[pr]) continue; /* no p-value, no z (the statistic's kernels handle these SNPs) */
-      const PStat x = a.ps[pr];
+      if (fl4[r]) continue; /* no p-value, no z (the statistic's kernels handle these SNPs) */
       const double xy = (double)acc[0][r] + 256.0 * (double)acc[1][r] + 65536.0 * (double)acc[2][r] +
                         16777216.0 * (double)acc[3][r];
-      const double sxy = (xy - x.sx * y.tsum * invN) * y.scale;
+      const double sxy = (xy - sx4[r] * y.tsum * invN) * y.scale;
       if (MODE == PF_SEP) {
         /* R^2 = 1 - rss / y'y; the residual sum of squares stays in FP64 */
-        const double r2 = sxy * sxy * x.inv_xx * inv_yy;
+        const double r2 = sxy * sxy * sc4[r] * inv_yy;
         if (!(1.0 - r2 >= 0.0)) { /* rss < 0: NaN statistic */
           if (q == 0) fb = true;  /* a NaN first statistic: general path */
           continue;
@@ -409,7 +429,7 @@ __global__ void __launch_bounds__(256) pf_gemm_kernel(const PfGemmArgs a) {
       } else {
         /* R = |x'y| / sqrt(x'x y'y); |t| / sqrt(nu) = R / sqrt(1 - R^2), so
          * s = asinh(|t| / sqrt(nu)) = atanh(R); 1 - R in FP64 */
-        const double R = fabs(sxy) * x.rs_xx * rs_yy;
+        const double R = fabs(sxy) * sc4[r] * rs_yy;
         const double omR = 1.0 - R;
         float zeta;
         if (!(omR > 0.0)) {
@@ -967,13 +987,14 @@ int perm_fast_run(eqbf_ctx* ctx, const eqbf_chunk* cp, const eqbf_perm_opts* po,
     ga.nu = nu;
     ga.nperm = nperm;
     ga.NPP = NPP;
+    ga.n_items = (int)gitems.size();
     const size_t lds = (size_t)Npad * 4 + (size_t)nks * 4 * 64 * 16;
     {
       ProfScope ps(ctx, "pf_gemm_kernel", st);
       if (join)
-        hipLaunchKernelGGL(pf_gemm_kernel<PF_JOIN_ALL>, dim3((unsigned)gitems.size()), dim3(256), lds, st, ga);
+        hipLaunchKernelGGL(pf_gemm_kernel<PF_JOIN_ALL>, dim3((unsigned)((gitems.size() + 7) / 8 * 8)), dim3(256), lds, st, ga);
       else
-        hipLaunchKernelGGL(pf_gemm_kernel<PF_SEP>, dim3((unsigned)gitems.size()), dim3(256), lds, st, ga);
+        hipLaunchKernelGGL(pf_gemm_kernel<PF_SEP>, dim3((unsigned)((gitems.size() + 7) / 8 * 8)), dim3(256), lds, st, ga);
     }
     EQBF_HIP(ctx, hipGetLastError());
     if (join) {
