@@ -421,7 +421,7 @@ def perm_alloc_leg(args, torch, result):
     """Learned permutation allocator at scale: trained on null draws of 2,000
     genes x 10,000 permutations from the permutation engine, then applied to
     20,000 genes (predicted p > 0.2: 1,000 permutations, else 10,000), with
-    1,000 of them also permuted fully to judge the p-values and eGene calls."""
+    1,000 of them also permuted fully to check the p-values and eGene calls."""
     from eqtlbma_amd import _abi, allocator, synth
     from eqtlbma_amd.engine import Engine, PermOpts, Rng, perm_pvalue
     gL, gS = synth.make_grid("general"), synth.make_grid("small")

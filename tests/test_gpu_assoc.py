@@ -5,7 +5,7 @@ Tolerances (north-star precision statement):
     within 1e-9 absolute (both sides FP64; the oracle factorises in long
     double).
   mixed path: betahat within 1e-6 of max(|betahat|, se) (relative error with
-    the standard error as floor, so that betahat ~ 0 is not judged by a
+    the standard error as floor, so that betahat ~ 0 is not assessed by a
     meaningless ratio), se / sigmahat / p-value within 1e-6 relative, pve
     within 1e-8 absolute, log10 ABFs (raw and grid-averaged) within 1e-4
     absolute.
