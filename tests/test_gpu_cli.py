@@ -299,3 +299,58 @@ def test_cli_inss(tmp_path, orc):
         rows = cli_case.read_table(out + "_sumstats_%s.txt.gz" % sg)
         assert rows == [["gene", "snp", "maf", "n", "pve", "sigmahat", "betahat.geno", "sebetahat.geno",
                          "betapval.geno"]]
+
+
+def test_cli_qnorm(tmp_path, orc):
+    # --qnorm: every (gene, subgroup) expression vector over its members (an
+    # expression level and a genotype column) replaced by normal quantiles of
+    # its ranks, (rank - a) / (n + 1 - 2a) with a = 3/8 for n <= 10 else 1/2
+    # (utils_math.cpp:80-96); continuous values, so no ties.  The expected
+    # side transforms with scipy's ndtri and runs the oracle's association.
+    from scipy import special
+    import copy
+    root = str(tmp_path)
+    c = cli_case.make_case(root, seed=17, S=3, N=36, covariates=False)
+    inp = cli_case.load_expected_inputs(c)
+    out = os.path.join(root, "qn")
+    _run(_common_args(root, out, "join") + ["--bfs", "all", "--outss", "--outw", "--qnorm"], root)
+    q = copy.deepcopy(inp)
+    for gname, per in q["explev"].items():
+        for s, y in per.items():
+            mem = np.nonzero(~np.isnan(y) & (q["geno_in"][s] > 0))[0]
+            n = mem.size
+            if n == 0:
+                continue
+            a = 0.375 if n <= 10 else 0.5
+            order = np.argsort(y[mem], kind="stable")
+            z = np.empty(n)
+            z[order] = special.ndtri((np.arange(1, n + 1) - a) / (n + 1 - 2 * a))
+            y2 = y.copy()
+            y2[mem] = z
+            per[s] = y2
+    S = len(q["sbgrps"])
+    gL, gS = c["gL"], c["gS"]
+    labels = cli_case.config_labels("all", S)
+    exp_ss = {s: [] for s in range(S)}
+    exp_avg = []
+    for names, hc, psn in cli_case.expected_chunks(q, 2500, 2):
+        ref = orc.assoc_uvlr(hc, "join", "all", gL, gS)
+        nc = len(labels)
+        for g, gname in enumerate(names):
+            for p in range(hc.pair_offset[g], hc.pair_offset[g + 1]):
+                for s in range(S):
+                    if ref.n[s, p] > 0:
+                        exp_ss[s].append([gname, psn[p], ref.n[s, p]] + list(ref.sstats[:, s, p]))
+                exp_avg.append([gname, psn[p], int((ref.n[:, p] > 0).sum()), ref.w[0, p], ref.w[1, p], ref.w[2, p],
+                                ref.w[3 + nc, p], ref.w[3 + nc + 1, p]] + [ref.w[3 + k, p] for k in range(nc)])
+    for s, sg in enumerate(q["sbgrps"]):
+        rows = cli_case.read_table(out + "_sumstats_%s.txt.gz" % sg)
+        assert len(rows) - 1 == len(exp_ss[s])
+        for got, want in zip(rows[1:], exp_ss[s]):
+            assert got[:2] == want[:2] and int(got[3]) == want[2]
+            assert all(_close(_num(a), b) for a, b in zip(got[4:], want[3:])), (got, want)
+    rows = cli_case.read_table(out + "_l10abfs_avg-grids.txt.gz")
+    assert len(rows) - 1 == len(exp_avg)
+    for got, want in zip(rows[1:], exp_avg):
+        assert got[:2] == want[:2] and int(got[2]) == want[2]
+        assert all(_close(_num(a), b) for a, b in zip(got[3:], want[3:])), (got, want)
