@@ -611,7 +611,8 @@ def main():
         torch.cuda.synchronize()
         log("calibration copy: %d MiB read + %d MiB written" % (args.calib_copy_mib, args.calib_copy_mib))
         del src, dst
-    opts = _abi.AssocOpts(_abi.ANALYSIS["join"], _abi.BFS["sin"], _abi.PREC["mixed"], _abi.F32)
+    # raw ABFs in FP32, 64-pair tiles (each workgroup writes one contiguous block)
+    opts = _abi.AssocOpts(_abi.ANALYSIS["join"], _abi.BFS["sin"], _abi.PREC["mixed"], _abi.F32_TILED)
     o = _abi.AssocOut()
     o.n, o.sstats, o.stdsstats = out_n.data_ptr(), None, None
     o.raw_abf, o.w_abf = out_raw.data_ptr(), out_w.data_ptr()
@@ -706,7 +707,8 @@ def main():
                                "individuals, cis SNPs~Pois(1500) in [200,6000], FP32 dosages, 3 covariates, "
                                "--bfs sin, 50-point grid" % args.genes,
                    "pairs": pairs, "genes": args.genes, "subgroups": S, "samples": N,
-                   "grid_points": GL, "chunk_genes": args.batch_genes},
+                   "grid_points": GL, "chunk_genes": args.batch_genes,
+                   "outputs": "sample sizes, FP32 raw ABFs (64-pair tiles), FP64 grid-averaged ABFs"},
         "roofline": roof,
         "roofline_other_kernels": others,
         "kernels_ms_per_launch": {k: v[0] / max(v[1], 1) for k, v in kern.items()},

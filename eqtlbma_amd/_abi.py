@@ -19,6 +19,19 @@ BFS = {"gen": 0, "sin": 1, "all": 2}
 PBF = {"gen": 0, "gen-sin": 1, "all": 2}
 PREC = {"fp64_plain": 0, "plain": 0, "mixed": 1}
 F32, F64 = 0, 1
+F32_TILED = 2  # raw ABFs only: FP32 in tiles of 64 pairs
+
+
+def untile_raw(flat, n_raw, n_pairs):
+    """[n_raw][n_pairs] view of raw ABFs written as EQBF_F32_TILED: element
+    (row, p) at (p & ~63) * n_raw + row * w + (p & 63), w = min(64, n_pairs -
+    (p & ~63))."""
+    flat = np.asarray(flat).ravel()
+    out = np.empty((n_raw, n_pairs), dtype=flat.dtype)
+    for tb in range(0, n_pairs, 64):
+        w = min(64, n_pairs - tb)
+        out[:, tb:tb + w] = flat[tb * n_raw:tb * n_raw + n_raw * w].reshape(n_raw, w)
+    return out
 
 
 class Chunk(C.Structure):

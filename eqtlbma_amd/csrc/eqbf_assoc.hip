@@ -57,7 +57,7 @@ struct AssocParams {
   void* raw;
   double* w;
   int* flag;
-  int GL, GS, n_cfg, n_w, KU, bfs, join, raw_f64, multiset, CH;
+  int GL, GS, n_cfg, n_w, KU, bfs, join, raw_f64, raw_tiled, multiset, CH;
 };
 
 __device__ inline double geno_load(const eqbf_chunk& c, int set, int64_t i, int64_t j) {
@@ -510,7 +510,7 @@ __device__ inline double l10_finish(float m, double s, bool nan, double weight) 
 }
 
 __device__ inline void store_raw(const AssocParams& kp, int64_t row, int64_t NP, int64_t p, float l) {
-  const int64_t ro = row * NP + p;
+  const int64_t ro = raw_index(row, p, NP, 3 * (int64_t)kp.GL + (int64_t)kp.n_cfg * kp.GS, kp.raw_tiled);
   if (kp.raw_f64)
     ((double*)kp.raw)[ro] = l;
   else
@@ -936,7 +936,8 @@ __device__ __forceinline__ void plain_pair_abfs(const AssocParams& kp, int S, in
       double o2 = type == 0 ? omaL[gi] : (type == 1 ? phiL[gi] + omaL[gi] : 0.0);
       buf[gi] = abf_plain(all, S, st_b, st_se, st_t, has, p2, o2);
       if (kp.raw) {
-        int64_t ro = (int64_t)(type * kp.GL + gi) * NP + p;
+        const int64_t ro = raw_index(type * kp.GL + gi, p, NP, 3 * (int64_t)kp.GL + (int64_t)kp.n_cfg * kp.GS,
+                                     kp.raw_tiled);
         if (kp.raw_f64)
           ((double*)kp.raw)[ro] = buf[gi];
         else
@@ -952,7 +953,8 @@ __device__ __forceinline__ void plain_pair_abfs(const AssocParams& kp, int S, in
     for (int gi = 0; gi < kp.GS; ++gi) {
       buf[gi] = any ? abf_plain(mk, S, st_b, st_se, st_t, has, phiS[gi], omaS[gi]) : 0.0;
       if (kp.raw) {
-        int64_t ro = (int64_t)(3 * kp.GL + cfg * kp.GS + gi) * NP + p;
+        const int64_t ro = raw_index(3 * kp.GL + cfg * kp.GS + gi, p, NP,
+                                     3 * (int64_t)kp.GL + (int64_t)kp.n_cfg * kp.GS, kp.raw_tiled);
         if (kp.raw_f64)
           ((double*)kp.raw)[ro] = buf[gi];
         else
@@ -1144,6 +1146,8 @@ int assoc_uvlr_run(eqbf_ctx* ctx, const eqbf_chunk* cp, const eqbf_assoc_opts* o
     return ctx_fail(ctx, EQBF_ERR_ARG, "chunk sizes out of range (S=%d N=%d G=%d Q=%d)", S, N, G, c.n_covars_max);
   if (!out || !out->n || (o->analysis == EQBF_ANALYSIS_JOIN && !out->w_abf))
     return ctx_fail(ctx, EQBF_ERR_ARG, "missing output buffers");
+  if (o->raw_dtype != EQBF_F32 && o->raw_dtype != EQBF_F64 && o->raw_dtype != EQBF_F32_TILED)
+    return ctx_fail(ctx, EQBF_ERR_ARG, "invalid raw_dtype %d", o->raw_dtype);
   const bool join = o->analysis == EQBF_ANALYSIS_JOIN;
   const int GL = (int)ctx->phiL.size(), GS = (int)ctx->phiS.size();
   if (join && GL < 1) return ctx_fail(ctx, EQBF_ERR_ARG, "--analys join needs the large grid (eqbf_set_grids)");
@@ -1233,6 +1237,7 @@ int assoc_uvlr_run(eqbf_ctx* ctx, const eqbf_chunk* cp, const eqbf_assoc_opts* o
   kp.bfs = o->bfs;
   kp.join = join;
   kp.raw_f64 = o->raw_dtype == EQBF_F64;
+  kp.raw_tiled = o->raw_dtype == EQBF_F32_TILED;
   kp.multiset = c.n_geno_sets > 1;
 
   if (o->precision == EQBF_PREC_FP64_PLAIN) {
@@ -1315,7 +1320,8 @@ int abf_from_sstats_run(eqbf_ctx* ctx, const eqbf_sstats_in* in, int bfs, int ra
   if (S > PLAIN_SMAX) return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED, "at most %d subgroups", PLAIN_SMAX);
   if (bfs != EQBF_BFS_GEN && bfs != EQBF_BFS_SIN && bfs != EQBF_BFS_ALL)
     return ctx_fail(ctx, EQBF_ERR_ARG, "invalid bfs %d", bfs);
-  if (raw_dtype != EQBF_F32 && raw_dtype != EQBF_F64) return ctx_fail(ctx, EQBF_ERR_ARG, "invalid raw_dtype");
+  if (raw_dtype != EQBF_F32 && raw_dtype != EQBF_F64 && raw_dtype != EQBF_F32_TILED)
+    return ctx_fail(ctx, EQBF_ERR_ARG, "invalid raw_dtype");
   const int GL = (int)ctx->phiL.size(), GS = (int)ctx->phiS.size();
   if (GL < 1) return ctx_fail(ctx, EQBF_ERR_ARG, "the large grid is empty (eqbf_set_grids)");
   if (bfs != EQBF_BFS_GEN && GS < 1) return ctx_fail(ctx, EQBF_ERR_ARG, "the small grid is empty (eqbf_set_grids)");
@@ -1358,6 +1364,7 @@ int abf_from_sstats_run(eqbf_ctx* ctx, const eqbf_sstats_in* in, int bfs, int ra
   kp.bfs = bfs;
   kp.join = 1;
   kp.raw_f64 = raw_dtype == EQBF_F64;
+  kp.raw_tiled = raw_dtype == EQBF_F32_TILED;
   {
     ProfScope ps(ctx, "inss_abf_kernel", st);
     hipLaunchKernelGGL(inss_abf_kernel, dim3((unsigned)((NP + 63) / 64)), dim3(64), 0, st, *in, kp);

@@ -276,6 +276,14 @@ __device__ inline double tz_eval(const double* __restrict__ tz, int nu, double a
 
 }  // namespace dm
 
+/* Index of raw ABF (row, p) of n_raw rows and NP pairs: [n_raw][NP], or in
+ * tiles of 64 pairs (EQBF_F32_TILED). */
+__device__ inline int64_t raw_index(int64_t row, int64_t p, int64_t NP, int64_t n_raw, bool tiled) {
+  if (!tiled) return row * NP + p;
+  const int64_t tb = p & ~(int64_t)63, tw = NP - tb < 64 ? NP - tb : 64;
+  return tb * n_raw + row * tw + (p & 63);
+}
+
 /* ---- per (pair, subgroup) summary statistics -------------------------------
  * From the sums x'x, x'y~ and |U'x|^2 over the members:
  * FitSingleGeneWithSingleSnp (utils_math.cpp:166-209) then

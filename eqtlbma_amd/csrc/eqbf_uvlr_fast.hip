@@ -521,7 +521,8 @@ struct AbfParams {
   const double* grid; /* phiL, omaL, phiS, omaS */
   void* raw;
   double* w;
-  int n_jobs, n_cfg, GL, GS, raw_f64;
+  int n_jobs, n_cfg, GL, GS, raw_f64, raw_tiled;
+  int64_t n_raw;
   /* small grid as FP32 in the kernel arguments (read through the scalar
    * cache): phi2 at [0, GS), omega2 at [ABF_GMAX, ABF_GMAX + GS) */
   float gS[2 * ABF_GMAX];
@@ -579,7 +580,7 @@ __device__ inline void put_raw(const AbfParams& ap, int64_t row, int64_t NP, int
   if (RAW64)
     ((double*)ap.raw)[row * NP + p] = l;
   else
-    ((float*)ap.raw)[row * NP + p] = l;
+    ((float*)ap.raw)[raw_index(row, p, NP, ap.n_raw, ap.raw_tiled)] = l;
 }
 
 constexpr int NT_ABF = 256;
@@ -982,6 +983,8 @@ int uvlr_fast_run(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_assoc_opts* o, 
   ap.GL = GL;
   ap.GS = GS;
   ap.raw_f64 = o->raw_dtype == EQBF_F64;
+  ap.raw_tiled = o->raw_dtype == EQBF_F32_TILED;
+  ap.n_raw = 3 * (int64_t)GL + (int64_t)n_cfg * GS;
   for (int k = 0; k < GS; ++k) {
     ap.gS[k] = (float)ctx->phiS[k];
     ap.gS[ABF_GMAX + k] = (float)ctx->omaS[k];

@@ -178,14 +178,20 @@ class Engine:
         self.GL, self.GS = gL.shape[0], gS.shape[0]
 
     def assoc_uvlr(self, hc, analysis="join", bfs="gen", precision="mixed", raw_dtype=np.float64,
-                   want_sstats=True, want_std=True, want_raw=True):
+                   want_sstats=True, want_std=True, want_raw=True, raw_tiled=False):
+        """raw_tiled: FP32 raw ABFs written in 64-pair tiles (EQBF_F32_TILED),
+        returned rearranged to [n_raw][n_pairs]."""
+        if raw_tiled:
+            raw_dtype = np.float32
         out = _abi.HostAssocOut(hc.S, hc.n_pairs, bfs, self.GL, self.GS, raw_dtype=raw_dtype,
                                 want_sstats=want_sstats, want_std=want_std, want_raw=want_raw)
-        opts = _abi.AssocOpts(_abi.ANALYSIS[analysis], _abi.BFS[bfs], _abi.PREC[precision],
-                              _abi.F64 if raw_dtype == np.float64 else _abi.F32)
+        code = _abi.F32_TILED if raw_tiled else (_abi.F64 if raw_dtype == np.float64 else _abi.F32)
+        opts = _abi.AssocOpts(_abi.ANALYSIS[analysis], _abi.BFS[bfs], _abi.PREC[precision], code)
         st = hc.struct()
         o = out.struct()
         self._check(lib().eqbf_assoc_uvlr(self._ctx, C.byref(st), C.byref(opts), C.byref(o)))
+        if raw_tiled and out.raw is not None:
+            out.raw = _abi.untile_raw(out.raw, out.n_raw, hc.n_pairs)
         return out
 
     def assoc_uvlr_device(self, chunk_struct, opts, out_struct, stream=None):

@@ -68,6 +68,11 @@ enum { EQBF_PBF_GEN = 0, EQBF_PBF_GENSIN = 1, EQBF_PBF_ALL = 2 };/* --pbf   */
  *              FP64 log10-sum-exp. */
 enum { EQBF_PREC_FP64_PLAIN = 0, EQBF_PREC_MIXED = 1 };
 enum { EQBF_F32 = 0, EQBF_F64 = 1 };
+/* raw_dtype only: FP32 raw ABFs in tiles of 64 pairs (no counterpart in the
+ * reference, whose raw ABFs live in per-pair maps): element (row, p) at
+ * (p & ~63) * n_raw + row * w + (p & 63), w = min(64, n_pairs - (p & ~63)),
+ * so that one workgroup's 64 pairs write one contiguous block. */
+enum { EQBF_F32_TILED = 2 };
 
 /* ---- opaque context ------------------------------------------------------ */
 typedef struct eqbf_ctx eqbf_ctx;
@@ -115,7 +120,7 @@ typedef struct eqbf_assoc_opts {
   int32_t analysis;    /* EQBF_ANALYSIS_*                                     */
   int32_t bfs;         /* EQBF_BFS_*  (join only)                             */
   int32_t precision;   /* EQBF_PREC_*                                         */
-  int32_t raw_dtype;   /* element type of out->raw_abf: EQBF_F32 / EQBF_F64   */
+  int32_t raw_dtype;   /* out->raw_abf: EQBF_F32 / EQBF_F64 / EQBF_F32_TILED  */
 } eqbf_assoc_opts;
 
 /* Number of configurations written after the consistent ones:

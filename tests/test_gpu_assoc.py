@@ -189,3 +189,26 @@ def test_mixed_bayes_factors_only(eng, orc, bfs):
         err = np.abs(a[fin] - b[fin])
         _say("[gpu]   %s max abs err %.2e" % (name, err.max(initial=0)))
         assert err.max(initial=0) <= 1e-4, name
+
+
+@pytest.mark.parametrize("mode", ["plain", "mixed"])
+def test_raw_tiled_layout(eng, orc, mode):
+    # EQBF_F32_TILED: the same FP32 raw ABFs, in tiles of 64 pairs (a last,
+    # narrower tile included); every other output unchanged.  Covers the
+    # batched path (LDS-direct sums), the register-staged sums (odd column
+    # count) and the path for two genotype sets.
+    eng.set_grids(GL25, GS10)
+    cases = [synth.batched(seed=3, S=4, N_all=80, n_genes=3, p_mean=70, p_lo=20, p_hi=150, ns_lo=30, ns_hi=80, Q=1),
+             synth.batched(seed=4, S=3, N_all=70, n_genes=3, p_mean=50, p_lo=10, p_hi=90, ns_lo=25, ns_hi=70, Q=2)]
+    hc = cases[0]
+    g2 = np.clip(hc.genotypes + np.float32(0.25), 0, 2).astype(np.float32)
+    cases.append(type(hc)(hc.explevels, hc.gene_in_sbgrp, hc.geno_in_sbgrp, np.stack([hc.genotypes[0], g2[0]]),
+                          hc.snp_in_sbgrp, hc.pair_offset, hc.pair_snp, covars=hc.covars, covar_count=hc.covar_count,
+                          covar_in_sbgrp=hc.covar_in_sbgrp, geno_set_of_sbgrp=np.array([0, 1, 0, 1], np.int32)))
+    assert any(hc.n_pairs % 64 for hc in cases)
+    for k, hc in enumerate(cases):
+        _say("[gpu] raw tiled layout mode=%s case=%d pairs=%d" % (mode, k, hc.n_pairs))
+        a = eng.assoc_uvlr(hc, "join", "all", precision=mode, raw_dtype=np.float32)
+        b = eng.assoc_uvlr(hc, "join", "all", precision=mode, raw_tiled=True)
+        assert np.array_equal(a.raw, b.raw, equal_nan=True)
+        assert np.array_equal(a.w, b.w, equal_nan=True) and np.array_equal(a.n, b.n)
