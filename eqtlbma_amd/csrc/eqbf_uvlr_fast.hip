@@ -524,9 +524,11 @@ struct AbfParams {
   int n_jobs, n_cfg, GL, GS, raw_f64, raw_tiled;
   int64_t n_raw;
   /* small grid as FP32 in the kernel arguments (read through the scalar
-   * cache): phi2 at [0, GS), omega2 at [ABF_GMAX, ABF_GMAX + GS) */
-  float gS[2 * ABF_GMAX];
+   * cache): phi2 at [0, GS), omega2 at [ABF_GMAX, ABF_GMAX + GS),
+   * phi2 + omega2 at [2 ABF_GMAX, 2 ABF_GMAX + GS) */
+  float gS[3 * ABF_GMAX];
 };
+static_assert(sizeof(AbfParams) + 16 <= 4096, "kernel arguments of uvlr_abf_kernel exceed 4 KiB");
 
 __device__ inline float lg10f(float x) { return __log2f(x) * 0.30102999566398120f; }
 
@@ -575,28 +577,53 @@ __device__ inline double acc_finish(float m, double s, double weight) {
   return res;
 }
 
+/* Where one lane stores its pair's raw ABFs: row r lies at base + r * stride
+ * bytes (row-major: stride NP elements; 64-pair tiles: stride the tile's
+ * width, base at the tile's start plus the pair's column).  One 32 x 32 ->
+ * 64-bit multiply-add per store. */
 template <bool RAW64>
-__device__ inline void put_raw(const AbfParams& ap, int64_t row, int64_t NP, int64_t p, float l) {
-  if (RAW64)
-    ((double*)ap.raw)[row * NP + p] = l;
-  else
-    ((float*)ap.raw)[raw_index(row, p, NP, ap.n_raw, ap.raw_tiled)] = l;
-}
+struct RawRow {
+  char* base;
+  uint32_t stride;
+  __device__ RawRow(const AbfParams& ap, int64_t p, int64_t NP) {
+    const int64_t es = RAW64 ? 8 : 4;
+    if (!RAW64 && ap.raw_tiled) {
+      const int64_t tb = p & ~(int64_t)63, tw = NP - tb < 64 ? NP - tb : 64;
+      base = (char*)ap.raw + (tb * ap.n_raw + (p & 63)) * es;
+      stride = (uint32_t)(tw * es);
+    } else {
+      base = (char*)ap.raw + p * es;
+      stride = (uint32_t)(NP * es);
+    }
+  }
+  __device__ void put(int row, float l) const {
+    char* a = base + (uint64_t)(uint32_t)row * stride;
+    if (RAW64)
+      *(double*)a = l;
+    else
+      *(float*)a = l;
+  }
+};
 
 constexpr int NT_ABF = 256;
 
 /* Bayes factors of 64 pairs, one lane per pair (four waves share the jobs).
- * Per (subgroup, pair) the LDS holds {v, b, z^2 / (2 ln 10), c} with c = 1
- * for a subgroup that enters the Bayes factors and {1, 0, 0, 0} otherwise,
- * so the sums over subgroups run without branches (an absent or zero-z
- * subgroup adds exact zeros).  Lanes past the end of the pairs repeat the
- * last pair (same inputs, same values, same addresses).  WITH_RAW: the raw
- * ABFs are stored (FP64 when RAW64). */
+ * Per (subgroup, pair) the LDS holds {v, z^2 / (2 ln 10)} and b for a
+ * subgroup that enters the Bayes factors, and {2^100, 0}, 0 otherwise, so
+ * the sums over subgroups run without branches: such a subgroup has
+ * v / (v + phi2) = 1 and b / (v + phi2) = 0, adding exact zeros to the log
+ * terms and to sum b/(v+phi2), and adds 2^-100 to sum 1/(v+phi2), below
+ * FP32's resolution of any present subgroup's term (with no subgroup
+ * present, b-bar = 0 and the ABF is 0 either way).  12 + 1 bytes per
+ * (subgroup, pair) keep four workgroups per CU at 44 subgroups.  Lanes past
+ * the end of the pairs repeat the last pair (same inputs, same values, same
+ * addresses).  WITH_RAW: the raw ABFs are stored (FP64 when RAW64). */
 template <bool WITH_RAW, bool RAW64>
 __global__ void __launch_bounds__(NT_ABF) uvlr_abf_kernel(const int S, const int64_t NP, const AbfParams ap) {
   extern __shared__ __align__(16) unsigned char sm2[];
-  float4* l4 = (float4*)sm2;                      /* [S][64]                         */
-  uint8_t* lf = (uint8_t*)(l4 + S * 64);          /* [S][64] role of the subgroup    */
+  float2* lvz = (float2*)sm2;                     /* [S][64] {v, z^2 / (2 ln 10)}    */
+  float* lbv = (float*)(lvz + S * 64);            /* [S][64] b                       */
+  uint8_t* lf = (uint8_t*)(lbv + S * 64);         /* [S][64] role of the subgroup    */
   double* part = (double*)sm2;                    /* [4 waves][3][2][64], after jobs */
   const int tid = threadIdx.x, lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -613,8 +640,9 @@ __global__ void __launch_bounds__(NT_ABF) uvlr_abf_kernel(const int S, const int
     const int64_t o = (int64_t)s * NP + p0 + min(j, cnt - 1);
     const uint8_t f = ap.st_f[o];
     lf[e] = f;
-    l4[e] = f == F_NORMAL ? make_float4(ap.st_v[o], ap.st_b[o], kHalfInvLn10f * ap.st_z2[o], 1.f)
-                          : make_float4(1.f, 0.f, 0.f, 0.f);
+    const bool on = f == F_NORMAL;
+    lvz[e] = on ? make_float2(ap.st_v[o], kHalfInvLn10f * ap.st_z2[o]) : make_float2(0x1p100f, 0.f);
+    lbv[e] = on ? ap.st_b[o] : 0.f;
   }
   __syncthreads();
   const int64_t p = p0 + min(lane, cnt - 1);
@@ -623,7 +651,9 @@ __global__ void __launch_bounds__(NT_ABF) uvlr_abf_kernel(const int S, const int
   for (int s = 0; s < S; ++s)
     if (lf[s * 64 + lane] == F_NAN) nanm |= 1ull << s;
   const bool any_nan = nanm != 0;
-  const float4* lq = l4 + lane;
+  const float2* lq = lvz + lane;
+  const float* lqb = lbv + lane;
+  const RawRow<RAW64> rr(ap, p, NP);
 
   Acc ta0, ta1, ta2;
   ta0.init();
@@ -639,25 +669,24 @@ __global__ void __launch_bounds__(NT_ABF) uvlr_abf_kernel(const int S, const int
       if (p2 != 0.f) {
 #pragma unroll 4
         for (int s = 0; s < S; ++s) {
-          const float4 q = lq[s * 64];
+          const float2 q = lq[s * 64];
           const float r = __builtin_amdgcn_rcpf(q.x + p2);
-          L += q.w * (0.5f * lg10f(q.x * r)) + q.z * p2 * r;
-          A += q.y * r;
-          Bs += q.w * r;
+          L += 0.5f * lg10f(q.x * r) + q.y * p2 * r;
+          A += lqb[s * 64] * r;
+          Bs += r;
         }
       } else {
 #pragma unroll 4
         for (int s = 0; s < S; ++s) {
-          const float4 q = lq[s * 64];
-          const float r = __builtin_amdgcn_rcpf(q.x);
-          A += q.y * r;
-          Bs += q.w * r;
+          const float r = __builtin_amdgcn_rcpf(lq[s * 64].x);
+          A += lqb[s * 64] * r;
+          Bs += r;
         }
       }
       for (int e = jb.e0; e < jb.e1; ++e) {
         const PhiEntry en = ap.ents[e];
         const float l = any_nan ? 0.f : abf_fin(L, A, Bs, en.oma2);
-        if (WITH_RAW) put_raw<RAW64>(ap, (int64_t)en.type * ap.GL + en.gi, NP, p, l);
+        if (WITH_RAW) rr.put(en.type * ap.GL + en.gi, l);
         if (en.type == 0)
           ta0.add(l);
         else if (en.type == 1)
@@ -668,15 +697,18 @@ __global__ void __launch_bounds__(NT_ABF) uvlr_abf_kernel(const int S, const int
     } else {
       const int cfg = j - ap.n_jobs;
       const uint64_t mk = ap.masks[cfg];
-      const int64_t row0 = 3 * (int64_t)ap.GL + (int64_t)cfg * ap.GS;
+      const int row0 = 3 * ap.GL + cfg * ap.GS;
       float am = -INFINITY, as = 0.f;
       if (__popcll(mk) == 1) {
-        /* singleton: bbar = b, V = v + phi2, and the log terms telescope;
-         * blocks of 8 grid points share one rescaling of the running sum */
+        /* singleton: bbar = b, V = v + phi2; the log terms telescope to
+         * log10(v / (v + w)) and the exponents add up to z^2 w / (v + w) / 2
+         * with w = phi2 + omega2 (the subgroup's and the average's terms of
+         * CalcLog10AbfUvlr); blocks of 8 grid points share one rescaling of
+         * the running sum */
         const int s = __ffsll((unsigned long long)mk) - 1;
-        const float4 q = lq[s * 64];
-        const bool on = q.w != 0.f;
-        const float v = q.x, hz = q.z;
+        const float2 q = lq[s * 64];
+        const float v = q.x, hz = q.y;
+        const bool on = hz > 0.f; /* z != 0 in a present subgroup */
         for (int g0 = 0; g0 < ap.GS; g0 += 8) {
           float lb[8];
           float bm = -INFINITY;
@@ -685,11 +717,10 @@ __global__ void __launch_bounds__(NT_ABF) uvlr_abf_kernel(const int S, const int
             const int gi = g0 + i;
             float l = -INFINITY;
             if (gi < ap.GS) {
-              const float p2 = ap.gS[gi], o2 = ap.gS[ABF_GMAX + gi];
-              const float vp = v + p2, va = vp + o2;
-              const float r1 = __builtin_amdgcn_rcpf(vp), r2 = __builtin_amdgcn_rcpf(va);
-              l = on ? 0.5f * lg10f(v * r2) + hz * (p2 * r1 + v * o2 * r1 * r2) : 0.f;
-              if (WITH_RAW) put_raw<RAW64>(ap, row0 + gi, NP, p, l);
+              const float wg = ap.gS[2 * ABF_GMAX + gi];
+              const float r = __builtin_amdgcn_rcpf(v + wg);
+              l = on ? 0.5f * lg10f(v * r) + hz * wg * r : 0.f;
+              if (WITH_RAW) rr.put(row0 + gi, l);
             }
             lb[i] = l;
             bm = fmaxf(bm, l);
@@ -712,14 +743,14 @@ __global__ void __launch_bounds__(NT_ABF) uvlr_abf_kernel(const int S, const int
           while (m) {
             const int s = __ffsll((unsigned long long)m) - 1;
             m &= m - 1ull;
-            const float4 q = lq[s * 64];
+            const float2 q = lq[s * 64];
             const float r = __builtin_amdgcn_rcpf(q.x + p2);
-            if (p2 != 0.f) L += q.w * (0.5f * lg10f(q.x * r)) + q.z * p2 * r;
-            A += q.y * r;
-            Bs += q.w * r;
+            if (p2 != 0.f) L += 0.5f * lg10f(q.x * r) + q.y * p2 * r;
+            A += lqb[s * 64] * r;
+            Bs += r;
           }
           const float l = nan ? 0.f : abf_fin(L, A, Bs, o2);
-          if (WITH_RAW) put_raw<RAW64>(ap, row0 + gi, NP, p, l);
+          if (WITH_RAW) rr.put(row0 + gi, l);
           a.add(l);
         }
         am = a.m;
@@ -966,6 +997,9 @@ int uvlr_fast_run(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_assoc_opts* o, 
   }
   if (GS > ABF_GMAX)
     return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED, "mixed path supports small grids of at most %d points", ABF_GMAX);
+  if (out->raw_abf && o->raw_dtype != EQBF_F32_TILED && NP * (o->raw_dtype == EQBF_F64 ? 8 : 4) > (int64_t)UINT32_MAX)
+    return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED, "row-major raw ABFs of more than %lld pairs per call",
+                    (long long)(UINT32_MAX / (o->raw_dtype == EQBF_F64 ? 8 : 4)));
   AbfParams ap;
   memset(&ap, 0, sizeof ap);
   ap.st_v = sp.st_v;
@@ -988,8 +1022,9 @@ int uvlr_fast_run(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_assoc_opts* o, 
   for (int k = 0; k < GS; ++k) {
     ap.gS[k] = (float)ctx->phiS[k];
     ap.gS[ABF_GMAX + k] = (float)ctx->omaS[k];
+    ap.gS[2 * ABF_GMAX + k] = (float)(ctx->phiS[k] + ctx->omaS[k]);
   }
-  const size_t lds2 = std::max((size_t)S * 64 * 17, (size_t)4 * 3 * 2 * 64 * 8);
+  const size_t lds2 = std::max((size_t)S * 64 * 13, (size_t)4 * 3 * 2 * 64 * 8);
   {
     ProfScope ps(ctx, "uvlr_abf_kernel", st);
     const dim3 grid((unsigned)(((NP + 63) / 64 + 7) / 8 * 8));
