@@ -605,9 +605,9 @@ struct RawRow {
   }
 };
 
-constexpr int NT_ABF = 256;
+constexpr int NT_ABF = 512;
 
-/* Bayes factors of 64 pairs, one lane per pair (four waves share the jobs).
+/* Bayes factors of 64 pairs, one lane per pair (eight waves share the jobs).
  * Per (subgroup, pair) the LDS holds {v, z^2 / (2 ln 10)} and b for a
  * subgroup that enters the Bayes factors, and {2^100, 0}, 0 otherwise, so
  * the sums over subgroups run without branches: such a subgroup has
@@ -615,7 +615,7 @@ constexpr int NT_ABF = 256;
  * terms and to sum b/(v+phi2), and adds 2^-100 to sum 1/(v+phi2), below
  * FP32's resolution of any present subgroup's term (with no subgroup
  * present, b-bar = 0 and the ABF is 0 either way).  12 + 1 bytes per
- * (subgroup, pair) keep four workgroups per CU at 44 subgroups.  Lanes past
+ * (subgroup, pair) keep four workgroups (32 waves) per CU at 44 subgroups.  Lanes past
  * the end of the pairs repeat the last pair (same inputs, same values, same
  * addresses).  WITH_RAW: the raw ABFs are stored (FP64 when RAW64). */
 template <bool WITH_RAW, bool RAW64>
@@ -624,7 +624,7 @@ __global__ void __launch_bounds__(NT_ABF) uvlr_abf_kernel(const int S, const int
   float2* lvz = (float2*)sm2;                     /* [S][64] {v, z^2 / (2 ln 10)}    */
   float* lbv = (float*)(lvz + S * 64);            /* [S][64] b                       */
   uint8_t* lf = (uint8_t*)(lbv + S * 64);         /* [S][64] role of the subgroup    */
-  double* part = (double*)sm2;                    /* [4 waves][3][2][64], after jobs */
+  double* part = (double*)sm2;                    /* [8 waves][3][2][64], after jobs */
   const int tid = threadIdx.x, lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
   /* XCD-aware order: the workgroups of one XCD (blockIdx & 7) take one
@@ -759,7 +759,7 @@ __global__ void __launch_bounds__(NT_ABF) uvlr_abf_kernel(const int S, const int
       ap.w[(int64_t)(3 + cfg) * NP + p] = acc_finish(am, (double)as, 1.0 / ap.GS);
     }
   }
-  /* combine the consistent-configuration partial sums of the four waves
+  /* combine the consistent-configuration partial sums of the eight waves
    * (part lies over the subgroup records, which are no longer read) */
   __syncthreads();
   part[((w * 3 + 0) * 2) * 64 + lane] = (double)ta0.m;
@@ -1024,7 +1024,7 @@ int uvlr_fast_run(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_assoc_opts* o, 
     ap.gS[ABF_GMAX + k] = (float)ctx->omaS[k];
     ap.gS[2 * ABF_GMAX + k] = (float)(ctx->phiS[k] + ctx->omaS[k]);
   }
-  const size_t lds2 = std::max((size_t)S * 64 * 13, (size_t)4 * 3 * 2 * 64 * 8);
+  const size_t lds2 = std::max((size_t)S * 64 * 13, (size_t)(NT_ABF / 64) * 3 * 2 * 64 * 8);
   {
     ProfScope ps(ctx, "uvlr_abf_kernel", st);
     const dim3 grid((unsigned)(((NP + 63) / 64 + 7) / 8 * 8));
