@@ -531,20 +531,22 @@ struct AbfParams {
 static_assert(sizeof(AbfParams) + 16 <= 4096, "kernel arguments of uvlr_abf_kernel exceed 4 KiB");
 
 __device__ inline float lg10f(float x) { return __log2f(x) * 0.30102999566398120f; }
+/* 2^x on the transcendental unit, without the library's rescaling of
+ * results below 2^-126: such terms are invisible in the log-sums here */
+__device__ inline float ex2f(float x) { return __builtin_amdgcn_exp2f(x); }
 
 /* Finishes one ABF from the single-subgroup sum L and A = sum b/(v+phi2),
- * Bs = sum 1/(v+phi2) (CalcLog10AbfUvlr, gene_snp_pair.cpp:336-354). */
+ * Bs = sum 1/(v+phi2) (CalcLog10AbfUvlr, gene_snp_pair.cpp:336-354): 0
+ * unless Bs > 0, b-bar != 0 and its variance V is finite; the average's
+ * term only when b-bar^2 / V != 0.  Evaluated without branches (the terms
+ * of the cases that return 0 are computed and dropped). */
 __device__ inline float abf_fin(float L, float A, float Bs, float oma2) {
-  if (!(Bs > 0.f)) return 0.f;
   const float V = __builtin_amdgcn_rcpf(Bs), bbar = A * V;
-  if (bbar == 0.f || !(V < INFINITY)) return 0.f;
   const float T2 = A * bbar; /* bbar^2 / V */
-  float lbar = 0.f;
-  if (T2 != 0.f) {
-    const float rVo = __builtin_amdgcn_rcpf(V + oma2);
-    lbar = 0.5f * lg10f(V * rVo) + kHalfInvLn10f * T2 * oma2 * rVo;
-  }
-  return lbar + L;
+  const float rVo = __builtin_amdgcn_rcpf(V + oma2);
+  const float lbar = 0.5f * lg10f(V * rVo) + kHalfInvLn10f * T2 * oma2 * rVo;
+  const bool ok = Bs > 0.f && bbar != 0.f && V < INFINITY;
+  return ok ? (T2 != 0.f ? lbar : 0.f) + L : 0.f;
 }
 
 /* log10 of the mean of 10^v over a grid: running maximum m and the sum of
@@ -558,7 +560,7 @@ struct Acc {
   }
   __device__ void add(float v) {
     const bool up = v > m;
-    const float e = exp2f(-fabsf(v - m) * kLog2_10);
+    const float e = ex2f(-fabsf(v - m) * kLog2_10);
     s = up ? fmaf(s, e, 1.f) : s + e;
     m = up ? v : m;
   }
@@ -566,7 +568,7 @@ struct Acc {
   __device__ void merge(const Acc& o) {
     const float M = fmaxf(m, o.m);
     if (M == -INFINITY) return;
-    s = s * exp2f((m - M) * kLog2_10) + o.s * exp2f((o.m - M) * kLog2_10);
+    s = s * ex2f((m - M) * kLog2_10) + o.s * ex2f((o.m - M) * kLog2_10);
     m = M;
   }
 };
@@ -598,10 +600,11 @@ struct RawRow {
   }
   __device__ void put(int row, float l) const {
     char* a = base + (uint64_t)(uint32_t)row * stride;
+    /* streaming stores: the raw ABFs are not read again by this launch */
     if (RAW64)
-      *(double*)a = l;
+      __builtin_nontemporal_store((double)l, (double*)a);
     else
-      *(float*)a = l;
+      __builtin_nontemporal_store(l, (float*)a);
   }
 };
 
@@ -619,7 +622,7 @@ constexpr int NT_ABF = 512;
  * the end of the pairs repeat the last pair (same inputs, same values, same
  * addresses).  WITH_RAW: the raw ABFs are stored (FP64 when RAW64). */
 template <bool WITH_RAW, bool RAW64>
-__global__ void __launch_bounds__(NT_ABF) uvlr_abf_kernel(const int S, const int64_t NP, const AbfParams ap) {
+__global__ void __launch_bounds__(NT_ABF) __attribute__((amdgpu_waves_per_eu(8))) uvlr_abf_kernel(const int S, const int64_t NP, const AbfParams ap) {
   extern __shared__ __align__(16) unsigned char sm2[];
   float2* lvz = (float2*)sm2;                     /* [S][64] {v, z^2 / (2 ln 10)}    */
   float* lbv = (float*)(lvz + S * 64);            /* [S][64] b                       */
@@ -704,34 +707,36 @@ __global__ void __launch_bounds__(NT_ABF) uvlr_abf_kernel(const int S, const int
          * log10(v / (v + w)) and the exponents add up to z^2 w / (v + w) / 2
          * with w = phi2 + omega2 (the subgroup's and the average's terms of
          * CalcLog10AbfUvlr); blocks of 8 grid points share one rescaling of
-         * the running sum */
+         * the running sum.  A subgroup outside the Bayes factors (v = 2^100,
+         * z = 0) gives log10(1) + 0 = 0 at every grid point. */
         const int s = __ffsll((unsigned long long)mk) - 1;
         const float2 q = lq[s * 64];
         const float v = q.x, hz = q.y;
-        const bool on = hz > 0.f; /* z != 0 in a present subgroup */
-        for (int g0 = 0; g0 < ap.GS; g0 += 8) {
+        auto block = [&](const int g0, const int n) {
           float lb[8];
           float bm = -INFINITY;
 #pragma unroll
           for (int i = 0; i < 8; ++i) {
-            const int gi = g0 + i;
             float l = -INFINITY;
-            if (gi < ap.GS) {
-              const float wg = ap.gS[2 * ABF_GMAX + gi];
+            if (i < n) {
+              const float wg = ap.gS[2 * ABF_GMAX + g0 + i];
               const float r = __builtin_amdgcn_rcpf(v + wg);
-              l = on ? 0.5f * lg10f(v * r) + hz * wg * r : 0.f;
-              if (WITH_RAW) rr.put(row0 + gi, l);
+              l = 0.5f * lg10f(v * r) + hz * wg * r;
+              if (WITH_RAW) rr.put(row0 + g0 + i, l);
             }
             lb[i] = l;
             bm = fmaxf(bm, l);
           }
           const float M = fmaxf(am, bm);
-          float acc = as * exp2f((am - M) * kLog2_10);
+          float acc = as * ex2f((am - M) * kLog2_10);
 #pragma unroll
-          for (int i = 0; i < 8; ++i) acc += exp2f((lb[i] - M) * kLog2_10);
+          for (int i = 0; i < 8; ++i) acc += ex2f((lb[i] - M) * kLog2_10);
           as = acc;
           am = M;
-        }
+        };
+        int g0 = 0;
+        for (; g0 + 8 <= ap.GS; g0 += 8) block(g0, 8);
+        if (g0 < ap.GS) block(g0, ap.GS - g0);
       } else {
         const bool nan = (mk & nanm) != 0ull;
         Acc a;
