@@ -106,6 +106,17 @@ def test_batched_covariates_uneven(eng, orc, mode):
     _run(eng, orc, hc, "sin", G50, G50, mode, tag="batched-cov")
 
 
+@pytest.mark.parametrize("n_small", [1, 3, 8, 16])
+def test_small_grid_sizes(eng, orc, n_small):
+    # the batched Bayes factors sweep the small grid in blocks of 8 points
+    # (full blocks, then a remainder): grids of one, part of one, exactly
+    # one and exactly two blocks, with absent subgroups and missing values
+    hc = synth.batched(seed=60 + n_small, S=5, N_all=100, n_genes=4, p_mean=40, p_lo=10, p_hi=80,
+                       ns_lo=20, ns_hi=90, Q=2, nan_frac=0.03, absent_frac=0.2)
+    gs = G50[3:3 + n_small]
+    _run(eng, orc, hc, "all", GL25, gs, "mixed", tag="small grid of %d" % n_small)
+
+
 @pytest.mark.parametrize("mode", ["plain", "mixed"])
 def test_missing_values_and_absent_genes(eng, orc, mode):
     hc = synth.batched(seed=7, S=5, N_all=90, n_genes=5, p_mean=50, p_lo=10, p_hi=90,
