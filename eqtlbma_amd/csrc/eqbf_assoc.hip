@@ -375,6 +375,178 @@ __global__ void __launch_bounds__(64) prep_wave_kernel(const eqbf_chunk c, PrepR
   }
 }
 
+/* The same computation with the columns in registers: member k of the
+ * (gene, subgroup) is slot k / 64 of lane k % 64, for N <= 64 NPL samples and
+ * at most PREP_REG_KU basis columns.  Slots past the members hold zeros,
+ * which every update keeps at zero, so the sums add the same terms in the
+ * same order as prep_wave_kernel and the results are identical; only the
+ * member list stays in LDS, so many more waves fit a CU. */
+constexpr int PREP_REG_KU = 4;
+template <int NPL>
+__global__ void __launch_bounds__(64) prep_reg_kernel(const eqbf_chunk c, PrepRec* __restrict__ rec,
+                                                      double* __restrict__ B, uint64_t* __restrict__ memb,
+                                                      int32_t* __restrict__ idxs, int* __restrict__ flag, int KU) {
+  constexpr int KCM = PREP_REG_KU + 1;
+  __shared__ int lidx[NPL * 64];
+  const int S = c.n_subgroups, N = c.n_samples;
+  const int g = blockIdx.x / S, s = blockIdx.x % S;
+  const int lane = threadIdx.x;
+  const int KC = KU + 1;
+  PrepRec r;
+  r.n = 0;
+  r.flags = 0;
+  r.q = c.covar_count[s];
+  r.rank = 0;
+  r.yy = 0;
+  r.tss = 0;
+  if (!c.gene_in_sbgrp[(int64_t)g * S + s]) {
+    if (lane == 0) rec[(int64_t)g * S + s] = r;
+    return;
+  }
+  const double* y = c.explevels + ((int64_t)g * S + s) * N;
+  const uint8_t* gin = c.geno_in_sbgrp + (int64_t)s * N;
+  int32_t* idx = idxs + ((int64_t)g * S + s) * N;
+  int n = 0;
+  for (int base = 0; base < N; base += 64) {
+    const int i = base + lane;
+    const bool m = (i < N) && !isnan(y[i]) && gin[i];
+    const uint64_t bal = __ballot(m);
+    const int pre = __popcll(bal & ((1ull << lane) - 1ull));
+    if (m) {
+      lidx[n + pre] = i;
+      idx[n + pre] = i;
+    }
+    n += __popcll(bal);
+  }
+  __syncthreads();
+  const int Q = r.q;
+  r.n = n;
+  if (n == 0) {
+    if (lane == 0) rec[(int64_t)g * S + s] = r;
+    return;
+  }
+  if (Q > 0) {
+    int miss = 0;
+    for (int k = lane; k < n; k += 64) miss |= !c.covar_in_sbgrp[(int64_t)s * N + lidx[k]];
+    if (__ballot(miss)) {
+      if (lane == 0) {
+        atomicOr(flag, FLAG_MISSING_COVAR);
+        r.n = 0;
+        rec[(int64_t)g * S + s] = r;
+      }
+      return;
+    }
+  }
+  if (n < Q + 3) { /* N < P + 1 with P = Q + 2 (utils_math.cpp:175-178) */
+    if (lane == 0) {
+      r.flags = PREP_TOO_FEW;
+      rec[(int64_t)g * S + s] = r;
+    }
+    return;
+  }
+  /* y and the columns [1, covariates] of the members */
+  int li[NPL];
+  double cv[KCM][NPL];
+  double sy = 0;
+#pragma unroll
+  for (int t = 0; t < NPL; ++t) {
+    const int k = t * 64 + lane;
+    const bool in = k < n;
+    const int i = in ? lidx[k] : 0;
+    li[t] = i;
+    const double yv = in ? y[i] : 0.0;
+    cv[0][t] = yv;
+    sy += yv;
+    cv[1][t] = in ? 1.0 : 0.0;
+#pragma unroll
+    for (int j = 1; j < KCM - 1; ++j)
+      cv[1 + j][t] = (in && j <= Q) ? c.covars[((int64_t)s * c.n_covars_max + (j - 1)) * N + i] : 0.0;
+  }
+  const double ybar = wave_sum(sy) / n;
+  double ts = 0;
+#pragma unroll
+  for (int t = 0; t < NPL; ++t) {
+    if (t * 64 + lane < n) {
+      const double d = cv[0][t] - ybar;
+      ts += d * d;
+    }
+  }
+  const double tss = wave_sum(ts);
+  uint32_t kept = 0;
+  int rank = 0;
+#pragma unroll
+  for (int j = 0; j < KCM - 1; ++j) {
+    if (j > Q) continue;
+    double nn = 0;
+#pragma unroll
+    for (int t = 0; t < NPL; ++t) nn += cv[1 + j][t] * cv[1 + j][t];
+    const double n0 = sqrt(wave_sum(nn));
+    for (int pass = 0; pass < 2; ++pass) {
+#pragma unroll
+      for (int l = 0; l < j; ++l) {
+        if (!((kept >> l) & 1u)) continue;
+        double d = 0;
+#pragma unroll
+        for (int t = 0; t < NPL; ++t) d += cv[1 + l][t] * cv[1 + j][t];
+        d = wave_sum(d);
+#pragma unroll
+        for (int t = 0; t < NPL; ++t) cv[1 + j][t] -= d * cv[1 + l][t];
+      }
+    }
+    double nv = 0;
+#pragma unroll
+    for (int t = 0; t < NPL; ++t) nv += cv[1 + j][t] * cv[1 + j][t];
+    nv = sqrt(wave_sum(nv));
+    const bool keep = nv > 1e-9 * n0 && nv > 0;
+    const double sc = keep ? 1.0 / nv : 0.0;
+#pragma unroll
+    for (int t = 0; t < NPL; ++t) cv[1 + j][t] *= sc;
+    if (keep) {
+      kept |= 1u << j;
+      ++rank;
+    }
+  }
+  for (int pass = 0; pass < 2; ++pass) {
+#pragma unroll
+    for (int l = 0; l < KCM - 1; ++l) {
+      if (l > Q || !((kept >> l) & 1u)) continue;
+      double d = 0;
+#pragma unroll
+      for (int t = 0; t < NPL; ++t) d += cv[1 + l][t] * cv[0][t];
+      d = wave_sum(d);
+#pragma unroll
+      for (int t = 0; t < NPL; ++t) cv[0][t] -= d * cv[1 + l][t];
+    }
+  }
+  double yy = 0;
+#pragma unroll
+  for (int t = 0; t < NPL; ++t) yy += cv[0][t] * cv[0][t];
+  yy = wave_sum(yy);
+#pragma unroll
+  for (int t = 0; t < NPL; ++t) {
+    if (t * 64 + lane < n) {
+      const int i = li[t];
+      double* dst = B + (((int64_t)g * N + i) * S + s) * KC;
+#pragma unroll
+      for (int cc = 0; cc < KCM; ++cc)
+        if (cc < KC) dst[cc] = cv[cc][t];
+      atomicOr((unsigned long long*)&memb[(int64_t)g * N + i], 1ull << s);
+    }
+  }
+  if (lane == 0) {
+    r.rank = rank;
+    r.yy = yy;
+    r.tss = tss;
+    rec[(int64_t)g * S + s] = r;
+  }
+}
+
+template <int NPL>
+void launch_prep_reg(int blocks, hipStream_t st, const eqbf_chunk& c, PrepRec* rec, double* B, uint64_t* memb,
+                     int32_t* idx, int* flag, int KU) {
+  hipLaunchKernelGGL((prep_reg_kernel<NPL>), dim3(blocks), dim3(64), 0, st, c, rec, B, memb, idx, flag, KU);
+}
+
 /* ---- packed work list: tiles of up to 64 pairs of one gene ---------------------- */
 __global__ void __launch_bounds__(1024) tiles_kernel(const int64_t* __restrict__ pair_offset, int G,
                                                      int4* __restrict__ tiles, int* __restrict__ n_tiles) {
@@ -1204,7 +1376,20 @@ int assoc_uvlr_run(eqbf_ctx* ctx, const eqbf_chunk* cp, const eqbf_assoc_opts* o
   {
     ProfScope ps(ctx, "prep_kernel", st);
     const size_t plds = (size_t)N * (KC * sizeof(double) + sizeof(int));
-    if (plds <= 64 * 1024)
+    int* fl = (int*)ctx->d_flag.ptr;
+    const int npl = (N + 63) / 64;
+    if (KU <= PREP_REG_KU && npl <= 8) {
+      switch (npl) {
+        case 1: launch_prep_reg<1>(G * S, st, c, rec, B, memb, idx, fl, KU); break;
+        case 2: launch_prep_reg<2>(G * S, st, c, rec, B, memb, idx, fl, KU); break;
+        case 3: launch_prep_reg<3>(G * S, st, c, rec, B, memb, idx, fl, KU); break;
+        case 4: launch_prep_reg<4>(G * S, st, c, rec, B, memb, idx, fl, KU); break;
+        case 5: launch_prep_reg<5>(G * S, st, c, rec, B, memb, idx, fl, KU); break;
+        case 6: launch_prep_reg<6>(G * S, st, c, rec, B, memb, idx, fl, KU); break;
+        case 7: launch_prep_reg<7>(G * S, st, c, rec, B, memb, idx, fl, KU); break;
+        default: launch_prep_reg<8>(G * S, st, c, rec, B, memb, idx, fl, KU); break;
+      }
+    } else if (plds <= 64 * 1024)
       hipLaunchKernelGGL(prep_wave_kernel, dim3(G * S), dim3(64), plds, st, c, rec, B, memb, idx,
                          (int*)ctx->d_flag.ptr, KU);
     else

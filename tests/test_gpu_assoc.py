@@ -106,6 +106,16 @@ def test_batched_covariates_uneven(eng, orc, mode):
     _run(eng, orc, hc, "sin", G50, G50, mode, tag="batched-cov")
 
 
+@pytest.mark.parametrize("mode", ["plain", "mixed"])
+@pytest.mark.parametrize("n_all,q", [(800, 3), (120, 5)])
+def test_prep_wave_path(eng, orc, mode, n_all, q):
+    # more than 512 samples or more than 3 covariates: the prep keeps its
+    # columns in LDS (prep_wave_kernel) instead of registers
+    hc = synth.batched(seed=40 + q, S=3, N_all=n_all, n_genes=3, p_mean=20, p_lo=5, p_hi=40,
+                       ns_lo=n_all // 3, ns_hi=n_all - 10, Q=q, nan_frac=0.02)
+    _run(eng, orc, hc, "all", GL25, GS10, mode, tag="prep N=%d Q=%d" % (n_all, q))
+
+
 @pytest.mark.parametrize("n_small", [1, 3, 8, 16])
 def test_small_grid_sizes(eng, orc, n_small):
     # the batched Bayes factors sweep the small grid in blocks of 8 points
