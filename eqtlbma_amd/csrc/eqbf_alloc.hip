@@ -26,6 +26,7 @@ namespace {
 constexpr int EIG_NMAX = 512;  /* samples for the eigenvalue kernel (LDS) */
 constexpr int EIG_P = 32;      /* subspace vectors                        */
 constexpr int EIG_ITERS = 20;
+constexpr int EIG_NT = 512;    /* threads of the eigenvalue kernel          */
 constexpr int GEV_NMAX = 16384; /* permutations per gene for the GEV fit    */
 
 __device__ inline double al_geno(const eqbf_chunk& c, int64_t i, int64_t j) {
@@ -114,7 +115,7 @@ __global__ void __launch_bounds__(256) al_gram_kernel(const eqbf_chunk c, const 
 }
 
 /* k largest eigenvalues of one gene's K (one workgroup per gene). */
-__global__ void __launch_bounds__(256) al_eig_kernel(const float* __restrict__ Kall, int N, int k,
+__global__ void __launch_bounds__(EIG_NT) al_eig_kernel(const float* __restrict__ Kall, int N, int k,
                                                      float* __restrict__ eig) {
   constexpr int p = EIG_P;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -125,7 +126,7 @@ __global__ void __launch_bounds__(256) al_eig_kernel(const float* __restrict__ K
   const float* K = Kall + (int64_t)blockIdx.x * N * N;
   const int tid = threadIdx.x;
   /* start: a fixed pseudo-random block */
-  for (int e = tid; e < N * p; e += 256) {
+  for (int e = tid; e < N * p; e += EIG_NT) {
     uint32_t h = (uint32_t)e * 2654435761u ^ 0x9e3779b9u;
     h ^= h >> 15;
     h *= 2246822519u;
@@ -135,7 +136,7 @@ __global__ void __launch_bounds__(256) al_eig_kernel(const float* __restrict__ K
   __syncthreads();
   for (int it = 0; it <= EIG_ITERS; ++it) {
     /* W = K V (K symmetric: row i read as column i, coalesced) */
-    for (int i = tid; i < N; i += 256) {
+    for (int i = tid; i < N; i += EIG_NT) {
       float acc[p];
 #pragma unroll
       for (int c2 = 0; c2 < p; ++c2) acc[c2] = 0.f;
@@ -164,7 +165,7 @@ __global__ void __launch_bounds__(256) al_eig_kernel(const float* __restrict__ K
     __syncthreads();
     /* G = V^T W (Rayleigh-Ritz on the last pass) or W^T W (orthonormalise) */
     const float* X = it == EIG_ITERS ? V : W;
-    for (int e = tid; e < p * p; e += 256) {
+    for (int e = tid; e < p * p; e += EIG_NT) {
       const int a = e / p, b = e - a * p;
       if (b > a && it < EIG_ITERS) continue;
       double s = 0;
@@ -183,18 +184,18 @@ __global__ void __launch_bounds__(256) al_eig_kernel(const float* __restrict__ K
       if (tid == 0) G[j * p + j] = sqrt(fmax(G[j * p + j], floor_));
       __syncthreads();
       const double d = G[j * p + j];
-      for (int i = j + 1 + tid; i < p; i += 256) G[i * p + j] /= d;
+      for (int i = j + 1 + tid; i < p; i += EIG_NT) G[i * p + j] /= d;
       __syncthreads();
       /* G[i][c] -= L[i][j] L[c][j] for j < c <= i */
       const int m = p - 1 - j;
-      for (int e = tid; e < m * m; e += 256) {
+      for (int e = tid; e < m * m; e += EIG_NT) {
         const int i = j + 1 + e / m, c2 = j + 1 + e % m;
         if (c2 <= i) G[i * p + c2] -= G[i * p + j] * G[c2 * p + j];
       }
       __syncthreads();
     }
     /* V = W L^-T (rows are independent: forward substitution per row) */
-    for (int i = tid; i < N; i += 256) {
+    for (int i = tid; i < N; i += EIG_NT) {
       float* w = W + i * p;
       float* v = V + i * p;
       for (int c2 = 0; c2 < p; ++c2) {
@@ -210,7 +211,7 @@ __global__ void __launch_bounds__(256) al_eig_kernel(const float* __restrict__ K
    * row updates and then their column updates are independent. */
   __shared__ double cs_[EIG_P / 2], sn_[EIG_P / 2];
   __shared__ int pa_[EIG_P / 2], pb_[EIG_P / 2];
-  for (int e = tid; e < p * p; e += 256) {
+  for (int e = tid; e < p * p; e += EIG_NT) {
     const int a = e / p, b = e % p;
     if (b > a) {
       const double m = 0.5 * (G[a * p + b] + G[b * p + a]);
@@ -248,14 +249,14 @@ __global__ void __launch_bounds__(256) al_eig_kernel(const float* __restrict__ K
         sn_[tid] = s2;
       }
       __syncthreads();
-      for (int e = tid; e < (p / 2) * p; e += 256) { /* columns a, b of every row */
+      for (int e = tid; e < (p / 2) * p; e += EIG_NT) { /* columns a, b of every row */
         const int q = e / p, rr = e % p, a = pa_[q], b = pb_[q];
         const double gra = G[rr * p + a], grb = G[rr * p + b];
         G[rr * p + a] = cs_[q] * gra - sn_[q] * grb;
         G[rr * p + b] = sn_[q] * gra + cs_[q] * grb;
       }
       __syncthreads();
-      for (int e = tid; e < (p / 2) * p; e += 256) { /* rows a, b of every column */
+      for (int e = tid; e < (p / 2) * p; e += EIG_NT) { /* rows a, b of every column */
         const int q = e / p, rr = e % p, a = pa_[q], b = pb_[q];
         const double gar = G[a * p + rr], gbr = G[b * p + rr];
         G[a * p + rr] = cs_[q] * gar - sn_[q] * gbr;
@@ -401,7 +402,7 @@ static int ld_eig_run(eqbf_ctx* ctx, const eqbf_chunk& d, const std::vector<int6
     }
     {
       ProfScope ps(ctx, "al_eig_kernel", st);
-      hipLaunchKernelGGL(al_eig_kernel, dim3((unsigned)nb), dim3(256), lds, st, dK, N, n_eig, deig);
+      hipLaunchKernelGGL(al_eig_kernel, dim3((unsigned)nb), dim3(EIG_NT), lds, st, dK, N, n_eig, deig);
     }
     EQBF_HIP(ctx, hipGetLastError());
     EQBF_HIP(ctx, hipMemcpyAsync(eig + (int64_t)g0 * n_eig, deig, (size_t)nb * n_eig * sizeof(float),
