@@ -547,26 +547,36 @@ void orc::pair_abfs(int32_t S, int32_t bfs, const std::vector<int>& has, const s
     }
     w[3 + c] = orc_log10_weighted_sum(l.data(), GS);
   }
-  /* CalcBMAlite (:552-570): singletons "1".."S" then gen, weights 1/(2S), 1/2.
-   * For 'all' the singletons are the first S configurations. */
+  orc_bma(S, bfs, w + 3, w[0], &w[3 + nc], bfs == EQBF_BFS_ALL ? &w[3 + nc + 1] : nullptr);
+}
+
+/* CalcBMAlite (gene_snp_pair.cpp:552-570): log10 of the weighted sum of the
+ * singleton averages "1".."S" (weight 1/(2S) each) and gen (weight 1/2).
+ * CalcBMA (:572-602): log10 of the weighted sum over every configuration of
+ * size k (weight (1/S) / C(S, k)); w_cfg holds the configuration averages in
+ * gsl_combination order (for 'all' the singletons are the first S).  Both go
+ * through log10_weighted_sum, so a NaN first term poisons the result. */
+extern "C" void orc_bma(int32_t S, int32_t bfs, const double* w_cfg, double gen, double* gen_sin, double* all) {
   {
     std::vector<double> v, wt;
     for (int32_t s = 0; s < S; ++s) {
-      v.push_back(w[3 + s]);
+      v.push_back(w_cfg[s]);
       wt.push_back((1.0 / 2.0) * (1.0 / S));
     }
-    v.push_back(w[0]);
+    v.push_back(gen);
     wt.push_back(1.0 / 2.0);
-    w[3 + nc] = orc_log10_weighted_sum_w(v.data(), wt.data(), (int64_t)v.size());
+    *gen_sin = orc_log10_weighted_sum_w(v.data(), wt.data(), (int64_t)v.size());
   }
-  if (bfs == EQBF_BFS_ALL) { /* CalcBMA (:572-602) */
+  if (bfs == EQBF_BFS_ALL && all) {
+    std::vector<uint64_t> masks;
+    config_masks(bfs, S, masks);
     std::vector<double> v, wt;
-    for (int64_t c = 0; c < nc; ++c) {
+    for (size_t c = 0; c < masks.size(); ++c) {
       int k = __builtin_popcountll(masks[c]);
-      v.push_back(w[3 + c]);
+      v.push_back(w_cfg[c]);
       wt.push_back((1.0 / (double)S) * (1.0 / choose(S, k)));
     }
-    w[3 + nc + 1] = orc_log10_weighted_sum_w(v.data(), wt.data(), (int64_t)v.size());
+    *all = orc_log10_weighted_sum_w(v.data(), wt.data(), (int64_t)v.size());
   }
 }
 

@@ -34,5 +34,6 @@ double orc_tdist_P(double t, double nu);
 double orc_gauss_Pinv(double p);
 double orc_fdist_Q_1(double F, double nu);
 double orc_chisq_Qinv_1(double q);
+void orc_bma(int32_t S, int32_t bfs, const double* w_cfg, double gen, double* gen_sin, double* all);
 }
 #endif

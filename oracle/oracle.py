@@ -73,6 +73,7 @@ def lib():
         L.orc_assoc_mvlr.restype = C.c_int
         L.orc_abf_from_sstats.argtypes = [i32, i64, vp, vp, vp, vp, i32, vp, vp, i32, vp, vp, i32, vp, vp, vp]
         L.orc_abf_from_sstats.restype = C.c_int
+        L.orc_bma.argtypes = [i32, i32, vp, d, vp, vp]
         L.orc_last_error.restype = C.c_char_p
         _lib = L
     return _lib
@@ -156,6 +157,16 @@ def l10abf_uvlr(gamma, stdsstats, phi2, oma2):
     g = np.ascontiguousarray(gamma, dtype=np.int32)
     st = np.ascontiguousarray(stdsstats, dtype=np.float64)
     return lib().orc_l10abf_uvlr(g.size, _p(g), _p(st), phi2, oma2)
+
+
+def bma(S, bfs, w_cfg, gen):
+    """CalcBMAlite / CalcBMA of one pair: (gen-sin, all) from the
+    configuration averages (gsl_combination order) and gen; all is NaN unless
+    bfs == 'all'."""
+    w = np.ascontiguousarray(w_cfg, dtype=np.float64)
+    gs, al = np.zeros(1), np.full(1, np.nan)
+    lib().orc_bma(S, _abi.BFS[bfs], _p(w), float(gen), _p(gs), _p(al))
+    return float(gs[0]), float(al[0])
 
 
 def standardize(N, Q, sigmahat, betahat, sebetahat):
