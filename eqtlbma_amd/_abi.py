@@ -17,7 +17,7 @@ EQBF_ERR_UNSUPPORTED = 4
 ANALYSIS = {"sep": 0, "join": 1}
 BFS = {"gen": 0, "sin": 1, "all": 2}
 PBF = {"gen": 0, "gen-sin": 1, "all": 2}
-PREC = {"fp64_plain": 0, "plain": 0, "mixed": 1}
+PREC = {"fp64_plain": 0, "plain": 0, "mixed": 1, "fp64_batched": 2, "fp64": 2}
 F32, F64 = 0, 1
 F32_TILED = 2  # raw ABFs only: FP32 in tiles of 64 pairs
 

@@ -214,6 +214,48 @@ extern "C" int eqbf_perm_stats(eqbf_ctx* ctx, const eqbf_chunk* c, const eqbf_pe
   return EQBF_OK;
 }
 
+extern "C" int eqbf_perm_stats_fp64_ties_device(eqbf_ctx* ctx, const eqbf_chunk* c, const eqbf_perm_opts* po,
+                                                const int32_t* perm_idx_dev, int32_t n_tables,
+                                                const int32_t* gene_table, int64_t nperm, double tie_bound,
+                                                double* stats_dev, double* true_stats_dev, int64_t* n_rechecked,
+                                                void* stream) {
+  if (!ctx || !c || !po || !perm_idx_dev || !gene_table || !stats_dev || !true_stats_dev || n_tables < 0)
+    return ctx_fail(ctx, EQBF_ERR_ARG, "null argument");
+  EQBF_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  return perm_ties_run(ctx, c, po, perm_idx_dev, n_tables, gene_table, nperm, tie_bound, stats_dev, true_stats_dev,
+                       n_rechecked, st);
+}
+
+extern "C" int eqbf_perm_stats_fp64_ties(eqbf_ctx* ctx, const eqbf_chunk* c, const eqbf_perm_opts* po,
+                                         const int32_t* perm_idx, int32_t n_tables, const int32_t* gene_table,
+                                         int64_t nperm, double tie_bound, double* stats, double* true_stats,
+                                         int64_t* n_rechecked) {
+  if (!ctx || !c || !po || !perm_idx || !gene_table || !stats || !true_stats || n_tables < 0)
+    return ctx_fail(ctx, EQBF_ERR_ARG, "null argument");
+  EQBF_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  Stager sg{ctx, st};
+  eqbf_chunk d;
+  int rc = stage_chunk(ctx, sg, c, &d);
+  if (rc) return rc;
+  const int32_t* dtab = sg.put("h_ptab", perm_idx, (size_t)n_tables * std::max<int64_t>(nperm, 0) * c->n_samples);
+  double* dstats = sg.alloc<double>("o_pstats", (size_t)c->n_genes * std::max<int64_t>(nperm, 1), true);
+  double* dtrue = sg.alloc<double>("o_ptrue", (size_t)std::max(c->n_genes, 1), true);
+  if (sg.rc || (n_tables > 0 && nperm > 0 && !dtab) || !dstats || !dtrue)
+    return ctx_fail(ctx, EQBF_ERR_HIP, "staging the tables failed");
+  rc = perm_ties_run(ctx, &d, po, dtab ? dtab : (const int32_t*)dstats, n_tables, gene_table, nperm, tie_bound, dstats,
+                     dtrue, n_rechecked, st);
+  if (rc) return rc;
+  rc = check_flag(ctx, st);
+  if (rc) return rc;
+  if (nperm > 0)
+    EQBF_HIP(ctx, hipMemcpyAsync(stats, dstats, (size_t)c->n_genes * nperm * sizeof(double), hipMemcpyDeviceToHost, st));
+  EQBF_HIP(ctx, hipMemcpyAsync(true_stats, dtrue, (size_t)c->n_genes * sizeof(double), hipMemcpyDeviceToHost, st));
+  EQBF_HIP(ctx, hipStreamSynchronize(st));
+  return EQBF_OK;
+}
+
 extern "C" int eqbf_assoc_mvlr_device(eqbf_ctx* ctx, const eqbf_chunk* c, const eqbf_mvlr_opts* o,
                                       eqbf_assoc_out* out, void* stream) {
   if (!ctx || !c || !o || !out) return ctx_fail(ctx, EQBF_ERR_ARG, "null argument");

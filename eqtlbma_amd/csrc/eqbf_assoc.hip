@@ -28,7 +28,9 @@
  *     residual sums of squares; t-tail by tabulated cubic Hermite in FP64;
  *     Bayes-factor terms in FP32; grid averages with FP64 accumulation.
  *   plain (EQBF_PREC_FP64_PLAIN): one launch per gene, one thread per pair,
- *     every quantity in FP64 with the reference's loop structure.
+ *     every quantity in FP64 with the reference's loop structure;
+ *     EQBF_PREC_FP64_BATCHED runs the same per-pair code for every gene of
+ *     the call in one launch.
  */
 #include <algorithm>
 #include <cstring>
@@ -1041,6 +1043,38 @@ __global__ void bma_kernel(double* __restrict__ w, int64_t NP, int S, int n_cfg,
   }
 }
 
+/* Configuration masks (eqbf_config_masks: gsl_combination order, sizes 1..S,
+ * lexicographic within a size) and 1 / C(S, k), on the device (the same
+ * arithmetic as the host's upload, so the same bits). */
+__global__ void config_masks_kernel(int bfs, int S, uint64_t* __restrict__ masks, double* __restrict__ inv_choose) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int64_t k = 0;
+  if (bfs == EQBF_BFS_SIN) {
+    for (int s = 0; s < S; ++s) masks[k++] = 1ull << s;
+  } else if (bfs == EQBF_BFS_ALL) {
+    int c[64];
+    for (int sz = 1; sz <= S; ++sz) {
+      for (int i = 0; i < sz; ++i) c[i] = i;
+      while (true) {
+        uint64_t m = 0;
+        for (int i = 0; i < sz; ++i) m |= 1ull << c[i];
+        masks[k++] = m;
+        int i = sz - 1;
+        while (i >= 0 && c[i] == S - sz + i) --i;
+        if (i < 0) break;
+        ++c[i];
+        for (int j = i + 1; j < sz; ++j) c[j] = c[j - 1] + 1;
+      }
+    }
+  }
+  for (int kk = 0; kk < 65; ++kk) inv_choose[kk] = 0.0;
+  for (int kk = 1; kk <= S; ++kk) {
+    double ch = 1;
+    for (int i = 1; i <= kk; ++i) ch = ch * (S - kk + i) / i;
+    inv_choose[kk] = 1.0 / ch;
+  }
+}
+
 /* ---- plain FP64 path: one launch per gene, one thread per pair ------------------ */
 __device__ double abf_plain(uint64_t mask, int S, const double* st_b, const double* st_se, const double* st_t,
                             const uint8_t* has, double phi2, double oma2) {
@@ -1137,11 +1171,30 @@ __device__ __forceinline__ void plain_pair_abfs(const AssocParams& kp, int S, in
   }
 }
 
+/* g >= 0: pairs of gene g (the plain path, one launch per gene); g < 0:
+ * every pair of the chunk in one launch, thread = pair, its gene found by
+ * binary search of pair_offset (EQBF_PREC_FP64_BATCHED).  Each pair's
+ * arithmetic is the same either way. */
 __global__ void __launch_bounds__(64) assoc_plain_kernel(const eqbf_chunk c, const AssocParams kp, int g) {
   const int S = c.n_subgroups, N = c.n_samples;
   const int64_t NP = c.n_pairs;
-  const int64_t p = c.pair_offset[g] + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (p >= c.pair_offset[g + 1]) return;
+  int64_t p;
+  if (g >= 0) {
+    p = c.pair_offset[g] + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= c.pair_offset[g + 1]) return;
+  } else {
+    p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= NP) return;
+    int lo = 0, hi = c.n_genes; /* pair_offset[lo] <= p < pair_offset[hi] */
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (c.pair_offset[mid] <= p)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    g = lo;
+  }
   const int64_t snp = c.pair_snp[p];
   const int KC = kp.KU + 1;
   double st_b[PLAIN_SMAX], st_se[PLAIN_SMAX], st_t[PLAIN_SMAX];
@@ -1311,7 +1364,7 @@ int launch_bma(eqbf_ctx* ctx, double* w, int64_t NP, int S, int bfs, hipStream_t
 }
 
 int assoc_uvlr_run(eqbf_ctx* ctx, const eqbf_chunk* cp, const eqbf_assoc_opts* o, eqbf_assoc_out* out,
-                   hipStream_t st) {
+                   hipStream_t st, ScratchArena* arena) {
   const eqbf_chunk& c = *cp;
   const int S = c.n_subgroups, N = c.n_samples, G = c.n_genes;
   if (S < 1 || S > 64 || N < 1 || G < 0 || c.n_covars_max < 0 || c.n_covars_max > 16)
@@ -1328,7 +1381,9 @@ int assoc_uvlr_run(eqbf_ctx* ctx, const eqbf_chunk* cp, const eqbf_assoc_opts* o
   int64_t n_cfg = join ? eqbf_n_configs(o->bfs, S) : 0;
   if (n_cfg < 0) return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED, "--bfs all with %d subgroups", S);
   if (G == 0 || c.n_pairs == 0) return EQBF_OK;
-  if (o->precision == EQBF_PREC_FP64_PLAIN && (GL > PLAIN_GMAX || GS > PLAIN_GMAX))
+  const bool plain = o->precision == EQBF_PREC_FP64_PLAIN || o->precision == EQBF_PREC_FP64_BATCHED;
+  if (!plain && o->precision != EQBF_PREC_MIXED) return ctx_fail(ctx, EQBF_ERR_ARG, "invalid precision %d", o->precision);
+  if (plain && (GL > PLAIN_GMAX || GS > PLAIN_GMAX))
     return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED, "plain path supports grids of at most %d points", PLAIN_GMAX);
   int KU = 1;
   while (KU < c.n_covars_max + 1) KU = (KU == 1) ? 4 : KU * 2;
@@ -1337,21 +1392,30 @@ int assoc_uvlr_run(eqbf_ctx* ctx, const eqbf_chunk* cp, const eqbf_assoc_opts* o
     return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED, "mixed path supports at most 7 covariates");
   const int KC = KU + 1;
 
-  PrepRec* rec = (PrepRec*)ctx_buf(ctx, "rec", (size_t)G * S * sizeof(PrepRec));
-  double* B = (double*)ctx_buf(ctx, "B", (size_t)G * N * S * KC * sizeof(double));
-  uint64_t* memb = (uint64_t*)ctx_buf(ctx, "memb", (size_t)G * N * sizeof(uint64_t));
-  int32_t* idx = (int32_t*)ctx_buf(ctx, "idx", (size_t)G * S * N * sizeof(int32_t));
+  if (arena && !plain)
+    return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED, "the entry point without allocation runs the FP64 precisions only");
+  /* scratch of the call: the context's named buffers, or (arena) slices of
+   * the caller-less static scratch array, sized by assoc_scratch_bytes */
+  auto buf = [&](const char* name, size_t bytes) -> void* { return arena ? arena->take(bytes) : ctx_buf(ctx, name, bytes); };
+  PrepRec* rec = (PrepRec*)buf("rec", (size_t)G * S * sizeof(PrepRec));
+  double* B = (double*)buf("B", (size_t)G * N * S * KC * sizeof(double));
+  uint64_t* memb = (uint64_t*)buf("memb", (size_t)G * N * sizeof(uint64_t));
+  int32_t* idx = (int32_t*)buf("idx", (size_t)G * S * N * sizeof(int32_t));
   const size_t max_tiles = (size_t)(c.n_pairs / TILE) + (size_t)G + 1;
-  int4* tiles = (int4*)ctx_buf(ctx, "tiles", max_tiles * sizeof(int4));
-  int* n_tiles = (int*)ctx_buf(ctx, "ntiles", 256);
-  uint64_t* d_masks = (uint64_t*)ctx_buf(ctx, "masks", (size_t)std::max<int64_t>(n_cfg, 1) * sizeof(uint64_t) + 65 * sizeof(double));
-  if (!rec || !B || !memb || !idx || !tiles || !n_tiles || !d_masks) return EQBF_ERR_HIP;
+  int4* tiles = arena ? nullptr : (int4*)buf("tiles", max_tiles * sizeof(int4));
+  int* n_tiles = arena ? nullptr : (int*)buf("ntiles", 256);
+  uint64_t* d_masks = (uint64_t*)buf("masks", (size_t)std::max<int64_t>(n_cfg, 1) * sizeof(uint64_t) + 65 * sizeof(double));
+  if (!rec || !B || !memb || !idx || (!arena && (!tiles || !n_tiles)) || !d_masks)
+    return arena ? ctx_fail(ctx, EQBF_ERR_ARG, "the call needs more scratch than the static array holds") : EQBF_ERR_HIP;
   double* d_inv_choose = (double*)(d_masks + std::max<int64_t>(n_cfg, 1));
   /* the masks stay on the device until (bfs, S) or the buffer changes, so
    * that a repeated call enqueues kernels only (hipGraph capture) */
   char mkey[96];
   snprintf(mkey, sizeof mkey, "%d/%d/%p/%llu", (int)o->bfs, S, (void*)d_masks, (unsigned long long)ctx->buf_epoch);
-  if (n_cfg > 0 && ctx->masks_key != mkey) {
+  if (arena) {
+    hipLaunchKernelGGL(config_masks_kernel, dim3(1), dim3(64), 0, st, n_cfg > 0 ? (int)o->bfs : (int)EQBF_BFS_GEN, S,
+                       d_masks, d_inv_choose);
+  } else if (n_cfg > 0 && ctx->masks_key != mkey) {
     ctx->masks_key = mkey;
     std::vector<uint64_t> hm(n_cfg);
     eqbf_config_masks(o->bfs, S, hm.data());
@@ -1365,7 +1429,7 @@ int assoc_uvlr_run(eqbf_ctx* ctx, const eqbf_chunk* cp, const eqbf_assoc_opts* o
     EQBF_HIP(ctx, hipMemcpyAsync(d_inv_choose, ic.data(), 65 * sizeof(double), hipMemcpyHostToDevice, st));
     EQBF_HIP(ctx, hipStreamSynchronize(st)); /* host vectors go out of scope */
   }
-  int rc = ctx_ensure_tz(ctx, N, st);
+  int rc = arena ? EQBF_OK : ctx_ensure_tz(ctx, N, st); /* the FP64 kernels do not read the tables */
   if (rc) return rc;
 
   const bool fast = o->precision == EQBF_PREC_MIXED && uvlr_fast_eligible(c, KU);
@@ -1425,16 +1489,21 @@ int assoc_uvlr_run(eqbf_ctx* ctx, const eqbf_chunk* cp, const eqbf_assoc_opts* o
   kp.raw_tiled = o->raw_dtype == EQBF_F32_TILED;
   kp.multiset = c.n_geno_sets > 1;
 
-  if (o->precision == EQBF_PREC_FP64_PLAIN) {
+  if (plain) {
     if (S > PLAIN_SMAX) return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED, "plain path supports at most 64 subgroups");
-    std::vector<int64_t> off(G + 1);
-    EQBF_HIP(ctx, hipMemcpyAsync(off.data(), c.pair_offset, (G + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, st));
-    EQBF_HIP(ctx, hipStreamSynchronize(st));
-    for (int g = 0; g < G; ++g) {
-      int64_t np = off[g + 1] - off[g];
-      if (np <= 0) continue;
+    if (o->precision == EQBF_PREC_FP64_BATCHED) {
       ProfScope ps(ctx, "assoc_plain_kernel", st);
-      hipLaunchKernelGGL(assoc_plain_kernel, dim3((unsigned)((np + 63) / 64)), dim3(64), 0, st, c, kp, g);
+      hipLaunchKernelGGL(assoc_plain_kernel, dim3((unsigned)((c.n_pairs + 63) / 64)), dim3(64), 0, st, c, kp, -1);
+    } else {
+      std::vector<int64_t> off(G + 1);
+      EQBF_HIP(ctx, hipMemcpyAsync(off.data(), c.pair_offset, (G + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+      EQBF_HIP(ctx, hipStreamSynchronize(st));
+      for (int g = 0; g < G; ++g) {
+        int64_t np = off[g + 1] - off[g];
+        if (np <= 0) continue;
+        ProfScope ps(ctx, "assoc_plain_kernel", st);
+        hipLaunchKernelGGL(assoc_plain_kernel, dim3((unsigned)((np + 63) / 64)), dim3(64), 0, st, c, kp, g);
+      }
     }
     EQBF_HIP(ctx, hipGetLastError());
     if (join && n_cfg > 0) {
@@ -1563,3 +1632,4 @@ int abf_from_sstats_run(eqbf_ctx* ctx, const eqbf_sstats_in* in, int bfs, int ra
   EQBF_HIP(ctx, hipGetLastError());
   return EQBF_OK;
 }
+

@@ -345,8 +345,22 @@ __device__ inline int uvlr_stats(const PrepRec& r, bool has, double sxx, double 
 }
 
 /* ---- kernels launched from the C-ABI layer ------------------------------- */
+/* Bump allocator over a fixed device region: 256-byte aligned slices,
+ * nullptr when the region is exhausted. */
+struct ScratchArena {
+  uint8_t* base = nullptr;
+  size_t cap = 0, used = 0;
+  void* take(size_t bytes) {
+    const size_t off = (used + 255) & ~(size_t)255;
+    if (bytes == 0) bytes = 256;
+    if (off + bytes > cap) return nullptr;
+    used = off + bytes;
+    return base + off;
+  }
+};
 int assoc_uvlr_run(eqbf_ctx* ctx, const eqbf_chunk* c, const eqbf_assoc_opts* o, eqbf_assoc_out* out,
-                   hipStream_t st);
+                   hipStream_t st, ScratchArena* arena = nullptr);
+
 /* packed work list of 64-pair tiles of every gene (one block scan) */
 void launch_tiles(const int64_t* pair_offset, int G, int4* tiles, int* n_tiles, hipStream_t st);
 void launch_zero_u64(uint64_t* p, int64_t n, hipStream_t st);
@@ -363,6 +377,11 @@ int abf_from_sstats_run(eqbf_ctx* ctx, const eqbf_sstats_in* in, int bfs, int ra
 int launch_bma(eqbf_ctx* ctx, double* w, int64_t n_pairs, int S, int bfs, hipStream_t st);
 int perm_stats_run(eqbf_ctx* ctx, const eqbf_chunk* c, const eqbf_perm_opts* po, const int32_t* tables_dev,
                    int32_t n_tables, const int32_t* gene_table, int64_t nperm, double* stats_dev, hipStream_t st);
+/* perm_stats_run, then the FP64 true statistic of every analysed gene and an
+ * FP64 re-computation of every permutation statistic within tie_bound of it */
+int perm_ties_run(eqbf_ctx* ctx, const eqbf_chunk* c, const eqbf_perm_opts* po, const int32_t* tables_dev,
+                  int32_t n_tables, const int32_t* gene_table, int64_t nperm, double tie_bound, double* stats_dev,
+                  double* true_dev, int64_t* n_rechecked, hipStream_t st);
 /* contraction path for the genes with full membership; done[g] = 1 for the
  * genes it finished (the others go to the general path) */
 int perm_fast_run(eqbf_ctx* ctx, const eqbf_chunk* c, const eqbf_perm_opts* po, const int32_t* tables_dev,

@@ -48,6 +48,8 @@ def lib():
         L.eqbf_debug_ttail.argtypes = [vp, vp, vp, i32, vp, vp, vp, vp]
         L.eqbf_perm_stats.argtypes = [vp, vp, vp, vp, i32, vp, i64, vp]
         L.eqbf_perm_stats_device.argtypes = [vp, vp, vp, vp, i32, vp, i64, vp, vp]
+        L.eqbf_perm_stats_fp64_ties.argtypes = [vp, vp, vp, vp, i32, vp, i64, C.c_double, vp, vp, vp]
+        L.eqbf_perm_stats_fp64_ties_device.argtypes = [vp, vp, vp, vp, i32, vp, i64, C.c_double, vp, vp, vp, vp]
         L.eqbf_rng_seed.argtypes = [vp, C.c_uint64]
         L.eqbf_rng_get.argtypes = [vp]
         L.eqbf_rng_get.restype = C.c_uint32
@@ -251,6 +253,37 @@ class Engine:
         self._check(lib().eqbf_perm_stats(self._ctx, C.byref(st), C.byref(po), _p(tables), tables.shape[0],
                                           _p(gt), nperm, _p(out)))
         return out
+
+    def perm_stats_fp64_ties(self, hc, tables, gene_table, analysis="join", pbf="all", use_max_bf=True,
+                             sep_sbgrp=-1, precision="mixed", tie_bound=5e-5):
+        """perm_stats, then the FP64 true statistic of every analysed gene and
+        an FP64 re-computation of the permutation statistics within tie_bound
+        of it (absolute for join, relative for sep).  Returns (stats
+        [n_genes][nperm], true_stats [n_genes], number re-computed)."""
+        tables = np.ascontiguousarray(tables, dtype=np.int32)
+        gt = np.ascontiguousarray(gene_table, dtype=np.int32)
+        nperm = tables.shape[1]
+        out = np.full((hc.G, nperm), np.nan)
+        true = np.full(hc.G, np.nan)
+        n_re = C.c_int64(0)
+        po = PermOpts(_abi.ANALYSIS[analysis], _abi.PBF[pbf], int(use_max_bf), sep_sbgrp, _abi.PREC[precision])
+        st = hc.struct()
+        self._check(lib().eqbf_perm_stats_fp64_ties(self._ctx, C.byref(st), C.byref(po), _p(tables), tables.shape[0],
+                                                    _p(gt), nperm, float(tie_bound), _p(out), _p(true),
+                                                    C.byref(n_re)))
+        return out, true, n_re.value
+
+    def perm_stats_fp64_ties_device(self, chunk_struct, opts, tables_ptr, n_tables, gene_table, nperm, tie_bound,
+                                    stats_ptr, true_ptr, stream=None):
+        """Device-pointer entry of perm_stats_fp64_ties; returns the number
+        of statistics re-computed."""
+        gt = np.ascontiguousarray(gene_table, dtype=np.int32)
+        n_re = C.c_int64(0)
+        self._check(lib().eqbf_perm_stats_fp64_ties_device(self._ctx, C.byref(chunk_struct), C.byref(opts),
+                                                           tables_ptr, int(n_tables), _p(gt), int(nperm),
+                                                           float(tie_bound), stats_ptr, true_ptr, C.byref(n_re),
+                                                           stream))
+        return n_re.value
 
     def perm_stats_device(self, chunk_struct, opts, tables_ptr, n_tables, gene_table, nperm, stats_ptr, stream=None):
         """Device-pointer entry: tables_ptr [n_tables][nperm][N] int32 and

@@ -5,8 +5,8 @@
  * output files.  Mirrors run() in src/eqtlbma_bf.cpp:1449-1582:
  * testForAssociations (:707-771) -> eqbf_assoc_uvlr / eqbf_assoc_mvlr;
  * makePermutations (:867-917) -> host index tables with the reference's
- * generator and reseeding, device statistics (eqbf_perm_stats), host
- * counting (eqbf_perm_pvalue); writeRes (:1401-1447) and its writers
+ * generator and reseeding, device statistics with FP64 true statistics and
+ * near-ties (eqbf_perm_stats_fp64_ties), host counting (eqbf_perm_pvalue); writeRes (:1401-1447) and its writers
  * (:919-1399) for the file formats.
  */
 #include <algorithm>
@@ -449,47 +449,129 @@ double l10_avg(const std::vector<double>& v) {
   return res;
 }
 
+/* Index tables of consecutive genes continuing one generator: each gene
+ * starts from the identity and takes nperm cumulative shuffles
+ * (gene.cpp:617,638-639).  The reference reseeds the generator once per
+ * chunk (eqtlbma_bf.cpp:889-916), so every chunk draws the same tables as
+ * long as no gene of it stops early (--trick 1) or loses permutations to NaN
+ * statistics: the tables and the generator state after each gene are kept
+ * across chunks, keyed by the generator state they start from, up to
+ * kMaxBytes of tables. */
+struct PermTableCache {
+  static constexpr size_t kMaxBytes = (size_t)1 << 30;
+  eqbf_rng start{};
+  int64_t N = -1, nperm = -1;
+  std::vector<int32_t> tab;    /* [count][nperm][N] */
+  std::vector<eqbf_rng> after; /* generator state after each gene */
+  static size_t cap_genes(int64_t n, int64_t np) {
+    return std::max<size_t>(1, kMaxBytes / ((size_t)np * n * sizeof(int32_t)));
+  }
+};
+
+bool same_rng(const eqbf_rng& a, const eqbf_rng& b) { return a.idx == b.idx && !std::memcmp(a.mt, b.mt, sizeof a.mt); }
+
+/* Tables for `n` genes from generator state `r`: a pointer to [n][nperm][N]
+ * and the state after each gene (from the cache when r is its start or the
+ * state after one of its genes; generated otherwise, into `local`). */
+const int32_t* tables_from(PermTableCache& cache, const eqbf_rng& r, int64_t N, int64_t nperm, size_t n,
+                           bool chunk_start, std::vector<int32_t>& local, std::vector<eqbf_rng>& states) {
+  const size_t per = (size_t)nperm * N;
+  long pos = -1;
+  if (cache.N == N && cache.nperm == nperm) {
+    if (same_rng(r, cache.start))
+      pos = 0;
+    else
+      for (size_t j = 0; j < cache.after.size() && pos < 0; ++j)
+        if (same_rng(r, cache.after[j])) pos = (long)j + 1;
+  }
+  if (pos < 0 && chunk_start) {
+    cache.start = r;
+    cache.N = N;
+    cache.nperm = nperm;
+    cache.tab.clear();
+    cache.after.clear();
+    pos = 0;
+  }
+  if (pos >= 0 && (size_t)pos + n <= PermTableCache::cap_genes(N, nperm)) {
+    while (cache.after.size() < (size_t)pos + n) {
+      eqbf_rng g = cache.after.empty() ? cache.start : cache.after.back();
+      const size_t k = cache.after.size();
+      cache.tab.resize((k + 1) * per);
+      eqbf_perm_tables(&g, N, 1, nperm, cache.tab.data() + k * per);
+      cache.after.push_back(g);
+    }
+    states.assign(cache.after.begin() + pos, cache.after.begin() + pos + (long)n);
+    return cache.tab.data() + (size_t)pos * per;
+  }
+  local.resize(n * per);
+  states.clear();
+  eqbf_rng g = r;
+  for (size_t j = 0; j < n; ++j) {
+    eqbf_perm_tables(&g, N, 1, nperm, local.data() + j * per);
+    states.push_back(g);
+  }
+  return local.data();
+}
+
+/* Near-tie bounds of the FP64 re-check: absolute on the log10 BF statistic
+ * (about 40 times the mixed path's largest measured error, 1.3e-6), relative
+ * on the sep p-value statistic. */
+constexpr double kTieJoin = 5e-5, kTieSep = 1e-6;
+
 /* Statistics of `genes` (chunk indices, in order) under the reference's
  * sequential use of one generator: each gene starts from the identity and
  * draws its shuffles from where the previous gene stopped; --trick 1 and
  * NaN statistics change how many draws a gene takes and how many the next
- * gene is allowed (nb_permutations is shared), so tables are generated
- * speculatively for the rest of the chunk and regenerated from the first
- * gene whose use differed.  count(g, stats) runs the counting loop of gene
- * g and returns the number of shuffles it drew. */
+ * gene is allowed (nb_permutations is shared), so tables are made for the
+ * rest of the chunk (in batches of at most the cache's size) and made again
+ * from the first gene whose use differed.  The library returns the FP64 true
+ * statistics (truth[g]) and recomputes in FP64 every permutation statistic
+ * within tie_bound of them.  count(g, stats, nb) runs the counting loop of
+ * gene g and returns the number of shuffles it drew. */
 template <class Count>
 void run_perms(Engine& eng, const Chunk& ch, const eqbf_perm_opts& po, const std::vector<int>& genes, int64_t N,
-               eqbf_rng& rng, int64_t& nb_perm, Count count) {
+               eqbf_rng& rng, int64_t& nb_perm, PermTableCache& cache, double tie_bound, std::vector<double>& truth,
+               Count count) {
+  truth.assign(ch.genes.size(), kNaN);
   size_t k = 0;
+  std::vector<int32_t> local;
+  std::vector<eqbf_rng> states;
+  std::vector<double> tr(ch.genes.size());
   while (k < genes.size()) {
     const int64_t nb_cur = nb_perm;
-    const size_t rest = genes.size() - k;
     if (nb_cur <= 0) {
       for (size_t j = k; j < genes.size(); ++j) count(genes[j], nullptr, nb_cur);
       return;
     }
-    eqbf_rng saved = rng;
-    std::vector<int32_t> tables((size_t)rest * nb_cur * N);
-    eqbf_perm_tables(&rng, N, (int64_t)rest, nb_cur, tables.data());
+    const size_t rest = genes.size() - k;
+    const size_t n = std::min(rest, PermTableCache::cap_genes(N, nb_cur));
+    const int32_t* tables = tables_from(cache, rng, N, nb_cur, n, k == 0, local, states);
     std::vector<int32_t> gene_table(ch.genes.size(), -1);
-    for (size_t j = 0; j < rest; ++j) gene_table[genes[k + j]] = (int32_t)j;
+    for (size_t j = 0; j < n; ++j) gene_table[genes[k + j]] = (int32_t)j;
     std::vector<double> stats((size_t)ch.genes.size() * nb_cur, kNaN);
-    eng.check(eqbf_perm_stats(eng.ctx, &ch.c, &po, tables.data(), (int32_t)rest, gene_table.data(), nb_cur,
-                              stats.data()));
-    rng = saved;
-    std::vector<int32_t> perm(N);
-    size_t j = k;
-    for (; j < genes.size(); ++j) {
+    int64_t n_re = 0;
+    eng.check(eqbf_perm_stats_fp64_ties(eng.ctx, &ch.c, &po, tables, (int32_t)n, gene_table.data(), nb_cur,
+                                       tie_bound, stats.data(), tr.data(), &n_re));
+    for (size_t j = 0; j < n; ++j) truth[genes[k + j]] = tr[genes[k + j]];
+    size_t j = 0;
+    for (; j < n; ++j) {
       const int64_t before = nb_perm;
-      const int64_t drawn = count(genes[j], &stats[(size_t)genes[j] * nb_cur], nb_cur);
-      std::iota(perm.begin(), perm.end(), 0);
-      for (int64_t p = 0; p < drawn; ++p) eqbf_rng_shuffle(&rng, perm.data(), N);
+      const eqbf_rng at = rng;
+      const int64_t drawn = count(genes[k + j], &stats[(size_t)genes[k + j] * nb_cur], nb_cur);
+      if (drawn == nb_cur) {
+        rng = states[j];
+      } else { /* stopped early: the stream advanced by the shuffles drawn */
+        rng = at;
+        std::vector<int32_t> perm(N);
+        std::iota(perm.begin(), perm.end(), 0);
+        for (int64_t p = 0; p < drawn; ++p) eqbf_rng_shuffle(&rng, perm.data(), N);
+      }
       if (drawn != nb_cur || nb_perm != before) {
         ++j;
         break;
       }
     }
-    k = j;
+    k += j;
   }
 }
 
@@ -743,8 +825,10 @@ int main(int argc, char** argv) {
   std::vector<const Gene*> all;
   for (auto& kv : d.genes) all.push_back(&kv.second);
   int64_t nb_perm = (int64_t)o.nb_permutations;
+  PermTableCache tcache;
   size_t n_genes_done = 0, n_pairs_done = 0;
-  const int prec = o.precision == "fp64" ? EQBF_PREC_FP64_PLAIN : EQBF_PREC_MIXED;
+  /* fp64: the plain path's arithmetic, all genes of a chunk in one launch */
+  const int prec = o.precision == "fp64" ? EQBF_PREC_FP64_BATCHED : EQBF_PREC_MIXED;
   for (size_t g0 = 0; g0 < all.size(); g0 += (size_t)o.write_group_size) {
     const size_t g1 = std::min(all.size(), g0 + (size_t)o.write_group_size);
     Chunk ch;
@@ -796,6 +880,7 @@ int main(int argc, char** argv) {
       };
       eqbf_perm_opts po{};
       po.precision = prec;
+      std::vector<double> truth;
       if (perm_join) {
         reseed();
         po.analysis = EQBF_ANALYSIS_JOIN;
@@ -818,10 +903,12 @@ int main(int argc, char** argv) {
           }
           pj[g].true_stat = t;
         }
-        run_perms(eng, ch, po, genes, N, rng, nb_perm, [&](int g, const double* st, int64_t n_all) -> int64_t {
+        run_perms(eng, ch, po, genes, N, rng, nb_perm, tcache, kTieJoin, truth,
+                  [&](int g, const double* st, int64_t n_all) -> int64_t {
           PermRes& pr = pj[g];
           pr.done = true;
           (void)n_all;
+          if (st) pr.true_stat = truth[g]; /* FP64, from the library */
           static const double none = 0.0; /* no statistic is read when no permutation is left */
           return eqbf_perm_pvalue(st ? st : &none, pr.true_stat, 0, &nb_perm, o.trick, (int64_t)o.trick_cutoff,
                                   o.trick ? &rngT : nullptr, &pr.pval, &pr.used, &pr.median);
@@ -843,9 +930,11 @@ int main(int argc, char** argv) {
               }
             ps[pass][g].true_stat = t;
           }
-          run_perms(eng, ch, po, genes, N, rng, nb_perm, [&](int g, const double* st, int64_t) -> int64_t {
+          run_perms(eng, ch, po, genes, N, rng, nb_perm, tcache, kTieSep, truth,
+                    [&](int g, const double* st, int64_t) -> int64_t {
             PermRes& pr = ps[pass][g];
             pr.done = true;
+            if (st) pr.true_stat = truth[g]; /* FP64, from the library */
             static const double none = 0.0; /* no statistic is read when no permutation is left */
             return eqbf_perm_pvalue(st ? st : &none, pr.true_stat, 1, &nb_perm, o.trick, (int64_t)o.trick_cutoff,
                                     o.trick ? &rngT : nullptr, &pr.pval, &pr.used, nullptr);

@@ -61,12 +61,14 @@ enum { EQBF_ANALYSIS_SEP = 0, EQBF_ANALYSIS_JOIN = 1 };         /* --analys */
 enum { EQBF_BFS_GEN = 0, EQBF_BFS_SIN = 1, EQBF_BFS_ALL = 2 };  /* --bfs    */
 enum { EQBF_PBF_GEN = 0, EQBF_PBF_GENSIN = 1, EQBF_PBF_ALL = 2 };/* --pbf   */
 /* precision / batching mode:
- *  FP64_PLAIN: every quantity in FP64, one kernel launch per gene, reference
- *              loop structure (the "plain GPU" comparison path);
- *  MIXED:      batched persistent launch over all genes of the chunk, FP64
- *              sums and residual sums of squares, FP32 Bayes-factor terms,
- *              FP64 log10-sum-exp. */
-enum { EQBF_PREC_FP64_PLAIN = 0, EQBF_PREC_MIXED = 1 };
+ *  FP64_PLAIN:   every quantity in FP64, one kernel launch per gene, reference
+ *                loop structure (the "plain GPU" comparison path);
+ *  MIXED:        batched persistent launch over all genes of the chunk, FP64
+ *                sums and residual sums of squares, FP32 Bayes-factor terms,
+ *                FP64 log10-sum-exp;
+ *  FP64_BATCHED: the FP64_PLAIN arithmetic, bit for bit, with every gene of
+ *                the call in one launch. */
+enum { EQBF_PREC_FP64_PLAIN = 0, EQBF_PREC_MIXED = 1, EQBF_PREC_FP64_BATCHED = 2 };
 enum { EQBF_F32 = 0, EQBF_F64 = 1 };
 /* raw_dtype only: FP32 raw ABFs in tiles of 64 pairs (no counterpart in the
  * reference, whose raw ABFs live in per-pair maps): element (row, p) at
@@ -242,6 +244,34 @@ int eqbf_perm_stats(eqbf_ctx* ctx, const eqbf_chunk* chunk, const eqbf_perm_opts
 int eqbf_perm_stats_device(eqbf_ctx* ctx, const eqbf_chunk* chunk_dev, const eqbf_perm_opts* opts,
                            const int32_t* perm_idx_dev, int32_t n_tables, const int32_t* gene_table,
                            int64_t nperm, double* stats_dev, void* hip_stream);
+
+/* Near-ties in FP64.  The reference computes every statistic in FP64 and
+ * counts a permutation when its statistic is >= the true one (join) or <= it
+ * (sep) (gene.cpp:685, 697, 435, 555).  With the MIXED
+ * precision a statistic carries a small error (about 1e-6 in log10 BF), so a
+ * permutation whose statistic lies that close to the true one could be
+ * counted differently.  This entry point is eqbf_perm_stats followed by:
+ *  - true_stats[g]: the true statistic of every analysed gene in FP64 (the
+ *    FP64_PLAIN arithmetic) with the reference's semantics:
+ *    Gene::FindMaxTrueL10Abf / AvgTrueL10Abfs (gene.cpp:575-596: NaN never
+ *    wins the maximum, -inf when every SNP is NaN; the average runs over
+ *    the non-NaN SNPs) and FindMinTruePvalue* (gene.cpp:369-378, 475-488:
+ *    starts from 1); NaN for a skipped gene or one without pairs;
+ *  - every permutation statistic within tie_bound of it (absolute for join,
+ *    tie_bound * |true| for sep) recomputed in FP64 and replaced.
+ * The counting loop (eqbf_perm_pvalue) then sees FP64 values wherever the
+ * comparison could depend on the precision.  n_rechecked (may be NULL):
+ * number of permutation statistics recomputed.  With an FP64 precision the
+ * statistics are FP64 already and only the true statistics are added.
+ * No counterpart in the reference (which has no mixed precision). */
+int eqbf_perm_stats_fp64_ties(eqbf_ctx* ctx, const eqbf_chunk* chunk, const eqbf_perm_opts* opts,
+                              const int32_t* perm_idx, int32_t n_tables, const int32_t* gene_table,
+                              int64_t nperm, double tie_bound, double* stats, double* true_stats,
+                              int64_t* n_rechecked);
+int eqbf_perm_stats_fp64_ties_device(eqbf_ctx* ctx, const eqbf_chunk* chunk_dev, const eqbf_perm_opts* opts,
+                                     const int32_t* perm_idx_dev, int32_t n_tables, const int32_t* gene_table,
+                                     int64_t nperm, double tie_bound, double* stats_dev, double* true_stats_dev,
+                                     int64_t* n_rechecked, void* hip_stream);
 
 /* ---- host random numbers with GSL's semantics ----------------------------------
  * gsl_rng_mt19937 + gsl_rng_set / gsl_rng_uniform_int / gsl_ran_shuffle /

@@ -90,6 +90,12 @@ class MT:
         self._st = (C.c_uint32 * 625)()
         lib().orc_mt_seed(C.addressof(self._st), seed)
 
+    def copy(self):
+        m = MT.__new__(MT)
+        m._st = (C.c_uint32 * 625)()
+        C.memmove(m._st, self._st, C.sizeof(self._st))
+        return m
+
     def next(self):
         return lib().orc_mt_next(C.addressof(self._st))
 

@@ -22,8 +22,10 @@ def _w(path, lines):
             f.write(l + "\n")
 
 
-def make_case(root, seed=3, S=3, N=36, mvlr=False, covariates=True):
-    """Writes the input files under root; returns a dict describing them."""
+def make_case(root, seed=3, S=3, N=36, mvlr=False, covariates=True, near_mono=None):
+    """Writes the input files under root; returns a dict describing them.
+    near_mono: index of a SNP made monomorphic but for one carrier (its
+    statistics are NaN whenever that carrier is not among the members)."""
     rng = np.random.default_rng(seed)
     samples = ["ind%02d" % (i + 1) for i in range(N)]
     sbgrps = ["s%d" % (s + 1) for s in range(S)]
@@ -34,6 +36,9 @@ def make_case(root, seed=3, S=3, N=36, mvlr=False, covariates=True):
             snps.append(("rs%s_%d" % (chrom[-1], k), chrom, 1000 + 400 * k))
     maf = rng.uniform(0.1, 0.5, size=len(snps))
     G = rng.binomial(2, maf[:, None], size=(len(snps), N)).astype(float)
+    if near_mono is not None:
+        G[near_mono] = 1.0
+        G[near_mono, 0] = 2.0
     genes = [("geneB", "chr1", 3000, 4000), ("geneA", "chr1", 9000, 9500), ("geneD", "chr2", 5000, 5200),
              ("geneC", "chr2", 12000, 13000), ("geneE", "chr2", 90000, 91000),  # no cis SNP
              ("geneF", "chr1", 6000, 7000)]
@@ -46,6 +51,8 @@ def make_case(root, seed=3, S=3, N=36, mvlr=False, covariates=True):
         else:
             e = rng.random(N) < 0.85
             g = rng.random(N) < 0.85
+        if near_mono is not None:
+            e[0] = g[0] = True  # the carrier is in every file
         exp_in.append(e)
         geno_in.append(g)
     Y = rng.normal(size=(len(genes), S, N))

@@ -5,10 +5,10 @@ like the reference, once per chunk); both sides evaluate the same tables.
 Tolerances: plain FP64 path 1e-9 absolute on each permutation statistic
 (log10 BF) or 1e-10 relative (p-values); mixed path 1e-4 absolute on the
 log10 BF statistic, 1e-6 relative on p-values.  Exceedance counts and
-p-values from the reference's counting loop must then agree exactly for the
-plain path; for the mixed path they agree except where a permutation
-statistic lies within the tolerance of the true statistic (near-ties,
-reported by the test).
+p-values from the reference's counting loop must then agree exactly: on the
+FP64 paths directly, on the mixed path after eqbf_perm_stats_fp64_ties has
+recomputed in FP64 the true statistics and every permutation statistic
+within 5e-5 of them (about 40 times the mixed path's measured error).
 """
 import sys
 
@@ -59,7 +59,29 @@ def _check(gpu, ref, mode, analysis):
         assert np.allclose(gpu[fin], ref[fin], rtol=rtol, atol=1e-300)
 
 
-@pytest.mark.parametrize("mode", ["plain", "mixed"])
+def _count_parity(eng, orc, hc, tables, gene_table, nperm, pbf, use_max, mode, ref):
+    """Counting loop on both sides: the library's FP64 true statistics and
+    re-checked statistics against the oracle's (FP64) ones; counts, used
+    permutations and p-values identical."""
+    gpu, true_gpu, n_re = eng.perm_stats_fp64_ties(hc, tables, gene_table, "join", pbf, use_max, precision=mode)
+    _check(gpu, ref, mode, "join")
+    true = orc.assoc_uvlr(hc, "join", "all", GL25, GS10)
+    nc = 2 ** hc.S - 1
+    row = {"all": 3 + nc + 1, "gen-sin": 3 + nc, "gen": 0}[pbf]
+    for g in range(hc.G):
+        if gene_table[g] < 0:
+            assert np.isnan(true_gpu[g])
+            continue
+        w = true.w[row, hc.pair_offset[g]:hc.pair_offset[g + 1]]
+        t = orc.true_stat(w, use_max)
+        assert abs(true_gpu[g] - t) <= 1e-9, (g, true_gpu[g], t)
+        a = engine.perm_pvalue(gpu[g], true_gpu[g], nperm)
+        b = orc.perm_count(ref[g], t, nperm)
+        assert a[0] == b[0] and a[1] == b[1], (g, a, b)
+    return n_re
+
+
+@pytest.mark.parametrize("mode", ["plain", "fp64_batched", "mixed"])
 @pytest.mark.parametrize("pbf,use_max", [("all", True), ("gen-sin", False)])
 def test_join_perm_stats(eng, orc, mode, pbf, use_max):
     hc = synth.plain(seed=7, S=3, N=40, n_genes=4, snps_per_gene=12, frac_egenes=0.5)
@@ -69,20 +91,50 @@ def test_join_perm_stats(eng, orc, mode, pbf, use_max):
     _say("[gpu] perm join mode=%s pbf=%s max=%s" % (mode, pbf, use_max))
     gpu = eng.perm_stats(hc, tables, gene_table, "join", pbf, use_max, precision=mode)
     ref = _oracle_stats(orc, hc, tables, gene_table, "join", pbf, use_max, -1)
-    _check(gpu, ref, mode, "join")
-    # the reference's counting loop on both sides
-    true = orc.assoc_uvlr(hc, "join", "all", GL25, GS10)
-    row = {"all": 3 + 7 + 1, "gen-sin": 3 + 7}[pbf]
-    for g in range(hc.G):
-        if gene_table[g] < 0:
-            continue
-        w = true.w[row, hc.pair_offset[g]:hc.pair_offset[g + 1]]
-        t = orc.true_stat(w, use_max)
-        a = engine.perm_pvalue(gpu[g], t, nperm)
-        b = orc.perm_count(ref[g], t, nperm)
-        near = np.sum(np.abs(ref[g] - t) <= 1e-4)
-        if mode == "plain" or near == 0:
-            assert a[0] == b[0] and a[1] == b[1]
+    _check(gpu, ref, "mixed" if mode == "mixed" else "plain", "join")
+    if mode == "fp64_batched":
+        plain = eng.perm_stats(hc, tables, gene_table, "join", pbf, use_max, precision="plain")
+        assert np.array_equal(gpu, plain, equal_nan=True)
+    n_re = _count_parity(eng, orc, hc, tables, gene_table, nperm, pbf, use_max, mode, ref)
+    _say("[gpu]   re-checked in FP64: %d" % n_re)
+    if mode != "mixed":
+        assert n_re == 0
+
+
+def test_exact_ties_count_on_both_sides(eng, orc):
+    # identity permutations reproduce the true statistic exactly: the
+    # reference counts them (>=); on the mixed path they are near-ties and
+    # their FP64 re-check gives the true statistic's own arithmetic
+    hc = synth.plain(seed=19, S=3, N=40, n_genes=3, snps_per_gene=10, frac_egenes=0.67)
+    nperm = 30
+    tables = engine.Rng(9).tables(hc.N, 3, nperm)
+    for k in range(3):
+        tables[k, 3] = np.arange(hc.N)
+        tables[k, 17] = np.arange(hc.N)
+    gene_table = np.arange(3, dtype=np.int32)
+    _say("[gpu] exact ties (identity permutations)")
+    ref = _oracle_stats(orc, hc, tables, gene_table, "join", "all", True, -1)
+    n_re = _count_parity(eng, orc, hc, tables, gene_table, nperm, "all", True, "mixed", ref)
+    assert n_re >= 6
+    gpu, true_gpu, _ = eng.perm_stats_fp64_ties(hc, tables, gene_table, "join", "all", True, precision="mixed")
+    for k in range(3):
+        assert gpu[k, 3] == true_gpu[k] and gpu[k, 17] == true_gpu[k]
+
+
+def test_every_statistic_rechecked_equals_fp64(eng, orc):
+    # a bound that covers every statistic: the mixed call returns the FP64
+    # path's statistics (contraction genes and general-path genes alike)
+    hc = synth.plain(seed=29, S=4, N=48, n_genes=3, snps_per_gene=14, frac_egenes=0.34)
+    hc.explevels[2, 1, 5] = np.nan  # gene 2 takes the general path
+    nperm = 24
+    tables = engine.Rng(4).tables(hc.N, 3, nperm)
+    gene_table = np.arange(3, dtype=np.int32)
+    _say("[gpu] every statistic re-checked")
+    plain = eng.perm_stats(hc, tables, gene_table, "join", "all", True, precision="plain")
+    gpu, true_gpu, n_re = eng.perm_stats_fp64_ties(hc, tables, gene_table, "join", "all", True,
+                                                   precision="mixed", tie_bound=1e6)
+    assert n_re == int(np.isfinite(plain).sum())
+    assert np.array_equal(gpu, plain, equal_nan=True)
 
 
 @pytest.mark.parametrize("mode", ["plain", "mixed"])
@@ -111,3 +163,21 @@ def test_sep_perm_stats(eng, orc, mode, sbgrp):
     gpu = eng.perm_stats(hc, tables, gene_table, "sep", "gen", True, sep_sbgrp=sbgrp, precision=mode)
     ref = _oracle_stats(orc, hc, tables, gene_table, "sep", "gen", True, sbgrp)
     _check(gpu, ref, mode, "sep")
+    # FP64 true statistics (FindMinTruePvalue*) and re-checked near-ties:
+    # the counting loop (<=) gives the oracle's counts
+    gpu2, true_gpu, n_re = eng.perm_stats_fp64_ties(hc, tables, gene_table, "sep", "gen", True, sep_sbgrp=sbgrp,
+                                                    precision=mode, tie_bound=1e-5)
+    true = orc.assoc_uvlr(hc, "sep", "gen", GL25, None)
+    pv = true.sstats[4]
+    for g in range(hc.G):
+        t = 1.0
+        for p in range(hc.pair_offset[g], hc.pair_offset[g + 1]):
+            for s in range(hc.S):
+                if (sbgrp >= 0 and s != sbgrp) or not hc.gene_in_sbgrp[g, s]:
+                    continue
+                if true.n[s, p] > 0 and pv[s, p] < t:
+                    t = pv[s, p]
+        assert true_gpu[g] == pytest.approx(t, rel=1e-10, abs=0), (g, true_gpu[g], t)
+        a = engine.perm_pvalue(gpu2[g], true_gpu[g], nperm, sep=True)
+        b = orc.perm_count(ref[g], t, nperm, sep=True)
+        assert a[0] == b[0] and a[1] == b[1], (g, a, b)

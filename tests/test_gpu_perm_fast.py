@@ -4,9 +4,9 @@ hard-call genotypes, no covariates) against the CPU oracle.
 The index tables come from the library's GSL-semantics generator; both sides
 evaluate the same tables.  Tolerances (mixed precision): 1e-4 absolute on
 each permutation's log10 BF statistic, 1e-6 relative on the sep p-value
-statistics; the reference's counting loop then gives identical counts except
-where a permutation statistic lies within the tolerance of the true one
-(near-ties, counted and reported).  Each test also checks from the live
+statistics.  With the FP64 true statistics and the FP64 re-check of every
+permutation statistic within 5e-5 of them (eqbf_perm_stats_fp64_ties) the
+reference's counting loop gives the oracle's counts exactly.  Each test also checks from the live
 kernel timing which path ran: the contraction kernels (pf_*) or the general
 per-permutation path (perm_gather_kernel).
 """
@@ -95,17 +95,19 @@ def test_join_contraction_matches_oracle(eng, orc, pbf, use_max):
     _check(gpu, ref, "join")
     row, bfs = _true_w_row(pbf, hc.S)
     true = orc.assoc_uvlr(hc, "join", bfs, GL25, GS10)
+    chk, true_gpu, n_re = eng.perm_stats_fp64_ties(hc, tables, gene_table, "join", pbf, use_max, precision="mixed")
+    _check(chk, ref, "join")
     near_total = 0
     for g in range(hc.G):
         w = true.w[row, hc.pair_offset[g]:hc.pair_offset[g + 1]]
         t = orc.true_stat(w, use_max)
-        a = engine.perm_pvalue(gpu[g], t, nperm)
+        assert abs(true_gpu[g] - t) <= 1e-9, (g, true_gpu[g], t)
+        near_total += int(np.sum(np.abs(gpu[g] - true_gpu[g]) <= 5e-5))
+        a = engine.perm_pvalue(chk[g], true_gpu[g], nperm)
         b = orc.perm_count(ref[g], t, nperm)
-        near = int(np.sum(np.abs(ref[g] - t) <= 1e-4))
-        near_total += near
-        if near == 0:
-            assert a[0] == b[0] and a[1] == b[1]
-    _say("[gpu]   near-ties: %d" % near_total)
+        assert a[0] == b[0] and a[1] == b[1], (g, a, b)
+    assert n_re == near_total
+    _say("[gpu]   near-ties re-checked in FP64: %d" % n_re)
 
 
 def test_mixed_eligibility_splits_the_genes(eng, orc):
