@@ -34,6 +34,7 @@
  */
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 
 #include "eqbf_internal.h"
 
@@ -1633,3 +1634,74 @@ int abf_from_sstats_run(eqbf_ctx* ctx, const eqbf_sstats_in* in, int bfs, int ra
   return EQBF_OK;
 }
 
+/* ---- association that allocates nothing ---------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc): eqbf_assoc_uvlr_device with the FP64 precisions, its
+ * per-call scratch (preparation records, basis and residual columns, member
+ * lists, configuration masks) carved from one static device array instead of
+ * context buffers.  Each device holds its own copy of the array from the
+ * first use of the library until the process ends; every call on a device
+ * shares it, so calls that overlap on one device race. */
+#define NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+#define NOALLOC_MAX_DEVICES 64
+static __device__ __align__(256) uint8_t scratch[NOALLOC_SCRATCH_BYTES];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[NOALLOC_MAX_DEVICES];
+
+/* Called by eqbf_create with the context's device current: the device
+ * address of `scratch` on that device (the array's name in host code is a
+ * host-side stand-in that no kernel may be given). */
+int assoc_noalloc_setup(eqbf_ctx* ctx) {
+  if (ctx->device < 0 || ctx->device >= NOALLOC_MAX_DEVICES) return EQBF_OK; /* the entry point reports it */
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[ctx->device]) return EQBF_OK;
+  void* p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return ctx_fail(ctx, EQBF_ERR_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+  g_scratch_ptr[ctx->device] = p;
+  return EQBF_OK;
+}
+
+/* Scratch bytes one call needs (the slices assoc_uvlr_run takes, each
+ * rounded up to 256 bytes). */
+static size_t assoc_scratch_bytes(const eqbf_chunk& c, int64_t n_cfg) {
+  const size_t G = (size_t)c.n_genes, S = (size_t)c.n_subgroups, N = (size_t)c.n_samples;
+  int KU = 1;
+  while (KU < c.n_covars_max + 1) KU = (KU == 1) ? 4 : KU * 2;
+  if (KU > 16) KU = 17;
+  const size_t KC = (size_t)KU + 1;
+  auto r = [](size_t b) { return (std::max<size_t>(b, 1) + 255) & ~(size_t)255; };
+  return r(G * S * sizeof(PrepRec)) + r(G * N * S * KC * sizeof(double)) + r(G * N * sizeof(uint64_t)) +
+         r(G * S * N * sizeof(int32_t)) + r((size_t)std::max<int64_t>(n_cfg, 1) * sizeof(uint64_t) + 65 * sizeof(double));
+}
+
+extern "C" int eqbf_assoc_uvlr_device_noalloc(eqbf_ctx* ctx, const eqbf_chunk* c, const eqbf_assoc_opts* o,
+                                              eqbf_assoc_out* out, void* stream) {
+  if (!ctx || !c || !o || !out) return ctx_fail(ctx, EQBF_ERR_ARG, "null argument");
+  if (o->precision != EQBF_PREC_FP64_PLAIN && o->precision != EQBF_PREC_FP64_BATCHED)
+    return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED, "the entry point without allocation runs the FP64 precisions only");
+  if (ctx->device < 0 || ctx->device >= NOALLOC_MAX_DEVICES || !g_scratch_ptr[ctx->device])
+    return ctx_fail(ctx, EQBF_ERR_HIP, "no static scratch address for device %d", ctx->device);
+  if (c->n_subgroups < 1 || c->n_subgroups > 64 || c->n_samples < 1 || c->n_genes < 0 || c->n_covars_max < 0 ||
+      c->n_covars_max > 16)
+    return ctx_fail(ctx, EQBF_ERR_ARG, "chunk sizes out of range (S=%d N=%d G=%d Q=%d)", c->n_subgroups, c->n_samples,
+                    c->n_genes, c->n_covars_max);
+  const int64_t n_cfg = o->analysis == EQBF_ANALYSIS_JOIN ? eqbf_n_configs(o->bfs, c->n_subgroups) : 0;
+  if (n_cfg < 0) return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED, "--bfs all with %d subgroups", c->n_subgroups);
+  const size_t need = assoc_scratch_bytes(*c, n_cfg);
+  if (need > NOALLOC_SCRATCH_BYTES)
+    return ctx_fail(ctx, EQBF_ERR_ARG,
+                    "the call needs %zu bytes of scratch; the static array holds %zu (fewer genes per call)", need,
+                    (size_t)NOALLOC_SCRATCH_BYTES);
+  EQBF_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  eqbf_assoc_opts ob = *o;
+  ob.precision = EQBF_PREC_FP64_BATCHED; /* one launch: no host copy of the pair offsets */
+  ScratchArena arena;
+  arena.base = (uint8_t*)g_scratch_ptr[ctx->device];
+  arena.cap = NOALLOC_SCRATCH_BYTES;
+  return assoc_uvlr_run(ctx, c, &ob, out, st, &arena);
+}

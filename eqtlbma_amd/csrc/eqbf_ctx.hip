@@ -64,6 +64,12 @@ extern "C" int eqbf_create(eqbf_ctx** out, int device) {
     return EQBF_ERR_HIP;
   }
   ctx->d_flag.bytes = 256;
+  if (assoc_noalloc_setup(ctx) != EQBF_OK) {
+    (void)hipFree(ctx->d_flag.ptr);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return EQBF_ERR_HIP;
+  }
   *out = ctx;
   return EQBF_OK;
 }

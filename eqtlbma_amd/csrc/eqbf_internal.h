@@ -345,8 +345,9 @@ __device__ inline int uvlr_stats(const PrepRec& r, bool has, double sxx, double 
 }
 
 /* ---- kernels launched from the C-ABI layer ------------------------------- */
-/* Bump allocator over a fixed device region: 256-byte aligned slices,
- * nullptr when the region is exhausted. */
+/* Bump allocator over a fixed device region (the static scratch array of
+ * eqbf_assoc_uvlr_device_noalloc): 256-byte aligned slices, nullptr when
+ * the region is exhausted. */
 struct ScratchArena {
   uint8_t* base = nullptr;
   size_t cap = 0, used = 0;
@@ -360,7 +361,9 @@ struct ScratchArena {
 };
 int assoc_uvlr_run(eqbf_ctx* ctx, const eqbf_chunk* c, const eqbf_assoc_opts* o, eqbf_assoc_out* out,
                    hipStream_t st, ScratchArena* arena = nullptr);
-
+/* Address of the static scratch array on the current device (set up once
+ * per device by eqbf_create). */
+int assoc_noalloc_setup(eqbf_ctx* ctx);
 /* packed work list of 64-pair tiles of every gene (one block scan) */
 void launch_tiles(const int64_t* pair_offset, int G, int4* tiles, int* n_tiles, hipStream_t st);
 void launch_zero_u64(uint64_t* p, int64_t n, hipStream_t st);

@@ -41,6 +41,7 @@ def lib():
         L.eqbf_n_weighted.restype = i64
         L.eqbf_assoc_uvlr.argtypes = [vp, vp, vp, vp]
         L.eqbf_assoc_uvlr_device.argtypes = [vp, vp, vp, vp, vp]
+        L.eqbf_assoc_uvlr_device_noalloc.argtypes = [vp, vp, vp, vp, vp]
         L.eqbf_assoc_mvlr.argtypes = [vp, vp, vp, vp]
         L.eqbf_assoc_mvlr_device.argtypes = [vp, vp, vp, vp, vp]
         L.eqbf_abf_from_sstats.argtypes = [vp, vp, i32, i32, vp]
@@ -201,6 +202,13 @@ class Engine:
         _abi.AssocOut whose pointers are device addresses."""
         self._check(lib().eqbf_assoc_uvlr_device(self._ctx, C.byref(chunk_struct), C.byref(opts),
                                                  C.byref(out_struct), stream))
+
+    def assoc_uvlr_device_noalloc(self, chunk_struct, opts, out_struct, stream=None):
+        """eqbf_assoc_uvlr_device_noalloc: device-pointer entry, FP64
+        precisions, scratch from the library's static array (no device
+        allocation)."""
+        self._check(lib().eqbf_assoc_uvlr_device_noalloc(self._ctx, C.byref(chunk_struct), C.byref(opts),
+                                                         C.byref(out_struct), stream))
 
     def assoc_mvlr(self, hc, bfs="gen", fiterr=0.0, want_sstats=False):
         """--error mvlr Bayes factors (FP64); raw log10 ABFs are FP64."""

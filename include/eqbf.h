@@ -162,6 +162,24 @@ int eqbf_assoc_uvlr_device(eqbf_ctx* ctx, const eqbf_chunk* chunk_dev,
                            const eqbf_assoc_opts* opts, eqbf_assoc_out* out_dev,
                            void* hip_stream);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  eqbf_assoc_uvlr_device with an FP64 precision
+ * (FP64_PLAIN or FP64_BATCHED, run as FP64_BATCHED: the same results bit for
+ * bit) whose per-call scratch comes from one static device array of 64 MiB
+ * instead of context buffers: no hipMalloc / hipFree, no host copies, no
+ * synchronisation.  The scratch of a call is about
+ *   n_genes * (n_samples * (8 * S * (1 + KU) + 4 * S + 8) + 32 * S) + 8 * n_cfg
+ * bytes (KU = 1, 4, 8, 16 or 17 basis columns for 0, <= 3, <= 7, <= 15, 16
+ * covariates); a call that needs more than the array holds fails with
+ * EQBF_ERR_ARG (split the chunk).  Every call on one device shares the
+ * array, which keeps the last call's contents: calls that overlap on one
+ * device (two streams, two contexts, two threads) race.  The grids must be
+ * set (eqbf_set_grids) beforehand; other precisions give
+ * EQBF_ERR_UNSUPPORTED. */
+int eqbf_assoc_uvlr_device_noalloc(eqbf_ctx* ctx, const eqbf_chunk* chunk_dev, const eqbf_assoc_opts* opts,
+                                   eqbf_assoc_out* out_dev, void* hip_stream);
+
 /* ---- multivariate (MVLR) Bayes factors ------------------------------------------
  * --error mvlr: GeneSnpPair::CalcAbfsMvlr (gene_snp_pair.cpp:730-758) and
  * class MVLR (MVLR.cpp:26-764) for one SNP per pair.  Every subgroup must use
