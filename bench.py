@@ -75,6 +75,9 @@ def parse():
     ap.add_argument("--alloc-holdout", type=int, default=1000)
     ap.add_argument("--alloc-epochs", type=int, default=200)
     ap.add_argument("--only-perm", action="store_true", help="run only the permutation workload")
+    ap.add_argument("--mvlr-genes", type=int, default=2000)
+    ap.add_argument("--no-mvlr", action="store_true", help="skip the MVLR workload")
+    ap.add_argument("--no-c0", action="store_true", help="skip the plain-method workload")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a kernel of known "
                          "traffic, to scale the HBM byte counters of a profiler run)")
@@ -83,6 +86,39 @@ def parse():
 
 def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def host_cpu():
+    """(CPU model, threads used for the CPU baselines): the host cores this
+    process may run on, at most 16 (one GPU's share of a box's cores)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return model, max(1, min(16, n))
+
+
+def threaded(fn, items, threads):
+    """fn over items on `threads` host threads (the oracle's ctypes calls
+    release the interpreter lock, so the threads run in parallel)."""
+    import concurrent.futures as cf
+    with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+        return list(ex.map(fn, items))
+
+
+def cpu_label(model, threads, what):
+    return {"cores": threads, "kind": "port", "label": "proxy, not reference", "cpu_model": model,
+            "note": "the reference needs GSL, absent from this image; this is the FP64 oracle restatement of "
+                    "its algorithm, %s, on %d host threads" % (what, threads)}
 
 
 class Workload:
@@ -280,6 +316,7 @@ class PermWorkload:
 
 
 WRTSIZE = 10  # genes per chunk of the reference's run loop (--wrtsize default); the RNG restarts per chunk
+TIE_BOUND = 5e-5  # FP64 re-check of permutation statistics this close to the true one (log10 BF)
 
 
 def perm_leg(args, torch, result):
@@ -314,26 +351,33 @@ def perm_leg(args, torch, result):
     gene_table = (np.arange(G) % WRTSIZE).astype(np.int32)
     stream = torch.cuda.current_stream()
     sptr = C.c_void_p(stream.cuda_stream)
+
+    true_d = torch.empty((G,), dtype=torch.float64, device=dev)
     w_all_row = 3 + n_cfg + 1
 
     def run(g1):
+        # association pass (testForAssociations), then every permutation's
+        # statistic with the FP64 true statistics and FP64 re-checks of the
+        # near-ties, then the reference's counting loop on the host
         c = wl.chunk(_abi, 0, g1)
         eng.assoc_uvlr_device(c, aopts, o, sptr)
-        eng.perm_stats_device(c, popts, C.c_void_p(tables_d.data_ptr()), WRTSIZE, gene_table[:g1], nperm,
-                              C.c_void_p(stats.data_ptr()), sptr)
         npc = int(wl.offs[g1])  # the outputs' leading dimension is this chunk's pair count
         w = out_w.view(-1)[w_all_row * npc:(w_all_row + 1) * npc].cpu().numpy()
-        true = np.maximum.reduceat(w, wl.offs[:g1].astype(np.int64))
+        true_d[:g1].copy_(torch.from_numpy(np.fmax.reduceat(w, wl.offs[:g1].astype(np.int64))))
+        n_re = eng.perm_stats_fp64_ties_device(c, popts, C.c_void_p(tables_d.data_ptr()), WRTSIZE, gene_table[:g1],
+                                               nperm, TIE_BOUND, C.c_void_p(stats.data_ptr()),
+                                               C.c_void_p(true_d.data_ptr()), sptr)
+        true = true_d[:g1].cpu().numpy()
         st = stats[:g1].cpu().numpy()
         pv = np.array([perm_pvalue(st[g], true[g], nperm)[0] for g in range(g1)])
-        return pv, st
+        return pv, st, true, n_re
 
     run(min(G, 2 * WRTSIZE))  # warm-up: buffers and tables
     torch.cuda.synchronize()
     eng.profile(True)
     before = eng.profile_read()
     t0 = time.perf_counter()
-    pv, st = run(G)
+    pv, st, true, n_re = run(G)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     after = eng.profile_read()
@@ -346,6 +390,10 @@ def perm_leg(args, torch, result):
                         "hard calls, %d joint permutations per gene (GSL mt19937 seed 7, one stream per %d-gene "
                         "chunk), --pbf all --maxbf, grids 25/10" % (G, S, N, nperm, WRTSIZE),
             "pairs": NP, "host_table_generation_s": t_tab,
+            "fp64_recheck": {"bound_abs_log10_bf": TIE_BOUND, "permutations_rechecked": int(n_re),
+                             "of": int(G) * int(nperm),
+                             "what": "permutation statistics within the bound of the true statistic, and the true "
+                                     "statistic of their genes, recomputed in FP64 before counting"},
             "kernels_ms_total": {k: v[0] for k, v in kern.items()},
             "kernels_launches": {k: v[1] for k, v in kern.items()},
             "egenes_p_below_0.01": int(np.sum(pv < 0.01))}
@@ -393,22 +441,26 @@ def perm_leg(args, torch, result):
         log("perm plain %.3e genes/s" % plain_rate)
     if not args.no_cpu_baseline:
         from oracle import oracle as orc
+        model, T = host_cpu()
         hc = wl.host_chunk(_abi, 0, 1)
         k = 2
         t = time.perf_counter()
-        ref = orc.perm_join_stats(hc, 0, "all", True, gL, gS, tables[0, :k])
+        orc.perm_join_stats(hc, 0, "all", True, gL, gS, tables[0, :k])
+        d1 = (time.perf_counter() - t) / k
+        kt = max(1, min(nperm // T, int(args.cpu_seconds / max(d1, 1e-4))))  # permutations per thread
+        parts = [tables[0, r * kt:(r + 1) * kt] for r in range(T)]
+        t = time.perf_counter()
+        refs = threaded(lambda tb: orc.perm_join_stats(hc, 0, "all", True, gL, gS, tb), parts, T)
         dtc = time.perf_counter() - t
-        k2 = max(k, min(nperm, int(args.cpu_seconds / max(dtc / k, 1e-4))))
-        if k2 > k:
-            t = time.perf_counter()
-            ref = orc.perm_join_stats(hc, 0, "all", True, gL, gS, tables[0, :k2])
-            dtc = time.perf_counter() - t
-            k = k2
-        perm["cpu_baseline"] = {"value": 1.0 / (dtc / k * nperm), "unit": "genes/s", "cores": 1, "kind": "port",
-                                "sample": "gene 0 (%d SNPs), first %d permutations, oracle restatement, single "
-                                          "thread, %.1f s; rate per gene at %d permutations" %
-                                          (wl.P[0], k, dtc, nperm)}
-        perm["error_vs_fp64_oracle"] = {"perm_stat_abs": float(np.max(np.abs(st[0, :k] - ref))), "sample_perms": k}
+        ref = np.concatenate(refs)
+        k = ref.size
+        perm["cpu_baseline"] = dict(value=1.0 / (dtc / k * nperm), unit="genes/s",
+                                    sample="gene 0 (%d SNPs), its first %d permutations split over the threads, "
+                                           "%.1f s wall; rate per gene at %d permutations" % (wl.P[0], k, dtc, nperm),
+                                    **cpu_label(model, T, "permutations of one gene sharded over threads"))
+        perm["error_vs_fp64_oracle"] = {"perm_stat_abs": float(np.max(np.abs(st[0, :k] - ref))), "sample_perms": k,
+                                        "true_stat_abs": float(abs(true[0] - orc.true_stat(
+                                            orc.assoc_uvlr(hc, "join", "all", gL, gS).w[3 + n_cfg + 1], True)))}
         log("perm cpu oracle %.3e genes/s; stat error %.3g" % (perm["cpu_baseline"]["value"],
                                                                  perm["error_vs_fp64_oracle"]["perm_stat_abs"]))
     eng.close()
@@ -511,11 +563,18 @@ def perm_alloc_leg(args, torch, result):
     eng.profile(False)
     kern = {k: round(v[0] - before.get(k, (0.0, 0))[0], 2) for k, v in after.items() if v[1] > before.get(k, (0, 0))[1]}
     value = n_inf / dt
-    # ---- held-out genes: full permutation of the first n_hold inference genes ----
+    # ---- every inference gene fully permuted: BH 5 % eGene calls over all of
+    # them, and the p-value error on the first n_hold (held out from nothing
+    # the network saw: it trained on other genes) ----
+    t1 = time.perf_counter()
+    perm(inf, nfull)
+    full_all = stats_rows(inf, nfull)
+    pv_full_all = np.array([perm_pvalue(full_all[r], true_inf[r], nfull)[0] for r in range(n_inf)])
+    t_full = time.perf_counter() - t1
+    calls_a_all = allocator.bh_reject(pv, 0.05)
+    calls_f_all = allocator.bh_reject(pv_full_all, 0.05)
     hold = np.arange(n_hold)
-    perm(inf[hold], nfull)
-    full_rows = stats_rows(inf[hold], nfull)
-    pv_full = np.array([perm_pvalue(full_rows[r], true_inf[h], nfull)[0] for r, h in enumerate(hold)])
+    pv_full = pv_full_all[hold]
     calls_a = allocator.bh_reject(pv[hold], 0.05)
     calls_f = allocator.bh_reject(pv_full, 0.05)
     lowk = k_gene[hold] == nlow
@@ -538,18 +597,236 @@ def perm_alloc_leg(args, torch, result):
                        else 0.0,
                        "max_abs_p_error_full_runs": float(np.max(np.abs(pv[hold] - pv_full)[~lowk])) if (~lowk).any()
                        else 0.0},
+           "all_genes": {"genes": int(n_inf), "egenes_bh5_full": int(calls_f_all.sum()),
+                         "egenes_bh5_allocated": int(calls_a_all.sum()),
+                         "calls_identical": bool(np.array_equal(calls_a_all, calls_f_all)),
+                         "call_disagreements": int(np.sum(calls_a_all != calls_f_all)),
+                         "full_permutation_seconds": t_full},
            "kernels_ms_total": kern}
     if "perm" in result and result["perm"].get("value"):
         res["speedup_vs_full_permutation_same_engine"] = value / result["perm"]["value"]
         if "plain_gpu" in result["perm"]:
             res["speedup_vs_plain_gpu"] = value / result["perm"]["plain_gpu"]["value"]
-    log("allocated: %d genes in %.2f s = %.1f genes/s; %.2f of full permutations; held-out calls %d vs %d" %
-        (n_inf, dt, value, res["permutations_used_fraction_of_full"], calls_a.sum(), calls_f.sum()))
+    log("allocated: %d genes in %.2f s = %.1f genes/s; %.2f of full permutations; BH 5%% calls over all genes "
+        "%d allocated vs %d full (%d differ)" % (n_inf, dt, value, res["permutations_used_fraction_of_full"],
+                                                 calls_a_all.sum(), calls_f_all.sum(),
+                                                 int(np.sum(calls_a_all != calls_f_all))))
     al.close()
     eng.close()
     del wl, stats, tab_full, tab_low, out_n, out_w
     torch.cuda.empty_cache()
     result["perm_allocated"] = res
+
+
+def to_device(torch, hc, abi):
+    """A HostChunk's arrays as device tensors and a chunk struct pointing at
+    them (the tensors are returned so that they stay alive)."""
+    dev = torch.device("cuda:0")
+    keep = {}
+    c = hc.struct()
+    for f in ("explevels", "gene_in_sbgrp", "geno_in_sbgrp", "covar_count", "covars", "covar_in_sbgrp",
+              "geno_set_of_sbgrp", "genotypes", "snp_in_sbgrp", "pair_offset", "pair_snp"):
+        t = torch.from_numpy(np.ascontiguousarray(getattr(hc, f))).to(dev)
+        keep[f] = t
+        setattr(c, f, t.data_ptr())
+    return c, keep
+
+
+def c0_leg(args, torch, result):
+    """The plain-method configuration (the reference's CPU-runnable case): 3
+    subgroups sharing N = 100 individuals, 200 genes with ~201 cis SNPs each,
+    hard calls, 10 % eGenes, --analys join --bfs all, grids 25 / 10, FP64,
+    seed 42.  Latency of one association call over the whole chunk."""
+    from eqtlbma_amd import _abi, synth
+    from eqtlbma_amd.engine import Engine
+    from oracle import oracle as orc
+    gL, gS = synth.make_grid("general"), synth.make_grid("small")
+    hc = synth.plain(seed=42, S=3, N=100, n_genes=200, snps_per_gene=201, frac_egenes=0.1)
+    S, NP = hc.S, hc.n_pairs
+    dev = torch.device("cuda:0")
+    c, keep = to_device(torch, hc, _abi)
+    eng = Engine(0)
+    eng.set_grids(gL, gS)
+    n_cfg = 2 ** S - 1
+    n_raw, n_w = 3 * gL.shape[0] + n_cfg * gS.shape[0], 3 + n_cfg + 2
+    outs = {"n": torch.empty((S, NP), dtype=torch.int32, device=dev),
+            "sstats": torch.empty((5, S, NP), dtype=torch.float64, device=dev),
+            "stdsstats": torch.empty((3, S, NP), dtype=torch.float64, device=dev),
+            "raw": torch.empty((n_raw, NP), dtype=torch.float64, device=dev),
+            "w": torch.empty((n_w, NP), dtype=torch.float64, device=dev)}
+    o = _abi.AssocOut()
+    o.n, o.sstats, o.stdsstats = outs["n"].data_ptr(), outs["sstats"].data_ptr(), outs["stdsstats"].data_ptr()
+    o.raw_abf, o.w_abf = outs["raw"].data_ptr(), outs["w"].data_ptr()
+    sptr = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    res = {"workload": "plain method: 200 genes, 3 subgroups sharing N=100, ~201 cis SNPs per gene, hard calls, "
+                       "--bfs all, grids 25/10, seed 42", "pairs": NP}
+    t = time.perf_counter()
+    ref = orc.assoc_uvlr(hc, "join", "all", gL, gS)
+    t_orc = time.perf_counter() - t
+    for name in ("fp64_batched", "mixed", "fp64_plain"):
+        opts = _abi.AssocOpts(_abi.ANALYSIS["join"], _abi.BFS["all"], _abi.PREC[name], _abi.F64)
+        eng.assoc_uvlr_device(c, opts, o, sptr)
+        torch.cuda.synchronize()
+        reps = 20 if name != "fp64_plain" else 3
+        t = time.perf_counter()
+        for _ in range(reps):
+            eng.assoc_uvlr_device(c, opts, o, sptr)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t) * 1e3 / reps
+        w = outs["w"].cpu().numpy()
+        raw = outs["raw"].cpu().numpy()
+        fw, fr = np.isfinite(ref.w), np.isfinite(ref.raw)
+        res[name] = {"ms_per_call": ms, "pairs_per_s": NP / (ms / 1e3),
+                     "log10_abf_weighted_abs_err": float(np.max(np.abs(w[fw] - ref.w[fw]))),
+                     "log10_abf_raw_abs_err": float(np.max(np.abs(raw[fr] - ref.raw[fr])))}
+    model, T = host_cpu()
+    # the oracle over the same 200 genes, sharded by gene over the threads
+    subs = [_abi.HostChunk(hc.explevels[g:g + 1], hc.gene_in_sbgrp[g:g + 1], hc.geno_in_sbgrp, hc.genotypes,
+                           hc.snp_in_sbgrp, hc.pair_offset[g:g + 2] - hc.pair_offset[g],
+                           hc.pair_snp[hc.pair_offset[g]:hc.pair_offset[g + 1]]) for g in range(hc.G)]
+    t = time.perf_counter()
+    threaded(lambda h: orc.assoc_uvlr(h, "join", "all", gL, gS), subs, T)
+    t_par = time.perf_counter() - t
+    res["cpu_baseline"] = dict(value=NP / t_par, unit="pairs/s", single_thread_pairs_per_s=NP / t_orc,
+                               sample="all 200 genes, %.2f s wall (%.2f s on one thread)" % (t_par, t_orc),
+                               **cpu_label(model, T, "genes sharded over threads"))
+    log("c0: %s" % {k: (round(v["ms_per_call"], 3) if isinstance(v, dict) and "ms_per_call" in v else None)
+                    for k, v in res.items()})
+    eng.close()
+    del keep, outs
+    result["plain_method_c0"] = res
+
+
+class MvlrWorkload:
+    """Device-resident synthetic data of the MVLR configuration: S = 9
+    subgroups, every one of N = 150 individuals in every subgroup, hard-call
+    genotypes under HWE with MAF ~ U(0.05, 0.5), residuals correlated across
+    subgroups (equicorrelation rho = 0.5), one eQTL in 10 % of genes with
+    multivariate normal effects (between-subgroup correlation 0.8, sd 0.5),
+    cis SNPs per gene ~ Poisson(800) clipped to [400, 1200]."""
+
+    def __init__(self, torch, G, seed, S=9, N=150, rho=0.5, r_eff=0.8):
+        dev = torch.device("cuda:0")
+        self.torch, self.dev, self.G, self.S, self.N = torch, dev, G, S, N
+        rng = np.random.default_rng(seed)
+        P = np.clip(rng.poisson(800, size=G), 400, 1200).astype(np.int64)
+        self.P = P
+        self.offs = np.concatenate([[0], np.cumsum(P)]).astype(np.int64)
+        n_snps = int(self.offs[-1])
+        self.n_snps = n_snps
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(seed)
+        maf = torch.rand(n_snps, device=dev, generator=gen) * 0.45 + 0.05
+        g = (torch.rand((N, n_snps), device=dev, generator=gen) < maf).float()
+        g += (torch.rand((N, n_snps), device=dev, generator=gen) < maf).float()
+        self.geno = g
+        L = torch.from_numpy(np.linalg.cholesky(np.full((S, S), rho) + (1 - rho) * np.eye(S))).to(dev)
+        Le = np.linalg.cholesky(np.full((S, S), r_eff) + (1 - r_eff) * np.eye(S))
+        y = torch.einsum("st,gtn->gsn", L, torch.randn((G, S, N), device=dev, generator=gen, dtype=torch.float64))
+        for gi in rng.choice(G, size=int(round(0.1 * G)), replace=False):
+            j = int(self.offs[gi] + rng.integers(P[gi]))
+            b = torch.from_numpy(0.5 * (Le @ rng.normal(size=S))).to(dev)
+            y[gi] += b[:, None] * self.geno[:, j].double()[None, :]
+        self.y = y.contiguous()
+        self.gene_in = torch.ones((G, S), dtype=torch.uint8, device=dev)
+        self.geno_in = torch.ones((S, N), dtype=torch.uint8, device=dev)
+        self.covars = torch.zeros((S, 1, N), dtype=torch.float64, device=dev)
+        self.covar_count = torch.zeros((S,), dtype=torch.int32, device=dev)
+        self.covar_in = torch.ones((S, N), dtype=torch.uint8, device=dev)
+        self.gset = torch.zeros((S,), dtype=torch.int32, device=dev)
+        self.snp_in = torch.ones((n_snps, S), dtype=torch.uint8, device=dev)
+        self.pair_snp = torch.arange(n_snps, dtype=torch.int32, device=dev)
+        self.rel = torch.from_numpy(self.offs.copy()).to(dev)
+        self._rels = {}
+        torch.cuda.synchronize()
+
+    chunk = PermWorkload.chunk
+    host_chunk = PermWorkload.host_chunk
+
+
+def mvlr_leg(args, torch, result):
+    """MVLR Bayes factors (--error mvlr --fiterr 0.5 --bfs gen, 50-point
+    grid): 2,000 genes x ~800 cis SNPs, 9 subgroups sharing 150 individuals.
+    Each (pair, grid point, gen / gen-fix / gen-maxh) is one 9 x 9 system:
+    M = I + Vg^-1 Wg, partial-pivot LU, log-determinant and a quadratic form
+    in FP64 (MVLR.cpp:559-605)."""
+    from eqtlbma_amd import _abi, synth
+    from eqtlbma_amd.engine import Engine, MvlrOpts
+    from oracle import oracle as orc
+    grid = synth.make_grid("general", n_aes=10)
+    GLn = grid.shape[0]
+    G = args.mvlr_genes
+    wl = MvlrWorkload(torch, G, 1003)
+    S, dev = wl.S, wl.dev
+    NP = int(wl.offs[-1])
+    eng = Engine(0)
+    eng.set_grids(grid, grid)
+    out_n = torch.empty((S, NP), dtype=torch.int32, device=dev)
+    out_raw = torch.empty((3 * GLn, NP), dtype=torch.float64, device=dev)
+    out_w = torch.empty((5, NP), dtype=torch.float64, device=dev)
+    o = _abi.AssocOut()
+    o.n, o.sstats, o.stdsstats, o.raw_abf, o.w_abf = out_n.data_ptr(), None, None, out_raw.data_ptr(), out_w.data_ptr()
+    mo = MvlrOpts(_abi.BFS["gen"], 0.5)
+    sptr = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    c = wl.chunk(_abi, 0, G)
+    eng.assoc_mvlr_device(c, mo, o, sptr)  # warm-up
+    torch.cuda.synchronize()
+    eng.profile(True)
+    before = eng.profile_read()
+    reps = 3
+    t = time.perf_counter()
+    for _ in range(reps):
+        eng.assoc_mvlr_device(c, mo, o, sptr)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / reps
+    after = eng.profile_read()
+    eng.profile(False)
+    kern = {k: ((v[0] - before.get(k, (0.0, 0))[0]) / reps, (v[1] - before.get(k, (0.0, 0))[1]) // reps)
+            for k, v in after.items() if v[1] > before.get(k, (0, 0))[1]}
+    systems = float(NP) * 3 * GLn
+    flops = systems * MVLR_FLOPS_PER_SYSTEM
+    gk = kern.get("mvlr_grid_kernel", (float("nan"), 0))
+    res = {"metric": "SNP-gene MVLR Bayes factors/s", "value": NP / dt, "unit": "pairs/s", "seconds_per_pass": dt,
+           "workload": "MVLR: %d genes x ~800 cis SNPs (%d pairs), 9 subgroups sharing N=150, residual "
+                       "equicorrelation 0.5, effects MVN with correlation 0.8, --fiterr 0.5 --bfs gen, 50-point "
+                       "grid (150 9x9 systems per pair), FP64" % (G, NP),
+           "kernels_ms_per_pass": {k: v[0] for k, v in kern.items()},
+           "roofline": {"bound": "fp64", "kernel": "mvlr_grid_kernel", "systems": systems,
+                        "alg_flops_per_system": MVLR_FLOPS_PER_SYSTEM,
+                        "achieved": flops / (gk[0] / 1e3) / 1e12, "peak": 78.6, "unit": "TFLOP/s",
+                        "frac": flops / (gk[0] / 1e3) / 1e12 / 78.6, "avg_launch_ms": gk[0] / max(gk[1], 1)}}
+    # oracle: parity on two genes, CPU baseline over the host threads
+    model, T = host_cpu()
+    h0 = wl.host_chunk(_abi, 0, 1)
+    t = time.perf_counter()
+    r0 = orc.assoc_mvlr(h0, "gen", 0.5, grid, grid)
+    d1 = time.perf_counter() - t
+    raw0 = out_raw[:, :h0.n_pairs].cpu().numpy()
+    w0 = out_w[:, :h0.n_pairs].cpu().numpy()
+    fr, fw = np.isfinite(r0.raw), np.isfinite(r0.w)
+    res["error_vs_fp64_oracle"] = {"sample_pairs": int(h0.n_pairs),
+                                   "log10_abf_raw_abs": float(np.max(np.abs(raw0[fr] - r0.raw[fr]))),
+                                   "log10_abf_weighted_abs": float(np.max(np.abs(w0[fw] - r0.w[fw])))}
+    k = min(G, T * max(1, int(args.cpu_seconds / max(d1, 1e-3))))
+    hcs = [wl.host_chunk(_abi, g, g + 1) for g in range(k)]
+    t = time.perf_counter()
+    threaded(lambda h: orc.assoc_mvlr(h, "gen", 0.5, grid, grid), hcs, T)
+    tc = time.perf_counter() - t
+    cp = sum(h.n_pairs for h in hcs)
+    res["cpu_baseline"] = dict(value=cp / tc, unit="pairs/s",
+                               sample="first %d genes (%d pairs), one gene per task, %.1f s wall" % (k, cp, tc),
+                               **cpu_label(model, T, "genes sharded over threads"))
+    log("mvlr: %d pairs in %.3f s = %.3e pairs/s; kernels %s; cpu %.3e pairs/s" %
+        (NP, dt, NP / dt, {k: round(v[0], 2) for k, v in kern.items()}, cp / tc))
+    eng.close()
+    del wl, out_n, out_raw, out_w
+    torch.cuda.empty_cache()
+    result["mvlr"] = res
+
+
+# 9 x 9 system per (pair, grid point): M = I + Vg^-1 Wg (2 S^3), LU (2/3 S^3),
+# solve and quadratic form (4 S^2), S = 9
+MVLR_FLOPS_PER_SYSTEM = 2 * 9 ** 3 + (2 * 9 ** 3) // 3 + 4 * 9 ** 2
 
 
 def alg_bytes(P_g, S, N, GL, GS):
@@ -740,52 +1017,48 @@ def main():
         result["diag_ms_per_launch_first_chunk"] = diag
         log("diag %s" % diag)
 
-    # ---- parity sample and CPU baseline (oracle, one core) ----
+    # ---- CPU baseline (oracle on the host cores) and parity sample ----
     if not args.no_cpu_baseline:
         from oracle import oracle as orc
-        k = 1
-        hc = wl.host_chunk(_abi, 0, k)
+        model, T = host_cpu()
+        hc1 = wl.host_chunk(_abi, 0, 1)
         t = time.perf_counter()
-        ref = orc.assoc_uvlr(hc, "join", "sin", grid, grid)
+        orc.assoc_uvlr(hc1, "join", "sin", grid, grid)
+        d1 = time.perf_counter() - t
+        k = min(wl.G, T * max(1, int(args.cpu_seconds / max(d1, 1e-3))))
+        hcs = [wl.host_chunk(_abi, g, g + 1) for g in range(k)]
+        t = time.perf_counter()
+        refs = threaded(lambda h: orc.assoc_uvlr(h, "join", "sin", grid, grid), hcs, T)
         dt = time.perf_counter() - t
-        per_gene = dt / k
-        k2 = max(1, min(40, int(args.cpu_seconds / max(per_gene, 1e-3))))
-        if k2 > k:
-            hc = wl.host_chunk(_abi, 0, k2)
-            t = time.perf_counter()
-            ref = orc.assoc_uvlr(hc, "join", "sin", grid, grid)
-            dt = time.perf_counter() - t
-            k = k2
-        cpu_pairs = hc.n_pairs
-        result["cpu_baseline"] = {"value": cpu_pairs / dt, "unit": "pairs/s", "cores": 1, "kind": "port",
-                                  "sample": "first %d genes of the workload (%d pairs), oracle FP64/long-double "
-                                            "restatement, single thread, %.1f s" % (k, cpu_pairs, dt)}
-        gpu = eng.assoc_uvlr(hc, analysis="join", bfs="sin", precision="mixed", raw_dtype=np.float32)
-        has = ref.n > 0
-        bet, se = ref.sstats[2], ref.sstats[3]
-        ok = has & ~np.isnan(bet)
-        e_beta = np.max(np.abs(gpu.sstats[2][ok] - bet[ok]) / np.maximum(np.abs(bet[ok]), se[ok]))
-        e_se = np.max(np.abs(gpu.sstats[3][ok] - se[ok]) / se[ok])
-        fin = np.isfinite(ref.raw)
-        e_raw = np.max(np.abs(gpu.raw.astype(np.float64)[fin] - ref.raw[fin]))
-        fin = np.isfinite(ref.w)
-        e_w = np.max(np.abs(gpu.w[fin] - ref.w[fin]))
-        pv = ref.sstats[4]
-        okp = ok & (pv > 1e-290)
-        e_p = np.max(np.abs(gpu.sstats[4][okp] - pv[okp]) / pv[okp])
-        # the timed configuration: Bayes factors without the FP64 statistic outputs
-        gbf = eng.assoc_uvlr(hc, analysis="join", bfs="sin", precision="mixed", raw_dtype=np.float32,
-                             want_sstats=False, want_std=False)
-        fin = np.isfinite(ref.raw)
-        e_raw_t = np.max(np.abs(gbf.raw.astype(np.float64)[fin] - ref.raw[fin]))
-        fin = np.isfinite(ref.w)
-        e_w_t = np.max(np.abs(gbf.w[fin] - ref.w[fin]))
-        result["error_vs_fp64_oracle"] = {
-            "sample_pairs": int(hc.n_pairs), "betahat_rel_to_max(|b|,se)": float(e_beta),
-            "se_rel": float(e_se), "pval_rel": float(e_p), "log10_abf_raw_abs": float(e_raw_t),
-            "log10_abf_weighted_abs": float(e_w_t), "log10_abf_raw_abs_with_fp64_statistics": float(e_raw),
-            "log10_abf_weighted_abs_with_fp64_statistics": float(e_w)}
-        log("cpu oracle %.1f pairs/s; errors %s" % (cpu_pairs / dt, result["error_vs_fp64_oracle"]))
+        cpu_pairs = sum(h.n_pairs for h in hcs)
+        result["cpu_baseline"] = dict(value=cpu_pairs / dt, unit="pairs/s",
+                                      sample="first %d genes of the workload (%d pairs), one gene per task, "
+                                             "%.1f s wall" % (k, cpu_pairs, dt),
+                                      **cpu_label(model, T, "genes sharded over threads"))
+        errs = {}
+        for h, ref in list(zip(hcs, refs))[:5]:
+            gpu = eng.assoc_uvlr(h, analysis="join", bfs="sin", precision="mixed", raw_dtype=np.float32)
+            # the timed configuration: Bayes factors without the FP64 statistic outputs
+            gbf = eng.assoc_uvlr(h, analysis="join", bfs="sin", precision="mixed", raw_dtype=np.float32,
+                                 want_sstats=False, want_std=False)
+            has = ref.n > 0
+            bet, se, pv = ref.sstats[2], ref.sstats[3], ref.sstats[4]
+            ok = has & ~np.isnan(bet)
+            okp = ok & (pv > 1e-290)
+            fr, fw = np.isfinite(ref.raw), np.isfinite(ref.w)
+            e = {"betahat_rel_to_max(|b|,se)": np.max(np.abs(gpu.sstats[2][ok] - bet[ok]) /
+                                                      np.maximum(np.abs(bet[ok]), se[ok])),
+                 "se_rel": np.max(np.abs(gpu.sstats[3][ok] - se[ok]) / se[ok]),
+                 "pval_rel": np.max(np.abs(gpu.sstats[4][okp] - pv[okp]) / pv[okp]),
+                 "log10_abf_raw_abs": np.max(np.abs(gbf.raw.astype(np.float64)[fr] - ref.raw[fr])),
+                 "log10_abf_weighted_abs": np.max(np.abs(gbf.w[fw] - ref.w[fw])),
+                 "log10_abf_raw_abs_with_fp64_statistics": np.max(np.abs(gpu.raw.astype(np.float64)[fr] - ref.raw[fr])),
+                 "log10_abf_weighted_abs_with_fp64_statistics": np.max(np.abs(gpu.w[fw] - ref.w[fw]))}
+            for kk, v in e.items():
+                errs[kk] = max(errs.get(kk, 0.0), float(v))
+        errs["sample_pairs"] = int(sum(h.n_pairs for h in hcs[:5]))
+        result["error_vs_fp64_oracle"] = errs
+        log("cpu oracle %.1f pairs/s on %d threads; errors %s" % (cpu_pairs / dt, T, errs))
 
     # ---- plain FP64 one-gene-per-launch path on a subset ----
     if not args.no_plain and args.plain_genes > 0:
@@ -817,6 +1090,10 @@ def main():
     eng.close()
     del wl, chunks, out_n, out_raw, out_w
     torch.cuda.empty_cache()
+    if not args.no_c0:
+        c0_leg(args, torch, result)
+    if not args.no_mvlr:
+        mvlr_leg(args, torch, result)
     if not args.no_perm:
         perm_leg(args, torch, result)
         if not args.no_alloc:

@@ -241,8 +241,8 @@ extern "C" int eqbf_perm_stats_fp64_ties(eqbf_ctx* ctx, const eqbf_chunk* c, con
   if (rc) return rc;
   const int32_t* dtab = sg.put("h_ptab", perm_idx, (size_t)n_tables * std::max<int64_t>(nperm, 0) * c->n_samples);
   double* dstats = sg.alloc<double>("o_pstats", (size_t)c->n_genes * std::max<int64_t>(nperm, 1), true);
-  double* dtrue = sg.alloc<double>("o_ptrue", (size_t)std::max(c->n_genes, 1), true);
-  if (sg.rc || (n_tables > 0 && nperm > 0 && !dtab) || !dstats || !dtrue)
+  double* dtrue = c->n_genes > 0 ? sg.put("h_ptrue", true_stats, (size_t)c->n_genes) : nullptr;
+  if (sg.rc || (n_tables > 0 && nperm > 0 && !dtab) || !dstats || (c->n_genes > 0 && !dtrue))
     return ctx_fail(ctx, EQBF_ERR_HIP, "staging the tables failed");
   rc = perm_ties_run(ctx, &d, po, dtab ? dtab : (const int32_t*)dstats, n_tables, gene_table, nperm, tie_bound, dstats,
                      dtrue, n_rechecked, st);
@@ -251,7 +251,8 @@ extern "C" int eqbf_perm_stats_fp64_ties(eqbf_ctx* ctx, const eqbf_chunk* c, con
   if (rc) return rc;
   if (nperm > 0)
     EQBF_HIP(ctx, hipMemcpyAsync(stats, dstats, (size_t)c->n_genes * nperm * sizeof(double), hipMemcpyDeviceToHost, st));
-  EQBF_HIP(ctx, hipMemcpyAsync(true_stats, dtrue, (size_t)c->n_genes * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (c->n_genes > 0)
+    EQBF_HIP(ctx, hipMemcpyAsync(true_stats, dtrue, (size_t)c->n_genes * sizeof(double), hipMemcpyDeviceToHost, st));
   EQBF_HIP(ctx, hipStreamSynchronize(st));
   return EQBF_OK;
 }

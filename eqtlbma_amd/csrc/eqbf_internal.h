@@ -380,8 +380,10 @@ int abf_from_sstats_run(eqbf_ctx* ctx, const eqbf_sstats_in* in, int bfs, int ra
 int launch_bma(eqbf_ctx* ctx, double* w, int64_t n_pairs, int S, int bfs, hipStream_t st);
 int perm_stats_run(eqbf_ctx* ctx, const eqbf_chunk* c, const eqbf_perm_opts* po, const int32_t* tables_dev,
                    int32_t n_tables, const int32_t* gene_table, int64_t nperm, double* stats_dev, hipStream_t st);
-/* perm_stats_run, then the FP64 true statistic of every analysed gene and an
- * FP64 re-computation of every permutation statistic within tie_bound of it */
+/* perm_stats_run, then for every gene with a permutation statistic within
+ * tie_bound of its true statistic (true_dev, the caller's): the true
+ * statistic in FP64 (overwriting true_dev[g]) and those permutation
+ * statistics in FP64 */
 int perm_ties_run(eqbf_ctx* ctx, const eqbf_chunk* c, const eqbf_perm_opts* po, const int32_t* tables_dev,
                   int32_t n_tables, const int32_t* gene_table, int64_t nperm, double tie_bound, double* stats_dev,
                   double* true_dev, int64_t* n_rechecked, hipStream_t st);

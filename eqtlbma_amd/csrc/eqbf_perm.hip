@@ -324,21 +324,9 @@ int perm_ties_run(eqbf_ctx* ctx, const eqbf_chunk* cp, const eqbf_perm_opts* po,
   if (!(tie_bound >= 0)) return ctx_fail(ctx, EQBF_ERR_ARG, "tie_bound must be >= 0");
   int rc = perm_stats_run(ctx, cp, po, tables_dev, n_tables, gene_table, nperm, stats_dev, st);
   if (rc) return rc;
-  std::vector<int64_t> off;
-  rc = gene_offsets(ctx, c, off, st);
-  if (rc) return rc;
-  /* the true statistic of every analysed gene, FP64 */
-  nan_fill(true_dev, G, st);
-  EQBF_HIP(ctx, hipGetLastError());
-  std::vector<std::pair<int32_t, int32_t>> items;
-  for (int g = 0; g < G; ++g)
-    if (gene_table[g] >= 0 && off[g + 1] > off[g]) items.push_back({g, -1});
-  rc = general_items_run(ctx, c, po, tables_dev, gene_table, std::max<int64_t>(nperm, 1), off, items,
-                         EQBF_PREC_FP64_BATCHED, true_dev, 1, st, true);
-  if (rc) return rc;
   const bool fp64 = po->precision == EQBF_PREC_FP64_PLAIN || po->precision == EQBF_PREC_FP64_BATCHED;
   if (fp64 || nperm <= 0) return EQBF_OK;
-  /* permutations near the true statistic, recomputed in FP64 */
+  /* permutations within the bound of the caller's true statistic */
   const int64_t n = (int64_t)G * nperm;
   unsigned long long* d_cnt = (unsigned long long*)ctx_buf(ctx, "tie_cnt", 64);
   int64_t* d_list = (int64_t*)ctx_buf(ctx, "tie_list", (size_t)n * sizeof(int64_t));
@@ -355,6 +343,19 @@ int perm_ties_run(eqbf_ctx* ctx, const eqbf_chunk* cp, const eqbf_perm_opts* po,
   EQBF_HIP(ctx, hipMemcpyAsync(list.data(), d_list, cnt * sizeof(int64_t), hipMemcpyDeviceToHost, st));
   EQBF_HIP(ctx, hipStreamSynchronize(st));
   std::sort(list.begin(), list.end());
+  std::vector<int64_t> off;
+  rc = gene_offsets(ctx, c, off, st);
+  if (rc) return rc;
+  /* the genes concerned: true statistic in FP64 (overwriting the caller's),
+   * then their near-tie permutations in FP64 */
+  std::vector<std::pair<int32_t, int32_t>> items;
+  for (int64_t i : list) {
+    const int32_t g = (int32_t)(i / nperm);
+    if (items.empty() || items.back().first != g) items.push_back({g, -1});
+  }
+  rc = general_items_run(ctx, c, po, tables_dev, gene_table, nperm, off, items, EQBF_PREC_FP64_BATCHED, true_dev, 1,
+                         st, true);
+  if (rc) return rc;
   items.clear();
   for (int64_t i : list) items.push_back({(int32_t)(i / nperm), (int32_t)(i % nperm)});
   if (n_rechecked) *n_rechecked = (int64_t)cnt;

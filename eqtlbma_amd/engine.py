@@ -262,17 +262,18 @@ class Engine:
                                           _p(gt), nperm, _p(out)))
         return out
 
-    def perm_stats_fp64_ties(self, hc, tables, gene_table, analysis="join", pbf="all", use_max_bf=True,
-                             sep_sbgrp=-1, precision="mixed", tie_bound=5e-5):
-        """perm_stats, then the FP64 true statistic of every analysed gene and
-        an FP64 re-computation of the permutation statistics within tie_bound
-        of it (absolute for join, relative for sep).  Returns (stats
-        [n_genes][nperm], true_stats [n_genes], number re-computed)."""
+    def perm_stats_fp64_ties(self, hc, tables, gene_table, true_stats, analysis="join", pbf="all",
+                             use_max_bf=True, sep_sbgrp=-1, precision="mixed", tie_bound=5e-5):
+        """perm_stats, then for every gene with a permutation statistic within
+        tie_bound of true_stats[g] (absolute for join, relative for sep): its
+        true statistic and those permutation statistics in FP64.  Returns
+        (stats [n_genes][nperm], true_stats [n_genes], number re-computed)."""
         tables = np.ascontiguousarray(tables, dtype=np.int32)
         gt = np.ascontiguousarray(gene_table, dtype=np.int32)
         nperm = tables.shape[1]
         out = np.full((hc.G, nperm), np.nan)
-        true = np.full(hc.G, np.nan)
+        true = np.array(true_stats, dtype=np.float64, copy=True)
+        assert true.shape == (hc.G,)
         n_re = C.c_int64(0)
         po = PermOpts(_abi.ANALYSIS[analysis], _abi.PBF[pbf], int(use_max_bf), sep_sbgrp, _abi.PREC[precision])
         st = hc.struct()
@@ -283,8 +284,9 @@ class Engine:
 
     def perm_stats_fp64_ties_device(self, chunk_struct, opts, tables_ptr, n_tables, gene_table, nperm, tie_bound,
                                     stats_ptr, true_ptr, stream=None):
-        """Device-pointer entry of perm_stats_fp64_ties; returns the number
-        of statistics re-computed."""
+        """Device-pointer entry of perm_stats_fp64_ties (true_ptr: the
+        caller's true statistics, updated in place); returns the number of
+        statistics re-computed."""
         gt = np.ascontiguousarray(gene_table, dtype=np.int32)
         n_re = C.c_int64(0)
         self._check(lib().eqbf_perm_stats_fp64_ties_device(self._ctx, C.byref(chunk_struct), C.byref(opts),

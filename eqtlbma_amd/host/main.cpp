@@ -524,15 +524,15 @@ constexpr double kTieJoin = 5e-5, kTieSep = 1e-6;
  * NaN statistics change how many draws a gene takes and how many the next
  * gene is allowed (nb_permutations is shared), so tables are made for the
  * rest of the chunk (in batches of at most the cache's size) and made again
- * from the first gene whose use differed.  The library returns the FP64 true
- * statistics (truth[g]) and recomputes in FP64 every permutation statistic
- * within tie_bound of them.  count(g, stats, nb) runs the counting loop of
- * gene g and returns the number of shuffles it drew. */
+ * from the first gene whose use differed.  truth[g] (in/out): the true
+ * statistics from the association results; the library replaces them by FP64
+ * values for the genes with a permutation statistic within tie_bound of them
+ * and recomputes those statistics in FP64.  count(g, stats, nb) runs the
+ * counting loop of gene g and returns the number of shuffles it drew. */
 template <class Count>
 void run_perms(Engine& eng, const Chunk& ch, const eqbf_perm_opts& po, const std::vector<int>& genes, int64_t N,
                eqbf_rng& rng, int64_t& nb_perm, PermTableCache& cache, double tie_bound, std::vector<double>& truth,
                Count count) {
-  truth.assign(ch.genes.size(), kNaN);
   size_t k = 0;
   std::vector<int32_t> local;
   std::vector<eqbf_rng> states;
@@ -550,6 +550,7 @@ void run_perms(Engine& eng, const Chunk& ch, const eqbf_perm_opts& po, const std
     for (size_t j = 0; j < n; ++j) gene_table[genes[k + j]] = (int32_t)j;
     std::vector<double> stats((size_t)ch.genes.size() * nb_cur, kNaN);
     int64_t n_re = 0;
+    tr = truth;
     eng.check(eqbf_perm_stats_fp64_ties(eng.ctx, &ch.c, &po, tables, (int32_t)n, gene_table.data(), nb_cur,
                                        tie_bound, stats.data(), tr.data(), &n_re));
     for (size_t j = 0; j < n; ++j) truth[genes[k + j]] = tr[genes[k + j]];
@@ -903,12 +904,14 @@ int main(int argc, char** argv) {
           }
           pj[g].true_stat = t;
         }
+        truth.assign(G, kNaN);
+        for (int g : genes) truth[g] = pj[g].true_stat;
         run_perms(eng, ch, po, genes, N, rng, nb_perm, tcache, kTieJoin, truth,
                   [&](int g, const double* st, int64_t n_all) -> int64_t {
           PermRes& pr = pj[g];
           pr.done = true;
           (void)n_all;
-          if (st) pr.true_stat = truth[g]; /* FP64, from the library */
+          if (st) pr.true_stat = truth[g]; /* FP64 where a permutation came near it */
           static const double none = 0.0; /* no statistic is read when no permutation is left */
           return eqbf_perm_pvalue(st ? st : &none, pr.true_stat, 0, &nb_perm, o.trick, (int64_t)o.trick_cutoff,
                                   o.trick ? &rngT : nullptr, &pr.pval, &pr.used, &pr.median);
@@ -930,11 +933,13 @@ int main(int argc, char** argv) {
               }
             ps[pass][g].true_stat = t;
           }
+          truth.assign(G, kNaN);
+          for (int g : genes) truth[g] = ps[pass][g].true_stat;
           run_perms(eng, ch, po, genes, N, rng, nb_perm, tcache, kTieSep, truth,
                     [&](int g, const double* st, int64_t) -> int64_t {
             PermRes& pr = ps[pass][g];
             pr.done = true;
-            if (st) pr.true_stat = truth[g]; /* FP64, from the library */
+            if (st) pr.true_stat = truth[g]; /* FP64 where a permutation came near it */
             static const double none = 0.0; /* no statistic is read when no permutation is left */
             return eqbf_perm_pvalue(st ? st : &none, pr.true_stat, 1, &nb_perm, o.trick, (int64_t)o.trick_cutoff,
                                     o.trick ? &rngT : nullptr, &pr.pval, &pr.used, nullptr);

@@ -265,22 +265,25 @@ int eqbf_perm_stats_device(eqbf_ctx* ctx, const eqbf_chunk* chunk_dev, const eqb
 
 /* Near-ties in FP64.  The reference computes every statistic in FP64 and
  * counts a permutation when its statistic is >= the true one (join) or <= it
- * (sep) (gene.cpp:685, 697, 435, 555).  With the MIXED
- * precision a statistic carries a small error (about 1e-6 in log10 BF), so a
- * permutation whose statistic lies that close to the true one could be
- * counted differently.  This entry point is eqbf_perm_stats followed by:
- *  - true_stats[g]: the true statistic of every analysed gene in FP64 (the
- *    FP64_PLAIN arithmetic) with the reference's semantics:
- *    Gene::FindMaxTrueL10Abf / AvgTrueL10Abfs (gene.cpp:575-596: NaN never
- *    wins the maximum, -inf when every SNP is NaN; the average runs over
- *    the non-NaN SNPs) and FindMinTruePvalue* (gene.cpp:369-378, 475-488:
- *    starts from 1); NaN for a skipped gene or one without pairs;
- *  - every permutation statistic within tie_bound of it (absolute for join,
- *    tie_bound * |true| for sep) recomputed in FP64 and replaced.
- * The counting loop (eqbf_perm_pvalue) then sees FP64 values wherever the
- * comparison could depend on the precision.  n_rechecked (may be NULL):
- * number of permutation statistics recomputed.  With an FP64 precision the
- * statistics are FP64 already and only the true statistics are added.
+ * (sep) (gene.cpp:685, 697, 435, 555).  With the MIXED precision a statistic
+ * carries a small error (about 1e-6 in log10 BF), so a permutation whose
+ * statistic lies that close to the true one could be counted differently.
+ * This entry point is eqbf_perm_stats followed by, for every gene g with a
+ * permutation statistic within tie_bound of true_stats[g] (absolute for
+ * join, tie_bound * |true| for sep; tie_bound must exceed the error of the
+ * statistic plus that of the caller's true statistic):
+ *  - true_stats[g] replaced by the true statistic in FP64 (the FP64_PLAIN
+ *    arithmetic) with the reference's semantics: Gene::FindMaxTrueL10Abf /
+ *    AvgTrueL10Abfs (gene.cpp:575-596: NaN never wins the maximum, the
+ *    average runs over the non-NaN SNPs) or FindMinTruePvalue*
+ *    (gene.cpp:369-378, 475-488: starts from 1);
+ *  - those permutation statistics recomputed in FP64.
+ * true_stats (in/out, [n_genes]): on entry the caller's true statistics,
+ * as its counting loop would use them (e.g. from eqbf_assoc_uvlr's
+ * grid-averaged Bayes factors); the other genes keep them.  The counting
+ * loop (eqbf_perm_pvalue) then compares FP64 values wherever the precision
+ * could decide the count.  n_rechecked (may be NULL): permutation
+ * statistics recomputed.  With an FP64 precision nothing is rechecked.
  * No counterpart in the reference (which has no mixed precision). */
 int eqbf_perm_stats_fp64_ties(eqbf_ctx* ctx, const eqbf_chunk* chunk, const eqbf_perm_opts* opts,
                               const int32_t* perm_idx, int32_t n_tables, const int32_t* gene_table,

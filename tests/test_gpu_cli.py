@@ -244,13 +244,13 @@ def test_cli_join_permutations_contraction_path(tmp_path, orc):
     assert rows[0] == ["# perm.bf=all seed=%d" % seed]
     assert len(rows) - 2 == len(exp_perm)
     for got, want in zip(rows[2:], exp_perm):
-        # counts exact: the true statistic is FP64 and near-ties are
-        # re-checked in FP64 by the library
+        # counts exact: near-ties are re-checked in FP64 by the library; the
+        # true statistic is the mixed path's (FP64 for the genes with a
+        # near-tie), within 1e-4
         assert got[0] == want[0] and int(got[1]) == want[1]
         assert int(got[3]) == want[3]
         assert _num(got[2]) == pytest.approx(want[2], rel=1e-5)
-        assert _close(_num(got[4]), want[4])
-        assert abs(_num(got[5]) - want[5]) <= 1e-4
+        assert abs(_num(got[4]) - want[4]) <= 1e-4 and abs(_num(got[5]) - want[5]) <= 1e-4
 
 
 def test_cli_inss(tmp_path, orc):

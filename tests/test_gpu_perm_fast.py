@@ -95,14 +95,19 @@ def test_join_contraction_matches_oracle(eng, orc, pbf, use_max):
     _check(gpu, ref, "join")
     row, bfs = _true_w_row(pbf, hc.S)
     true = orc.assoc_uvlr(hc, "join", bfs, GL25, GS10)
-    chk, true_gpu, n_re = eng.perm_stats_fp64_ties(hc, tables, gene_table, "join", pbf, use_max, precision="mixed")
+    a_gpu = eng.assoc_uvlr(hc, "join", bfs, precision="mixed")
+    t_in = np.array([orc.true_stat(a_gpu.w[row, hc.pair_offset[g]:hc.pair_offset[g + 1]], use_max)
+                     for g in range(hc.G)])
+    chk, true_gpu, n_re = eng.perm_stats_fp64_ties(hc, tables, gene_table, t_in, "join", pbf, use_max,
+                                                   precision="mixed")
     _check(chk, ref, "join")
     near_total = 0
     for g in range(hc.G):
         w = true.w[row, hc.pair_offset[g]:hc.pair_offset[g + 1]]
         t = orc.true_stat(w, use_max)
-        assert abs(true_gpu[g] - t) <= 1e-9, (g, true_gpu[g], t)
-        near_total += int(np.sum(np.abs(gpu[g] - true_gpu[g]) <= 5e-5))
+        near = int(np.sum(np.abs(gpu[g] - t_in[g]) <= 5e-5))
+        near_total += near
+        assert abs(true_gpu[g] - t) <= (1e-9 if near else 1e-4), (g, true_gpu[g], t)
         a = engine.perm_pvalue(chk[g], true_gpu[g], nperm)
         b = orc.perm_count(ref[g], t, nperm)
         assert a[0] == b[0] and a[1] == b[1], (g, a, b)
