@@ -405,14 +405,17 @@ def perm_leg(args, torch, result):
                                  "unit": "TOP/s (int8)", "frac": ops / gm / 1e12 / 5000.0, "kernel": "pf_gemm_kernel",
                                  "alg_ops": ops, "total_ms": gm * 1e3}
     if "pf_bma_kernel" in kern:
-        terms = float(NP) * nperm * n_cfg * gS.shape[0]
+        # --maxbf: every z of every (SNP, subgroup, permutation) is read once to
+        # bound the SNP's statistic; only the SNPs whose bound reaches the
+        # running maximum are evaluated (310 exponentials each), so the floor
+        # is the read of the z buffer (FP32)
+        zbytes = float(NP) * S * nperm * 4
         bm = kern["pf_bma_kernel"][0] / 1e3
-        # one exp2 per (SNP, permutation, configuration, grid point): transcendental rate
-        # of 256 CUs x 16 lanes per clock x 2.4 GHz
-        peak = 256 * 16 * 2.4e9
-        perm["roofline_bma"] = {"bound": "transcendental", "achieved": terms / bm / 1e12, "peak": peak / 1e12,
-                                "unit": "Tterm/s", "frac": terms / bm / peak, "kernel": "pf_bma_kernel",
-                                "terms": terms, "total_ms": bm * 1e3}
+        terms = float(NP) * nperm * n_cfg * gS.shape[0]
+        perm["roofline_bma"] = {"bound": "hbm", "achieved": zbytes / bm / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                "frac": zbytes / bm / 1e9 / HBM_PEAK_GBS, "kernel": "pf_bma_kernel",
+                                "alg_bytes": zbytes, "total_ms": bm * 1e3,
+                                "terms_if_every_snp_were_evaluated_per_s": terms / bm}
     log("perm: %d genes x %d permutations in %.2f s = %.1f genes/s; kernels %s" %
         (G, nperm, dt, value, {k: round(v[0], 1) for k, v in kern.items()}))
 

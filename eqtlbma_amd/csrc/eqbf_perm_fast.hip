@@ -517,6 +517,7 @@ struct PfBmaArgs {
   double* psum; /* [slot][NPP] */
   int* fallback;
   float lw[2][9]; /* log2 weights: [0] large-grid terms by m, [1] small-grid terms by m */
+  float lwc[9];   /* log2 C(S, m) (bounds of pf_bma_max_kernel) */
   int GL, GS, use_max, n_items;
   int64_t NPP, nperm;
 };
@@ -694,6 +695,7 @@ struct PfFinalArgs {
   const double* tz;
   double* stats; /* [G][nperm] */
   int nb, S, nu, mode, use_max, sep_sbgrp;
+  int one_slot; /* the maximum of each (gene, permutation) is in its first slot */
   int64_t nperm, NPP;
 };
 
@@ -720,7 +722,7 @@ __global__ void pf_final_kernel(const PfFinalArgs a) {
       res = pv < 1.0 ? pv : 1.0;
     }
   } else {
-    const int nsb = (int)((np + PF_SB - 1) / PF_SB);
+    const int nsb = a.one_slot ? 1 : (int)((np + PF_SB - 1) / PF_SB);
     const int slot0 = a.bslot[bi];
     if (a.use_max) {
       float m = 0.f;
@@ -734,6 +736,229 @@ __global__ void pf_final_kernel(const PfFinalArgs a) {
     if (fabs(res) <= 2.220446049250313e-16) res = 0.0;
   }
   a.stats[(int64_t)a.bgene[bi] * a.nperm + p] = res;
+}
+
+/* ---- --pbf all --maxbf: the maximum over SNPs without evaluating every SNP --------------
+ * Work item = (batch gene, 4 x 64 permutations), one wave per 64
+ * permutations (lane = permutation), the gene's SNPs in blocks of 64 whose
+ * Bayes-factor coefficients are staged in LDS.  Every configuration of
+ * size m and grid point j has an exponent (log2) of at most
+ *   E_mj = lw_m + C_mj + q_j Z2max_m + D_mj Z1max_m^2,
+ * Z2max_m = the sum of the m largest z^2, Z1max_m^2 = the larger square of
+ * the sums of the m largest and of the m smallest z (q, D >= 0).  Two bounds
+ * of log2(statistic) follow: log2(number of terms) + max E_mj (a few
+ * operations per grid point, every SNP), and log2 sum_m,j C(S, m) 2^E_mj (one
+ * exponential per (m, j), the SNPs that pass the first).  Only a SNP whose
+ * bounds reach the lane's running maximum can raise it: those go through two
+ * queues in LDS, 64 entries at a time, and are evaluated exactly by
+ * pf_sum_all (the arithmetic of pf_bma_kernel), so the running maxima, and the
+ * final maximum, are exactly pf_bma_kernel's.  A SNP with some z = 0 (its
+ * subgroup drops out of the configurations) skips the bounds. */
+constexpr int PFQ = 128;  /* entries per queue and wave      */
+constexpr int PF_WG = 4;  /* waves (permutation blocks) per item */
+
+template <int S>
+struct PfSorted {
+  float Z2[S + 1], Q1[S + 1];
+};
+
+template <int S>
+__device__ inline PfSorted<S> pf_sorted(const float (&z)[S]) {
+  float a[S], b[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    a[s] = z[s];
+    b[s] = z[s] * z[s];
+  }
+#pragma unroll
+  for (int i = 1; i < S; ++i)
+#pragma unroll
+    for (int k = i; k > 0; --k) {
+      const float lo = fminf(a[k - 1], a[k]), hi = fmaxf(a[k - 1], a[k]);
+      a[k - 1] = lo;
+      a[k] = hi;
+      const float bl = fmaxf(b[k - 1], b[k]), bs = fminf(b[k - 1], b[k]);
+      b[k - 1] = bl;
+      b[k] = bs;
+    }
+  PfSorted<S> r;
+  float top = 0.f, bot = 0.f, sq = 0.f;
+#pragma unroll
+  for (int m = 1; m <= S; ++m) {
+    top += a[S - m];
+    bot += a[m - 1];
+    sq += b[m - 1];
+    r.Z2[m] = sq;
+    r.Q1[m] = fmaxf(top * top, bot * bot);
+  }
+  return r;
+}
+
+/* first bound: max over (m, j) of E_mj (without the log2 of the term count) */
+template <int S>
+__device__ inline float pf_bound_max(const PfSorted<S>& t, const float* cf, int GS, const float* lw) {
+  constexpr int NCF = 1 + 2 * S;
+  float u = -INFINITY;
+  for (int j = 0; j < GS; ++j) {
+    const float* row = cf + j * NCF;
+    const float q = row[0];
+#pragma unroll
+    for (int m = 1; m <= S; ++m) u = fmaxf(u, fmaf(row[S + m], t.Q1[m], fmaf(q, t.Z2[m], row[m] + lw[m])));
+  }
+  return u;
+}
+
+/* second bound: log2 sum over (m, j) of C(S, m) 2^E_mj; lwc_m = lw_m + log2 C(S, m) */
+template <int S>
+__device__ inline float pf_bound_sum(const PfSorted<S>& t, const float* cf, int GS, const float* lwc) {
+  constexpr int NCF = 1 + 2 * S;
+  float sum = 0.f;
+  for (int j = 0; j < GS; ++j) {
+    const float* row = cf + j * NCF;
+    const float q = row[0];
+#pragma unroll
+    for (int m = 1; m <= S; ++m)
+      sum += __builtin_amdgcn_exp2f(fmaf(row[S + m], t.Q1[m], fmaf(q, t.Z2[m], row[m] + lwc[m])));
+  }
+  return __builtin_amdgcn_logf(sum);
+}
+
+template <int S>
+__global__ void __launch_bounds__(64 * PF_WG) pf_bma_max_kernel(const PfBmaArgs a) {
+  extern __shared__ float cfs[]; /* [64 SNPs][GS][1 + 2S] */
+  __shared__ uint32_t q1s[PF_WG][PFQ], q2s[PF_WG][PFQ], mbs[PF_WG][64];
+  constexpr int NC = (1 << S) - 1;
+  constexpr int NCF = 1 + 2 * S;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int4 it = a.items[blockIdx.x];
+  const int bi = it.x, slot = it.w;
+  const int64_t P0 = a.bpoff[bi], np = a.bpoff[bi + 1] - P0;
+  const int64_t pbase = ((int64_t)it.y + wv) * 64;
+  const int64_t p = pbase + lane;
+  const bool valid = p < a.nperm;
+  const int GS = a.GS, rowf = GS * NCF;
+  uint32_t* q1 = q1s[wv];
+  uint32_t* q2 = q2s[wv];
+  uint32_t* mb = mbs[wv];
+  float lw[S + 1], lwc[S + 1];
+#pragma unroll
+  for (int m = 1; m <= S; ++m) {
+    lw[m] = a.lw[1][m];
+    lwc[m] = a.lw[1][m] + a.lwc[m];
+  }
+  const float lg_terms = __builtin_amdgcn_logf((float)(NC * GS));
+  mb[lane] = 0u; /* +0.0f: statistics are > 0 */
+  float lm = -INFINITY; /* log2 of this lane's running maximum */
+  int n1 = 0, n2 = 0;
+  bool bad = false;
+  int64_t b0 = 0; /* first SNP of the staged block */
+
+  auto zload = [&](int64_t pr, int64_t pp, float (&z)[S]) {
+#pragma unroll
+    for (int s = 0; s < S; ++s) z[s] = a.zeta[(pr * S + s) * a.NPP + pp];
+  };
+  /* exact statistics of the first k entries of queue 2 */
+  auto drain2 = [&](int k) {
+    if (lane < k) {
+      const uint32_t e = q2[lane];
+      const int ql = (int)(e >> 6), le = (int)(e & 63u);
+      float z[S];
+      zload(P0 + b0 + ql, pbase + le, z);
+      int nzm = 0;
+#pragma unroll
+      for (int s = 0; s < S; ++s) nzm |= (z[s] != 0.f) << s;
+      const float val = pf_sum_all<S>(z, cfs + ql * rowf, GS, a.lw[1], nzm);
+      bad |= !(val < INFINITY) || !(val > 1e-30f);
+      atomicMax(&mb[le], __float_as_uint(val));
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    const int rest = n2 - k;
+    if (lane < rest) {
+      const uint32_t v = q2[k + lane];
+      q2[lane] = v;
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    n2 = rest;
+    lm = __builtin_amdgcn_logf(__uint_as_float(mb[lane]));
+  };
+  /* second bound of the first k entries of queue 1; survivors to queue 2 */
+  auto drain1 = [&](int k) {
+    bool pass = false;
+    const uint32_t e = lane < k ? q1[lane] : 0u;
+    const int le = lane < k ? (int)(e & 63u) : lane;
+    const float lmq = __shfl(lm, le, 64); /* every lane takes part */
+    if (lane < k) {
+      const int ql = (int)(e >> 6);
+      float z[S];
+      zload(P0 + b0 + ql, pbase + le, z);
+      bool zero = false;
+#pragma unroll
+      for (int s = 0; s < S; ++s) zero |= z[s] == 0.f;
+      pass = zero || !(pf_bound_sum<S>(pf_sorted<S>(z), cfs + ql * rowf, GS, lwc) + 0.00390625f < lmq);
+    }
+    const uint64_t bal = __ballot(pass);
+    const int pos = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0));
+    if (pass) q2[n2 + pos] = e;
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    n2 += __popcll(bal);
+    if (n2 >= 64) drain2(64);
+    const int rest = n1 - k;
+    if (lane < rest) {
+      const uint32_t v = q1[k + lane];
+      q1[lane] = v;
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    n1 = rest;
+  };
+
+  for (b0 = 0; b0 < np; b0 += 64) {
+    const int nq = (int)min((int64_t)64, np - b0);
+    __syncthreads(); /* every wave is done with the previous block */
+    {
+      const float* src = a.coef + (P0 + b0) * (int64_t)rowf;
+      for (int e = threadIdx.x; e < nq * rowf; e += 64 * PF_WG) cfs[e] = src[e];
+    }
+    __syncthreads();
+    if (pbase >= a.nperm) continue; /* a wave past the last permutation only joins the barriers */
+    float zn[S];
+    zload(P0 + b0, valid ? p : pbase, zn);
+    for (int ql = 0; ql < nq; ++ql) {
+      float z[S];
+#pragma unroll
+      for (int s = 0; s < S; ++s) z[s] = zn[s];
+      if (ql + 1 < nq) zload(P0 + b0 + ql + 1, valid ? p : pbase, zn);
+      if (a.pflag[P0 + b0 + ql]) { /* every ABF of a non-estimable SNP is 1 */
+        if (valid) atomicMax(&mb[lane], __float_as_uint(1.f));
+        __builtin_amdgcn_s_waitcnt(0);
+        lm = __builtin_amdgcn_logf(__uint_as_float(mb[lane]));
+        continue;
+      }
+      bool zero = false;
+#pragma unroll
+      for (int s = 0; s < S; ++s) zero |= z[s] == 0.f;
+      const float u = pf_bound_max<S>(pf_sorted<S>(z), cfs + ql * rowf, GS, lw) + lg_terms;
+      /* a margin of 2^-8 for the FP32 rounding of the bound and the maximum */
+      const bool cand = valid && (zero || !(u + 0.00390625f < lm));
+      const uint64_t bal = __ballot(cand);
+      if (bal) {
+        const int pos =
+            __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0));
+        if (cand) q1[n1 + pos] = ((uint32_t)ql << 6) | (uint32_t)lane;
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_wave_barrier();
+        n1 += __popcll(bal);
+        if (n1 >= 64) drain1(64);
+      }
+    }
+    while (n1 > 0) drain1(min(n1, 64));
+    while (n2 > 0) drain2(min(n2, 64));
+  }
+  if (bad) atomicOr(&a.fallback[bi], 1);
+  if (valid) a.pmax[(int64_t)slot * a.NPP + p] = __uint_as_float(mb[lane]);
 }
 
 template <int MODE>
@@ -762,6 +987,17 @@ void launch_bma_s<PF_JOIN_GENSIN>(int S, dim3 grid, hipStream_t st, const PfBmaA
 #define PF_CASE(SS) \
   case SS: hipLaunchKernelGGL((pf_bma_kernel<SS, PF_JOIN_GENSIN>), grid, dim3(256), 0, st, a); break;
     PF_CASE(1) PF_CASE(2) PF_CASE(3) PF_CASE(4) PF_CASE(5) PF_CASE(6) PF_CASE(7) PF_CASE(8)
+#undef PF_CASE
+    default: break;
+  }
+}
+
+void launch_bma_max(int S, dim3 grid, hipStream_t st, const PfBmaArgs& a) {
+  const size_t lds = (size_t)64 * a.GS * (1 + 2 * S) * sizeof(float);
+  switch (S) {
+#define PF_CASE(SS) \
+  case SS: hipLaunchKernelGGL((pf_bma_max_kernel<SS>), grid, dim3(64 * PF_WG), lds, st, a); break;
+    PF_CASE(1) PF_CASE(2) PF_CASE(3) PF_CASE(4) PF_CASE(5) PF_CASE(6)
 #undef PF_CASE
     default: break;
   }
@@ -847,6 +1083,7 @@ int perm_fast_run(eqbf_ctx* ctx, const eqbf_chunk* cp, const eqbf_perm_opts* po,
       double ch = 1;
       for (int k = 0; k < m; ++k) ch = ch * (S - k) / (k + 1);
       ba.lw[1][m] = (float)lg2(1.0 / ((double)S * ch * GS));
+      ba.lwc[m] = (float)lg2(ch);
     }
     if (mode == PF_JOIN_GEN) ba.lw[0][m] = (float)lg2(1.0 / GL);
     if (mode == PF_JOIN_GENSIN) {
@@ -885,8 +1122,16 @@ int perm_fast_run(eqbf_ctx* ctx, const eqbf_chunk* cp, const eqbf_perm_opts* po,
       for (int pb = 0; pb < npb16; ++pb)
         for (int s = s_lo; s < s_hi; ++s) gitems.push_back(make_int4(b, pb, s, 0));
     std::vector<int4> bitems;
+    /* --pbf all --maxbf: one item per (gene, 64 permutations) over all the
+     * gene's SNPs (pf_bma_max_kernel); otherwise blocks of PF_SB SNPs */
+    const bool prune = join && mode == PF_JOIN_ALL && po->use_max_bf && S <= 6 &&
+                       (size_t)64 * GS * (1 + 2 * S) * sizeof(float) <= 48 * 1024;
     if (join)
       for (int b = 0; b < nb; ++b) {
+        if (prune) {
+          for (int pb = 0; pb < npb64; pb += PF_WG) bitems.push_back(make_int4(b, pb, 0, bslot[b]));
+          continue;
+        }
         const int nsb = (int)((bpoff[b + 1] - bpoff[b] + PF_SB - 1) / PF_SB);
         for (int pb = 0; pb < npb64; ++pb)
           for (int sb = 0; sb < nsb; ++sb) bitems.push_back(make_int4(b, pb, sb, bslot[b] + sb));
@@ -1014,7 +1259,9 @@ int perm_fast_run(eqbf_ctx* ctx, const eqbf_chunk* cp, const eqbf_perm_opts* po,
       ba.nperm = nperm;
       const dim3 grid((unsigned)((bitems.size() + 3) / 4));
       ProfScope ps(ctx, "pf_bma_kernel", st);
-      if (mode == PF_JOIN_ALL)
+      if (prune)
+        launch_bma_max(S, dim3((unsigned)bitems.size()), st, ba);
+      else if (mode == PF_JOIN_ALL)
         launch_bma_s<PF_JOIN_ALL>(S, grid, st, ba);
       else if (mode == PF_JOIN_GEN)
         launch_bma_s<PF_JOIN_GEN>(S, grid, st, ba);
@@ -1040,6 +1287,7 @@ int perm_fast_run(eqbf_ctx* ctx, const eqbf_chunk* cp, const eqbf_perm_opts* po,
       fa.mode = mode;
       fa.use_max = po->use_max_bf;
       fa.sep_sbgrp = po->sep_sbgrp;
+      fa.one_slot = prune ? 1 : 0;
       fa.nperm = nperm;
       fa.NPP = NPP;
       const int64_t ne = (int64_t)nb * nperm;
