@@ -91,15 +91,20 @@ __global__ void __launch_bounds__(64) perm_reduce_kernel(
     const int64_t* __restrict__ off_v, const int32_t* __restrict__ vg_gene, const int32_t* __restrict__ vg_perm,
     const int32_t* __restrict__ pair_snp_v, const int32_t* __restrict__ n_out, const double* __restrict__ sstats,
     const double* __restrict__ w, const uint8_t* __restrict__ gene_in, const uint8_t* __restrict__ snp_in,
-    int64_t NP, int S, int join, int w_row, int use_max, int sep_sbgrp, int64_t out_stride,
+    int64_t NP, int S, int join, int w_row, int use_max, int sep_sbgrp, int64_t out_stride, int mvlr,
     double* __restrict__ stats) {
   const bool truth = vg_perm[blockIdx.x] < 0;
   const int v = blockIdx.x, lane = threadIdx.x;
   const int g = vg_gene[v];
   const int64_t a = off_v[v], b = off_v[v + 1];
   auto val = [&](int64_t q) -> double {
-    if (join) return w[(int64_t)w_row * NP + q];
     const int64_t snp = pair_snp_v[q];
+    if (join && mvlr && !truth) { /* a SNP skipped by --error mvlr keeps 0.0 */
+      bool all = true;
+      for (int s = 0; s < S; ++s) all &= snp_in[snp * S + s] != 0;
+      if (!all) return 0.0;
+    }
+    if (join) return w[(int64_t)w_row * NP + q];
     if (sep_sbgrp >= 0) {
       const int s = sep_sbgrp;
       if (!(gene_in[(int64_t)g * S + s] && snp_in[snp * S + s])) return 1.0;
@@ -190,8 +195,11 @@ static int general_items_run(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_perm
   if (n_cfg < 0) return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED, "--pbf all with %d subgroups", S);
   const int64_t n_w = 3 + n_cfg + 2;
   const int w_row = !join ? 0 : (po->pbf == EQBF_PBF_GEN ? 0 : (po->pbf == EQBF_PBF_GENSIN ? (int)(3 + n_cfg) : (int)(3 + n_cfg + 1)));
+  const bool mvlr = join && po->error_model == EQBF_ERROR_MVLR;
+  const int GL = (int)ctx->phiL.size(), GS = (int)ctx->phiS.size();
+  const int64_t n_raw = mvlr ? 3 * (int64_t)GL + n_cfg * GS : 0;
   /* batch: at most 64k virtual genes and a bounded number of virtual pairs */
-  const int64_t per_pair_bytes = (int64_t)S * 4 + (join ? n_w * 8 : (int64_t)5 * S * 8) + 4;
+  const int64_t per_pair_bytes = (int64_t)S * 4 + (join ? (n_w + n_raw) * 8 : (int64_t)5 * S * 8) + 4;
   const int64_t pmax = std::max<int64_t>(1 << 16, std::min<int64_t>(8 << 20, ((int64_t)3 << 30) / per_pair_bytes));
   std::vector<int32_t> vg_gene, vg_table, vg_perm;
   std::vector<int64_t> off_v;
@@ -212,7 +220,9 @@ static int general_items_run(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_perm
     int32_t* d_n = (int32_t*)ctx_buf(ctx, "p_n", (size_t)S * std::max<int64_t>(NPv, 1) * sizeof(int32_t));
     double* d_w = join ? (double*)ctx_buf(ctx, "p_w", (size_t)n_w * std::max<int64_t>(NPv, 1) * sizeof(double)) : nullptr;
     double* d_ss = !join ? (double*)ctx_buf(ctx, "p_ss", (size_t)5 * S * std::max<int64_t>(NPv, 1) * sizeof(double)) : nullptr;
-    if (!d_vg || !d_off || !d_y || !d_gin || !d_psnp || !d_n || (join && !d_w) || (!join && !d_ss)) return EQBF_ERR_HIP;
+    double* d_raw = mvlr ? (double*)ctx_buf(ctx, "p_raw", (size_t)n_raw * std::max<int64_t>(NPv, 1) * sizeof(double)) : nullptr;
+    if (!d_vg || !d_off || !d_y || !d_gin || !d_psnp || !d_n || (join && !d_w) || (!join && !d_ss) || (mvlr && !d_raw))
+      return EQBF_ERR_HIP;
     std::vector<int32_t> packed;
     packed.insert(packed.end(), vg_gene.begin(), vg_gene.end());
     packed.insert(packed.end(), vg_table.begin(), vg_table.end());
@@ -241,13 +251,22 @@ static int general_items_run(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_perm
     vo.n = d_n;
     vo.sstats = d_ss;
     vo.w_abf = d_w;
-    int rc = assoc_uvlr_run(ctx, &vc, &ao, &vo, st);
+    int rc;
+    if (mvlr) {
+      eqbf_mvlr_opts mo;
+      mo.bfs = bfs;
+      mo.fiterr = po->fiterr;
+      vo.raw_abf = d_raw;
+      rc = mvlr_run(ctx, &vc, &mo, &vo, st);
+    } else {
+      rc = assoc_uvlr_run(ctx, &vc, &ao, &vo, st);
+    }
     if (rc) return rc;
     {
       ProfScope ps(ctx, fp64_check ? "fp64_check_reduce_kernel" : "perm_reduce_kernel", st);
       hipLaunchKernelGGL(perm_reduce_kernel, dim3(VB), dim3(64), 0, st, d_off, d_vg, d_vg + 2 * VB, d_psnp, d_n,
                          d_ss, d_w, c.gene_in_sbgrp, c.snp_in_sbgrp, NPv, S, (int)join, w_row, po->use_max_bf,
-                         po->sep_sbgrp, out_stride, out);
+                         po->sep_sbgrp, out_stride, (int)mvlr, out);
     }
     EQBF_HIP(ctx, hipGetLastError());
     /* the host vectors are reused for the next batch */
@@ -303,9 +322,14 @@ int perm_stats_run(eqbf_ctx* ctx, const eqbf_chunk* cp, const eqbf_perm_opts* po
   nan_fill(stats_dev, (int64_t)G * nperm, st);
   EQBF_HIP(ctx, hipGetLastError());
   /* genes that qualify go through the contraction path first */
-  std::vector<uint8_t> done;
-  rc = perm_fast_run(ctx, cp, po, tables_dev, n_tables, gene_table, nperm, stats_dev, st, done);
-  if (rc) return rc;
+  const bool mvlr = po->analysis == EQBF_ANALYSIS_JOIN && po->error_model == EQBF_ERROR_MVLR;
+  if (po->error_model != EQBF_ERROR_UVLR && !mvlr)
+    return ctx_fail(ctx, EQBF_ERR_ARG, "error_model %d is not valid for this analysis", po->error_model);
+  std::vector<uint8_t> done(G, 0);
+  if (!mvlr) { /* the contraction path is the uvlr model's */
+    rc = perm_fast_run(ctx, cp, po, tables_dev, n_tables, gene_table, nperm, stats_dev, st, done);
+    if (rc) return rc;
+  }
   std::vector<std::pair<int32_t, int32_t>> items;
   for (int g = 0; g < G; ++g) {
     if (gene_table[g] < 0 || done[g] || off[g + 1] <= off[g]) continue;
@@ -324,7 +348,9 @@ int perm_ties_run(eqbf_ctx* ctx, const eqbf_chunk* cp, const eqbf_perm_opts* po,
   if (!(tie_bound >= 0)) return ctx_fail(ctx, EQBF_ERR_ARG, "tie_bound must be >= 0");
   int rc = perm_stats_run(ctx, cp, po, tables_dev, n_tables, gene_table, nperm, stats_dev, st);
   if (rc) return rc;
-  const bool fp64 = po->precision == EQBF_PREC_FP64_PLAIN || po->precision == EQBF_PREC_FP64_BATCHED;
+  /* FP64 already: the FP64 precisions and the MVLR engine */
+  const bool fp64 = po->precision == EQBF_PREC_FP64_PLAIN || po->precision == EQBF_PREC_FP64_BATCHED ||
+                    (po->analysis == EQBF_ANALYSIS_JOIN && po->error_model == EQBF_ERROR_MVLR);
   if (fp64 || nperm <= 0) return EQBF_OK;
   /* permutations within the bound of the caller's true statistic */
   const int64_t n = (int64_t)G * nperm;

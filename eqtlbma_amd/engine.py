@@ -86,7 +86,8 @@ def lib():
 
 class PermOpts(C.Structure):
     _fields_ = [("analysis", C.c_int32), ("pbf", C.c_int32), ("use_max_bf", C.c_int32),
-                ("sep_sbgrp", C.c_int32), ("precision", C.c_int32)]
+                ("sep_sbgrp", C.c_int32), ("precision", C.c_int32), ("error_model", C.c_int32),
+                ("fiterr", C.c_double)]
 
 
 class MvlrOpts(C.Structure):
@@ -249,14 +250,16 @@ class Engine:
                                                       C.byref(out_struct), stream))
 
     def perm_stats(self, hc, tables, gene_table, analysis="join", pbf="all", use_max_bf=True, sep_sbgrp=-1,
-                   precision="mixed"):
+                   precision="mixed", error_model="uvlr", fiterr=0.5):
         """Statistic of every permutation: tables [n_tables][nperm][N],
-        gene_table [n_genes] (-1 skips a gene); returns [n_genes][nperm]."""
+        gene_table [n_genes] (-1 skips a gene); returns [n_genes][nperm].
+        error_model 'mvlr': MVLR Bayes factors (join)."""
         tables = np.ascontiguousarray(tables, dtype=np.int32)
         gt = np.ascontiguousarray(gene_table, dtype=np.int32)
         nperm = tables.shape[1]
         out = np.full((hc.G, nperm), np.nan)
-        po = PermOpts(_abi.ANALYSIS[analysis], _abi.PBF[pbf], int(use_max_bf), sep_sbgrp, _abi.PREC[precision])
+        po = PermOpts(_abi.ANALYSIS[analysis], _abi.PBF[pbf], int(use_max_bf), sep_sbgrp, _abi.PREC[precision],
+                      {"uvlr": 0, "mvlr": 1}[error_model], float(fiterr))
         st = hc.struct()
         self._check(lib().eqbf_perm_stats(self._ctx, C.byref(st), C.byref(po), _p(tables), tables.shape[0],
                                           _p(gt), nperm, _p(out)))

@@ -56,7 +56,7 @@ struct LoadOptions {
   std::string geno_list, snp_coords, exp_list, gene_coords, covar_list, snps_to_keep;
   std::vector<std::string> subgroups_to_keep;
   std::string error_model = "uvlr";
-  double min_maf = 0.0;
+  float min_maf = 0.0f; /* compared as the reference does, maf < (float) cutoff */
   int verbose = 1;
 };
 

@@ -871,9 +871,16 @@ int main(int argc, char** argv) {
     std::vector<std::vector<PermRes>> ps(S, std::vector<PermRes>(G));
     if (perm_join || perm_sep) {
       const int64_t N = sh.N;
+      /* makePermutationsJoin (eqtlbma_bf.cpp:851-858): genes with a cis SNP;
+       * with --error mvlr also expression in every subgroup */
       std::vector<int> genes;
-      for (int g = 0; g < G; ++g)
-        if (ch.pair_offset[g + 1] > ch.pair_offset[g]) genes.push_back(g);
+      for (int g = 0; g < G; ++g) {
+        if (ch.pair_offset[g + 1] <= ch.pair_offset[g]) continue;
+        bool all_expr = true;
+        for (int s = 0; s < S; ++s) all_expr &= ch.genes[g]->exp_row[s] >= 0;
+        if (mvlr && !all_expr) continue;
+        genes.push_back(g);
+      }
       eqbf_rng rng, rngT;
       auto reseed = [&]() {
         eqbf_rng_seed(&rng, o.seed);
@@ -881,6 +888,8 @@ int main(int argc, char** argv) {
       };
       eqbf_perm_opts po{};
       po.precision = prec;
+      po.error_model = mvlr ? EQBF_ERROR_MVLR : EQBF_ERROR_UVLR;
+      po.fiterr = o.prop_cov_errors;
       std::vector<double> truth;
       if (perm_join) {
         reseed();

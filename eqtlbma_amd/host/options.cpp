@@ -133,13 +133,13 @@ Options parse_options(int argc, char** argv) {
     else if (n == "lik") o.likelihood = a;
     else if (n == "analys") o.analysis = a;
     else if (n == "qnorm") o.need_qnorm = true;
-    else if (n == "maf") o.min_maf = std::atof(a);
+    else if (n == "maf") o.min_maf = (float)std::atof(a);
     else if (n == "covar") o.covar = a;
     else if (n == "gridL") o.gridL = a;
     else if (n == "gridS") o.gridS = a;
     else if (n == "bfs") o.bfs = a;
     else if (n == "error") o.error_model = a;
-    else if (n == "fiterr") o.prop_cov_errors = std::atof(a);
+    else if (n == "fiterr") o.prop_cov_errors = (float)std::atof(a);
     else if (n == "nperm") o.nb_permutations = (uint64_t)std::atol(a);
     else if (n == "seed") o.seed = (uint64_t)std::atol(a);
     else if (n == "trick") o.trick = std::atoi(a);
@@ -214,8 +214,6 @@ Options parse_options(int argc, char** argv) {
     bad(o, prog, "if --analys join --bfs gen --nperm > 0, --pbf should be 'gen'");
   if (o.analysis == "join" && o.nb_permutations > 0 && o.permbf != "none" && o.bfs == "sin" && o.permbf == "all")
     bad(o, prog, "if --analys join --bfs sin --nperm > 0, --pbf should be 'gen' or 'gen-sin'");
-  if (o.analysis == "join" && o.error_model == "mvlr" && o.nb_permutations > 0 && o.permbf != "none")
-    bad(o, prog, "permutations with --error mvlr are not supported by this build");
   if (o.nb_threads <= 0) bad(o, prog, "--thread " + std::to_string(o.nb_threads) + " is invalid");
   if (!o.snps_to_keep.empty() && !file_exists(o.snps_to_keep)) bad(o, prog, "can't find " + o.snps_to_keep);
   if (o.precision != "mixed" && o.precision != "fp64") bad(o, prog, "--precision should be mixed or fp64");

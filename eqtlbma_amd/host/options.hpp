@@ -15,7 +15,9 @@ namespace host {
 struct Options {
   int verbose = 1, trick = 0, perm_sep = 0, nb_threads = 1, write_group_size = 10;
   uint64_t radius = 100000, nb_permutations = 0, seed = UINT64_MAX, trick_cutoff = 10;
-  double min_maf = 0.0, prop_cov_errors = 0.5;
+  /* float, as the reference parses them (eqtlbma_bf.cpp:1590): --maf 0.05
+   * compares against 0.05f, --fiterr 0.3 is 0.3f */
+  float min_maf = 0.0f, prop_cov_errors = 0.5f;
   bool save_sstats = false, save_weighted_abfs = false, need_qnorm = false, use_max_bf = false;
   std::string geno, scoord, exp, gcoord, anchor = "TSS", inss, out, likelihood = "normal", analysis, covar,
       gridL, gridS, bfs = "gen", error_model = "uvlr", permbf = "none", snps_to_keep;

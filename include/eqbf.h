@@ -247,7 +247,16 @@ typedef struct eqbf_perm_opts {
   int32_t use_max_bf;  /* join: 1 = max over SNPs (--maxbf), 0 = average   */
   int32_t sep_sbgrp;   /* sep: subgroup (--permsep 2) or -1 (--permsep 1)  */
   int32_t precision;   /* EQBF_PREC_*                                      */
+  /* join: EQBF_ERROR_UVLR (0) or EQBF_ERROR_MVLR (--error mvlr,
+   * gene.cpp:662-672): every permutation's Bayes factors come from the MVLR
+   * engine (FP64, eqbf_assoc_mvlr) on the permuted levels; a SNP without
+   * genotypes in every subgroup contributes 0 to the statistic (its initial
+   * value in the reference's loop, gene.cpp:650, 663-664); the subgroups of
+   * a gene must keep the same samples under every permutation. */
+  int32_t error_model;
+  double fiterr;       /* --fiterr with EQBF_ERROR_MVLR                    */
 } eqbf_perm_opts;
+enum { EQBF_ERROR_UVLR = 0, EQBF_ERROR_MVLR = 1 };
 
 /* Statistic of every permutation of every gene of the chunk.
  * perm_idx:   [n_tables][nperm][n_samples]; sample i of permutation p of

@@ -59,6 +59,8 @@ def lib():
         L.orc_perm_join_stats.argtypes = [vp, i64, i32, i32, vp, vp, i32, vp, vp, i32,
                                           vp, i64, vp]
         L.orc_perm_join_stats.restype = C.c_int
+        L.orc_perm_join_stats_mvlr.argtypes = [vp, i64, i32, i32, d, vp, vp, i32, vp, vp, i32, vp, i64, vp]
+        L.orc_perm_join_stats_mvlr.restype = C.c_int
         L.orc_true_stat.argtypes = [vp, i64, i32]
         L.orc_true_stat.restype = d
         L.orc_perm_count.argtypes = [vp, d, vp, i32, i64, vp, vp, vp, vp]
@@ -276,6 +278,23 @@ def perm_join_stats(hc, g, pbf, use_max_bf, gridL, gridS, table):
     rc = lib().orc_perm_join_stats(C.addressof(st), g, _abi.PBF[pbf], int(use_max_bf),
                                    _p(phiL), _p(omaL), gL.shape[0], _p(phiS), _p(omaS),
                                    gS.shape[0], _p(table), nperm, _p(stats))
+    if rc != 0:
+        raise RuntimeError("oracle: %s" % lib().orc_last_error().decode())
+    return stats
+
+
+def perm_join_stats_mvlr(hc, g, pbf, use_max_bf, fiterr, gridL, gridS, table):
+    """--error mvlr permutation statistics of gene g for every row of table."""
+    gL = np.ascontiguousarray(gridL, dtype=np.float64)
+    gS = np.ascontiguousarray(gridS, dtype=np.float64)
+    phiL, omaL = np.ascontiguousarray(gL[:, 0]), np.ascontiguousarray(gL[:, 1])
+    phiS, omaS = np.ascontiguousarray(gS[:, 0]), np.ascontiguousarray(gS[:, 1])
+    table = np.ascontiguousarray(table, dtype=np.int32)
+    stats = np.zeros(table.shape[0])
+    st = hc.struct()
+    rc = lib().orc_perm_join_stats_mvlr(C.addressof(st), g, _abi.PBF[pbf], int(use_max_bf), float(fiterr),
+                                        _p(phiL), _p(omaL), gL.shape[0], _p(phiS), _p(omaS), gS.shape[0],
+                                        _p(table), table.shape[0], _p(stats))
     if rc != 0:
         raise RuntimeError("oracle: %s" % lib().orc_last_error().decode())
     return stats

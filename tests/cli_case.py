@@ -225,8 +225,9 @@ def load_expected_inputs(case):
                 covar_in=covar_in)
 
 
-def expected_chunks(inp, radius, wrtsize, mvlr=False):
-    """Oracle chunks in the reference's order: [(gene names, HostChunk, snp names)]."""
+def expected_chunks(inp, radius, wrtsize, mvlr=False, anchor_tes=False):
+    """Oracle chunks in the reference's order: [(gene names, HostChunk, snp names)].
+    anchor_tes: cis window [start - radius, end + radius] (--anchor TSS+TES)."""
     S = len(inp["sbgrps"])
     N = len(inp["samples"])
     genes = sorted(g for g in inp["gcoord"] if g in inp["explev"])
@@ -248,9 +249,10 @@ def expected_chunks(inp, radius, wrtsize, mvlr=False):
             for s, y in inp["explev"][gname].items():
                 expl[gi, s] = y
                 gene_in[gi, s] = 1
-            chrom, start, _end = inp["gcoord"][gname]
+            chrom, start, end = inp["gcoord"][gname]
             lo = start - radius if start >= radius else -1
-            cis = [n for p, n in snps_by_chr.get(chrom, []) if lo <= p <= start + radius]
+            hi = (end if anchor_tes else start) + radius
+            cis = [n for p, n in snps_by_chr.get(chrom, []) if lo <= p <= hi]
             has = any(gene_in[gi, s] and n in inp["geno_vals"][s] for n in cis for s in range(S))
             if not has or (mvlr and not gene_in[gi].all()):
                 cis = []

@@ -83,6 +83,48 @@ def _expected_join(orc, c, inp, seed, nperm, trick, cutoff, wrtsize):
     return {0: out}, 0
 
 
+def _expected_mvlr(orc, c, inp, seed, nperm, trick, cutoff, wrtsize):
+    gL, gS = c["gL"], c["gS"]
+    nb = nperm
+    out = []
+    for names, hc, _psn in cli_case.expected_chunks(inp, 2500, wrtsize, mvlr=True):
+        ref = orc.assoc_mvlr(hc, "all", 0.5, gL, gS) if hc.n_pairs else None
+        mt = orc.MT(seed)
+        mtT = orc.MT(seed) if trick else None
+        for g, gname in enumerate(names):
+            p0, p1 = hc.pair_offset[g], hc.pair_offset[g + 1]
+            if p1 == p0:
+                continue
+            true = orc.true_stat(ref.w[3 + 7 + 1, p0:p1], True)
+            st = _stats_from(orc, mt, nb, hc.N,
+                             lambda t: orc.perm_join_stats_mvlr(hc, g, "all", True, 0.5, gL, gS, t))
+            pv, used, _med, nb, drawn = orc.perm_count(st, true, nb, trick, cutoff, mtT)
+            _advance(mt, drawn, hc.N)
+            out.append([gname, p1 - p0, pv, used, true])
+    return {0: out}, 0
+
+
+def test_cli_mvlr_permutations_trick2(tmp_path, orc):
+    # --error mvlr --nperm --pbf all --maxbf --trick 2 (the manual's recipe)
+    root = str(tmp_path)
+    c = cli_case.make_case(root, seed=14, S=3, N=30, mvlr=True)
+    inp = cli_case.load_expected_inputs(c)
+    out = os.path.join(root, "mvp")
+    nperm, seed, cutoff = 30, 9, 3
+    _run(_common_args(root, out, "join") + ["--covar", os.path.join(root, "list_cov.txt"), "--error", "mvlr",
+                                            "--fiterr", "0.5", "--bfs", "all", "--nperm", str(nperm), "--seed",
+                                            str(seed), "--pbf", "all", "--maxbf", "--trick", "2", "--tricut",
+                                            str(cutoff)], root)
+    exp, _ = _expected_mvlr(orc, c, inp, seed, nperm, 2, cutoff, 2)
+    rows = cli_case.read_table(out + "_joinPermPvals.txt.gz")
+    assert len(rows) - 2 == len(exp[0])
+    for got, want in zip(rows[2:], exp[0]):
+        assert got[0] == want[0] and int(got[1]) == want[1]
+        assert int(got[3]) == want[3], (got, want)
+        assert _num(got[2]) == pytest.approx(want[2], rel=1e-5), (got, want)
+        assert _close(_num(got[4]), want[4]), (got, want)
+
+
 def _expected_sep2(orc, c, inp, seed, nperm, trick, cutoff, wrtsize):
     S = len(inp["sbgrps"])
     nb = nperm
