@@ -1525,7 +1525,7 @@ int assoc_uvlr_run(eqbf_ctx* ctx, const eqbf_chunk* cp, const eqbf_assoc_opts* o
     hipLaunchKernelGGL(tiles_kernel, dim3(1), dim3(1024), 0, st, c.pair_offset, G, tiles, n_tiles);
   }
   if (fast) {
-    rc = uvlr_fast_run(ctx, c, o, out, st, rec, B, memb, tiles, n_tiles, KU, d_masks, (int)n_cfg);
+    rc = uvlr_fast_run(ctx, c, o, out, st, rec, B, memb, idx, tiles, n_tiles, KU, d_masks, (int)n_cfg);
     if (rc) return rc;
     if (join) {
       ProfScope ps(ctx, "bma_kernel", st);

@@ -370,8 +370,14 @@ void launch_zero_u64(uint64_t* p, int64_t n, hipStream_t st);
 /* batched mixed-precision uvlr path (one genotype set, matrix-core sums) */
 bool uvlr_fast_eligible(const eqbf_chunk& c, int KU);
 int uvlr_fast_run(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_assoc_opts* o, eqbf_assoc_out* out, hipStream_t st,
-                  const PrepRec* rec, const double* B, const uint64_t* memb, const int4* tiles, const int* n_tiles,
-                  int KU, const uint64_t* d_masks, int n_cfg);
+                  const PrepRec* rec, const double* B, const uint64_t* memb, const int32_t* idx, const int4* tiles,
+                  const int* n_tiles, int KU, const uint64_t* d_masks, int n_cfg);
+/* int8-digit sums of the batched path (eqbf_uvlr_i8.hip): x~'x~ and x'y~
+ * into sx2, and a bit in redo for each item to recompute in FP64 */
+bool uvlr_i8_eligible(const eqbf_chunk& c, int KU);
+int uvlr_i8_sums_run(eqbf_ctx* ctx, const eqbf_chunk& c, hipStream_t st, const PrepRec* rec, const double* B,
+                     const uint64_t* memb, const int4* tiles, const int* n_tiles, int KU, double* sx2,
+                     uint32_t* redo);
 int mvlr_run(eqbf_ctx* ctx, const eqbf_chunk* c, const eqbf_mvlr_opts* o, eqbf_assoc_out* out, hipStream_t st);
 /* --inss: Bayes factors from given summary statistics (device pointers) */
 int abf_from_sstats_run(eqbf_ctx* ctx, const eqbf_sstats_in* in, int bfs, int raw_dtype, eqbf_assoc_out* out,

@@ -31,6 +31,7 @@
  *   configurations sweep their active subgroups per grid point.
  */
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <type_traits>
 #include <vector>
@@ -61,6 +62,8 @@ struct SumsParams {
   uint8_t* st_f; /* [S][NP] role of the subgroup (F_*)                  */
   double* sx2;   /* [2][S][NP] x~'x~ (NaN: not estimable) and x'y~, when the
                     statistics are a kernel of their own                  */
+  const int32_t* idx; /* [G][S][N] members of (g, s), for the FP64 sums  */
+  uint32_t* redo;     /* [S * NP bits] items sums_repair_kernel recomputes */
   int KU, KC, CX, NXT, KCH, SPP;
 };
 
@@ -115,6 +118,9 @@ __device__ __forceinline__ void sums_epilogue(const eqbf_chunk& c, const SumsPar
       if (SPLIT) {
         sp.sx2[o] = uvlr_xx(E[j * ecols + NL + (s - sa)], proj);
         sp.sx2[(int64_t)S * NP + o] = e[0];
+        /* a NaN genotype somewhere in the SNP's column reaches every
+         * subgroup through the zero rows of non-members */
+        if (isnan(e[0])) atomicOr(&sp.redo[o >> 5], 1u << (o & 31));
         continue;
       }
       const int64_t snp = c.pair_snp[p];
@@ -273,7 +279,7 @@ __global__ void __launch_bounds__(NT_SUMS) uvlr_sums_kernel(const eqbf_chunk c, 
       __syncthreads();
     }
 
-    sums_epilogue<NXW, NMW, MULTI, false>(c, sp, buf, ax, am, g, p0, cnt);
+    sums_epilogue<NXW, NMW, MULTI, true>(c, sp, buf, ax, am, g, p0, cnt);
   }
 }
 
@@ -449,6 +455,78 @@ __device__ inline bool uvlr_bf_inputs_fast(const PrepRec& r, double xx, double s
   b = copysignf(az * sqrtf(v), (float)beta);
   z2 = az * az;
   return true;
+}
+
+/* x~'x~ and x'y~ in FP64 of the items the sums kernel marked, one wave per
+ * item: the uncentred sums over the members, as the FP64 product forms
+ * them (FitSingleGeneWithSingleSnp, utils_math.cpp:166-209).  A genotype
+ * that is NaN for a member makes them NaN; one that is NaN for a
+ * non-member is not read (GeneSnpPair::FillStlContainers reads the members'
+ * genotypes only, gene_snp_pair.cpp:79-170). */
+constexpr int NT_REPAIR = 256;
+__global__ void redo_zero_kernel(uint32_t* __restrict__ p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = 0u;
+}
+__device__ inline double repair_wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__global__ void __launch_bounds__(NT_REPAIR) sums_repair_kernel(const eqbf_chunk c, const SumsParams sp) {
+  const int N = c.n_samples, S = c.n_subgroups, G = c.n_genes;
+  const int64_t NP = c.n_pairs, W = ((int64_t)S * NP + 31) / 32;
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (int64_t)blockIdx.x * (NT_REPAIR / 64) + (threadIdx.x >> 6);
+  const int64_t n_waves = (int64_t)gridDim.x * (NT_REPAIR / 64);
+  for (int64_t word = wave; word < W; word += n_waves) {
+    uint32_t bits = sp.redo[word];
+    while (bits) {
+      const int b = __builtin_ctz(bits);
+      bits &= bits - 1u;
+      const int64_t o = word * 32 + b;
+      const int s = (int)(o / NP);
+      const int64_t p = o - (int64_t)s * NP;
+      int lo = 0, hi = G - 1; /* gene of pair p: pair_offset[g] <= p < pair_offset[g + 1] */
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (c.pair_offset[mid] <= p)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      const int g = lo;
+      const int64_t snp = c.pair_snp[p];
+      const PrepRec r = sp.rec[(int64_t)g * S + s];
+      if (r.n == 0 || (r.flags & PREP_TOO_FEW) || !c.snp_in_sbgrp[snp * S + s]) continue; /* no statistics */
+      const int n = r.n;
+      const int32_t* idx = sp.idx + ((int64_t)g * S + s) * N;
+      double sxx = 0.0, sy = 0.0, u[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) u[k] = 0.0;
+      for (int m = lane; m < n; m += 64) {
+        const int i = idx[m];
+        const double x = geno_at(c, i, snp);
+        const double* bb = sp.B + ((int64_t)g * N + i) * sp.CX + (int64_t)s * sp.KC;
+        sxx += x * x;
+        sy += x * bb[0];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+          if (k < sp.KU) u[k] += x * bb[1 + k];
+      }
+      sxx = repair_wave_sum(sxx);
+      sy = repair_wave_sum(sy);
+      double proj = 0.0;
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (k < sp.KU) {
+          const double t = repair_wave_sum(u[k]);
+          proj += t * t;
+        }
+      if (lane == 0) {
+        sp.sx2[o] = uvlr_xx(sxx, proj);
+        sp.sx2[(int64_t)S * NP + o] = sy;
+      }
+    }
+  }
 }
 
 /* FULL: the FP64 summary statistics are written too (sstats or stdsstats
@@ -828,8 +906,8 @@ bool uvlr_fast_eligible(const eqbf_chunk& c, int KU) {
 
 
 int uvlr_fast_run(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_assoc_opts* o, eqbf_assoc_out* out, hipStream_t st,
-                  const PrepRec* rec, const double* B, const uint64_t* memb, const int4* tiles, const int* n_tiles,
-                  int KU, const uint64_t* d_masks, int n_cfg) {
+                  const PrepRec* rec, const double* B, const uint64_t* memb, const int32_t* idx, const int4* tiles,
+                  const int* n_tiles, int KU, const uint64_t* d_masks, int n_cfg) {
   const int S = c.n_subgroups, G = c.n_genes;
   const int64_t NP = c.n_pairs;
   const bool join = o->analysis == EQBF_ANALYSIS_JOIN;
@@ -838,6 +916,7 @@ int uvlr_fast_run(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_assoc_opts* o, 
   sp.rec = rec;
   sp.B = B;
   sp.memb = memb;
+  sp.idx = idx;
   sp.tiles = tiles;
   sp.n_tiles = n_tiles;
   sp.KU = KU;
@@ -868,12 +947,19 @@ int uvlr_fast_run(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_assoc_opts* o, 
   /* LDS-direct staging when the coefficient rows split into 16-byte units;
    * the statistics are then a kernel of their own (the sums kernel keeps
    * its registers for the products) */
-  const bool dma = (sp.CX % 2) == 0;
+  /* int8-digit sums when only Bayes factors are written (their inputs are
+   * FP32); EQBF_I8_SUMS=0 in the environment keeps the FP64 product */
+  const char* i8env = getenv("EQBF_I8_SUMS");
+  const bool i8 = join && !sp.sstats && !sp.stdsstats && uvlr_i8_eligible(c, KU) && !(i8env && i8env[0] == '0');
+  const bool dma = i8 || (sp.CX % 2) == 0;
   const size_t max_tiles = (size_t)(NP / 64) + (size_t)G + 1;
-  if (dma) {
-    sp.sx2 = (double*)ctx_buf(ctx, "sums_x2", (size_t)2 * S * std::max<int64_t>(NP, 1) * sizeof(double));
-    if (!sp.sx2) return EQBF_ERR_HIP; /* ctx_buf has set the message */
-  }
+  /* the sums go through sx2 to the statistics kernel; items the sums
+   * kernel marks in the redo bitmap are recomputed in FP64 in between */
+  const int64_t redo_words = ((int64_t)S * NP + 31) / 32;
+  sp.sx2 = (double*)ctx_buf(ctx, "sums_x2", (size_t)2 * S * std::max<int64_t>(NP, 1) * sizeof(double));
+  sp.redo = (uint32_t*)ctx_buf(ctx, "sums_redo", (size_t)(redo_words + 2) * sizeof(uint32_t));
+  if (!sp.sx2 || !sp.redo) return EQBF_ERR_HIP; /* ctx_buf has set the message */
+  hipLaunchKernelGGL(redo_zero_kernel, dim3(256), dim3(256), 0, st, sp.redo, redo_words);
   const size_t xsz = c.geno_dtype == EQBF_F32 ? 4 : 8;
   auto stage_bytes = [&](int kch) {
     return dma ? (size_t)kch * ((size_t)sp.CX * 8 + 64 * xsz + 8) : (size_t)8 * kch * (65 + CXPP);
@@ -913,7 +999,10 @@ int uvlr_fast_run(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_assoc_opts* o, 
   snprintf(key, sizeof key, "sums/%d/%d/%d/%d/%d/%s", S, sp.KC, nxw, nmw, c.n_samples,
            dma ? (xsz == 4 ? "dma32" : "dma64") : "regs");
   auto tk = ctx->tuned.find(key);
-  if (tk != ctx->tuned.end()) {
+  if (i8) {
+    const int rc = uvlr_i8_sums_run(ctx, c, st, rec, B, memb, tiles, n_tiles, KU, sp.sx2, sp.redo);
+    if (rc) return rc;
+  } else if (tk != ctx->tuned.end()) {
     choice = tk->second;
   } else if (ctx->autotune_on && !ctx->capturing) {
     std::vector<eqbf_ctx::TuneChoice> cands;
@@ -951,12 +1040,17 @@ int uvlr_fast_run(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_assoc_opts* o, 
     (void)hipEventDestroy(e1);
     ctx->tuned[key] = choice;
   }
-  {
+  if (!i8) {
     ProfScope ps(ctx, "uvlr_sums_kernel", st);
     if (launch(choice)) return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED, "too many subgroup columns for the batched path");
   }
   EQBF_HIP(ctx, hipGetLastError());
-  if (dma) {
+  {
+    ProfScope ps(ctx, "sums_repair_kernel", st);
+    hipLaunchKernelGGL(sums_repair_kernel, dim3(1024), dim3(NT_REPAIR), 0, st, c, sp);
+  }
+  EQBF_HIP(ctx, hipGetLastError());
+  {
     ProfScope ps(ctx, "uvlr_stats_kernel", st);
     const dim3 sgrid((unsigned)((max_tiles + 7) / 8 * 8));
     if (sp.sstats || sp.stdsstats)

@@ -11,7 +11,10 @@ floor(80 KiB / (64 x 8 x 6 B)) = 26 of the 44 per pass, the tuned shapes 22
 or 11), and with more 64-pair tiles than the 512 workgroups of the
 persistent grid some workgroups loop over several tiles.
 
-The whole chunk runs on the GPU; the oracle (FP64 / long double, Gene::
+With Bayes factors as the only outputs the sums are the int8-digit product
+(exact integer digit products, FP64 recompute of the ill-conditioned
+items); with the FP64 statistics written, or EQBF_I8_SUMS=0, the FP64
+product.  The whole chunk runs on the GPU; the oracle (FP64 / long double, Gene::
 TestForAssociations, gene.cpp:263-332, GeneSnpPair::CalcAbfsUvlr,
 gene_snp_pair.cpp:364-464, 552-570) runs on genes spread across the chunk.
 Tolerances are those of the mixed path (north-star precision statement):
@@ -91,27 +94,33 @@ def _cmp(gpu, refs, with_sstats):
             assert err <= 1e-4, name
 
 
-@pytest.mark.parametrize("with_sstats", [False, True])
-def test_c1_shape_default_launch(case, with_sstats):
+@pytest.mark.parametrize("with_sstats,i8", [(False, True), (False, False), (True, True)])
+def test_c1_shape_default_launch(case, with_sstats, i8, monkeypatch):
+    """Bayes factors only: the int8-digit sums (or, with EQBF_I8_SUMS=0, the
+    FP64 product); with the FP64 statistics written: the FP64 product."""
     from eqtlbma_amd.engine import Engine
     hc, refs = case
+    if not i8:
+        monkeypatch.setenv("EQBF_I8_SUMS", "0")
     eng = Engine(0)
     try:
         eng.set_grids(G50, G50)
         eng.profile(True)
-        _say("[gpu] c1 shape, default launch shape, FP64 statistic outputs=%s" % with_sstats)
+        _say("[gpu] c1 shape, default launch shape, FP64 statistic outputs=%s, int8 sums allowed=%s" %
+             (with_sstats, i8))
         gpu = eng.assoc_uvlr(hc, "join", "sin", precision="mixed", raw_tiled=True,
                              want_sstats=with_sstats, want_std=with_sstats)
         prof = eng.profile_read()
-        # the LDS-direct sums kernel and the separate statistics kernel ran
+        # the batched sums kernel and the separate statistics kernel ran
         assert prof["uvlr_sums_kernel"][1] == 1 and prof["uvlr_stats_kernel"][1] == 1
         assert "assoc_mixed_kernel" not in prof
+        assert ("b8_convert_kernel" in prof) == (i8 and not with_sstats)
         _cmp(gpu, refs, with_sstats)
     finally:
         eng.close()
 
 
-def test_c1_shape_tuned_launch_shapes(case):
+def test_c1_shape_tuned_launch_shapes(case, monkeypatch):
     """The autotuner's key names the instance (44 subgroups, 5 coefficient
     columns each, 7 column blocks per wave, 2 column waves, 500 samples,
     FP32 LDS-direct staging); every shape it times and the one it keeps
@@ -119,6 +128,7 @@ def test_c1_shape_tuned_launch_shapes(case):
     default shape bit for bit."""
     from eqtlbma_amd.engine import Engine
     hc, refs = case
+    monkeypatch.setenv("EQBF_I8_SUMS", "0")  # the tuner times the FP64 product's launch shapes
     a = Engine(0)
     b = Engine(0)
     try:
