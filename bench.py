@@ -849,6 +849,10 @@ def abf_bytes(P_g, S, GL, GS):
 
 
 SUMS_FLOPS_PER_MEMBER = 2 * (4 + 2)  # x.x, x.y~ and x.U for 4 basis columns (3 covariates + 1)
+# int8-digit sums: per (pair, member) the y~ and 4 U columns take ten digit
+# products each and x'^2 four, two ops per multiply-add
+SUMS_I8_OPS_PER_MEMBER = 2 * (10 * (1 + 4) + 4)
+INT8_MATRIX_PEAK_TOPS = 5000.0
 
 
 def main():
@@ -935,7 +939,19 @@ def main():
     dominant = max(kern, key=lambda k: kern[k][0])
     dm_ms, dm_n = kern[dominant]
     avg_ms = dm_ms / max(dm_n, 1)
-    if dominant == "uvlr_sums_kernel":
+    i8_sums = "b8_convert_kernel" in kern
+    if dominant == "uvlr_sums_kernel" and i8_sums:
+        # int8-digit sums: the digit products of the member entries (the
+        # non-member rows of the dense product, padding and empty column
+        # slots are not counted)
+        ops_per_launch = pairs * members * SUMS_I8_OPS_PER_MEMBER / launches_per_step
+        achieved = ops_per_launch / (avg_ms / 1e3) / 1e12
+        roof = {"bound": "mfma", "achieved": achieved, "peak": INT8_MATRIX_PEAK_TOPS, "unit": "TOP/s (int8)",
+                "frac": achieved / INT8_MATRIX_PEAK_TOPS, "traffic": None, "kernel": dominant,
+                "avg_launch_ms": avg_ms, "alg_ops_per_launch": ops_per_launch,
+                "alg_ops_per_pair": members * SUMS_I8_OPS_PER_MEMBER,
+                "fp64_equivalent_tflops": pairs * members * SUMS_FLOPS_PER_MEMBER / launches_per_step / (avg_ms / 1e3) / 1e12}
+    elif dominant == "uvlr_sums_kernel":
         # FP64 matrix-core sums: algorithmic flops = 12 per (pair, member of a subgroup)
         flops_per_launch = pairs * members * SUMS_FLOPS_PER_MEMBER / launches_per_step
         achieved = flops_per_launch / (avg_ms / 1e3) / 1e12
@@ -961,6 +977,11 @@ def main():
     traffic = _load_traffic(dominant, args)
     if traffic is not None:
         roof["traffic"] = traffic
+    # the whole association pass against the HBM bytes it must move (genotype
+    # rows, raw and grid-averaged Bayes factors, expression levels)
+    path_bytes = sum(alg_bytes(int(p), S, N, GL, GS) for p in wl.P)
+    path_roof = {"bound": "hbm", "achieved": path_bytes / (ms_step / 1e3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": path_bytes / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS, "alg_bytes_per_step": path_bytes}
     log("step %.2f ms, %.3e pairs/s; kernels: %s" % (ms_step, value, {k: (round(v[0] / max(v[1], 1), 3), v[1]) for k, v in kern.items()}))
 
     # the same steps replayed from hipGraphs (first pass eager, second captured)
@@ -982,7 +1003,9 @@ def main():
     result = {
         "metric": METRIC, "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64 sums + f32 Bayes-factor terms (mixed)", "data": "synthetic",
+        "vs_baseline": None,
+        "dtype": ("exact int8-digit sums (FP64 recompute of ill-conditioned items) + f32 Bayes-factor terms"
+                  if i8_sums else "f64 sums + f32 Bayes-factor terms (mixed)"), "data": "synthetic",
         "config": {"workload": "batched ABFs: %d genes, 44 subgroups of N_s~U(80,400) from 500 "
                                "individuals, cis SNPs~Pois(1500) in [200,6000], FP32 dosages, 3 covariates, "
                                "--bfs sin, 50-point grid" % args.genes,
@@ -990,6 +1013,7 @@ def main():
                    "grid_points": GL, "chunk_genes": args.batch_genes,
                    "outputs": "sample sizes, FP32 raw ABFs (64-pair tiles), FP64 grid-averaged ABFs"},
         "roofline": roof,
+        "roofline_path": path_roof,
         "roofline_other_kernels": others,
         "kernels_ms_per_launch": {k: v[0] / max(v[1], 1) for k, v in kern.items()},
         "autotune": eng.autotune_read(),

@@ -65,6 +65,7 @@ constexpr int I8_PLANE = I8_ROWS * 64; /* bytes of one digit plane: 32 rows x 64
 constexpr int I8_ABUF = 8 * I8_PLANE;  /* the eight planes of one step              */
 /* s_waitcnt vmcnt(n) lgkmcnt(0) (expcnt not waited for) */
 constexpr int I8_VMCNT4 = 4 | (7 << 4);
+constexpr int I8_LGKM0 = 15 | (7 << 4) | (3 << 14); /* lgkmcnt(0) only */
 constexpr int I8_VMCNT8 = 8 | (7 << 4);
 
 /* 16 bytes per lane from global memory to LDS (M0 + 16 lane), by the load
@@ -135,10 +136,10 @@ __device__ __forceinline__ void digits4(int X, int (&d)[4]) {
 }
 
 /* scale exponent of the y~ column of (g, s): |y~_i| <= ||y~|| < 2^k */
-__device__ __forceinline__ int ytilde_exp(const PrepRec& r) {
-  if (!(r.yy > 0.0)) return 30;
+__device__ __forceinline__ int ytilde_exp(double yy) {
+  if (!(yy > 0.0)) return 30;
   int k;
-  (void)frexp(sqrt(r.yy), &k);
+  (void)frexp(sqrt(yy), &k);
   return 30 - k;
 }
 
@@ -152,14 +153,16 @@ __global__ void __launch_bounds__(256) b8_convert_kernel(const PrepRec* __restri
   const int g = blockIdx.x / NCK, ck = blockIdx.x % NCK;
   const int CT = CXP + SXP;
   uint8_t* dst = B8 + ((int64_t)g * NCK + ck) * CKB;
+  /* the four 16-sample units of a column row on adjacent lanes: each row
+   * is one 64-byte write */
   for (int it = threadIdx.x; it < 4 * CT; it += 256) {
-    const int u = it / CT, col = it % CT;
+    const int u = it & 3, col = it >> 2;
     const int i0 = ck * 64 + 16 * u;
     if (col < CXP) {
       uint32_t w[4][4];
       if (col < CX) {
         const int s = col / KC, k = col - s * KC;
-        const double sc = ldexp(1.0, k == 0 ? ytilde_exp(rec[(int64_t)g * S + s]) : 30);
+        const double sc = ldexp(1.0, k == 0 ? ytilde_exp(rec[(int64_t)g * S + s].yy) : 30);
 #pragma unroll
         for (int t4 = 0; t4 < 4; ++t4) {
           uint32_t pw[4] = {0u, 0u, 0u, 0u};
@@ -214,9 +217,14 @@ __global__ void __launch_bounds__(I8_NT) uvlr_sums_i8_kernel(const eqbf_chunk c,
   const int CKB = q.CKB, NCK = q.NCK, CXP = q.CXP, NXB = q.NXB;
   unsigned char* Abase = smem + 2 * CKB;
   const int small_off = max(2 * CKB + 2 * I8_ABUF, I8_ROWS * q.ES * 8);
-  int* cx_l = (int*)(smem + small_off); /* centre of each SNP row, in units of 2^-e */
-  int* ex_l = cx_l + I8_ROWS;            /* scale exponent e of each SNP row          */
-  int* bad_l = ex_l + I8_ROWS;           /* row to recompute in FP64                  */
+  double* ryy_l = (double*)(smem + small_off); /* y~'y~ of each subgroup of the gene      */
+  int* rn_l = (int*)(ryy_l + 64);          /* members of each subgroup                  */
+  int* rf_l = rn_l + 64;                   /* prep flags of each subgroup               */
+  int* yexp_l = rf_l + 64;                 /* scale exponent of the y~ column           */
+  int* cx_l = yexp_l + 64;                 /* centre of each SNP row, in units of 2^-e  */
+  int* ex_l = cx_l + I8_ROWS;              /* scale exponent e of each SNP row          */
+  int* bad_l = ex_l + I8_ROWS;             /* row to recompute in FP64                  */
+  uint8_t* snpin_l = (uint8_t*)(bad_l + I8_ROWS); /* [32][64]: SNP of the row kept in s */
   const int sw = swz(l16);
   /* every wave runs NBX y~/U slots and one x'^2 slot without branches (an
    * empty slot multiplies block 0 and is not written), so that the
@@ -230,13 +238,30 @@ __global__ void __launch_bounds__(I8_NT) uvlr_sums_i8_kernel(const eqbf_chunk c,
   const int T = 2 * *q.n_tiles; /* half tiles of 32 pairs */
   const int span = (T + 7) / 8;
   const int t0 = xcd * span, t1 = min(T, t0 + span);
+  /* the SNP of this thread's row in half tile t (a valid pair of the tile
+   * when the half is empty) */
+  auto row_snp = [&](int t, const int4& tl) -> int64_t {
+    const int cn = tl.z - 32 * (t & 1);
+    return c.pair_snp[cn > 0 ? tl.y + 32 * (t & 1) + min(jx, cn - 1) : tl.y];
+  };
+  /* the next half tile's descriptor and SNP are loaded one half tile ahead */
+  int4 tile_n = make_int4(0, 0, 0, 0);
+  int64_t snp_n = 0;
+  if (t0 + slot < t1) {
+    tile_n = q.tiles[(t0 + slot) >> 1];
+    snp_n = row_snp(t0 + slot, tile_n);
+  }
   for (int t = t0 + slot; t < t1; t += per_xcd) {
-    const int4 tile = q.tiles[t >> 1];
+    const int4 tile = tile_n;
+    const int64_t snp = snp_n;
+    if (t + per_xcd < t1) {
+      tile_n = q.tiles[(t + per_xcd) >> 1];
+      snp_n = row_snp(t + per_xcd, tile_n);
+    }
     const int g = tile.x;
     const int64_t p0 = tile.y + 32 * (t & 1);
     const int cnt = tile.z - 32 * (t & 1);
     if (cnt <= 0) continue; /* uniform over the workgroup */
-    const int64_t snp = c.pair_snp[p0 + min(jx, cnt - 1)];
     const xt* gcol = (const xt*)c.genotypes + snp;
     const uint8_t* Bg = q.B8 + (int64_t)g * NCK * CKB;
 
@@ -309,6 +334,21 @@ __global__ void __launch_bounds__(I8_NT) uvlr_sums_i8_kernel(const eqbf_chunk c,
         for (int k = 0; k < 4; ++k) acc[b][rb][k] = v4i{0, 0, 0, 0};
 
     issue_b(0);
+    /* what the epilogue reads, staged in LDS with the first step: the
+     * subgroup records of the gene and whether each row's SNP is kept in
+     * each subgroup */
+    double r_yy = 0.0;
+    int r_n = 0, r_fl = 0;
+    if (tid < S) {
+      const PrepRec* rp = q.rec + (int64_t)g * S + tid;
+      r_yy = rp->yy;
+      r_n = rp->n;
+      r_fl = rp->flags;
+    }
+    uint32_t sin4 = 0u; /* byte k: row jx's SNP kept in subgroup gq + 16 k */
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (gq + 16 * k < S) sin4 |= (uint32_t)c.snp_in_sbgrp[snp * S + gq + 16 * k] << (8 * k);
     load_x(0, xr);
     load_x(1, xn);
     load_x(2, xl);
@@ -333,7 +373,16 @@ __global__ void __launch_bounds__(I8_NT) uvlr_sums_i8_kernel(const eqbf_chunk c,
       rs[(1 * 16 + gq) * 32 + jx] = mn;
       rs[(2 * 16 + gq) * 32 + jx] = mx;
       rs[(3 * 16 + gq) * 32 + jx] = (double)nf;
-      __syncthreads();
+      if (tid < S) {
+        ryy_l[tid] = r_yy;
+        rn_l[tid] = r_n;
+        rf_l[tid] = r_fl;
+        yexp_l[tid] = ytilde_exp(r_yy);
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (gq + 16 * k < S) snpin_l[jx * 64 + gq + 16 * k] = (uint8_t)(sin4 >> (8 * k));
+      i8_barrier<I8_LGKM0>();
       if (tid < 32) {
         double s2 = 0.0, n2 = 0.0, lo = INFINITY, hi = -INFINITY;
         for (int k = 0; k < 16; ++k) {
@@ -351,7 +400,7 @@ __global__ void __launch_bounds__(I8_NT) uvlr_sums_i8_kernel(const eqbf_chunk c,
         ex_l[tid] = 29 - kx;
         bad_l[tid] = fabs(cs) < 1073741824.0 ? 0 : 1; /* centre beyond the fixed-point range */
       }
-      __syncthreads();
+      i8_barrier<I8_LGKM0>();
     }
     Cx = cx_l[jx];
     sxf = ldexpf(1.f, ex_l[jx]);
@@ -375,15 +424,12 @@ __global__ void __launch_bounds__(I8_NT) uvlr_sums_i8_kernel(const eqbf_chunk c,
       __builtin_amdgcn_sched_barrier(0);
       const unsigned char* Bs = smem + (ck & 1) * CKB;
       const unsigned char* A = Abase + (ck & 1) * I8_ABUF;
-      v4i ax[2][4], ay[2][4];
+      v4i ax[2][4];
 #pragma unroll
       for (int rb = 0; rb < 2; ++rb) {
         const unsigned char* ap = A + (rb * 16 + l16) * 64 + 16 * (kq ^ sw);
 #pragma unroll
-        for (int p = 0; p < 4; ++p) {
-          ax[rb][p] = *(const v4i*)(ap + p * I8_PLANE);
-          ay[rb][p] = *(const v4i*)(ap + (4 + p) * I8_PLANE);
-        }
+        for (int p = 0; p < 4; ++p) ax[rb][p] = *(const v4i*)(ap + p * I8_PLANE);
       }
 #pragma unroll
       for (int b = 0; b < NBX; ++b) {
@@ -409,10 +455,13 @@ __global__ void __launch_bounds__(I8_NT) uvlr_sums_i8_kernel(const eqbf_chunk c,
       {
         const v4i bmf = *(const v4i*)(Bs + (4 * CXP + bms * 16 + l16) * 64 + 16 * (kq ^ sw));
 #pragma unroll
-        for (int rb = 0; rb < 2; ++rb)
+        for (int rb = 0; rb < 2; ++rb) {
+          const unsigned char* ap = A + (rb * 16 + l16) * 64 + 16 * (kq ^ sw) + 4 * I8_PLANE;
 #pragma unroll
           for (int p = 0; p < 4; ++p)
-            acc[NBX][rb][p] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ay[rb][p], bmf, acc[NBX][rb][p], 0, 0, 0);
+            acc[NBX][rb][p] =
+                __builtin_amdgcn_mfma_i32_16x16x64_i8(*(const v4i*)(ap + p * I8_PLANE), bmf, acc[NBX][rb][p], 0, 0, 0);
+        }
       }
       convert(ck + 1, P); /* past the last step: digits nobody reads */
       /* the next step's B copy has landed (only the four genotype loads
@@ -439,14 +488,14 @@ __global__ void __launch_bounds__(I8_NT) uvlr_sums_i8_kernel(const eqbf_chunk c,
 #pragma unroll
     for (int b = 0; b < NBX + 1; ++b) {
       const bool xb = b < NBX;
-      const int bk = xb ? q.bx[w * I8_NBMAX + b] : q.bm[w];
+      const int bk = xb ? q.bx[w * I8_NBMAX + (b < NBX ? b : 0)] : q.bm[w];
       if (bk < 0) continue;
       int ecol, eb;
       if (xb) {
         ecol = bk * 16 + l16;
         if (ecol >= q.CX) continue;
         const int s = ecol / q.KC, k = ecol - s * q.KC;
-        eb = 24 - (k == 0 ? ytilde_exp(q.rec[(int64_t)g * S + s]) : 30);
+        eb = 24 - (k == 0 ? yexp_l[s] : 30);
       } else {
         const int s = bk * 16 + l16;
         if (s >= S) continue;
@@ -463,7 +512,7 @@ __global__ void __launch_bounds__(I8_NT) uvlr_sums_i8_kernel(const eqbf_chunk c,
           E[row * ES + ecol] = ldexp(v, xb ? eb - ex_l[row] : eb - 2 * ex_l[row]);
         }
     }
-    __syncthreads();
+    i8_barrier<I8_LGKM0>();
     for (int it = tid; it < I8_ROWS * S; it += I8_NT) {
       const int j = it & 31, s = it >> 5;
       if (j >= cnt) continue;
@@ -475,12 +524,12 @@ __global__ void __launch_bounds__(I8_NT) uvlr_sums_i8_kernel(const eqbf_chunk c,
       const int64_t o = (int64_t)s * NP + p0 + j;
       q.sx2[o] = xx;
       q.sx2[(int64_t)S * NP + o] = sxy;
-      const PrepRec r = q.rec[(int64_t)g * S + s];
-      const bool ok = !bad_l[j] && xx > I8_TAU * sxx && r.yy - sxy * (sxy / xx) > I8_RSS * r.yy;
-      if (!ok && r.n > 0 && !(r.flags & PREP_TOO_FEW) && c.snp_in_sbgrp[(int64_t)c.pair_snp[p0 + j] * S + s])
+      const double yy = ryy_l[s];
+      const bool ok = !bad_l[j] && xx > I8_TAU * sxx && yy - sxy * (sxy / xx) > I8_RSS * yy;
+      if (!ok && rn_l[s] > 0 && !(rf_l[s] & PREP_TOO_FEW) && snpin_l[j * 64 + s])
         atomicOr(&q.redo[o >> 5], 1u << (o & 31));
     }
-    __syncthreads();
+    i8_barrier<I8_LGKM0>(); /* E is read; the stores need not land */
   }
 }
 
@@ -509,7 +558,7 @@ I8Shape i8_shape(const eqbf_chunk& c, int KU) {
   sh.ES = (CX + S) | 1;
   sh.NB = (sh.NXB + 7) / 8; /* y~/U slots per wave; the x'^2 blocks take one slot each (NMB <= 4) */
   const size_t body = std::max<size_t>((size_t)2 * sh.CKB + 2 * I8_ABUF, (size_t)I8_ROWS * sh.ES * 8);
-  sh.lds = body + I8_ROWS * 3 * 4;
+  sh.lds = body + 64 * (8 + 3 * 4) + I8_ROWS * 3 * 4 + I8_ROWS * 64;
   return sh;
 }
 
