@@ -1,0 +1,716 @@
+/*
+ * EM of eqtlbma_hm's hierarchical model on raw log10 ABFs
+ * (Controller::run_EM_classic / run_EM_square, eqtlbma_hm.cpp:925-1346,
+ * over the per-SNP and per-gene log10 sums of hm_methods.cpp:74-781).
+ *
+ * One EM pass over the data evaluates the model at the current parameters
+ * and yields everything an EM update needs, in three kernels:
+ *  hm_pair_kernel  one wave per (gene, SNP) pair: the grid-averaged BF of
+ *                  each configuration (or type), the SNP's BF, the
+ *                  configuration-averaged BF of each grid point and, for the
+ *                  types model, each (type, subgroup)'s numerator of the
+ *                  subgroup update (hm_methods.cpp:74-300);
+ *  hm_gene_kernel  one workgroup per gene: the gene's BF over its SNPs
+ *                  (weight 1/P), its log10 observed likelihood, and each
+ *                  column's SNP average minus that likelihood -- the gene's
+ *                  contribution to each weight's update (hm_methods.cpp:
+ *                  392-704, gene_eQTL::em_update_*);
+ *  hm_col_kernel   one workgroup per column: the log10 sum over genes of
+ *                  those contributions, the likelihood and pi0's sum
+ *                  (eqtlbma_hm.cpp:617-868).
+ * The host then normalises the new weights exactly as the reference does
+ * (log10 sums with the previous weight added, divided by their log10 sum)
+ * and runs the reference's iteration logic; one pass per EM iteration, all
+ * in FP64.  Every log10 sum keeps log10_weighted_sum's rules
+ * (utils_math.cpp:133-159): the maximum starts from the first element (a
+ * NaN first element makes the sum NaN), NaN terms are skipped, and a result
+ * within DBL_EPSILON of zero is zero.
+ */
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "eqbf_internal.h"
+
+namespace {
+
+constexpr int HM_TYPES_KMAX = 16; /* types model: types per wave in registers */
+constexpr int HM_NT = 256;
+
+__device__ inline double hm_finish(double m, double s) {
+  const double r = m + log10(s);
+  return fabs(r) <= DBL_EPSILON ? 0.0 : r;
+}
+
+/* log10 sum w 10^v over the wave, v given per lane (valid lanes only):
+ * the first element's NaN decides, as the reference's sequential max */
+__device__ inline double hm_wave_lws(double v, double w, bool valid, double first) {
+  double m = (valid && !isnan(v)) ? v : -INFINITY;
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  if (isnan(first)) m = NAN;
+  double s = (valid && !isnan(v)) ? w * exp10(v - m) : 0.0;
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  return hm_finish(m, s);
+}
+
+struct HmDev {
+  const double* raw; /* [NP][D][L] */
+  const int64_t* off;
+  int64_t NP, G;
+  int D, L, K, S;  /* D = K (configs) or S (types) */
+  const double* par; /* pi0, grid[L], dimw[K], sub[K][S] */
+  double* cfg;       /* [NP][K] grid-averaged BF per configuration / type  */
+  double* grd;       /* [NP][L] configuration-averaged BF per grid point    */
+  double* num;       /* [NP][K][S] types: subgroup-update numerators        */
+  double* snp;       /* [NP] SNP BF                                         */
+  double* rows;      /* [G][NC + 3]: per-gene columns, NC = K + L (+ K S), then
+                        log10 obslik, pi0's term and the gene's BF          */
+  double* red;       /* [NC + 2] over genes                                 */
+  int stale;         /* 1: keep each gene's stored log10 obslik (row[NC])   */
+};
+
+/* log10(q 10^b + 1 - q), a subgroup's term of the types model
+ * (hm_methods.cpp:143-146) */
+__device__ inline double hm_sub_term(double q, double b) { return log10(exp10(log10(q) + b) + 1 - q); }
+
+template <bool TYPES>
+__global__ void __launch_bounds__(HM_NT) hm_pair_kernel(const HmDev h) {
+  const int lane = threadIdx.x & 63;
+  const int64_t p = (int64_t)blockIdx.x * (HM_NT / 64) + (threadIdx.x >> 6);
+  if (p >= h.NP) return;
+  const int L = h.L, K = h.K, S = h.S, D = h.D;
+  const double* R = h.raw + (size_t)p * D * L;
+  const double* grid = h.par + 1;
+  const double* dw = h.par + 1 + L;
+  if (!TYPES) {
+    /* per configuration: log10 sum_l grid_l 10^R[k][l] (hm_methods.cpp:81-84) */
+    for (int k = lane; k < K; k += 64) {
+      const double* r = R + (size_t)k * L;
+      double m = r[0];
+      for (int l = 1; l < L; ++l)
+        if (r[l] > m) m = r[l];
+      double s = 0.0;
+      for (int l = 0; l < L; ++l)
+        if (!isnan(r[l])) s += grid[l] * exp10(r[l] - m);
+      h.cfg[(size_t)p * K + k] = hm_finish(m, s);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    /* the SNP's BF: log10 sum_k w_k 10^cfg[k] (hm_methods.cpp:102-104) */
+    const double* c = h.cfg + (size_t)p * K;
+    const double first = c[0];
+    double m = -INFINITY;
+    for (int k = lane; k < K; k += 64)
+      if (!isnan(c[k])) m = fmax(m, c[k]);
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    if (isnan(first)) m = NAN;
+    double s = 0.0;
+    for (int k = lane; k < K; k += 64)
+      if (!isnan(c[k])) s += dw[k] * exp10(c[k] - m);
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) h.snp[p] = hm_finish(m, s);
+    /* per grid point: log10 sum_k w_k 10^R[k][l] (hm_methods.cpp:261-275) */
+    for (int l = lane; l < L; l += 64) {
+      double mm = R[l];
+      for (int k = 1; k < K; ++k)
+        if (R[(size_t)k * L + l] > mm) mm = R[(size_t)k * L + l];
+      double ss = 0.0;
+      for (int k = 0; k < K; ++k) {
+        const double v = R[(size_t)k * L + l];
+        if (!isnan(v)) ss += dw[k] * exp10(v - mm);
+      }
+      h.grd[(size_t)p * L + l] = hm_finish(mm, ss);
+    }
+    return;
+  }
+  /* types model, one lane per grid point (L <= 64): t[k] = sum_s
+   * log10(q_ks 10^R[s][l] + 1 - q_ks) */
+  const double* sub = h.par + 1 + L + K;
+  const bool on = lane < L;
+  const int l = on ? lane : 0;
+  double t[HM_TYPES_KMAX];
+#pragma unroll
+  for (int k = 0; k < HM_TYPES_KMAX; ++k) {
+    double acc = 0.0;
+    if (k < K)
+      for (int s = 0; s < S; ++s) acc += hm_sub_term(sub[(size_t)k * S + s], R[(size_t)s * L + l]);
+    t[k] = acc;
+  }
+  /* per type: log10 sum_l grid_l 10^t[k][l] (hm_methods.cpp:131-160) */
+  for (int k = 0; k < HM_TYPES_KMAX && k < K; ++k) {
+    const double first = __shfl(t[k], 0, 64);
+    const double v = hm_wave_lws(t[k], grid[l], on, first);
+    if (lane == 0) h.cfg[(size_t)p * K + k] = v;
+    /* subgroup-update numerators: R[s][l] + sum over the other subgroups
+     * (hm_methods.cpp:236-247) */
+    for (int s = 0; s < S; ++s) {
+      double x = 0.0;
+      for (int s2 = 0; s2 < S; ++s2)
+        if (s2 != s) x += hm_sub_term(sub[(size_t)k * S + s2], R[(size_t)s2 * L + l]);
+      x = R[(size_t)s * L + l] + x;
+      const double f = __shfl(x, 0, 64);
+      const double nv = hm_wave_lws(x, grid[l], on, f);
+      if (lane == 0) h.num[((size_t)p * K + k) * S + s] = nv;
+    }
+  }
+  /* per grid point: log10 sum_k w_k 10^t[k] (hm_methods.cpp:277-300) */
+  if (on) {
+    double mm = t[0];
+    for (int k = 1; k < HM_TYPES_KMAX && k < K; ++k)
+      if (t[k] > mm) mm = t[k];
+    double ss = 0.0;
+    for (int k = 0; k < HM_TYPES_KMAX && k < K; ++k)
+      if (!isnan(t[k])) ss += dw[k] * exp10(t[k] - mm);
+    h.grd[(size_t)p * L + l] = hm_finish(mm, ss);
+  }
+  /* the SNP's BF over the types */
+  if (lane == 0) {
+    const double* c = h.cfg + (size_t)p * K;
+    double m = c[0];
+    for (int k = 1; k < K; ++k)
+      if (c[k] > m) m = c[k];
+    double s = 0.0;
+    for (int k = 0; k < K; ++k)
+      if (!isnan(c[k])) s += dw[k] * exp10(c[k] - m);
+    h.snp[p] = hm_finish(m, s);
+  }
+}
+
+/* block-wide log10 sum of w 10^v, v strided over the block's threads; the
+ * first element decides NaN */
+__device__ double hm_block_lws(const double* v, int64_t n, int64_t stride, double w, double* red) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  double m = -INFINITY;
+  for (int64_t i = tid; i < n; i += HM_NT) {
+    const double x = v[i * stride];
+    if (!isnan(x)) m = fmax(m, x);
+  }
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  __syncthreads();
+  if (lane == 0) red[wv] = m;
+  __syncthreads();
+  m = red[0];
+  for (int k = 1; k < HM_NT / 64; ++k) m = fmax(m, red[k]);
+  if (n > 0 && isnan(v[0])) m = NAN;
+  double s = 0.0;
+  for (int64_t i = tid; i < n; i += HM_NT) {
+    const double x = v[i * stride];
+    if (!isnan(x)) s += w * exp10(x - m);
+  }
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  __syncthreads();
+  if (lane == 0) red[wv] = s;
+  __syncthreads();
+  s = 0.0;
+  for (int k = 0; k < HM_NT / 64; ++k) s += red[k];
+  return hm_finish(m, s);
+}
+
+/* per gene: BF, observed likelihood and the contributions to the updates
+ * (one thread per column, SNPs in order, as gene_eQTL::em_update_* sum) */
+template <bool TYPES>
+__global__ void __launch_bounds__(HM_NT) hm_gene_kernel(const HmDev h) {
+  __shared__ double red[HM_NT / 64];
+  __shared__ double s_ol;
+  const int64_t g = blockIdx.x;
+  const int64_t p0 = h.off[g], np = h.off[g + 1] - p0;
+  const int K = h.K, L = h.L, S = h.S;
+  const int NC = K + L + (TYPES ? K * S : 0);
+  double* row = h.rows + (size_t)g * (NC + 3);
+  const double pi0 = h.par[0];
+  const double gbf = hm_block_lws(h.snp + p0, np, 1, 1.0 / (double)np, red);
+  if (threadIdx.x == 0) {
+    /* log10(pi0 10^0 + (1 - pi0) 10^gbf) (hm_methods.cpp:504-520), or the
+     * likelihood stored by the last evaluation */
+    double ol;
+    if (h.stale) {
+      ol = row[NC];
+    } else {
+      const double m = gbf > 0.0 ? gbf : 0.0;
+      double s = pi0 * exp10(0.0 - m);
+      if (!isnan(gbf)) s += (1 - pi0) * exp10(gbf - m);
+      ol = hm_finish(m, s);
+      row[NC] = ol;
+    }
+    s_ol = ol;
+    row[NC + 1] = exp10(log10(pi0) - ol); /* em_update_pi0 (hm_methods.cpp:542-545) */
+    row[NC + 2] = gbf;
+  }
+  __syncthreads();
+  const double ol = s_ol;
+  const double w = 1.0 / (double)np;
+  for (int c = threadIdx.x; c < NC; c += HM_NT) {
+    const double* v;
+    int64_t stride;
+    if (c < K) {
+      v = h.cfg + (size_t)p0 * K + c;
+      stride = K;
+    } else if (c < K + L) {
+      v = h.grd + (size_t)p0 * L + (c - K);
+      stride = L;
+    } else {
+      v = h.num + (size_t)p0 * K * S + (c - K - L);
+      stride = (int64_t)K * S;
+    }
+    double m = v[0];
+    for (int64_t i = 1; i < np; ++i)
+      if (v[i * stride] > m) m = v[i * stride];
+    double s = 0.0;
+    for (int64_t i = 0; i < np; ++i) {
+      const double x = v[i * stride];
+      if (!isnan(x)) s += w * exp10(x - m);
+    }
+    row[c] = hm_finish(m, s) - ol;
+  }
+}
+
+/* over genes: log10 sum of each contribution column (weights 1), the sum
+ * of the log10 likelihoods and pi0's sum */
+__global__ void __launch_bounds__(HM_NT) hm_col_kernel(const HmDev h, int NC) {
+  __shared__ double red[HM_NT / 64];
+  const int j = blockIdx.x;
+  const double* col = h.rows + j;
+  const int64_t stride = NC + 3;
+  if (j < NC) {
+    const double v = hm_block_lws(col, h.G, stride, 1.0, red);
+    if (threadIdx.x == 0) h.red[j] = v;
+    return;
+  }
+  double s = 0.0;
+  for (int64_t g = threadIdx.x; g < h.G; g += HM_NT) s += col[g * stride];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int k = 0; k < HM_NT / 64; ++k) t += red[k];
+    h.red[j] = t;
+  }
+}
+
+/* ---- host side ------------------------------------------------------------ */
+
+struct HmParams {
+  double pi0 = NAN;
+  std::vector<double> grid, dimw, sub; /* dimw: configs or types; sub [K][S] */
+};
+
+struct HmPass {
+  double lik = NAN;    /* log10 likelihood at the parameters (a fresh pass)  */
+  bool lik_ok = false; /* finite (compute_log10_obs_lik throws otherwise)    */
+  bool upd_ok = false; /* no NaN unnormalised configuration or grid weight   */
+  HmParams next;       /* one EM update from the parameters                  */
+};
+
+/* log10_weighted_sum, sequential (utils_math.cpp:133-159) */
+double h_lws(const double* v, const double* w, size_t n) {
+  double m = v[0];
+  for (size_t i = 0; i < n; ++i)
+    if (v[i] > m) m = v[i];
+  double s = 0.0;
+  for (size_t i = 0; i < n; ++i)
+    if (!std::isnan(v[i])) s += w[i] * std::pow(10.0, v[i] - m);
+  const double r = m + std::log10(s);
+  return std::fabs(r) <= DBL_EPSILON ? 0.0 : r;
+}
+
+struct HmRun {
+  eqbf_ctx* ctx;
+  hipStream_t st;
+  HmDev h;
+  const eqbf_hm_opts* o;
+  bool types;
+  int NC;
+  std::vector<double> hpar, hred;
+
+  int upload(const HmParams& P) {
+    const int K = h.K, L = h.L, S = h.S;
+    hpar.assign(1 + L + K + (types ? K * S : 0), 0.0);
+    hpar[0] = P.pi0;
+    std::memcpy(&hpar[1], P.grid.data(), L * sizeof(double));
+    std::memcpy(&hpar[1 + L], P.dimw.data(), K * sizeof(double));
+    if (types) std::memcpy(&hpar[1 + L + K], P.sub.data(), (size_t)K * S * sizeof(double));
+    EQBF_HIP(ctx, hipMemcpyAsync((void*)h.par, hpar.data(), hpar.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    return EQBF_OK;
+  }
+
+  int launch() {
+    const unsigned pg = (unsigned)((h.NP + HM_NT / 64 - 1) / (HM_NT / 64));
+    {
+      ProfScope ps(ctx, "hm_pair_kernel", st);
+      if (types)
+        hipLaunchKernelGGL(hm_pair_kernel<true>, dim3(pg), dim3(HM_NT), 0, st, h);
+      else
+        hipLaunchKernelGGL(hm_pair_kernel<false>, dim3(pg), dim3(HM_NT), 0, st, h);
+    }
+    {
+      ProfScope ps(ctx, "hm_gene_kernel", st);
+      if (types)
+        hipLaunchKernelGGL(hm_gene_kernel<true>, dim3((unsigned)h.G), dim3(HM_NT), 0, st, h);
+      else
+        hipLaunchKernelGGL(hm_gene_kernel<false>, dim3((unsigned)h.G), dim3(HM_NT), 0, st, h);
+    }
+    EQBF_HIP(ctx, hipGetLastError());
+    return EQBF_OK;
+  }
+
+  /* one pass: the likelihood at P and the EM update from P
+   * (Controller::em_update_*, eqtlbma_hm.cpp:659-868); stale: the update
+   * reads the genes' likelihoods stored by the previous pass */
+  int pass(const HmParams& P, HmPass& out, bool stale = false) {
+    h.stale = stale ? 1 : 0;
+    int rc = upload(P);
+    if (rc) return rc;
+    if ((rc = launch())) return rc;
+    {
+      ProfScope ps(ctx, "hm_col_kernel", st);
+      hipLaunchKernelGGL(hm_col_kernel, dim3((unsigned)(NC + 2)), dim3(HM_NT), 0, st, h, NC);
+    }
+    EQBF_HIP(ctx, hipGetLastError());
+    hred.resize(NC + 2);
+    EQBF_HIP(ctx, hipMemcpyAsync(hred.data(), h.red, (NC + 2) * sizeof(double), hipMemcpyDeviceToHost, st));
+    EQBF_HIP(ctx, hipStreamSynchronize(st));
+    const int K = h.K, L = h.L, S = h.S;
+    out.lik = hred[NC];
+    out.lik_ok = std::isfinite(out.lik);
+    HmParams& Q = out.next;
+    Q = P;
+    bool ok = true;
+    if (!o->fixed[0]) Q.pi0 = hred[NC + 1] / (double)h.G;
+    std::vector<double> ones(std::max(K, L), 1.0), t;
+    auto normalise = [&](const double* contrib, const std::vector<double>& old, std::vector<double>& nw,
+                         bool check) {
+      const size_t D = old.size();
+      t.assign(D, 0.0);
+      for (size_t k = 0; k < D; ++k) {
+        t[k] = contrib[k] + std::log10(old[k]);
+        /* the reference throws here for configurations and grid points
+         * (eqtlbma_hm.cpp:697-700, 844-847), not for types */
+        if (check && std::isnan(t[k])) ok = false;
+      }
+      const double den = h_lws(t.data(), ones.data(), D);
+      nw.resize(D);
+      for (size_t k = 0; k < D; ++k) nw[k] = std::pow(10.0, t[k] - den);
+    };
+    if (!types) {
+      if (!o->fixed[2] && K > 1) normalise(&hred[0], P.dimw, Q.dimw, true);
+    } else {
+      if (!o->fixed[3]) normalise(&hred[0], P.dimw, Q.dimw, false);
+      if (!o->fixed[4])
+        for (int k = 0; k < K; ++k)
+          for (int s = 0; s < S; ++s)
+            Q.sub[(size_t)k * S + s] =
+                std::pow(10.0, hred[K + L + k * S + s] + std::log10(P.sub[(size_t)k * S + s]) - hred[k]);
+    }
+    if (!o->fixed[1]) normalise(&hred[K], P.grid, Q.grid, true);
+    out.upd_ok = ok;
+    return EQBF_OK;
+  }
+};
+
+double sq(double x) { return x * x; }
+
+int hm_throw(HmRun& R) {
+  return ctx_fail(R.ctx, EQBF_ERR_DATA, "log10(obslik) or an unnormalised weight is NaN or infinite");
+}
+
+/* run_EM_fixedpoint from cur's parameters: the update cur.next and, in
+ * nxt, its likelihood and the update after it */
+int hm_fixedpoint(HmRun& R, const HmPass& cur, HmPass& nxt) {
+  if (!cur.upd_ok) return hm_throw(R);
+  const int rc = R.pass(cur.next, nxt);
+  if (rc) return rc;
+  return nxt.lik_ok ? EQBF_OK : hm_throw(R);
+}
+
+bool hm_stop(const HmRun& R, double a, double b, int64_t iter) {
+  return std::fabs(a - b) < R.o->thresh || (R.o->max_iters > 0 && iter == R.o->max_iters - 1);
+}
+
+/* run_EM_classic (eqtlbma_hm.cpp:1076-1107) with passes: pass(P_t) gives
+ * lik(P_t) and P_{t+1} */
+int hm_classic(HmRun& R, HmParams& P, double* lik_out, int64_t* iters_out) {
+  HmPass cur;
+  int rc = R.pass(P, cur);
+  if (rc) return rc;
+  if (!cur.lik_ok) return hm_throw(R);
+  int64_t iter = 0;
+  while (true) {
+    ++iter;
+    HmPass nxt;
+    if ((rc = hm_fixedpoint(R, cur, nxt))) return rc;
+    if (nxt.lik < cur.lik)
+      return ctx_fail(R.ctx, EQBF_ERR_DATA, "observed log-likelihood is decreasing (%f < %f)", nxt.lik, cur.lik);
+    const bool stop = hm_stop(R, nxt.lik, cur.lik, iter);
+    P = cur.next;
+    cur = nxt;
+    if (stop) break;
+  }
+  /* the last update_params, one more iteration and update_params */
+  ++iter;
+  HmPass fin;
+  if ((rc = hm_fixedpoint(R, cur, fin))) return rc;
+  P = cur.next;
+  *lik_out = fin.lik;
+  *iters_out = iter;
+  return EQBF_OK;
+}
+
+/* run_EM_square (eqtlbma_hm.cpp:1215-1324): step length rule 3
+ * (alpha = sqrt(|r|^2 / |v|^2), Varadhan and Roland 2008), stepmin0 =
+ * stepmax0 = 1, mstep = 4, maxdist = 1.  After an extrapolation rejected for
+ * falling too far below the start, the parameters return to the second EM
+ * iterate while the genes keep the likelihoods of the last evaluation, which
+ * the next update reads (a stale pass). */
+int hm_square(HmRun& R, HmParams& P, double* lik_out, int64_t* iters_out) {
+  const double stepmin0 = 1, stepmax0 = 1, mstep = 4, maxdist = 1, stepmax_user = R.o->stepmax;
+  const bool dimv = R.types || R.h.K > 1;
+  int64_t iter = 0;
+  double steplen = 1.0, stepmin = stepmin0, stepmax = stepmax0;
+  HmPass cur;
+  int rc = R.pass(P, cur);
+  if (rc) return rc;
+  if (!cur.lik_ok) return hm_throw(R);
+  double lik = cur.lik;
+  while (true) {
+    const double lik0 = lik;
+    const HmParams P0 = P;
+    ++iter;
+    HmPass c1;
+    if ((rc = hm_fixedpoint(R, cur, c1))) return rc;
+    P = cur.next;
+    lik = c1.lik;
+    cur = c1;
+    const HmParams P1 = P;
+    const double lik1 = lik;
+    if (hm_stop(R, lik1, lik0, iter)) break;
+    ++iter;
+    HmPass c2;
+    if ((rc = hm_fixedpoint(R, cur, c2))) return rc;
+    P = cur.next;
+    lik = c2.lik;
+    cur = c2;
+    const HmParams P2 = P;
+    const double lik2 = lik;
+    if (hm_stop(R, lik2, lik1, iter)) break;
+    /* compute_steplength */
+    double sr2 = sq(P1.pi0 - P0.pi0), sv2 = sq(P2.pi0 - 2 * P1.pi0 + P0.pi0);
+    auto acc = [&](const std::vector<double>& a0, const std::vector<double>& a1, const std::vector<double>& a2) {
+      for (size_t k = 0; k < a0.size(); ++k) {
+        sr2 += sq(a1[k] - a0[k]);
+        sv2 += sq(a2[k] - 2 * a1[k] + a0[k]);
+      }
+    };
+    if (dimv) acc(P0.dimw, P1.dimw, P2.dimw);
+    if (R.types) acc(P0.sub, P1.sub, P2.sub);
+    acc(P0.grid, P1.grid, P2.grid);
+    double alpha = std::sqrt(sr2 / sv2);
+    alpha = std::max(stepmin, std::min(stepmax, alpha));
+    alpha = std::min(stepmax_user, alpha);
+    steplen = alpha;
+    /* proposals_squarem */
+    HmParams Pn = P2;
+    auto prop = [&](double a0, double a1, double a2) {
+      return a0 + 2.0 * alpha * (a1 - a0) + std::pow(alpha, 2) * (a2 - 2.0 * a1 + a0);
+    };
+    auto propv = [&](const std::vector<double>& a0, const std::vector<double>& a1, const std::vector<double>& a2,
+                     std::vector<double>& out) {
+      for (size_t k = 0; k < a0.size(); ++k) out[k] = prop(a0[k], a1[k], a2[k]);
+    };
+    Pn.pi0 = prop(P0.pi0, P1.pi0, P2.pi0);
+    if (dimv) propv(P0.dimw, P1.dimw, P2.dimw, Pn.dimw);
+    if (R.types) propv(P0.sub, P1.sub, P2.sub, Pn.sub);
+    propv(P0.grid, P1.grid, P2.grid, Pn.grid);
+    HmPass cn;
+    if ((rc = R.pass(Pn, cn))) return rc;
+    if (!cn.lik_ok) return hm_throw(R);
+    P = Pn;
+    lik = cn.lik;
+    cur = cn;
+    bool extrap = true;
+    if (std::fabs(steplen - 1) > 0.01) {
+      /* consistency check: one EM iteration from the proposal, which on
+       * failure goes back to the second EM iterate */
+      ++iter;
+      HmPass cc;
+      bool failed = !cn.upd_ok;
+      if (!failed) {
+        if ((rc = R.pass(cn.next, cc))) return rc;
+        failed = !cc.lik_ok;
+      }
+      if (failed) {
+        P = P2;
+        HmPass cb;
+        if ((rc = R.pass(P, cb))) return rc;
+        if (!cb.lik_ok) return hm_throw(R);
+        lik = cb.lik;
+        cur = cb;
+        if (steplen == stepmax) stepmax = std::max(stepmax0, stepmax / mstep);
+        steplen = 1;
+        if (steplen == stepmax) stepmax = mstep * stepmax;
+        if (stepmin < 0 && steplen == stepmin) stepmin = mstep * stepmin;
+        continue;
+      }
+      P = cn.next;
+      lik = cc.lik;
+      cur = cc;
+    }
+    if (extrap && lik < lik0 - maxdist) {
+      lik = lik2;
+      P = P2;
+      if ((rc = R.pass(P, cur, true))) return rc;
+      cur.lik = lik2;
+      if (steplen == stepmax) stepmax = std::max(stepmax0, stepmax / mstep);
+      steplen = 1;
+    }
+    if (steplen == stepmax) stepmax = mstep * stepmax;
+    if (stepmin < 0 && steplen == stepmin) stepmin = mstep * stepmin;
+  }
+  ++iter;
+  HmPass fin;
+  if ((rc = hm_fixedpoint(R, cur, fin))) return rc;
+  P = cur.next;
+  *lik_out = fin.lik;
+  *iters_out = iter;
+  return EQBF_OK;
+}
+
+int hm_setup(eqbf_ctx* ctx, HmRun& R, const double* raw_dev, int64_t n_pairs, const int64_t* off_dev, int64_t G,
+             const eqbf_hm_opts* o, hipStream_t st) {
+  if (!o || G < 1 || n_pairs < 1 || o->n_grid < 1 || o->dim < 1)
+    return ctx_fail(ctx, EQBF_ERR_ARG, "eqbf_hm: empty data or invalid options");
+  const bool types = o->model == EQBF_HM_TYPES;
+  if (!types && o->model != EQBF_HM_CONFIGS) return ctx_fail(ctx, EQBF_ERR_ARG, "eqbf_hm: unknown model %d", o->model);
+  if (types && (o->dim > HM_TYPES_KMAX || o->n_grid > 64 || o->n_subgroups < 1))
+    return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED, "types model: at most %d types and 64 grid points", HM_TYPES_KMAX);
+  R.ctx = ctx;
+  R.st = st;
+  R.o = o;
+  R.types = types;
+  HmDev& h = R.h;
+  memset(&h, 0, sizeof h);
+  h.raw = raw_dev;
+  h.off = off_dev;
+  h.NP = n_pairs;
+  h.G = G;
+  h.K = o->dim;
+  h.L = o->n_grid;
+  h.S = types ? o->n_subgroups : 0;
+  h.D = types ? h.S : h.K;
+  R.NC = h.K + h.L + (types ? h.K * h.S : 0);
+  h.par = (const double*)ctx_buf(ctx, "hm_par", (size_t)(1 + h.L + h.K + h.K * h.S) * sizeof(double));
+  h.cfg = (double*)ctx_buf(ctx, "hm_cfg", (size_t)n_pairs * h.K * sizeof(double));
+  h.grd = (double*)ctx_buf(ctx, "hm_grd", (size_t)n_pairs * h.L * sizeof(double));
+  h.num = types ? (double*)ctx_buf(ctx, "hm_num", (size_t)n_pairs * h.K * h.S * sizeof(double)) : nullptr;
+  h.snp = (double*)ctx_buf(ctx, "hm_snp", (size_t)n_pairs * sizeof(double));
+  h.rows = (double*)ctx_buf(ctx, "hm_rows", (size_t)G * (R.NC + 3) * sizeof(double));
+  h.red = (double*)ctx_buf(ctx, "hm_red", (size_t)(R.NC + 2) * sizeof(double));
+  if (!h.par || !h.cfg || !h.grd || (types && !h.num) || !h.snp || !h.rows || !h.red) return EQBF_ERR_HIP;
+  return EQBF_OK;
+}
+
+void hm_load(const eqbf_hm_opts* o, const eqbf_hm_params* p, HmParams& P) {
+  const bool types = o->model == EQBF_HM_TYPES;
+  P.pi0 = p->pi0;
+  P.grid.assign(p->grid, p->grid + o->n_grid);
+  const double* d = types ? p->type : p->config;
+  P.dimw.assign(d, d + o->dim);
+  if (types) P.sub.assign(p->subgroup, p->subgroup + (size_t)o->dim * o->n_subgroups);
+}
+
+void hm_store(const eqbf_hm_opts* o, const HmParams& P, eqbf_hm_params* p) {
+  const bool types = o->model == EQBF_HM_TYPES;
+  p->pi0 = P.pi0;
+  std::memcpy(p->grid, P.grid.data(), P.grid.size() * sizeof(double));
+  std::memcpy(types ? p->type : p->config, P.dimw.data(), P.dimw.size() * sizeof(double));
+  if (types) std::memcpy(p->subgroup, P.sub.data(), P.sub.size() * sizeof(double));
+}
+
+int hm_check_params(eqbf_ctx* ctx, const eqbf_hm_opts* o, const eqbf_hm_params* p) {
+  if (!p || !p->grid || (o->model == EQBF_HM_TYPES ? (!p->type || !p->subgroup) : !p->config))
+    return ctx_fail(ctx, EQBF_ERR_ARG, "eqbf_hm: missing parameter arrays");
+  return EQBF_OK;
+}
+
+int hm_em_run(eqbf_ctx* ctx, const double* raw_dev, int64_t n_pairs, const int64_t* off_dev, int64_t G,
+              const eqbf_hm_opts* o, eqbf_hm_params* p, double* lik, int64_t* iters, hipStream_t st) {
+  HmRun R;
+  int rc = hm_setup(ctx, R, raw_dev, n_pairs, off_dev, G, o, st);
+  if (rc) return rc;
+  if ((rc = hm_check_params(ctx, o, p))) return rc;
+  HmParams P;
+  hm_load(o, p, P);
+  double l = NAN;
+  int64_t it = 0;
+  rc = o->stepmax == 1.0 ? hm_classic(R, P, &l, &it) : hm_square(R, P, &l, &it);
+  if (rc) return rc;
+  hm_store(o, P, p);
+  if (lik) *lik = l;
+  if (iters) *iters = it;
+  return EQBF_OK;
+}
+
+}  // namespace
+
+extern "C" int eqbf_hm_em_device(eqbf_ctx* ctx, const double* raw_dev, int64_t n_pairs, const int64_t* gene_offset_dev,
+                                 int64_t n_genes, const eqbf_hm_opts* opts, eqbf_hm_params* params, double* log10_lik,
+                                 int64_t* n_iters) {
+  if (!ctx) return EQBF_ERR_ARG;
+  return hm_em_run(ctx, raw_dev, n_pairs, gene_offset_dev, n_genes, opts, params, log10_lik, n_iters, ctx->stream);
+}
+
+extern "C" int eqbf_hm_em(eqbf_ctx* ctx, const double* raw, int64_t n_pairs, const int64_t* gene_offset,
+                          int64_t n_genes, const eqbf_hm_opts* opts, eqbf_hm_params* params, double* log10_lik,
+                          int64_t* n_iters) {
+  if (!ctx || !raw || !gene_offset || !opts) return EQBF_ERR_ARG;
+  const int D = opts->model == EQBF_HM_TYPES ? opts->n_subgroups : opts->dim;
+  const size_t rb = (size_t)n_pairs * D * opts->n_grid * sizeof(double);
+  double* d_raw = (double*)ctx_buf(ctx, "hm_raw", rb);
+  int64_t* d_off = (int64_t*)ctx_buf(ctx, "hm_off", (size_t)(n_genes + 1) * sizeof(int64_t));
+  if (!d_raw || !d_off) return EQBF_ERR_HIP;
+  EQBF_HIP(ctx, hipMemcpyAsync(d_raw, raw, rb, hipMemcpyHostToDevice, ctx->stream));
+  EQBF_HIP(ctx, hipMemcpyAsync(d_off, gene_offset, (size_t)(n_genes + 1) * sizeof(int64_t), hipMemcpyHostToDevice,
+                               ctx->stream));
+  return hm_em_run(ctx, d_raw, n_pairs, d_off, n_genes, opts, params, log10_lik, n_iters, ctx->stream);
+}
+
+extern "C" int eqbf_hm_bfs(eqbf_ctx* ctx, const double* raw, int64_t n_pairs, const int64_t* gene_offset,
+                           int64_t n_genes, const eqbf_hm_opts* opts, const eqbf_hm_params* params, double* gene_post,
+                           double* gene_l10bf, double* snp_l10bf, double* dim_l10bf) {
+  if (!ctx || !raw || !gene_offset || !opts) return EQBF_ERR_ARG;
+  int rc = hm_check_params(ctx, opts, params);
+  if (rc) return rc;
+  const int D = opts->model == EQBF_HM_TYPES ? opts->n_subgroups : opts->dim;
+  const size_t rb = (size_t)n_pairs * D * opts->n_grid * sizeof(double);
+  double* d_raw = (double*)ctx_buf(ctx, "hm_raw", rb);
+  int64_t* d_off = (int64_t*)ctx_buf(ctx, "hm_off", (size_t)(n_genes + 1) * sizeof(int64_t));
+  if (!d_raw || !d_off) return EQBF_ERR_HIP;
+  hipStream_t st = ctx->stream;
+  EQBF_HIP(ctx, hipMemcpyAsync(d_raw, raw, rb, hipMemcpyHostToDevice, st));
+  EQBF_HIP(ctx, hipMemcpyAsync(d_off, gene_offset, (size_t)(n_genes + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  HmRun R;
+  if ((rc = hm_setup(ctx, R, d_raw, n_pairs, d_off, n_genes, opts, st))) return rc;
+  HmParams P;
+  hm_load(opts, params, P);
+  if ((rc = R.upload(P))) return rc;
+  if ((rc = R.launch())) return rc;
+  const int NC = R.NC;
+  std::vector<double> rows((size_t)n_genes * (NC + 3));
+  EQBF_HIP(ctx, hipMemcpyAsync(rows.data(), R.h.rows, rows.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (snp_l10bf) EQBF_HIP(ctx, hipMemcpyAsync(snp_l10bf, R.h.snp, n_pairs * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (dim_l10bf)
+    EQBF_HIP(ctx, hipMemcpyAsync(dim_l10bf, R.h.cfg, (size_t)n_pairs * opts->dim * sizeof(double), hipMemcpyDeviceToHost,
+                                 st));
+  EQBF_HIP(ctx, hipStreamSynchronize(st));
+  for (int64_t g = 0; g < n_genes; ++g) {
+    const double ol = rows[(size_t)g * (NC + 3) + NC], gb = rows[(size_t)g * (NC + 3) + NC + 2];
+    if (gene_l10bf) gene_l10bf[g] = gb;
+    if (gene_post) {
+      const double pp = std::pow(10.0, std::log10(1.0 - P.pi0) + gb - ol);
+      gene_post[g] = pp > 1.0 ? 1.0 : pp;
+    }
+  }
+  return EQBF_OK;
+}

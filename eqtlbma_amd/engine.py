@@ -79,6 +79,9 @@ def lib():
         L.eqbf_ld_eigenvalues.argtypes = [vp, vp, i32, vp]
         L.eqbf_ld_eigenvalues_device.argtypes = [vp, vp, i32, vp]
         L.eqbf_gev_fit.argtypes = [vp, vp, i64, i64, vp]
+        L.eqbf_hm_em.argtypes = [vp, vp, i64, vp, i64, vp, vp, vp, vp]
+        L.eqbf_hm_em_device.argtypes = [vp, vp, i64, vp, i64, vp, vp, vp, vp]
+        L.eqbf_hm_bfs.argtypes = [vp, vp, i64, vp, i64, vp, vp, vp, vp, vp, vp]
         L.eqbf_perm_pvalue.restype = i64
         _lib = L
     return _lib
@@ -92,6 +95,20 @@ class PermOpts(C.Structure):
 
 class MvlrOpts(C.Structure):
     _fields_ = [("bfs", C.c_int32), ("fiterr", C.c_double)]
+
+
+class HmOpts(C.Structure):
+    """eqbf_hm_opts: model (0 configs, 1 types), its dimension, subgroups,
+    grid size, stopping threshold, iteration limit, SQUAREM step bound (1:
+    classic EM) and the fixed flags (pi0, grid, configs, types, subgroups)."""
+    _fields_ = [("model", C.c_int32), ("dim", C.c_int32), ("n_subgroups", C.c_int32), ("n_grid", C.c_int32),
+                ("thresh", C.c_double), ("max_iters", C.c_int64), ("stepmax", C.c_double),
+                ("fixed", C.c_int32 * 5)]
+
+
+class HmParamsC(C.Structure):
+    _fields_ = [("pi0", C.c_double), ("grid", C.c_void_p), ("config", C.c_void_p), ("type", C.c_void_p),
+                ("subgroup", C.c_void_p)]
 
 
 class SstatsIn(C.Structure):
@@ -324,6 +341,63 @@ class Engine:
         out = np.zeros((st.shape[0], 3))
         self._check(lib().eqbf_gev_fit(self._ctx, _p(st), st.shape[0], st.shape[1], _p(out)))
         return out
+
+    def _hm_setup(self, raw, off, pi0, grid, cfg, type_, sub, thresh, maxit, stepmax, fixed):
+        raw = np.ascontiguousarray(raw, dtype=np.float64)
+        off = np.ascontiguousarray(off, dtype=np.int64)
+        types = type_ is not None
+        o = HmOpts()
+        o.model = 1 if types else 0
+        o.n_grid = raw.shape[2]
+        if types:
+            sub = np.ascontiguousarray(sub, dtype=np.float64).copy()
+            o.dim, o.n_subgroups = sub.shape
+            dimw = np.ascontiguousarray(type_, dtype=np.float64).copy()
+        else:
+            o.dim, o.n_subgroups = raw.shape[1], 0
+            dimw = np.ascontiguousarray(cfg, dtype=np.float64).copy()
+        o.thresh, o.max_iters, o.stepmax = float(thresh), int(maxit), float(stepmax)
+        for i in range(5):
+            o.fixed[i] = int(fixed[i])
+        grid = np.ascontiguousarray(grid, dtype=np.float64).copy()
+        p = HmParamsC()
+        p.pi0 = float(pi0)
+        p.grid = grid.ctypes.data
+        if types:
+            p.type, p.subgroup = dimw.ctypes.data, sub.ctypes.data
+        else:
+            p.config = dimw.ctypes.data
+        return raw, off, o, p, grid, dimw, sub
+
+    def hm_em(self, raw, off, pi0, grid, cfg=None, type_=None, sub=None, thresh=0.05, maxit=0, stepmax=1.0,
+              fixed=(0, 0, 0, 0, 0)):
+        """EM of eqtlbma_hm's model on raw log10 ABFs [pairs][D][L] grouped by
+        gene (off: [G+1]).  Returns (pi0, grid, configuration or type
+        weights, subgroup probabilities or None, log10 likelihood,
+        iterations)."""
+        raw, off, o, p, grid, dimw, sub = self._hm_setup(raw, off, pi0, grid, cfg, type_, sub, thresh, maxit,
+                                                         stepmax, fixed)
+        lik, it = C.c_double(), C.c_int64()
+        self._check(lib().eqbf_hm_em(self._ctx, _p(raw), raw.shape[0], _p(off), off.size - 1, C.byref(o),
+                                     C.byref(p), C.byref(lik), C.byref(it)))
+        return p.pi0, grid, dimw, (sub if type_ is not None else None), lik.value, it.value
+
+    def hm_em_device(self, raw_ptr, n_pairs, off_ptr, n_genes, opts, params):
+        lik, it = C.c_double(), C.c_int64()
+        self._check(lib().eqbf_hm_em_device(self._ctx, raw_ptr, int(n_pairs), off_ptr, int(n_genes), C.byref(opts),
+                                            C.byref(params), C.byref(lik), C.byref(it)))
+        return lik.value, it.value
+
+    def hm_bfs(self, raw, off, pi0, grid, cfg=None, type_=None, sub=None):
+        """(gene posterior, gene log10 BF, SNP log10 BF, [pairs][dim] log10
+        BFs averaged over the grid) at the given parameters."""
+        raw, off, o, p, grid, dimw, sub = self._hm_setup(raw, off, pi0, grid, cfg, type_, sub, 0.05, 0, 1.0,
+                                                         (0,) * 5)
+        G, NP = off.size - 1, raw.shape[0]
+        gp, gb, sb, db = np.zeros(G), np.zeros(G), np.zeros(NP), np.zeros((NP, o.dim))
+        self._check(lib().eqbf_hm_bfs(self._ctx, _p(raw), NP, _p(off), G, C.byref(o), C.byref(p), _p(gp), _p(gb),
+                                      _p(sb), _p(db)))
+        return gp, gb, sb, db
 
     def sync(self):
         self._check(lib().eqbf_sync(self._ctx))

@@ -414,6 +414,60 @@ int eqbf_ld_eigenvalues(eqbf_ctx* ctx, const eqbf_chunk* chunk, int32_t n_eig, f
 int eqbf_ld_eigenvalues_device(eqbf_ctx* ctx, const eqbf_chunk* chunk_dev, int32_t n_eig, float* eig);
 int eqbf_gev_fit(eqbf_ctx* ctx, const double* stats, int64_t n_genes, int64_t nperm, double* params);
 
+/* ---- eqtlbma_hm: EM of the hierarchical model on raw log10 ABFs --------
+ * Replaces Controller::run_EM (eqtlbma_hm.cpp:925-1346) and the per-gene /
+ * per-SNP log10 sums it calls (hm_methods.cpp:74-781).
+ * raw[pair][d][l]: the log10 ABFs of the file eqtlbma_bf writes
+ * (_l10abfs_raw), one row d per kept configuration (configs model, d <
+ * dim) or per subgroup (types model, d < n_subgroups), n_grid values each;
+ * the pairs of gene g are gene_offset[g] .. gene_offset[g+1]-1 (SNP order
+ * irrelevant, each SNP weighted 1/P as set_snp_prior does).
+ * The parameters are the starting point on input (Controller::init_params)
+ * and the estimates on output.  FP64 throughout.  log10_lik: the log10
+ * likelihood of the output parameters (the last EM iterate, as the reference
+ * keeps it); n_iters: the reference's iteration count (the iteration after
+ * the stopping rule included).  EQBF_ERR_DATA where the reference stops
+ * with an error: the likelihood decreases under classic EM, the log10
+ * likelihood is NaN or infinite, or an unnormalised configuration or grid
+ * weight is NaN (eqtlbma_hm.cpp:640-647, 697-700, 844-847, 1088-1093).
+ * stepmax (--msl): 1 = classic EM (run_EM_classic); any other value =
+ * SQUAREM with step-length rule 3 (run_EM_square, eqtlbma_hm.cpp:1109-1324;
+ * run_EM selects classic only for exactly 1). */
+enum { EQBF_HM_CONFIGS = 0, EQBF_HM_TYPES = 1 };
+typedef struct eqbf_hm_opts {
+  int32_t model;       /* EQBF_HM_CONFIGS (--model configs) or EQBF_HM_TYPES  */
+  int32_t dim;         /* --dim: configurations kept, or types                */
+  int32_t n_subgroups; /* --nsubgrp                                           */
+  int32_t n_grid;      /* --ngrid                                             */
+  double thresh;       /* --thresh (stopping rule on the log10 likelihood)    */
+  int64_t max_iters;   /* --maxit; <= 0: no limit                             */
+  double stepmax;      /* --msl                                               */
+  int32_t fixed[5];    /* held fixed: pi0, grid-points, configs, types,
+                          subgroups-per-type (the --init file's third column) */
+} eqbf_hm_opts;
+typedef struct eqbf_hm_params {
+  double pi0;
+  double* grid;     /* [n_grid]                                  */
+  double* config;   /* [dim]          configs model (else NULL)  */
+  double* type;     /* [dim]          types model (else NULL)    */
+  double* subgroup; /* [dim][n_subgroups] types model (else NULL) */
+} eqbf_hm_params;
+int eqbf_hm_em(eqbf_ctx* ctx, const double* raw, int64_t n_pairs, const int64_t* gene_offset, int64_t n_genes,
+               const eqbf_hm_opts* opts, eqbf_hm_params* params, double* log10_lik, int64_t* n_iters);
+/* raw_dev and gene_offset_dev in HBM (e.g. eqtlbma_bf's raw ABFs left on the
+ * device); the parameters stay host arrays */
+int eqbf_hm_em_device(eqbf_ctx* ctx, const double* raw_dev, int64_t n_pairs, const int64_t* gene_offset_dev,
+                      int64_t n_genes, const eqbf_hm_opts* opts, eqbf_hm_params* params, double* log10_lik,
+                      int64_t* n_iters);
+/* --getbf outputs (gene_eQTL::compute_posterior, hm_methods.cpp:752-781;
+ * Controller::save_result, eqtlbma_hm.cpp:1683-1740) for the given
+ * parameters: gene_post[g] = min(1, (1 - pi0) 10^(BF_g - obslik_g)),
+ * gene_l10bf[g], snp_l10bf[pair], dim_l10bf[pair][dim] (per configuration
+ * or type, averaged over the grid).  Host arrays. */
+int eqbf_hm_bfs(eqbf_ctx* ctx, const double* raw, int64_t n_pairs, const int64_t* gene_offset, int64_t n_genes,
+                const eqbf_hm_opts* opts, const eqbf_hm_params* params, double* gene_post, double* gene_l10bf,
+                double* snp_l10bf, double* dim_l10bf);
+
 #ifdef __cplusplus
 }
 #endif

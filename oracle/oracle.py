@@ -73,6 +73,11 @@ def lib():
         L.orc_mvlr_sigma.argtypes = [i32, i32, i32, vp, vp, vp, d, vp, vp, vp]
         L.orc_assoc_mvlr.argtypes = [vp, i32, d, vp, vp, i32, vp, vp, i32, vp, vp, vp]
         L.orc_assoc_mvlr.restype = C.c_int
+        L.orc_hm_em.argtypes = [vp, vp, i64, i32, i32, i32, i32, d, i64, d, vp, vp, vp, vp, vp, vp, vp, i64]
+        L.orc_hm_em.restype = i64
+        L.orc_hm_loglik.argtypes = [vp, vp, i64, i32, i32, i32, i32, vp, vp, vp, vp, vp]
+        L.orc_hm_loglik.restype = d
+        L.orc_hm_bfs.argtypes = [vp, vp, i64, i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.orc_abf_from_sstats.argtypes = [i32, i64, vp, vp, vp, vp, i32, vp, vp, i32, vp, vp, i32, vp, vp, vp]
         L.orc_abf_from_sstats.restype = C.c_int
         L.orc_bma.argtypes = [i32, i32, vp, d, vp, vp]
@@ -332,3 +337,76 @@ def perm_count(stats, true_stat_value, nb_permutations, trick=0, trick_cutoff=10
     drawn = lib().orc_perm_count(_p(stats), true_stat_value, C.addressof(nb), trick, trick_cutoff,
                                  addr, C.addressof(pv), C.addressof(used), C.addressof(med))
     return pv.value, used.value, med.value, nb.value, drawn
+
+
+class HmParams:
+    """Parameters of eqtlbma_hm's model: pi0, grid weights, and either the
+    configuration weights (configs model) or the type weights and the
+    per-type subgroup probabilities [K][S] (types model)."""
+
+    def __init__(self, pi0, grid, cfg=None, type_=None, sub=None):
+        self.pi0 = float(pi0)
+        self.grid = np.ascontiguousarray(grid, dtype=np.float64).copy()
+        self.cfg = None if cfg is None else np.ascontiguousarray(cfg, dtype=np.float64).copy()
+        self.type = None if type_ is None else np.ascontiguousarray(type_, dtype=np.float64).copy()
+        self.sub = None if sub is None else np.ascontiguousarray(sub, dtype=np.float64).copy()
+
+    def copy(self):
+        return HmParams(self.pi0, self.grid, self.cfg, self.type, self.sub)
+
+
+def _hm_args(raw, off, prm):
+    raw = np.ascontiguousarray(raw, dtype=np.float64)
+    off = np.ascontiguousarray(off, dtype=np.int64)
+    types = prm.type is not None
+    L = raw.shape[2]
+    if types:
+        K, S = prm.sub.shape
+        cfg = np.zeros(1)
+        typ, sub = prm.type, np.ascontiguousarray(prm.sub)
+    else:
+        K, S = raw.shape[1], 0
+        cfg, typ, sub = prm.cfg, np.zeros(1), np.zeros(1)
+    return raw, off, int(types), int(K), int(S), int(L), cfg, typ, sub
+
+
+def hm_em(raw, off, prm, thresh=0.05, maxit=0, fixed=(0, 0, 0, 0, 0), stepmax=1.0, max_trace=100000):
+    """EM of eqtlbma_hm from prm (not modified): classic for stepmax == 1,
+    SQUAREM otherwise.  raw: [pairs][D][L] (D = configurations, or subgroups
+    for the types model).  Returns (params, iterations, log10-likelihood
+    trace)."""
+    out = prm.copy()
+    raw, off, types, K, S, L, cfg, typ, sub = _hm_args(raw, off, out)
+    fx = np.ascontiguousarray(fixed, dtype=np.int32)
+    pi0 = np.array([out.pi0])
+    trace = np.full(max_trace, np.nan)
+    it = lib().orc_hm_em(_p(raw), _p(off), off.size - 1, types, K, S, L, thresh, maxit, float(stepmax), _p(fx),
+                         _p(pi0), _p(out.grid), _p(cfg), _p(typ), _p(sub), _p(trace), max_trace)
+    if it == -1:
+        raise RuntimeError("oracle: the log10 likelihood decreased")
+    if it < 0:
+        raise RuntimeError("oracle: NaN or infinite log10 likelihood or weight")
+    out.pi0 = float(pi0[0])
+    if types:
+        out.type, out.sub = typ, sub
+    else:
+        out.cfg = cfg
+    return out, int(it), trace[:min(it + 1, max_trace)]
+
+
+def hm_loglik(raw, off, prm):
+    raw, off, types, K, S, L, cfg, typ, sub = _hm_args(raw, off, prm)
+    pi0 = np.array([prm.pi0])
+    return lib().orc_hm_loglik(_p(raw), _p(off), off.size - 1, types, K, S, L, _p(pi0), _p(prm.grid), _p(cfg),
+                               _p(typ), _p(sub))
+
+
+def hm_bfs(raw, off, prm):
+    """(gene posterior, gene log10 BF, SNP log10 BF, [pairs][dim] log10 BFs)."""
+    raw, off, types, K, S, L, cfg, typ, sub = _hm_args(raw, off, prm)
+    G, NP = off.size - 1, raw.shape[0]
+    gp, gb, sb, db = np.zeros(G), np.zeros(G), np.zeros(NP), np.zeros((NP, K))
+    pi0 = np.array([prm.pi0])
+    lib().orc_hm_bfs(_p(raw), _p(off), G, types, K, S, L, _p(pi0), _p(prm.grid), _p(cfg), _p(typ), _p(sub),
+                     _p(gp), _p(gb), _p(sb), _p(db))
+    return gp, gb, sb, db
