@@ -1,8 +1,8 @@
-"""Build recipe for libeqbf_hip.so (HIP for gfx950) and the eqtlbma_bf host
-program.  Objects go to eqtlbma_amd/_build/; the shared library and the
+"""Build recipe for libeqbf_hip.so (HIP for gfx950) and the eqtlbma_bf and
+eqtlbma_hm host programs.  Objects go to eqtlbma_amd/_build/; the shared library and the
 program are written next to this file so they travel with the source tree.
 
-    python -m eqtlbma_amd.build            # library + program
+    python -m eqtlbma_amd.build            # library + programs
 """
 import concurrent.futures as cf
 import glob
@@ -17,6 +17,8 @@ BUILD = os.path.join(HERE, "_build")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 LIB = os.path.join(HERE, "libeqbf_hip.so")
 CLI = os.path.join(HERE, "eqtlbma_bf")
+HOST_HM = os.path.join(HERE, "host_hm")
+CLI_HM = os.path.join(HERE, "eqtlbma_hm")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("EQBF_ARCH", "gfx950")
@@ -70,9 +72,20 @@ def build_cli():
     return CLI
 
 
+def build_hm():
+    srcs = sorted(glob.glob(os.path.join(HOST_HM, "*.cpp"))) + [os.path.join(HOST, "util.cpp")]
+    tmp = CLI_HM + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I" + INCLUDE, "-o", tmp] + srcs +
+         ["-L" + HERE, "-leqbf_hip", "-Wl,-rpath,$ORIGIN", "-L/opt/rocm/lib", "-Wl,-rpath-link,/opt/rocm/lib",
+          "-lz"])
+    os.replace(tmp, CLI_HM)
+    return CLI_HM
+
+
 def build(jobs=8):
     build_lib(jobs)
     build_cli()
+    build_hm()
 
 
 if __name__ == "__main__":

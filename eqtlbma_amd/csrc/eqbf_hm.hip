@@ -68,6 +68,7 @@ struct HmDev {
                         log10 obslik, pi0's term and the gene's BF          */
   double* red;       /* [NC + 2] over genes                                 */
   int stale;         /* 1: keep each gene's stored log10 obslik (row[NC])   */
+  int lik_only;      /* 1: the likelihood only, no update columns           */
 };
 
 /* log10(q 10^b + 1 - q), a subgroup's term of the types model
@@ -111,6 +112,7 @@ __global__ void __launch_bounds__(HM_NT) hm_pair_kernel(const HmDev h) {
       if (!isnan(c[k])) s += dw[k] * exp10(c[k] - m);
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
     if (lane == 0) h.snp[p] = hm_finish(m, s);
+    if (h.lik_only) return;
     /* per grid point: log10 sum_k w_k 10^R[k][l] (hm_methods.cpp:261-275) */
     for (int l = lane; l < L; l += 64) {
       double mm = R[l];
@@ -145,7 +147,7 @@ __global__ void __launch_bounds__(HM_NT) hm_pair_kernel(const HmDev h) {
     if (lane == 0) h.cfg[(size_t)p * K + k] = v;
     /* subgroup-update numerators: R[s][l] + sum over the other subgroups
      * (hm_methods.cpp:236-247) */
-    for (int s = 0; s < S; ++s) {
+    for (int s = 0; s < S && !h.lik_only; ++s) {
       double x = 0.0;
       for (int s2 = 0; s2 < S; ++s2)
         if (s2 != s) x += hm_sub_term(sub[(size_t)k * S + s2], R[(size_t)s2 * L + l]);
@@ -156,7 +158,7 @@ __global__ void __launch_bounds__(HM_NT) hm_pair_kernel(const HmDev h) {
     }
   }
   /* per grid point: log10 sum_k w_k 10^t[k] (hm_methods.cpp:277-300) */
-  if (on) {
+  if (on && !h.lik_only) {
     double mm = t[0];
     for (int k = 1; k < HM_TYPES_KMAX && k < K; ++k)
       if (t[k] > mm) mm = t[k];
@@ -241,7 +243,7 @@ __global__ void __launch_bounds__(HM_NT) hm_gene_kernel(const HmDev h) {
   __syncthreads();
   const double ol = s_ol;
   const double w = 1.0 / (double)np;
-  for (int c = threadIdx.x; c < NC; c += HM_NT) {
+  for (int c = threadIdx.x; c < NC && !h.lik_only; c += HM_NT) {
     const double* v;
     int64_t stride;
     if (c < K) {
@@ -268,9 +270,9 @@ __global__ void __launch_bounds__(HM_NT) hm_gene_kernel(const HmDev h) {
 
 /* over genes: log10 sum of each contribution column (weights 1), the sum
  * of the log10 likelihoods and pi0's sum */
-__global__ void __launch_bounds__(HM_NT) hm_col_kernel(const HmDev h, int NC) {
+__global__ void __launch_bounds__(HM_NT) hm_col_kernel(const HmDev h, int NC, int j0) {
   __shared__ double red[HM_NT / 64];
-  const int j = blockIdx.x;
+  const int j = j0 + blockIdx.x;
   const double* col = h.rows + j;
   const int64_t stride = NC + 3;
   if (j < NC) {
@@ -361,12 +363,13 @@ struct HmRun {
    * reads the genes' likelihoods stored by the previous pass */
   int pass(const HmParams& P, HmPass& out, bool stale = false) {
     h.stale = stale ? 1 : 0;
+    h.lik_only = 0;
     int rc = upload(P);
     if (rc) return rc;
     if ((rc = launch())) return rc;
     {
       ProfScope ps(ctx, "hm_col_kernel", st);
-      hipLaunchKernelGGL(hm_col_kernel, dim3((unsigned)(NC + 2)), dim3(HM_NT), 0, st, h, NC);
+      hipLaunchKernelGGL(hm_col_kernel, dim3((unsigned)(NC + 2)), dim3(HM_NT), 0, st, h, NC, 0);
     }
     EQBF_HIP(ctx, hipGetLastError());
     hred.resize(NC + 2);
@@ -409,6 +412,24 @@ struct HmRun {
     return EQBF_OK;
   }
 };
+
+/* compute_log10_obs_lik(P) alone (eqtlbma_hm.cpp:617-650) */
+int hm_loglik(HmRun& R, const HmParams& P, double* lik) {
+  HmDev& h = R.h;
+  h.stale = 0;
+  h.lik_only = 1;
+  int rc = R.upload(P);
+  if (!rc) rc = R.launch();
+  if (rc) return rc;
+  {
+    ProfScope ps(R.ctx, "hm_col_kernel", R.st);
+    hipLaunchKernelGGL(hm_col_kernel, dim3(1), dim3(HM_NT), 0, R.st, h, R.NC, R.NC);
+  }
+  EQBF_HIP(R.ctx, hipGetLastError());
+  EQBF_HIP(R.ctx, hipMemcpyAsync(lik, h.red + R.NC, sizeof(double), hipMemcpyDeviceToHost, R.st));
+  EQBF_HIP(R.ctx, hipStreamSynchronize(R.st));
+  return EQBF_OK;
+}
 
 double sq(double x) { return x * x; }
 
@@ -652,6 +673,130 @@ int hm_em_run(eqbf_ctx* ctx, const double* raw_dev, int64_t n_pairs, const int64
   return EQBF_OK;
 }
 
+/* raw log10 ABFs and gene offsets from host arrays into the context's
+ * buffers */
+int hm_upload(eqbf_ctx* ctx, const double* raw, int64_t n_pairs, const int64_t* gene_offset, int64_t n_genes,
+              const eqbf_hm_opts* opts, const double** d_raw_out, const int64_t** d_off_out) {
+  const int D = opts->model == EQBF_HM_TYPES ? opts->n_subgroups : opts->dim;
+  if (n_pairs < 1 || n_genes < 1 || D < 1 || opts->n_grid < 1)
+    return ctx_fail(ctx, EQBF_ERR_ARG, "eqbf_hm: empty data or invalid options");
+  const size_t rb = (size_t)n_pairs * D * opts->n_grid * sizeof(double);
+  double* d_raw = (double*)ctx_buf(ctx, "hm_raw", rb);
+  int64_t* d_off = (int64_t*)ctx_buf(ctx, "hm_off", (size_t)(n_genes + 1) * sizeof(int64_t));
+  if (!d_raw || !d_off) return EQBF_ERR_HIP;
+  EQBF_HIP(ctx, hipMemcpyAsync(d_raw, raw, rb, hipMemcpyHostToDevice, ctx->stream));
+  EQBF_HIP(ctx, hipMemcpyAsync(d_off, gene_offset, (size_t)(n_genes + 1) * sizeof(int64_t), hipMemcpyHostToDevice,
+                               ctx->stream));
+  *d_raw_out = d_raw;
+  *d_off_out = d_off;
+  return EQBF_OK;
+}
+
+/* Controller::estimate_profile_ci (eqtlbma_hm.cpp:1348-1546): from the
+ * estimates, each of pi0, the configuration weights and the grid weights
+ * walks down and up in steps of 0.001 (the other weights of a simplex
+ * rescaled to keep their sum) while the natural-log likelihood stays
+ * within 2 of its maximum; types and subgroup probabilities keep their
+ * estimate as both bounds. */
+int hm_ci(HmRun& R, const HmParams& P, double* left, double* right) {
+  const double tick = 0.001, le = std::log10(std::exp(1.0));
+  double mx;
+  int rc = hm_loglik(R, P, &mx);
+  if (rc) return rc;
+  if (!std::isfinite(mx)) return hm_throw(R);
+  HmParams Q = P;
+  auto eval = [&](double* v) -> int {
+    const int e = hm_loglik(R, Q, v);
+    if (e) return e;
+    return std::isfinite(*v) ? EQBF_OK : hm_throw(R);
+  };
+  auto below = [&](double v) { return v / le < mx / le - 2.0; };
+  {
+    double l = P.pi0, r = P.pi0, v;
+    while (l >= 0) {
+      l -= tick;
+      if (l < 0) {
+        l = 0;
+        break;
+      }
+      Q.pi0 = l;
+      if ((rc = eval(&v))) return rc;
+      if (below(v)) {
+        l += tick;
+        break;
+      }
+    }
+    while (r <= 1) {
+      r += tick;
+      if (r > 1) {
+        r = 1;
+        break;
+      }
+      Q.pi0 = r;
+      if ((rc = eval(&v))) return rc;
+      if (below(v)) {
+        r -= tick;
+        break;
+      }
+    }
+    Q.pi0 = P.pi0;
+    left[0] = l;
+    right[0] = r;
+  }
+  /* one weight of a simplex, the others rescaled by (mle_i - x) / (1 - mle_i) */
+  auto walk = [&](std::vector<double>& w, const std::vector<double>& mle, double* lo, double* hi) -> int {
+    for (size_t i = 0; i < mle.size(); ++i) {
+      double l = mle[i], r = mle[i], v;
+      const double cp = mle[i], st = 1 - cp;
+      auto set = [&](double x) {
+        const double diff = cp - x;
+        for (size_t j = 0; j < mle.size(); ++j)
+          if (j != i) w[j] = mle[j] + diff * mle[j] / st;
+        w[i] = x;
+      };
+      while (l >= 0) {
+        l -= tick;
+        if (l < 0) {
+          l = 0;
+          break;
+        }
+        set(l);
+        if ((rc = eval(&v))) return rc;
+        if (below(v)) {
+          l += tick;
+          break;
+        }
+      }
+      while (r <= 1) {
+        r += tick;
+        if (r > 1) {
+          r = 1;
+          break;
+        }
+        set(r);
+        if ((rc = eval(&v))) return rc;
+        if (below(v)) {
+          r -= tick;
+          break;
+        }
+      }
+      lo[i] = l;
+      hi[i] = r;
+    }
+    w = mle;
+    return EQBF_OK;
+  };
+  const int K = R.h.K, S = R.h.S, L = R.h.L;
+  if (!R.types) {
+    if ((rc = walk(Q.dimw, P.dimw, left + 1, right + 1))) return rc;
+  } else {
+    for (int k = 0; k < K; ++k) left[1 + k] = right[1 + k] = P.dimw[k];
+    for (int i = 0; i < K * S; ++i) left[1 + K + i] = right[1 + K + i] = P.sub[i];
+  }
+  const int g0 = 1 + K + (R.types ? K * S : 0);
+  return walk(Q.grid, P.grid, left + g0, right + g0);
+}
+
 }  // namespace
 
 extern "C" int eqbf_hm_em_device(eqbf_ctx* ctx, const double* raw_dev, int64_t n_pairs, const int64_t* gene_offset_dev,
@@ -665,14 +810,10 @@ extern "C" int eqbf_hm_em(eqbf_ctx* ctx, const double* raw, int64_t n_pairs, con
                           int64_t n_genes, const eqbf_hm_opts* opts, eqbf_hm_params* params, double* log10_lik,
                           int64_t* n_iters) {
   if (!ctx || !raw || !gene_offset || !opts) return EQBF_ERR_ARG;
-  const int D = opts->model == EQBF_HM_TYPES ? opts->n_subgroups : opts->dim;
-  const size_t rb = (size_t)n_pairs * D * opts->n_grid * sizeof(double);
-  double* d_raw = (double*)ctx_buf(ctx, "hm_raw", rb);
-  int64_t* d_off = (int64_t*)ctx_buf(ctx, "hm_off", (size_t)(n_genes + 1) * sizeof(int64_t));
-  if (!d_raw || !d_off) return EQBF_ERR_HIP;
-  EQBF_HIP(ctx, hipMemcpyAsync(d_raw, raw, rb, hipMemcpyHostToDevice, ctx->stream));
-  EQBF_HIP(ctx, hipMemcpyAsync(d_off, gene_offset, (size_t)(n_genes + 1) * sizeof(int64_t), hipMemcpyHostToDevice,
-                               ctx->stream));
+  const double* d_raw;
+  const int64_t* d_off;
+  const int rc = hm_upload(ctx, raw, n_pairs, gene_offset, n_genes, opts, &d_raw, &d_off);
+  if (rc) return rc;
   return hm_em_run(ctx, d_raw, n_pairs, d_off, n_genes, opts, params, log10_lik, n_iters, ctx->stream);
 }
 
@@ -682,14 +823,10 @@ extern "C" int eqbf_hm_bfs(eqbf_ctx* ctx, const double* raw, int64_t n_pairs, co
   if (!ctx || !raw || !gene_offset || !opts) return EQBF_ERR_ARG;
   int rc = hm_check_params(ctx, opts, params);
   if (rc) return rc;
-  const int D = opts->model == EQBF_HM_TYPES ? opts->n_subgroups : opts->dim;
-  const size_t rb = (size_t)n_pairs * D * opts->n_grid * sizeof(double);
-  double* d_raw = (double*)ctx_buf(ctx, "hm_raw", rb);
-  int64_t* d_off = (int64_t*)ctx_buf(ctx, "hm_off", (size_t)(n_genes + 1) * sizeof(int64_t));
-  if (!d_raw || !d_off) return EQBF_ERR_HIP;
+  const double* d_raw;
+  const int64_t* d_off;
+  if ((rc = hm_upload(ctx, raw, n_pairs, gene_offset, n_genes, opts, &d_raw, &d_off))) return rc;
   hipStream_t st = ctx->stream;
-  EQBF_HIP(ctx, hipMemcpyAsync(d_raw, raw, rb, hipMemcpyHostToDevice, st));
-  EQBF_HIP(ctx, hipMemcpyAsync(d_off, gene_offset, (size_t)(n_genes + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
   HmRun R;
   if ((rc = hm_setup(ctx, R, d_raw, n_pairs, d_off, n_genes, opts, st))) return rc;
   HmParams P;
@@ -713,4 +850,20 @@ extern "C" int eqbf_hm_bfs(eqbf_ctx* ctx, const double* raw, int64_t n_pairs, co
     }
   }
   return EQBF_OK;
+}
+
+extern "C" int eqbf_hm_ci(eqbf_ctx* ctx, const double* raw, int64_t n_pairs, const int64_t* gene_offset,
+                          int64_t n_genes, const eqbf_hm_opts* opts, const eqbf_hm_params* params, double* left,
+                          double* right) {
+  if (!ctx || !raw || !gene_offset || !opts || !left || !right) return EQBF_ERR_ARG;
+  int rc = hm_check_params(ctx, opts, params);
+  if (rc) return rc;
+  const double* d_raw;
+  const int64_t* d_off;
+  if ((rc = hm_upload(ctx, raw, n_pairs, gene_offset, n_genes, opts, &d_raw, &d_off))) return rc;
+  HmRun R;
+  if ((rc = hm_setup(ctx, R, d_raw, n_pairs, d_off, n_genes, opts, ctx->stream))) return rc;
+  HmParams P;
+  hm_load(opts, params, P);
+  return hm_ci(R, P, left, right);
 }

@@ -82,6 +82,7 @@ def lib():
         L.eqbf_hm_em.argtypes = [vp, vp, i64, vp, i64, vp, vp, vp, vp]
         L.eqbf_hm_em_device.argtypes = [vp, vp, i64, vp, i64, vp, vp, vp, vp]
         L.eqbf_hm_bfs.argtypes = [vp, vp, i64, vp, i64, vp, vp, vp, vp, vp, vp]
+        L.eqbf_hm_ci.argtypes = [vp, vp, i64, vp, i64, vp, vp, vp, vp]
         L.eqbf_perm_pvalue.restype = i64
         _lib = L
     return _lib
@@ -398,6 +399,17 @@ class Engine:
         self._check(lib().eqbf_hm_bfs(self._ctx, _p(raw), NP, _p(off), G, C.byref(o), C.byref(p), _p(gp), _p(gb),
                                       _p(sb), _p(db)))
         return gp, gb, sb, db
+
+    def hm_ci(self, raw, off, pi0, grid, cfg=None, type_=None, sub=None):
+        """--getci bounds (left, right): pi0, configs or types, subgroups per
+        type (types model), grid points."""
+        raw, off, o, p, grid, dimw, sub = self._hm_setup(raw, off, pi0, grid, cfg, type_, sub, 0.05, 0, 1.0,
+                                                         (0,) * 5)
+        n = 1 + o.dim + (o.dim * o.n_subgroups if type_ is not None else 0) + o.n_grid
+        left, right = np.zeros(n), np.zeros(n)
+        self._check(lib().eqbf_hm_ci(self._ctx, _p(raw), raw.shape[0], _p(off), off.size - 1, C.byref(o),
+                                     C.byref(p), _p(left), _p(right)))
+        return left, right
 
     def sync(self):
         self._check(lib().eqbf_sync(self._ctx))

@@ -467,6 +467,17 @@ int eqbf_hm_em_device(eqbf_ctx* ctx, const double* raw_dev, int64_t n_pairs, con
 int eqbf_hm_bfs(eqbf_ctx* ctx, const double* raw, int64_t n_pairs, const int64_t* gene_offset, int64_t n_genes,
                 const eqbf_hm_opts* opts, const eqbf_hm_params* params, double* gene_post, double* gene_l10bf,
                 double* snp_l10bf, double* dim_l10bf);
+/* --getci (Controller::estimate_profile_ci, eqtlbma_hm.cpp:1348-1546):
+ * profile-likelihood bounds in steps of 0.001 where the natural-log
+ * likelihood stays within 2 of its value at the given parameters; pi0,
+ * configuration and grid weights are walked (a simplex's other weights
+ * rescaled to keep their sum), types and subgroup probabilities keep their
+ * value as both bounds.  left/right: [1 + dim + (types model: dim *
+ * n_subgroups) + n_grid] in the order pi0, configurations or types,
+ * subgroups per type, grid points.  Host arrays.  EQBF_ERR_DATA if a log10
+ * likelihood on the way is NaN or infinite. */
+int eqbf_hm_ci(eqbf_ctx* ctx, const double* raw, int64_t n_pairs, const int64_t* gene_offset, int64_t n_genes,
+               const eqbf_hm_opts* opts, const eqbf_hm_params* params, double* left, double* right);
 
 #ifdef __cplusplus
 }

@@ -490,3 +490,105 @@ extern "C" void orc_hm_bfs(const double* raw, const int64_t* off, int64_t G, int
     }
   }
 }
+
+/* Profile-likelihood bounds of --getci (Controller::estimate_profile_ci and
+ * estimate_profile_ci_pi0/configs/types/subgroups/grids,
+ * eqtlbma_hm.cpp:1348-1546), at the given parameters: steps of 0.001 while
+ * log10 lik / log10(e) >= max / log10(e) - 2.  left/right: pi0, configs or
+ * types, subgroups per type (types model), grid points.  Returns 0, or -2
+ * where the reference throws (a NaN or infinite log10 likelihood). */
+extern "C" int32_t orc_hm_ci(const double* raw, const int64_t* off, int64_t G, int32_t types, int32_t K, int32_t S,
+                             int32_t L, const double* pi0, const double* grid, const double* cfg, const double* type,
+                             const double* sub, double* left, double* right) {
+  HmData d{raw, off, G, types ? S : K, L};
+  const Params P = load_params(types, K, S, L, pi0, grid, cfg, type, sub);
+  std::vector<double> olik(G);
+  const double tick = 0.001, le = std::log10(std::exp(1.0));
+  const double mx = total_lik(d, types, K, P, olik);
+  if (!std::isfinite(mx)) return -2;
+  Params Q = P;
+  bool bad = false;
+  auto below = [&]() {
+    const double v = total_lik(d, types, K, Q, olik);
+    if (!std::isfinite(v)) bad = true;
+    return v / le < mx / le - 2.0;
+  };
+  /* pi0 (estimate_profile_ci_pi0) */
+  double l = P.pi0, r = P.pi0;
+  while (l >= 0 && !bad) {
+    l -= tick;
+    if (l < 0) {
+      l = 0;
+      break;
+    }
+    Q.pi0 = l;
+    if (below()) {
+      l += tick;
+      break;
+    }
+  }
+  while (r <= 1 && !bad) {
+    r += tick;
+    if (r > 1) {
+      r = 1;
+      break;
+    }
+    Q.pi0 = r;
+    if (below()) {
+      r -= tick;
+      break;
+    }
+  }
+  Q.pi0 = P.pi0;
+  left[0] = l;
+  right[0] = r;
+  /* configs and grids: one weight moves, the others share the difference
+   * in proportion to their estimate */
+  auto walk = [&](std::vector<double>& w, const std::vector<double>& mle, double* lo, double* hi) {
+    for (size_t i = 0; i < mle.size() && !bad; ++i) {
+      const double cp = mle[i], st = 1 - cp;
+      double a = cp, b = cp;
+      auto set = [&](double x) {
+        for (size_t j = 0; j < mle.size(); ++j)
+          if (j != i) w[j] = mle[j] + (cp - x) * mle[j] / st;
+        w[i] = x;
+      };
+      while (a >= 0 && !bad) {
+        a -= tick;
+        if (a < 0) {
+          a = 0;
+          break;
+        }
+        set(a);
+        if (below()) {
+          a += tick;
+          break;
+        }
+      }
+      while (b <= 1 && !bad) {
+        b += tick;
+        if (b > 1) {
+          b = 1;
+          break;
+        }
+        set(b);
+        if (below()) {
+          b -= tick;
+          break;
+        }
+      }
+      lo[i] = a;
+      hi[i] = b;
+    }
+    w = mle;
+  };
+  if (!types) {
+    walk(Q.cfg, P.cfg, left + 1, right + 1);
+  } else {
+    for (int32_t k = 0; k < K; ++k) left[1 + k] = right[1 + k] = P.type[k];
+    for (int32_t i = 0; i < K * S; ++i) left[1 + K + i] = right[1 + K + i] = P.sub[i];
+  }
+  const int32_t g0 = 1 + K + (types ? K * S : 0);
+  walk(Q.grid, P.grid, left + g0, right + g0);
+  return bad ? -2 : 0;
+}

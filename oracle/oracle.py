@@ -78,6 +78,8 @@ def lib():
         L.orc_hm_loglik.argtypes = [vp, vp, i64, i32, i32, i32, i32, vp, vp, vp, vp, vp]
         L.orc_hm_loglik.restype = d
         L.orc_hm_bfs.argtypes = [vp, vp, i64, i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.orc_hm_ci.argtypes = [vp, vp, i64, i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp]
+        L.orc_hm_ci.restype = i32
         L.orc_abf_from_sstats.argtypes = [i32, i64, vp, vp, vp, vp, i32, vp, vp, i32, vp, vp, i32, vp, vp, vp]
         L.orc_abf_from_sstats.restype = C.c_int
         L.orc_bma.argtypes = [i32, i32, vp, d, vp, vp]
@@ -410,3 +412,17 @@ def hm_bfs(raw, off, prm):
     lib().orc_hm_bfs(_p(raw), _p(off), G, types, K, S, L, _p(pi0), _p(prm.grid), _p(cfg), _p(typ), _p(sub),
                      _p(gp), _p(gb), _p(sb), _p(db))
     return gp, gb, sb, db
+
+
+def hm_ci(raw, off, prm):
+    """--getci bounds (left, right): pi0, configs or types, subgroups per
+    type (types model), grid points."""
+    raw, off, types, K, S, L, cfg, typ, sub = _hm_args(raw, off, prm)
+    n = 1 + K + (K * S if types else 0) + L
+    left, right = np.zeros(n), np.zeros(n)
+    pi0 = np.array([prm.pi0])
+    rc = lib().orc_hm_ci(_p(raw), _p(off), off.size - 1, types, K, S, L, _p(pi0), _p(prm.grid), _p(cfg), _p(typ),
+                         _p(sub), _p(left), _p(right))
+    if rc:
+        raise RuntimeError("oracle: NaN or infinite log10 likelihood")
+    return left, right

@@ -140,3 +140,20 @@ def test_hm_bfs_vs_oracle(eng, orc, model):
         err = float(np.max(np.abs(a[fin] - b[fin]), initial=0))
         _say("[gpu]   %s max abs err %.2e" % (name, err))
         assert err <= 1e-10, name
+
+
+@pytest.mark.parametrize("model", ["configs", "types"])
+def test_hm_ci_vs_oracle(eng, orc, model):
+    """--getci bounds: the same 0.001 steps, so the same values."""
+    if model == "configs":
+        raw, off = _case(67, G=400, K=3, L=4)
+        prm = orc.HmParams(0.5, np.full(4, 0.25), cfg=np.full(3, 1 / 3))
+    else:
+        raw, off = _case(71, G=200, K=2, L=3, S=3)
+        prm = orc.HmParams(0.5, np.full(3, 1 / 3), type_=np.full(2, 0.5), sub=np.full((2, 3), 0.5))
+    em, _, _ = orc.hm_em(raw, off, prm, thresh=1e-6)
+    _say("[gpu] hm profile CI %s pairs=%d" % (model, raw.shape[0]))
+    left, right = eng.hm_ci(raw, off, em.pi0, em.grid, cfg=em.cfg, type_=em.type, sub=em.sub)
+    rl, rr = orc.hm_ci(raw, off, em)
+    _say("[gpu]   left %s\n[gpu]   right %s" % (np.round(left, 4), np.round(right, 4)))
+    assert np.allclose(left, rl, rtol=0, atol=1e-12) and np.allclose(right, rr, rtol=0, atol=1e-12)

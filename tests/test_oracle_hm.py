@@ -181,3 +181,47 @@ def test_nan_likelihood_is_an_error():
     prm = orc.HmParams(0.5, np.full(3, 1 / 3), cfg=np.full(3, 1 / 3))
     with pytest.raises(RuntimeError, match="NaN"):
         orc.hm_em(raw, off, prm)
+
+
+def test_profile_ci_bounds_are_where_the_likelihood_drops_by_two():
+    """--getci: each bound is the last 0.001 step at which the natural-log
+    likelihood (one weight moved, the other weights of its simplex rescaled
+    in proportion) stays within 2 of the maximum; the next step leaves it."""
+    raw, off = _case(29, G=120, K=3, L=3)
+    prm = orc.HmParams(0.5, np.full(3, 1 / 3), cfg=np.full(3, 1 / 3))
+    em, _, _ = orc.hm_em(raw, off, prm, thresh=1e-8)
+    left, right = orc.hm_ci(raw, off, em)
+    ln = np.log10(np.e)
+    mx = orc.hm_loglik(raw, off, em)
+
+    def inside(q):
+        return orc.hm_loglik(raw, off, q) / ln >= mx / ln - 2.0
+
+    def moved(name, i, x):
+        q = em.copy()
+        if name == "pi0":
+            q.pi0 = x
+            return q
+        w = getattr(q, name)
+        mle = getattr(em, name)
+        for j in range(w.size):
+            w[j] = x if j == i else mle[j] + (mle[i] - x) * mle[j] / (1 - mle[i])
+        return q
+
+    items = [("pi0", 0, 0)] + [("cfg", i, 1 + i) for i in range(3)] + [("grid", i, 4 + i) for i in range(3)]
+    n_interior = 0
+    for name, i, j in items:
+        est = em.pi0 if name == "pi0" else getattr(em, name)[i]
+        assert left[j] <= est + 1e-12 <= right[j] + 2e-12
+        for b, step in [(left[j], -1e-3), (right[j], 1e-3)]:
+            if 1e-9 < b < 1 - 1e-9:
+                assert inside(moved(name, i, b))
+                assert not inside(moved(name, i, b + step))
+                n_interior += 1
+    assert n_interior >= 4
+    # the types model keeps its type and subgroup estimates as both bounds
+    raw, off = _case(31, G=60, L=2, S=3)
+    prm = orc.HmParams(0.5, np.full(2, 0.5), type_=np.full(2, 0.5), sub=np.full((2, 3), 0.4))
+    left, right = orc.hm_ci(raw, off, prm)
+    assert np.array_equal(left[1:9], np.concatenate([prm.type, prm.sub.ravel()]))
+    assert np.array_equal(right[1:9], left[1:9])
