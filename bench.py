@@ -51,6 +51,19 @@ def _load_traffic(kernel, args):
     return e.get("bytes_per_launch")
 
 
+def _load_traffic_hm(kernel, genes):
+    """Counter-measured HBM bytes per launch of an eqtlbma_hm kernel on this
+    workload (same file, keyed by kernel and gene count), if recorded."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            e = json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
+    if not e or e.get("genes") != genes:
+        return None
+    return e.get("bytes_per_launch")
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +91,10 @@ def parse():
     ap.add_argument("--mvlr-genes", type=int, default=2000)
     ap.add_argument("--no-mvlr", action="store_true", help="skip the MVLR workload")
     ap.add_argument("--no-c0", action="store_true", help="skip the plain-method workload")
+    ap.add_argument("--hm-genes", type=int, default=20000)
+    ap.add_argument("--hm-iters", type=int, default=6, help="EM iterations timed in the eqtlbma_hm workload")
+    ap.add_argument("--no-hm", action="store_true", help="skip the eqtlbma_hm workload")
+    ap.add_argument("--only-hm", action="store_true", help="run only the eqtlbma_hm workload")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a kernel of known "
                          "traffic, to scale the HBM byte counters of a profiler run)")
@@ -832,6 +849,128 @@ def mvlr_leg(args, torch, result):
 MVLR_FLOPS_PER_SYSTEM = 2 * 9 ** 3 + (2 * 9 ** 3) // 3 + 4 * 9 ** 2
 
 
+def hm_leg(args, torch, result):
+    """eqtlbma_hm's EM (configs model) on device-resident raw log10 ABFs: the
+    rows eqtlbma_bf --bfs all writes for 5 subgroups (31 configurations) on a
+    10-point grid, for 20,000 genes with cis SNPs ~ Poisson(1500) clipped to
+    [200, 6000] (3.0e7 pairs, 74 GB of FP64).  Values N(-0.6, 0.4) plus, in
+    15 % of genes, one SNP with U(1, 3) added to 60 % of its entries.  Each EM
+    iteration is one pass over the raw ABFs; the timed run makes --hm-iters
+    iterations (thresh below reach, --maxit iters + 1 as the reference counts
+    them) from the default start."""
+    from eqtlbma_amd.engine import Engine, HmOpts, HmParamsC
+    from oracle import oracle as orc
+    K, L, G = 31, 10, args.hm_genes
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(1005)
+    P = np.clip(rng.poisson(1500, G), 200, 6000).astype(np.int64)
+    off = np.concatenate([[0], np.cumsum(P)]).astype(np.int64)
+    NP = int(off[-1])
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(1005)
+    raw = torch.empty((NP, K, L), dtype=torch.float64, device=dev)
+    raw.normal_(-0.6, 0.4, generator=gen)
+    eg = np.nonzero(rng.random(G) < 0.15)[0]
+    pe = torch.from_numpy(off[eg] + rng.integers(0, P[eg])).to(dev)
+    bump = torch.empty((eg.size, K, L), dtype=torch.float64, device=dev).uniform_(1.0, 3.0, generator=gen)
+    bump *= (torch.rand((eg.size, K, L), device=dev, generator=gen) < 0.6)
+    raw[pe] += bump
+    off_d = torch.from_numpy(off).to(dev)
+    torch.cuda.synchronize()
+    eng = Engine(0)
+
+    def params():
+        grid, cfg = np.full(L, 1.0 / L), np.full(K, 1.0 / K)
+        p = HmParamsC()
+        p.pi0, p.grid, p.config = 0.5, grid.ctypes.data, cfg.ctypes.data
+        return p, grid, cfg
+
+    o = HmOpts()
+    o.model, o.dim, o.n_subgroups, o.n_grid = 0, K, 5, L
+    o.thresh, o.max_iters, o.stepmax = 1e-300, args.hm_iters, 1.0
+    p, grid, cfg = params()
+    o.max_iters = 2
+    eng.hm_em_device(raw.data_ptr(), NP, off_d.data_ptr(), G, o, p)  # warm-up
+    o.max_iters = args.hm_iters
+    p, grid, cfg = params()
+    eng.profile(True)
+    before = eng.profile_read()
+    t = time.perf_counter()
+    lik, iters = eng.hm_em_device(raw.data_ptr(), NP, off_d.data_ptr(), G, o, p)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t
+    after = eng.profile_read()
+    eng.profile(False)
+    kern = {k: ((v[0] - before.get(k, (0.0, 0))[0]), (v[1] - before.get(k, (0, 0))[1]))
+            for k, v in after.items() if v[1] > before.get(k, (0, 0))[1]}
+    passes = kern.get("hm_pair_kernel", (0.0, 1))[1]
+    pk = kern.get("hm_pair_kernel", (float("nan"), 1))
+    pair_ms = pk[0] / max(pk[1], 1)
+    # algorithmic bytes of one pair-kernel launch: the raw ABFs read once, the
+    # per-pair configuration, grid-point and SNP log10 sums written once
+    alg = NP * (K * L + K + L + 1) * 8.0
+    res = {"metric": "EM iterations x SNP-gene pairs per second", "value": iters * NP / dt, "unit": "pairs/s",
+           "seconds": dt, "iterations": int(iters), "passes": int(passes), "log10_lik": lik,
+           "ms_per_pass": dt * 1e3 / max(passes, 1),
+           "workload": "eqtlbma_hm configs model: %d genes, cis SNPs~Pois(1500) in [200,6000] (%d pairs), 31 "
+                       "configurations x 10 grid points of FP64 raw log10 ABFs (%.1f GB) resident in HBM, "
+                       "default start, %d iterations" % (G, NP, NP * K * L * 8 / 1e9, iters),
+           "kernels_ms_per_pass": {k: v[0] / max(v[1], 1) for k, v in kern.items()},
+           "roofline": {"bound": "hbm", "kernel": "hm_pair_kernel", "alg_bytes_per_launch": alg,
+                        "achieved": alg / (pair_ms / 1e3) / 1e9, "peak": 8000.0, "unit": "GB/s",
+                        "frac": alg / (pair_ms / 1e3) / 1e9 / 8000.0, "avg_launch_ms": pair_ms,
+                        "traffic": _load_traffic_hm("hm_pair_kernel", G)}}
+    if args.no_cpu_baseline:
+        eng.close()
+        del raw, bump, off_d
+        torch.cuda.empty_cache()
+        result["hm"] = res
+        log("hm: %d pairs, %d iterations in %.3f s (%.2f ms per pass); kernels %s" %
+            (NP, iters, dt, dt * 1e3 / max(passes, 1), {k: round(v[0] / max(v[1], 1), 3) for k, v in kern.items()}))
+        return
+    # parity on the first genes and the oracle's rate over the host threads
+    model, T = host_cpu()
+    gs = 40
+    raw_h = raw[:int(off[gs])].cpu().numpy()
+    off_h = off[:gs + 1].copy()
+    prm = orc.HmParams(0.5, np.full(L, 1.0 / L), cfg=np.full(K, 1.0 / K))
+    ref, it_ref, _ = orc.hm_em(raw_h, off_h, prm, thresh=1e-300, maxit=3)
+    e2 = Engine(0)
+    g_pi0, g_grid, g_cfg, _s, g_lik, g_it = e2.hm_em(raw_h, off_h, 0.5, np.full(L, 1.0 / L), cfg=np.full(K, 1.0 / K),
+                                                    thresh=1e-300, maxit=3)
+    e2.close()
+    res["error_vs_fp64_oracle"] = {"sample_genes": gs, "iterations": int(it_ref), "same_iterations": it_ref == g_it,
+                                   "pi0_abs": abs(g_pi0 - ref.pi0),
+                                   "grid_abs": float(np.max(np.abs(g_grid - ref.grid))),
+                                   "config_abs": float(np.max(np.abs(g_cfg - ref.cfg)))}
+    t = time.perf_counter()
+    orc.hm_loglik(raw_h, off_h, prm)
+    d1 = time.perf_counter() - t
+    k = min(G, max(gs, int(gs * args.cpu_seconds * T / max(4 * d1, 1e-3))))
+    k = max(gs, int(np.searchsorted(off, 4e9 / (K * L * 8.0), side="right")) - 1) if off[k] * K * L * 8.0 > 4e9 else k
+    bounds = [(int(c[0]), int(c[-1]) + 1) for c in np.array_split(np.arange(k), min(T, k))]
+    shards = [(raw[int(off[a]):int(off[b])].cpu().numpy(), off[a:b + 1] - off[a]) for a, b in bounds]
+
+    def one(i):
+        r_, o_ = shards[i]
+        orc.hm_em(r_, o_, prm, thresh=1e-300, maxit=2)  # 3 passes: initial + 2 iterations
+    t = time.perf_counter()
+    threaded(one, list(range(len(shards))), T)
+    tc = time.perf_counter() - t
+    cp = int(off[k])
+    res["cpu_baseline"] = dict(value=2 * cp / tc, unit="pairs/s",
+                               sample="first %d genes (%d pairs), 2 EM iterations (3 likelihood passes) per gene "
+                                      "shard, %.1f s wall" % (k, cp, tc),
+                               **cpu_label(model, T, "gene ranges sharded over threads"))
+    log("hm: %d pairs, %d iterations in %.3f s (%.2f ms per pass); kernels %s; cpu %.3e pairs/s" %
+        (NP, iters, dt, dt * 1e3 / max(passes, 1), {k: round(v[0] / max(v[1], 1), 3) for k, v in kern.items()},
+         2 * cp / tc))
+    eng.close()
+    del raw, bump, off_d
+    torch.cuda.empty_cache()
+    result["hm"] = res
+
+
 def alg_bytes(P_g, S, N, GL, GS):
     """Algorithmic HBM bytes of the association pass for one gene: per pair
     the genotype row (4N), the raw Bayes factors in FP32 (3 GL + S GS), the
@@ -865,6 +1004,18 @@ def main():
     from eqtlbma_amd import _abi, synth
     from eqtlbma_amd.engine import Engine, lib
 
+    if args.only_hm:
+        result = {"metric": "eqtlbma_hm EM only"}
+        if args.calib_copy_mib > 0:
+            src = torch.ones(args.calib_copy_mib << 18, dtype=torch.float32, device="cuda:0")
+            dst = torch.empty_like(src)
+            torch.neg(src, out=dst)  # reads and writes every byte once: the counters' scale
+            torch.cuda.synchronize()
+            log("calibration copy: %d MiB read + %d MiB written" % (args.calib_copy_mib, args.calib_copy_mib))
+            del src, dst
+        hm_leg(args, torch, result)
+        print(json.dumps(result), flush=True)
+        return
     if args.only_perm:
         result = {"metric": METRIC, "n_gpus": 1, "data": "synthetic"}
         perm_leg(args, torch, result)
@@ -1121,6 +1272,8 @@ def main():
         c0_leg(args, torch, result)
     if not args.no_mvlr:
         mvlr_leg(args, torch, result)
+    if not args.no_hm:
+        hm_leg(args, torch, result)
     if not args.no_perm:
         perm_leg(args, torch, result)
         if not args.no_alloc:

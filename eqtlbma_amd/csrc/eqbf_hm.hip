@@ -5,14 +5,23 @@
  *
  * One EM pass over the data evaluates the model at the current parameters
  * and yields everything an EM update needs, in three kernels:
- *  hm_pair_kernel  one wave per (gene, SNP) pair: the grid-averaged BF of
- *                  each configuration (or type), the SNP's BF, the
+ *  hm_pair_kernel  per (gene, SNP) pair: the grid-averaged BF of each
+ *                  configuration (or type), the SNP's BF, the
  *                  configuration-averaged BF of each grid point and, for the
  *                  types model, each (type, subgroup)'s numerator of the
- *                  subgroup update (hm_methods.cpp:74-300);
- *  hm_gene_kernel  one workgroup per gene: the gene's BF over its SNPs
- *                  (weight 1/P), its log10 observed likelihood, and each
- *                  column's SNP average minus that likelihood -- the gene's
+ *                  subgroup update (hm_methods.cpp:74-300).  Configs model
+ *                  (hm_cfg_tile_kernel): persistent workgroups over tiles of
+ *                  pairs, the next tile's raw ABFs loaded into registers
+ *                  while the current one is computed from LDS; one thread
+ *                  per configuration row (sum over the grid), then one per
+ *                  grid column and one per SNP (sums over the
+ *                  configurations), with one exponential per raw value where
+ *                  the pair's range allows it.  Types model, and
+ *                  configurations too many for a tile: one wave per pair;
+ *  hm_gene_kernel  one workgroup per gene, lanes over the columns and the
+ *                  four waves over the gene's SNPs: the gene's BF (weight
+ *                  1/P), its log10 observed likelihood, and each column's
+ *                  SNP average minus that likelihood -- the gene's
  *                  contribution to each weight's update (hm_methods.cpp:
  *                  392-704, gene_eQTL::em_update_*);
  *  hm_col_kernel   one workgroup per column: the log10 sum over genes of
@@ -69,6 +78,8 @@ struct HmDev {
   double* red;       /* [NC + 2] over genes                                 */
   int stale;         /* 1: keep each gene's stored log10 obslik (row[NC])   */
   int lik_only;      /* 1: the likelihood only, no update columns           */
+  int tp;            /* configs model: pairs per workgroup tile (0: none)   */
+  double range;      /* configs model: widest pair span for factored sums  */
 };
 
 /* log10(q 10^b + 1 - q), a subgroup's term of the types model
@@ -180,6 +191,214 @@ __global__ void __launch_bounds__(HM_NT) hm_pair_kernel(const HmDev h) {
   }
 }
 
+/* log10 sum_i w_i 10^v[i*stride] over n values, two passes as
+ * log10_weighted_sum does (max from the first value, NaN terms skipped) */
+__device__ inline double hm_lws_seq(const double* v, int n, int stride, const double* w) {
+  double m = v[0];
+  for (int i = 1; i < n; ++i)
+    if (v[i * stride] > m) m = v[i * stride];
+  double s = 0.0;
+  for (int i = 0; i < n; ++i) {
+    const double x = v[i * stride];
+    if (!isnan(x)) s += w[i] * exp10(x - m);
+  }
+  return hm_finish(m, s);
+}
+
+/* i / d for 0 <= i < 2^22 through a float reciprocal, corrected both ways */
+__device__ inline int hm_div(int i, int d, float inv) {
+  int q = (int)((float)i * inv);
+  if (q * d > i)
+    --q;
+  else if ((q + 1) * d <= i)
+    ++q;
+  return q;
+}
+
+constexpr int HM_PRE = 16;                            /* raw values per thread in flight  */
+constexpr int HM_LGKM0 = 15 | (7 << 4) | (3 << 14);  /* s_waitcnt lgkmcnt(0) only       */
+
+/* Workgroup barrier that waits for this wave's LDS operations only: the
+ * release fence of __syncthreads would also wait (vmcnt(0)) for the next
+ * tile's loads, which are meant to stay in flight over the computation.
+ * The empty asm statements keep the compiler from moving memory accesses
+ * across it. */
+__device__ __forceinline__ void hm_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(HM_LGKM0);
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+/* Configs model: persistent workgroups over tiles of h.tp pairs; the next
+ * tile's raw ABFs are loaded into registers (coalesced, HM_PRE values per
+ * thread) while the current tile is computed from LDS.
+ *
+ * A pair whose raw ABFs are all finite and span at most h.range (log10
+ * units) takes factored sums: with M the pair's maximum, each value becomes
+ * T = 10^(R - M) once, and
+ *   configuration k: M + log10 sum_l grid_l T[k][l],
+ *   grid point l:    M + log10 sum_k w_k T[k][l],
+ *   SNP:             M + log10 sum_k w_k sum_l grid_l T[k][l],
+ * the same quantities as the reference's two-pass sums (each against its
+ * own row or column maximum), one exponential per value instead of two.
+ * h.range keeps every product of a weight and T a normal double (it is 290
+ * plus the log10 of the smallest positive grid and configuration weights),
+ * so no term underflows.  Any other pair takes the reference's two-pass
+ * sums.
+ *
+ * LDS: grid and configuration weights [L + K], raw ABFs [tp*K][L|1] (odd
+ * row stride), row maxima then configuration sums [tp*K], row minima then
+ * row sums [tp*K], pair maxima [tp], pair flags [tp]. */
+__global__ void __launch_bounds__(HM_NT) __attribute__((amdgpu_waves_per_eu(4, 8))) hm_cfg_tile_kernel(const HmDev h) {
+  extern __shared__ double hm_lds[];
+  const int K = h.K, L = h.L, TP = h.tp, tid = threadIdx.x;
+  const int Lp = L | 1, KL = K * L;
+  double* wg = hm_lds;
+  double* wd = wg + L;
+  double* rl = wd + K;
+  double* rm = rl + (size_t)TP * K * Lp;
+  double* rs = rm + (size_t)TP * K;
+  double* pm = rs + (size_t)TP * K;
+  double* pf = pm + TP;
+  for (int i = tid; i < L + K; i += HM_NT) wg[i] = h.par[1 + i];
+  const float invL = 1.0f / (float)L, invK = 1.0f / (float)K;
+  const int64_t ntiles = (h.NP + TP - 1) / TP;
+  double pre[HM_PRE];
+  auto fetch = [&](int64_t t) {
+    const int64_t q0 = t * TP;
+    const int nv = (int)min((int64_t)TP, h.NP - q0) * KL;
+    const double* src = h.raw + (size_t)q0 * KL;
+#pragma unroll
+    for (int j = 0; j < HM_PRE; ++j) {
+      const int i = tid + j * HM_NT;
+      pre[j] = i < nv ? src[i] : 0.0;
+    }
+  };
+  int64_t tile = blockIdx.x;
+  if (tile < ntiles) fetch(tile);
+  for (; tile < ntiles; tile += gridDim.x) {
+    const int64_t p0 = tile * TP;
+    const int np = (int)min((int64_t)TP, h.NP - p0);
+    const int nv = np * KL, nrow = np * K;
+    {
+      /* element i = tid + j * HM_NT is row r, column c: stepped by
+       * HM_NT = dq rows + dr columns */
+      const int dq = HM_NT / L, dr = HM_NT - dq * L;
+      int r = hm_div(tid, L, invL), c = tid - r * L;
+#pragma unroll
+      for (int j = 0; j < HM_PRE; ++j) {
+        if (tid + j * HM_NT < nv) rl[r * Lp + c] = pre[j];
+        r += dq;
+        c += dr;
+        if (c >= L) {
+          c -= L;
+          ++r;
+        }
+      }
+    }
+    hm_barrier();
+    if (tile + gridDim.x < ntiles) fetch(tile + gridDim.x);
+    /* rows: maximum as the reference takes it (from the first value, NaN
+     * never greater), minimum, finiteness */
+    for (int r = tid; r < nrow; r += HM_NT) {
+      const double* x = rl + r * Lp;
+      double m = x[0], lo = x[0];
+      bool fin = isfinite(x[0]);
+      for (int l = 1; l < L; ++l) {
+        const double v = x[l];
+        if (v > m) m = v;
+        lo = fmin(lo, v);
+        fin = fin && isfinite(v);
+      }
+      rm[r] = m;
+      rs[r] = fin ? lo : NAN;
+    }
+    hm_barrier();
+    for (int q = tid; q < np; q += HM_NT) {
+      double M = -INFINITY, lo = INFINITY;
+      bool ok = true;
+      for (int k = 0; k < K; ++k) {
+        const double b = rs[q * K + k];
+        M = fmax(M, rm[q * K + k]);
+        if (isnan(b))
+          ok = false;
+        else
+          lo = fmin(lo, b);
+      }
+      pm[q] = M;
+      pf[q] = (ok && M - lo <= h.range) ? 1.0 : 0.0;
+    }
+    hm_barrier();
+    /* configuration sums (hm_methods.cpp:81-84) */
+    for (int r = tid; r < nrow; r += HM_NT) {
+      const int q = hm_div(r, K, invK);
+      double* x = rl + r * Lp;
+      double c;
+      if (pf[q] != 0.0) {
+        const double M = pm[q];
+        double s0 = 0.0, s1 = 0.0;
+        int l = 0;
+        for (; l + 1 < L; l += 2) {
+          const double t0 = exp10(x[l] - M), t1 = exp10(x[l + 1] - M);
+          x[l] = t0;
+          x[l + 1] = t1;
+          s0 += wg[l] * t0;
+          s1 += wg[l + 1] * t1;
+        }
+        if (l < L) {
+          const double t0 = exp10(x[l] - M);
+          x[l] = t0;
+          s0 += wg[l] * t0;
+        }
+        const double sum = s0 + s1;
+        rs[r] = sum;
+        c = hm_finish(M, sum);
+      } else {
+        const double m = rm[r];
+        double sum = 0.0;
+        for (int l = 0; l < L; ++l) {
+          const double v = x[l];
+          if (!isnan(v)) sum += wg[l] * exp10(v - m);
+        }
+        c = hm_finish(m, sum);
+      }
+      rm[r] = c;
+      h.cfg[(size_t)p0 * K + r] = c;
+    }
+    hm_barrier();
+    /* the SNP's BF (log10 sum_k w_k 10^cfg[k], hm_methods.cpp:102-104) and
+     * the grid columns (log10 sum_k w_k 10^R[k][l], hm_methods.cpp:261-275),
+     * one task per thread */
+    const int ntask = np + (h.lik_only ? 0 : np * L);
+    for (int t = tid; t < ntask; t += HM_NT) {
+      const bool snp = t < np;
+      const int q = snp ? t : hm_div(t - np, L, invL);
+      const int l = snp ? 0 : t - np - q * L;
+      double out;
+      if (pf[q] != 0.0) {
+        const double* a = snp ? rs + q * K : rl + (size_t)q * K * Lp + l;
+        const int st = snp ? 1 : Lp;
+        double s0 = 0.0, s1 = 0.0;
+        int k = 0;
+        for (; k + 1 < K; k += 2) {
+          s0 += wd[k] * a[k * st];
+          s1 += wd[k + 1] * a[(k + 1) * st];
+        }
+        if (k < K) s0 += wd[k] * a[k * st];
+        out = hm_finish(pm[q], s0 + s1);
+      } else {
+        out = snp ? hm_lws_seq(rm + q * K, K, 1, wd) : hm_lws_seq(rl + (size_t)q * K * Lp + l, K, Lp, wd);
+      }
+      if (snp)
+        h.snp[p0 + q] = out;
+      else
+        h.grd[(size_t)(p0 + q) * L + l] = out;
+    }
+    hm_barrier();
+  }
+}
+
 /* block-wide log10 sum of w 10^v, v strided over the block's threads; the
  * first element decides NaN */
 __device__ double hm_block_lws(const double* v, int64_t n, int64_t stride, double w, double* red) {
@@ -242,29 +461,68 @@ __global__ void __launch_bounds__(HM_NT) hm_gene_kernel(const HmDev h) {
   }
   __syncthreads();
   const double ol = s_ol;
+  if (h.lik_only) return;
+  /* lanes over columns, the four waves over the gene's SNPs: column max,
+   * then the sum against it (the reference's two passes); the first SNP's
+   * NaN makes the column NaN */
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, NW = HM_NT / 64;
   const double w = 1.0 / (double)np;
-  for (int c = threadIdx.x; c < NC && !h.lik_only; c += HM_NT) {
-    const double* v;
-    int64_t stride;
-    if (c < K) {
-      v = h.cfg + (size_t)p0 * K + c;
-      stride = K;
-    } else if (c < K + L) {
-      v = h.grd + (size_t)p0 * L + (c - K);
-      stride = L;
-    } else {
-      v = h.num + (size_t)p0 * K * S + (c - K - L);
-      stride = (int64_t)K * S;
+  __shared__ double part[HM_NT / 64][64];
+  for (int c0 = 0; c0 < NC; c0 += 64) {
+    const int c = c0 + lane;
+    const bool on = c < NC;
+    const double* v = h.cfg;
+    int64_t stride = K;
+    if (on) {
+      if (c < K) {
+        v = h.cfg + (size_t)p0 * K + c;
+        stride = K;
+      } else if (c < K + L) {
+        v = h.grd + (size_t)p0 * L + (c - K);
+        stride = L;
+      } else {
+        v = h.num + (size_t)p0 * K * S + (c - K - L);
+        stride = (int64_t)K * S;
+      }
     }
-    double m = v[0];
-    for (int64_t i = 1; i < np; ++i)
-      if (v[i * stride] > m) m = v[i * stride];
-    double s = 0.0;
-    for (int64_t i = 0; i < np; ++i) {
-      const double x = v[i * stride];
-      if (!isnan(x)) s += w * exp10(x - m);
+    double m = -INFINITY;
+    if (on) {
+      double m1 = -INFINITY;
+      int64_t i = wv;
+      for (; i + NW < np; i += 2 * NW) {
+        const double x0 = v[i * stride], x1 = v[(i + NW) * stride];
+        if (x0 > m) m = x0;
+        if (x1 > m1) m1 = x1;
+      }
+      if (i < np && v[i * stride] > m) m = v[i * stride];
+      m = fmax(m, m1);
     }
-    row[c] = hm_finish(m, s) - ol;
+    part[wv][lane] = m;
+    __syncthreads();
+    m = part[0][lane];
+    for (int k = 1; k < NW; ++k) m = fmax(m, part[k][lane]);
+    if (on && isnan(v[0])) m = NAN;
+    __syncthreads();
+    double sum = 0.0;
+    if (on) {
+      double sum1 = 0.0;
+      int64_t i = wv;
+      for (; i + NW < np; i += 2 * NW) {
+        const double x0 = v[i * stride], x1 = v[(i + NW) * stride];
+        if (!isnan(x0)) sum += w * exp10(x0 - m);
+        if (!isnan(x1)) sum1 += w * exp10(x1 - m);
+      }
+      if (i < np && !isnan(v[i * stride])) sum += w * exp10(v[i * stride] - m);
+      sum += sum1;
+    }
+    part[wv][lane] = sum;
+    __syncthreads();
+    if (wv == 0 && on) {
+      double t = 0.0;
+      for (int k = 0; k < NW; ++k) t += part[k][lane];
+      row[c] = hm_finish(m, t) - ol;
+    }
+    __syncthreads();
   }
 }
 
@@ -334,6 +592,19 @@ struct HmRun {
     std::memcpy(&hpar[1], P.grid.data(), L * sizeof(double));
     std::memcpy(&hpar[1 + L], P.dimw.data(), K * sizeof(double));
     if (types) std::memcpy(&hpar[1 + L + K], P.sub.data(), (size_t)K * S * sizeof(double));
+    /* factored sums only while every weight x 10^-range product stays a
+     * normal double: 290 + log10 of the smallest positive weights */
+    double lo_g = 1.0, lo_d = 1.0;
+    bool bad = false;
+    for (double g : P.grid) {
+      if (std::isnan(g) || g < 0) bad = true;
+      if (g > 0) lo_g = std::min(lo_g, g);
+    }
+    for (double d : P.dimw) {
+      if (std::isnan(d) || d < 0) bad = true;
+      if (d > 0) lo_d = std::min(lo_d, d);
+    }
+    h.range = bad ? -1.0 : 290.0 + std::log10(lo_g) + std::log10(lo_d);
     EQBF_HIP(ctx, hipMemcpyAsync((void*)h.par, hpar.data(), hpar.size() * sizeof(double), hipMemcpyHostToDevice, st));
     return EQBF_OK;
   }
@@ -342,10 +613,18 @@ struct HmRun {
     const unsigned pg = (unsigned)((h.NP + HM_NT / 64 - 1) / (HM_NT / 64));
     {
       ProfScope ps(ctx, "hm_pair_kernel", st);
-      if (types)
+      if (types) {
         hipLaunchKernelGGL(hm_pair_kernel<true>, dim3(pg), dim3(HM_NT), 0, st, h);
-      else
+      } else if (h.tp > 0) {
+        const int64_t nt = (h.NP + h.tp - 1) / h.tp;
+        const size_t lds = ((size_t)h.tp * (h.K * (h.L | 1) + 2 * h.K + 2) + h.K + h.L) * sizeof(double);
+        /* persistent: as many workgroups as fit the CUs at once (LDS-bound) */
+        const int per_cu = std::max(1, std::min(8, (int)((160u << 10) / lds)));
+        const unsigned nb = (unsigned)std::min<int64_t>(nt, (int64_t)std::max(ctx->n_cu, 1) * per_cu);
+        hipLaunchKernelGGL(hm_cfg_tile_kernel, dim3(nb), dim3(HM_NT), lds, st, h);
+      } else {
         hipLaunchKernelGGL(hm_pair_kernel<false>, dim3(pg), dim3(HM_NT), 0, st, h);
+      }
     }
     {
       ProfScope ps(ctx, "hm_gene_kernel", st);
@@ -621,6 +900,14 @@ int hm_setup(eqbf_ctx* ctx, HmRun& R, const double* raw_dev, int64_t n_pairs, co
   h.S = types ? o->n_subgroups : 0;
   h.D = types ? h.S : h.K;
   R.NC = h.K + h.L + (types ? h.K * h.S : 0);
+  /* configs model: tiles of up to 16 pairs within 40 KB of LDS (four
+   * workgroups per CU) and HM_PRE raw values per thread; a pair past that
+   * takes the wave kernel */
+  if (!types) {
+    const size_t per = (size_t)(h.K * (h.L | 1) + 2 * h.K + 2) * sizeof(double);
+    const size_t room = (40u << 10) - std::min<size_t>(40u << 10, (size_t)(h.K + h.L) * sizeof(double));
+    h.tp = (int)std::min<size_t>(std::min<size_t>(16, room / per), (size_t)(HM_PRE * HM_NT) / (h.K * h.L));
+  }
   h.par = (const double*)ctx_buf(ctx, "hm_par", (size_t)(1 + h.L + h.K + h.K * h.S) * sizeof(double));
   h.cfg = (double*)ctx_buf(ctx, "hm_cfg", (size_t)n_pairs * h.K * sizeof(double));
   h.grd = (double*)ctx_buf(ctx, "hm_grd", (size_t)n_pairs * h.L * sizeof(double));

@@ -157,3 +157,34 @@ def test_hm_ci_vs_oracle(eng, orc, model):
     rl, rr = orc.hm_ci(raw, off, em)
     _say("[gpu]   left %s\n[gpu]   right %s" % (np.round(left, 4), np.round(right, 4)))
     assert np.allclose(left, rl, rtol=0, atol=1e-12) and np.allclose(right, rr, rtol=0, atol=1e-12)
+
+
+def test_hm_many_configurations_vs_oracle(eng, orc):
+    """1,023 configurations (10 subgroups) x 6 grid points: one pair's raw
+    ABFs exceed the LDS tile, so the per-pair kernel runs instead."""
+    raw, off = _case(73, G=30, K=1023, L=6, pmin=2, pmax=6)
+    prm = orc.HmParams(0.5, np.full(6, 1 / 6), cfg=np.full(1023, 1 / 1023))
+    _say("[gpu] hm 1023 configurations pairs=%d" % raw.shape[0])
+    _em_both(eng, orc, raw, off, prm, thresh=1e-3)
+
+
+def test_hm_wide_ranges_and_tiny_weights_vs_oracle(eng, orc):
+    """Pairs whose log10 ABFs span more than ~290 units, and a grid weight of
+    1e-200, leave the factored sums (the reference's two-pass sums are kept
+    where a product of a weight and 10^(R - max) would underflow)."""
+    raw, off = _case(79, G=200, K=7, L=5)
+    rng = np.random.default_rng(80)
+    wide = rng.choice(raw.shape[0], 40, replace=False)
+    raw[wide, 2, 1] += 350.0
+    raw[wide[:10], 0, 0] -= 400.0
+    prm = orc.HmParams(0.5, np.full(5, 0.2), cfg=np.full(7, 1 / 7))
+    _say("[gpu] hm wide ranges pairs=%d" % raw.shape[0])
+    _em_both(eng, orc, raw, off, prm, thresh=1e-4)
+    prm = orc.HmParams(0.5, np.array([1e-200, 0.25, 0.25, 0.25, 0.25 - 1e-200]), cfg=np.full(7, 1 / 7))
+    _em_both(eng, orc, raw, off, prm, thresh=1e-4, fixed=(0, 1, 0, 0, 0))
+    got = eng.hm_bfs(raw, off, prm.pi0, prm.grid, cfg=prm.cfg)
+    ref = orc.hm_bfs(raw, off, prm)
+    for a, b in zip(got, ref):
+        fin = np.isfinite(b)
+        assert np.array_equal(np.isfinite(a), fin)
+        assert np.max(np.abs(a[fin] - b[fin]), initial=0) <= 1e-10
