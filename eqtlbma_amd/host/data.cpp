@@ -1,8 +1,14 @@
 #include "data.hpp"
 
+#include <sys/stat.h>
+#include <zlib.h>
+
 #include <algorithm>
+#include <climits>
 #include <cstdlib>
+#include <cstring>
 #include <iostream>
+#include <unordered_map>
 
 #include "util.hpp"
 
@@ -170,6 +176,171 @@ void load_snp_coords(const std::string& file, const std::set<std::string>& keep,
     s.chr = tok[0];
     s.pos = std::atol(tok[2].c_str());
     d.snps.emplace(s.name, s);
+  }
+}
+
+/* ---- tabix-indexed BED (loadSnpInfo with a .tbi, data_loader.cpp:1345-1400)
+ *
+ * The reference queries the index once per gene (genes in name order) for
+ * the region Gene::GetRegionInTabixFormat gives (gene.cpp:94-104:
+ * chr:start-r-start+r, or start-r-end+r for TSS+TES, in size_t arithmetic,
+ * so start < r wraps), parsed as ti_parse_region does (atoi of each bound,
+ * begin - 1 when positive, a region with begin > end dropped, begin clamped
+ * at 0 by the query), and keeps every record overlapping it (half-open
+ * intervals from the index's column configuration, tabix/index.c:128-208,
+ * 860-874); a SNP name seen before is skipped.  Here one pass over the
+ * BGZF file gives the same set: a record is kept if it overlaps the region
+ * of some gene, and of records sharing a name the one returned first by the
+ * per-gene queries -- smallest gene index, then file order -- is kept.  The
+ * index file is read for its header only (sequence names, columns,
+ * meta character, skipped lines). */
+struct TbiHeader {
+  int32_t preset = 0, sc = 1, bc = 2, ec = 3, meta = '#', skip = 0;
+  std::unordered_map<std::string, int> tid;
+};
+
+bool read_tbi(const std::string& path, TbiHeader& h) {
+  gzFile f = gzopen(path.c_str(), "rb");
+  if (!f) return false;
+  char magic[4];
+  int32_t v[8];
+  bool ok = gzread(f, magic, 4) == 4 && std::memcmp(magic, "TBI\1", 4) == 0 &&
+            gzread(f, v, sizeof v) == (int)sizeof v;
+  if (ok) {
+    h.preset = v[1];
+    h.sc = v[2];
+    h.bc = v[3];
+    h.ec = v[4];
+    h.meta = v[5];
+    h.skip = v[6];
+    const int32_t l_nm = v[7];
+    std::vector<char> nm((size_t)std::max(l_nm, 0));
+    ok = l_nm >= 0 && gzread(f, nm.data(), (unsigned)l_nm) == l_nm;
+    for (int32_t i = 0, start = 0, k = 0; ok && i < l_nm; ++i)
+      if (nm[i] == 0) {
+        h.tid.emplace(std::string(nm.data() + start, nm.data() + i), k++);
+        start = i + 1;
+      }
+  }
+  gzclose(f);
+  return ok;
+}
+
+struct TbxRegion {
+  int beg, end, gene;
+};
+
+void load_snp_coords_tabix(const std::string& file, const std::string& idx, const std::set<std::string>& keep,
+                           const Data& d, const std::string& anchor, uint64_t radius, Data& out) {
+  struct stat st_bed, st_tbi;
+  stat(file.c_str(), &st_bed);
+  stat(idx.c_str(), &st_tbi);
+  if (st_bed.st_mtime > st_tbi.st_mtime) fail("index file (" + idx + ") is older than data file (" + file + ")");
+  TbiHeader h;
+  if (!read_tbi(idx, h)) fail("failed to load the index file (tabix)");
+  if ((h.preset & 0xffff) != 0) fail("failed to load the index file (tabix): not a BED (generic) index");
+  /* regions, per sequence of the index */
+  std::vector<std::vector<TbxRegion>> reg(h.tid.size());
+  int gi = 0;
+  for (const auto& kv : d.genes) {
+    const Gene& g = kv.second;
+    const int gene = gi++;
+    auto t = h.tid.find(g.chr);
+    if (t == h.tid.end()) continue;
+    const uint64_t a = (uint64_t)g.start - radius;
+    const uint64_t b = (uint64_t)(anchor == "TSS+TES" ? g.end : g.start) + radius;
+    int beg = (int)std::strtol(std::to_string(a).c_str(), nullptr, 10);
+    int end = (int)std::strtol(std::to_string(b).c_str(), nullptr, 10);
+    if (anchor != "TSS" && anchor != "TSS+TES") {
+      beg = 0;
+      end = 1 << 29;
+    }
+    if (beg > 0) --beg;
+    if (beg > end) continue;
+    if (beg < 0) beg = 0;
+    if (end < beg) continue;
+    reg[t->second].push_back({beg, end, gene});
+  }
+  std::vector<std::vector<int>> pmax(reg.size());
+  for (size_t t = 0; t < reg.size(); ++t) {
+    std::sort(reg[t].begin(), reg[t].end(), [](const TbxRegion& x, const TbxRegion& y) { return x.beg < y.beg; });
+    pmax[t].resize(reg[t].size());
+    int m = INT_MIN;
+    for (size_t i = 0; i < reg[t].size(); ++i) pmax[t][i] = m = std::max(m, reg[t][i].end);
+  }
+  struct Cand {
+    int gene;
+    int64_t line;
+    std::string chr, coord;
+  };
+  std::unordered_map<std::string, Cand> best;
+  gzFile f = gzopen(file.c_str(), "rb");
+  if (!f) fail("fail to open the data file (tabix)");
+  gzbuffer(f, 1 << 20);
+  std::vector<char> buf(1 << 16);
+  std::string line;
+  std::vector<std::string> fld, tok;
+  int64_t nl = 0;
+  while (true) {
+    line.clear();
+    bool got = false;
+    while (gzgets(f, buf.data(), (int)buf.size())) {
+      got = true;
+      const size_t n = std::strlen(buf.data());
+      if (n > 0 && buf[n - 1] == '\n') {
+        line.append(buf.data(), n - 1);
+        break;
+      }
+      line.append(buf.data(), n);
+    }
+    if (!got) break;
+    ++nl;
+    if (nl <= h.skip || (!line.empty() && (unsigned char)line[0] == (unsigned char)h.meta)) continue;
+    /* the record's interval as ti_get_intv forms it (fields split at each
+     * tab, empty ones included) */
+    fld.clear();
+    size_t p0 = 0;
+    while (true) {
+      const size_t q = line.find('\t', p0);
+      fld.emplace_back(line, p0, q == std::string::npos ? std::string::npos : q - p0);
+      if (q == std::string::npos) break;
+      p0 = q + 1;
+    }
+    if ((int)fld.size() < h.sc || (int)fld.size() < h.bc) continue;
+    auto t = h.tid.find(fld[h.sc - 1]);
+    if (t == h.tid.end()) continue;
+    long beg = std::strtol(fld[h.bc - 1].c_str(), nullptr, 0), end = beg;
+    if (!(h.preset & 0x10000))
+      --beg;
+    else
+      ++end;
+    if (beg < 0) beg = 0;
+    if (end < 1) end = 1;
+    if (h.ec > 0 && h.ec != h.bc && h.ec <= (int)fld.size()) end = std::strtol(fld[h.ec - 1].c_str(), nullptr, 0);
+    /* smallest gene index among the regions overlapping [beg, end) */
+    const std::vector<TbxRegion>& R = reg[t->second];
+    const std::vector<int>& M = pmax[t->second];
+    size_t ub = std::lower_bound(R.begin(), R.end(), end, [](const TbxRegion& x, long v) { return x.beg < v; }) -
+                R.begin();
+    int gmin = INT_MAX;
+    for (size_t i = ub; i-- > 0;) {
+      if (M[i] <= beg) break;
+      if (R[i].end > beg && R[i].gene < gmin) gmin = R[i].gene;
+    }
+    if (gmin == INT_MAX) continue;
+    tokenize(line, "\t", tok);
+    if (tok.size() < 4) fail("file " + file + " is not in the BED format");
+    if (!keep.empty() && !keep.count(tok[3])) continue;
+    auto it = best.find(tok[3]);
+    if (it == best.end() || gmin < it->second.gene) best[tok[3]] = Cand{gmin, nl, tok[0], tok[2]};
+  }
+  gzclose(f);
+  for (const auto& kv : best) {
+    Snp sn;
+    sn.name = kv.first;
+    sn.chr = kv.second.chr;
+    sn.pos = std::atol(kv.second.coord.c_str());
+    out.snps.emplace(sn.name, sn);
   }
 }
 
@@ -346,7 +517,14 @@ bool load_data(const LoadOptions& o, Data& d) {
     if (keep.empty()) return false;
   }
   const bool custom = !o.snp_coords.empty();
-  if (custom) load_snp_coords(o.snp_coords, keep, d);
+  if (custom) {
+    if (file_exists(o.snp_coords + ".tbi")) {
+      if (verbose > 0) std::cout << "load SNP coordinates (tabix-indexed BED file) ..." << std::endl;
+      load_snp_coords_tabix(o.snp_coords, o.snp_coords + ".tbi", keep, d, o.anchor, o.radius, d);
+    } else {
+      load_snp_coords(o.snp_coords, keep, d);
+    }
+  }
   /* files are read in subgroup order until the first subgroup (after the
    * first) naming the first subgroup's file; every subgroup from there on
    * takes the first subgroup's genotypes (data_loader.cpp:733-738, 691-715) */

@@ -56,6 +56,8 @@ struct LoadOptions {
   std::string geno_list, snp_coords, exp_list, gene_coords, covar_list, snps_to_keep;
   std::vector<std::string> subgroups_to_keep;
   std::string error_model = "uvlr";
+  std::string anchor = "TSS"; /* cis anchor and radius: the regions of a   */
+  uint64_t radius = 100000;   /* tabix-indexed --scoord are queried by them */
   float min_maf = 0.0f; /* compared as the reference does, maf < (float) cutoff */
   int verbose = 1;
 };

@@ -710,6 +710,8 @@ int main(int argc, char** argv) {
   lo.subgroups_to_keep = o.subgroups_to_keep;
   lo.error_model = o.analysis == "join" ? o.error_model : "uvlr";
   lo.min_maf = o.min_maf;
+  lo.anchor = o.anchor;
+  lo.radius = o.radius;
   lo.verbose = verbose;
   Data d;
   if (!load_data(lo, d)) return EXIT_SUCCESS;

@@ -98,3 +98,26 @@ def test_inss_missing_column_and_empty(root):
     r = _run(["--inss", lst, "--out", os.path.join(root, "empty")] + args, root)
     assert r.returncode == 0, r.stderr
     assert not [f for f in os.listdir(root) if f.startswith("empty_")]
+
+
+def test_tabix_index_older_than_data(root, tmp_path):
+    """A .tbi older than its BED file is refused (data_loader.cpp:1358-1363)."""
+    import shutil
+    import time
+    from eqtlbma_amd import REPO_DIR
+    ref = os.path.join(REPO_DIR, "oracle", "_ref")
+    if not os.path.exists(os.path.join(ref, "tabix")):
+        pytest.skip("oracle/_ref has no tabix build")
+    bed = str(tmp_path / "snps.bed")
+    lines = sorted(open(os.path.join(root, "snps.bed")), key=lambda l: (l.split()[0], int(l.split()[1])))
+    open(bed, "w").write("".join(lines))
+    subprocess.run([os.path.join(ref, "bgzip"), "-f", bed], check=True)
+    subprocess.run([os.path.join(ref, "tabix"), "-f", "-p", "bed", bed + ".gz"], check=True)
+    t = os.path.getmtime(bed + ".gz.tbi")
+    os.utime(bed + ".gz", (t + 10, t + 10))
+    args = _base(root, str(tmp_path / "out"))
+    args[args.index("--scoord") + 1] = bed + ".gz"
+    r = _run(args + ["--analys", "sep"], str(tmp_path))
+    assert r.returncode == 1
+    assert "is older than data file" in r.stderr
+
