@@ -446,3 +446,341 @@ extern "C" int orc_assoc_mvlr(const eqbf_chunk* c, int32_t bfs, double fiterr, c
   }
   return EQBF_OK;
 }
+
+/* ======================================================================== */
+/* --error hybrid (GeneSnpPair::CalcAbfsHybrid, gene_snp_pair.cpp:1384-1423, */
+/* and what it calls, :760-1382; utils_math.cpp:249-348;                     */
+/* samples.cpp:148-192).  Same dense steps as the reference:                 */
+/*  - per subgroup, X = [1, genotype, covariates] over the members           */
+/*    (FillStlContainers): pseudo-inverses from the eigen-decomposition of   */
+/*    X'X (X's squared singular values, every one inverted as the reference  */
+/*    does), rank = singular values > DBL_EPSILON, betahat, se, p, pve,      */
+/*    sigmahat, sigma2 = rss / N for the full and the null ([1, covariates]) */
+/*    model, Sigma_ss = f s2_full + (1 - f) s2_null, Vg_ss = Sigma_ss        */
+/*    [(X'X)^-1]_11 (CalcBetahatsAndDiagsPerSubgroup :760-880);              */
+/*  - per pair of subgroups, individuals in both / only the first / only the */
+/*    second (present: genotyped and a non-NaN expression level), rows       */
+/*    [1, genotype, covariates of the FIRST subgroup read at the             */
+/*    individual's position in the union of samples] (:882-975),            */
+/*    A_s = pinv(X12'X12 + Xs'Xs) X12' (:977-1013), the ML error covariance  */
+/*    Y'(I - X pinv(X'X) X')Y / n12 of the full and null model               */
+/*    (CalcMleErrorCovariance utils_math.cpp:306-348), Sigma_12 = f full_01 */
+/*    + (1 - f) null_01, Vg_12 = Sigma_12 (A_1 A_2')_11 (:1035-1131);        */
+/*  - CalcLog10AbfMvlr (:1165-1256) with explicit inverses.                  */
+/* covars_pos: [S][Qmax][N], the covariate value at file column i of each   */
+/* subgroup (NaN past the file's columns); NULL: the chunk's covariates.    */
+/* expl_pair: [G][S][N] expression levels of the pairwise step, which the   */
+/* reference reads without --qnorm's normalisation (Gene::GetExplevel);     */
+/* NULL: the chunk's.                                                       */
+/* Pinned in tests/test_oracle_hybrid.py: per-subgroup quantities by numpy  */
+/* least squares, the pairwise covariances by residual cross-products, Vg   */
+/* off the diagonal by a different algebra, the Bayes factor by the ratio   */
+/* of two multivariate normal densities (scipy).                            */
+/* ======================================================================== */
+namespace {
+
+/* pseudo-inverse of a symmetric positive semi-definite matrix (its SVD is
+ * its eigen-decomposition), every eigenvalue inverted */
+static Mat pinv_sym(const Mat& A, int n, std::vector<ldouble>* evs = nullptr) {
+  std::vector<ldouble> ev;
+  Mat V;
+  jacobi_eig(A, n, ev, V);
+  Mat inv(n * n, 0);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      ldouble acc = 0;
+      for (int k = 0; k < n; ++k) acc += V[i * n + k] * V[j * n + k] / ev[k];
+      inv[i * n + j] = acc;
+    }
+  if (evs) *evs = ev;
+  return inv;
+}
+
+/* [Y'(I - X pinv(X'X) X')Y / n]_01 for Y n x 2, X n x P */
+static ldouble mle_cov01(const Mat& Y, const Mat& X, int n, int P) {
+  Mat XtX = matmul(X, X, P, n, P, true, false);
+  Mat Xi = pinv_sym(XtX, P);
+  Mat t1 = matmul(X, Xi, n, P, P);              /* n x P */
+  Mat t2 = matmul(t1, X, n, P, n, false, true); /* n x n */
+  ldouble acc = 0;
+  for (int i = 0; i < n; ++i) {
+    ldouble Ty = Y[i * 2 + 1];
+    for (int j = 0; j < n; ++j) Ty -= t2[i * n + j] * Y[j * 2 + 1];
+    acc += Y[i * 2 + 0] * Ty;
+  }
+  return acc / n;
+}
+
+/* CalcLog10AbfMvlr (gene_snp_pair.cpp:1165-1256) */
+static double hybrid_l10abf(int S, const std::vector<int>& gamma, const std::vector<ldouble>& b, const Mat& Sig,
+                            const Mat& Vg, double phi2, double oma2) {
+  Mat Wg(S * S);
+  for (int i = 0; i < S; ++i)
+    for (int j = 0; j < S; ++j) Wg[i * S + j] = ((i == j) ? phi2 + oma2 : oma2) * gamma[i] * gamma[j];
+  Mat Vi = invert(Vg, S);
+  std::vector<ldouble> bV(S, 0);
+  for (int j = 0; j < S; ++j)
+    for (int k = 0; k < S; ++k) bV[j] += b[k] * Vi[k * S + j];
+  for (int i = 0; i < S; ++i)
+    for (int j = 0; j < S; ++j) Wg[i * S + j] *= sqrtl(Sig[i * S + i]) * sqrtl(Sig[j * S + j]);
+  Mat ivw = matmul(Vi, Wg, S, S, S);
+  for (int i = 0; i < S; ++i) ivw[i * S + i] += 1;
+  Mat LU = ivw;
+  std::vector<int> piv;
+  int sg;
+  lu_decomp(LU, S, piv, sg);
+  ldouble lnd = 0;
+  for (int i = 0; i < S; ++i) lnd += logl(fabsl(LU[i * S + i]));
+  Mat ivi = lu_invert(LU, S, piv);
+  std::vector<ldouble> t4(S, 0), t5(S, 0);
+  for (int j = 0; j < S; ++j)
+    for (int k = 0; k < S; ++k) t4[j] += bV[k] * Wg[k * S + j];
+  for (int j = 0; j < S; ++j)
+    for (int k = 0; k < S; ++k) t5[j] += t4[k] * ivi[k * S + j];
+  ldouble t6 = 0;
+  for (int j = 0; j < S; ++j) t6 += t5[j] * bV[j];
+  return (double)((-0.5L * lnd + 0.5L * t6) / logl(10.0L));
+}
+
+/* b, Sigma and Vg of one pair (CalcSstatsHybrid, :1133-1163); writes n and
+ * the summary statistics of the pair when n_out / sstats are given.
+ * Returns 0, or EQBF_ERR_DATA (two subgroups without a common individual,
+ * where the reference exits). */
+static int hybrid_pair(const eqbf_chunk* c, int64_t g, int64_t p, double fiterr, const double* covars_pos,
+                       const double* expl_pair, std::vector<ldouble>& b, Mat& Sig, Mat& Vg, int32_t* n_out, double* sstats) {
+  const int32_t S = c->n_subgroups;
+  const int64_t NP = c->n_pairs, N = c->n_samples, QM = c->n_covars_max;
+  const int64_t snp = c->pair_snp[p];
+  const ldouble f = fiterr;
+  auto present = [&](int32_t s, int64_t i) {
+    return c->geno_in_sbgrp[(int64_t)s * N + i] && !std::isnan(expl_at(c, g, s, i));
+  };
+  b.assign(S, 0);
+  std::vector<ldouble> sd(S), vd(S);
+  /* per subgroup (CalcBetahatsAndDiagsPerSubgroup) */
+  for (int32_t s = 0; s < S; ++s) {
+    std::vector<int64_t> idx;
+    std::vector<double> y;
+    members(c, g, s, nullptr, idx, y);
+    const int n = (int)idx.size();
+    const int Q = c->covar_count[s], P = Q + 2;
+    Mat X((size_t)n * P), Xc((size_t)n * (P - 1));
+    for (int r = 0; r < n; ++r) {
+      X[r * P + 0] = 1;
+      X[r * P + 1] = geno_at(c, s, idx[r], snp);
+      Xc[r * (P - 1) + 0] = 1;
+      for (int q = 0; q < Q; ++q) {
+        const ldouble v = c->covars[((int64_t)s * QM + q) * N + idx[r]];
+        X[r * P + 2 + q] = v;
+        Xc[r * (P - 1) + 1 + q] = v;
+      }
+    }
+    std::vector<ldouble> ev;
+    Mat XtX = matmul(X, X, P, n, P, true, false);
+    Mat Xi = pinv_sym(XtX, P, &ev);
+    int rank = 0;
+    for (int k = 0; k < P; ++k)
+      if (sqrtl(ev[k]) > 2.220446049250313e-16L) ++rank;
+    std::vector<ldouble> Xty(P, 0), B(P, 0);
+    for (int j = 0; j < P; ++j)
+      for (int r = 0; r < n; ++r) Xty[j] += X[r * P + j] * y[r];
+    for (int i = 0; i < P; ++i)
+      for (int j = 0; j < P; ++j) B[i] += Xi[i * P + j] * Xty[j];
+    ldouble rss = 0, ym = 0, tss = 0;
+    for (int r = 0; r < n; ++r) {
+      ldouble fit = 0;
+      for (int j = 0; j < P; ++j) fit += X[r * P + j] * B[j];
+      rss += (y[r] - fit) * (y[r] - fit);
+      ym += y[r];
+    }
+    ym /= n;
+    for (int r = 0; r < n; ++r) tss += (y[r] - ym) * (y[r] - ym);
+    const ldouble sig = sqrtl(rss / (n - rank));
+    const ldouble se = sig * sqrtl(Xi[1 * P + 1]);
+    if (sstats) {
+      double* o = sstats + (int64_t)s * NP + p;
+      o[0] = (double)(1 - rss / tss);
+      o[(int64_t)1 * S * NP] = (double)sig;
+      o[(int64_t)2 * S * NP] = (double)B[1];
+      o[(int64_t)3 * S * NP] = (double)se;
+      o[(int64_t)4 * S * NP] = (double)(2 * tdist_lower_neg(fabsl(B[1] / se), (ldouble)(n - rank)));
+    }
+    if (n_out) n_out[(int64_t)s * NP + p] = n;
+    /* null model */
+    Mat XcX = matmul(Xc, Xc, P - 1, n, P - 1, true, false);
+    Mat Xci = pinv_sym(XcX, P - 1);
+    std::vector<ldouble> Xcy(P - 1, 0), Bn(P - 1, 0);
+    for (int j = 0; j < P - 1; ++j)
+      for (int r = 0; r < n; ++r) Xcy[j] += Xc[r * (P - 1) + j] * y[r];
+    for (int i = 0; i < P - 1; ++i)
+      for (int j = 0; j < P - 1; ++j) Bn[i] += Xci[i * (P - 1) + j] * Xcy[j];
+    ldouble rn = 0;
+    for (int r = 0; r < n; ++r) {
+      ldouble fit = 0;
+      for (int j = 0; j < P - 1; ++j) fit += Xc[r * (P - 1) + j] * Bn[j];
+      rn += (y[r] - fit) * (y[r] - fit);
+    }
+    b[s] = B[1];
+    sd[s] = f * rss / n + (1 - f) * rn / n;
+    vd[s] = sd[s] * Xi[1 * P + 1];
+  }
+  /* pairs of subgroups (CalcOffDiagCovarsFromPairsOfSubgroups) */
+  Sig.assign(S * S, 0);
+  Vg.assign(S * S, 0);
+  for (int32_t s = 0; s < S; ++s) {
+    Sig[s * S + s] = sd[s];
+    Vg[s * S + s] = vd[s];
+  }
+  for (int32_t s1 = 0; s1 + 1 < S; ++s1)
+    for (int32_t s2 = s1 + 1; s2 < S; ++s2) {
+      const int Q = c->covar_count[s1], P = Q + 2;
+      std::vector<int64_t> i12, i1, i2;
+      for (int64_t i = 0; i < N; ++i) {
+        const bool a = present(s1, i), bb = present(s2, i);
+        if (a && bb)
+          i12.push_back(i);
+        else if (a)
+          i1.push_back(i);
+        else if (bb)
+          i2.push_back(i);
+      }
+      if (i12.empty()) {
+        snprintf(g_err, sizeof g_err, "subgroups %d and %d have no individuals in common", s1, s2);
+        return EQBF_ERR_DATA;
+      }
+      auto rows = [&](const std::vector<int64_t>& ii, Mat& X) {
+        X.assign(ii.size() * P, 0);
+        for (size_t r = 0; r < ii.size(); ++r) {
+          const int64_t i = ii[r];
+          X[r * P + 0] = 1;
+          X[r * P + 1] = c->geno_in_sbgrp[(int64_t)s1 * N + i] ? geno_at(c, s1, i, snp) : kNaN;
+          for (int q = 0; q < Q; ++q)
+            X[r * P + 2 + q] = covars_pos ? covars_pos[((int64_t)s1 * QM + q) * N + i]
+                                          : c->covars[((int64_t)s1 * QM + q) * N + i];
+        }
+      };
+      Mat X12, X1, X2;
+      rows(i12, X12);
+      rows(i1, X1);
+      rows(i2, X2);
+      const int n12 = (int)i12.size();
+      Mat Y12((size_t)n12 * 2);
+      for (int r = 0; r < n12; ++r) {
+        if (expl_pair) {
+          Y12[r * 2 + 0] = expl_pair[(g * S + s1) * N + i12[r]];
+          Y12[r * 2 + 1] = expl_pair[(g * S + s2) * N + i12[r]];
+        } else {
+          Y12[r * 2 + 0] = expl_at(c, g, s1, i12[r]);
+          Y12[r * 2 + 1] = expl_at(c, g, s2, i12[r]);
+        }
+      }
+      Mat G12 = matmul(X12, X12, P, n12, P, true, false);
+      auto amat = [&](const Mat& Xu, int nu) {
+        Mat Gs = G12;
+        if (nu > 0) {
+          Mat Gu = matmul(Xu, Xu, P, nu, P, true, false);
+          for (int k = 0; k < P * P; ++k) Gs[k] += Gu[k];
+        }
+        return matmul(pinv_sym(Gs, P), X12, P, P, n12, false, true); /* P x n12 */
+      };
+      Mat A1 = amat(X1, (int)i1.size()), A2 = amat(X2, (int)i2.size());
+      const ldouble full = mle_cov01(Y12, X12, n12, P);
+      Mat Xc12((size_t)n12 * (P - 1));
+      for (int r = 0; r < n12; ++r) {
+        Xc12[r * (P - 1)] = 1;
+        for (int j = 2; j < P; ++j) Xc12[r * (P - 1) + j - 1] = X12[r * P + j];
+      }
+      const ldouble null = mle_cov01(Y12, Xc12, n12, P - 1);
+      const ldouble s12 = f * full + (1 - f) * null;
+      ldouble a = 0;
+      for (int r = 0; r < n12; ++r) a += A1[1 * n12 + r] * A2[1 * n12 + r];
+      Sig[s1 * S + s2] = Sig[s2 * S + s1] = s12;
+      Vg[s1 * S + s2] = Vg[s2 * S + s1] = s12 * a;
+    }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int orc_assoc_hybrid(const eqbf_chunk* c, int32_t bfs, double fiterr, const double* covars_pos,
+                                const double* expl_pair, const double* phiL, const double* omaL, int32_t GL, const double* phiS,
+                                const double* omaS, int32_t GS, int32_t* n_out, double* sstats, double* raw,
+                                double* w_abf) {
+  const int32_t S = c->n_subgroups;
+  const int64_t NP = c->n_pairs;
+  const int64_t nc = n_configs(bfs, S);
+  const int64_t n_raw = 3 * (int64_t)GL + nc * GS, n_w = 3 + nc + 2;
+  std::vector<uint64_t> masks;
+  config_masks(bfs, S, masks);
+  for (int64_t g = 0; g < c->n_genes; ++g) {
+    bool all_expr = true;
+    for (int32_t s = 0; s < S; ++s) all_expr &= c->gene_in_sbgrp[g * S + s] != 0;
+    for (int64_t p = c->pair_offset[g]; p < c->pair_offset[g + 1]; ++p) {
+      const int64_t snp = c->pair_snp[p];
+      bool all_geno = true;
+      for (int32_t s = 0; s < S; ++s) all_geno &= c->snp_in_sbgrp[snp * S + s] != 0;
+      for (int32_t s = 0; s < S; ++s) {
+        n_out[(int64_t)s * NP + p] = 0;
+        for (int k = 0; k < 5; ++k) sstats[((int64_t)k * S + s) * NP + p] = kNaN;
+      }
+      for (int64_t k = 0; k < n_raw; ++k) raw[k * NP + p] = kNaN;
+      for (int64_t k = 0; k < n_w; ++k) w_abf[k * NP + p] = kNaN;
+      if (!all_expr || !all_geno) continue;
+      std::vector<ldouble> b;
+      Mat Sig, Vg;
+      const int rc = hybrid_pair(c, g, p, fiterr, covars_pos, expl_pair, b, Sig, Vg, n_out, sstats);
+      if (rc) return rc;
+      /* Bayes factors (CalcAbfsHybridFor*) */
+      std::vector<int> gm(S, 1);
+      std::vector<double> gen(GL), fix(GL), mx(GL);
+      for (int k = 0; k < GL; ++k) {
+        gen[k] = hybrid_l10abf(S, gm, b, Sig, Vg, phiL[k], omaL[k]);
+        fix[k] = hybrid_l10abf(S, gm, b, Sig, Vg, 0.0, phiL[k] + omaL[k]);
+        mx[k] = hybrid_l10abf(S, gm, b, Sig, Vg, phiL[k] + omaL[k], 0.0);
+        raw[(int64_t)k * NP + p] = gen[k];
+        raw[(int64_t)(GL + k) * NP + p] = fix[k];
+        raw[(int64_t)(2 * GL + k) * NP + p] = mx[k];
+      }
+      std::vector<double> wv(n_w, kNaN);
+      wv[0] = orc_log10_weighted_sum(gen.data(), GL);
+      wv[1] = orc_log10_weighted_sum(fix.data(), GL);
+      wv[2] = orc_log10_weighted_sum(mx.data(), GL);
+      if (bfs != EQBF_BFS_GEN) {
+        std::vector<double> l(GS);
+        for (int64_t ci = 0; ci < nc; ++ci) {
+          for (int32_t s = 0; s < S; ++s) gm[s] = (masks[ci] >> s) & 1;
+          for (int k = 0; k < GS; ++k) {
+            l[k] = hybrid_l10abf(S, gm, b, Sig, Vg, phiS[k], omaS[k]);
+            raw[(3 * GL + ci * GS + k) * NP + p] = l[k];
+          }
+          wv[3 + ci] = orc_log10_weighted_sum(l.data(), GS);
+        }
+        double gs, al;
+        orc_bma(S, bfs, wv.data() + 3, wv[0], &gs, &al);
+        wv[3 + nc] = gs;
+        if (bfs == EQBF_BFS_ALL) wv[3 + nc + 1] = al;
+      }
+      for (int64_t k = 0; k < n_w; ++k) w_abf[k * NP + p] = wv[k];
+    }
+  }
+  return 0;
+}
+
+/* b [S], Sigma [S][S] and Vg [S][S] of pair p of gene g (test access). */
+extern "C" int orc_hybrid_pair_stats(const eqbf_chunk* c, int64_t g, int64_t p, double fiterr,
+                                     const double* covars_pos, const double* expl_pair, double* b, double* Sig,
+                                     double* Vg) {
+  std::vector<ldouble> bb;
+  Mat Sg, V;
+  const int rc = hybrid_pair(c, g, p, fiterr, covars_pos, expl_pair, bb, Sg, V, nullptr, nullptr);
+  if (rc) return rc;
+  const int S = c->n_subgroups;
+  for (int i = 0; i < S; ++i) {
+    b[i] = (double)bb[i];
+    for (int j = 0; j < S; ++j) {
+      Sig[i * S + j] = (double)Sg[i * S + j];
+      Vg[i * S + j] = (double)V[i * S + j];
+    }
+  }
+  return 0;
+}

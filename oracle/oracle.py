@@ -78,6 +78,10 @@ def lib():
         L.orc_hm_loglik.argtypes = [vp, vp, i64, i32, i32, i32, i32, vp, vp, vp, vp, vp]
         L.orc_hm_loglik.restype = d
         L.orc_hm_bfs.argtypes = [vp, vp, i64, i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.orc_assoc_hybrid.argtypes = [vp, i32, d, vp, vp, vp, vp, i32, vp, vp, i32, vp, vp, vp, vp]
+        L.orc_assoc_hybrid.restype = i32
+        L.orc_hybrid_pair_stats.argtypes = [vp, i64, i64, d, vp, vp, vp, vp, vp]
+        L.orc_hybrid_pair_stats.restype = i32
         L.orc_hm_ci.argtypes = [vp, vp, i64, i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp]
         L.orc_hm_ci.restype = i32
         L.orc_abf_from_sstats.argtypes = [i32, i64, vp, vp, vp, vp, i32, vp, vp, i32, vp, vp, i32, vp, vp, vp]
@@ -243,6 +247,43 @@ def assoc_mvlr(hc, bfs, fiterr, gridL, gridS):
     if rc != 0:
         raise RuntimeError("oracle: %s" % lib().orc_last_error().decode())
     return out
+
+
+def assoc_hybrid(hc, bfs, fiterr, gridL, gridS, covars_pos=None, expl_pair=None):
+    """--error hybrid on a chunk: n, summary statistics [5][S][NP], raw and
+    grid-averaged log10 ABFs.  covars_pos [S][Qmax][N]: covariate values by
+    file column (the pairwise designs read them there); None: the chunk's.
+    expl_pair [G][S][N]: expression levels of the pairwise step (the levels
+    before --qnorm); None: the chunk's."""
+    gL = np.ascontiguousarray(gridL, dtype=np.float64)
+    gS = np.ascontiguousarray(gridS if gridS is not None else gridL, dtype=np.float64)
+    out = _abi.HostAssocOut(hc.S, hc.n_pairs, bfs, gL.shape[0], gS.shape[0], want_sstats=True, want_std=False)
+    st = hc.struct()
+    phiL, omaL = np.ascontiguousarray(gL[:, 0]), np.ascontiguousarray(gL[:, 1])
+    phiS, omaS = np.ascontiguousarray(gS[:, 0]), np.ascontiguousarray(gS[:, 1])
+    cp = None if covars_pos is None else np.ascontiguousarray(covars_pos, dtype=np.float64)
+    ep = None if expl_pair is None else np.ascontiguousarray(expl_pair, dtype=np.float64)
+    rc = lib().orc_assoc_hybrid(C.addressof(st), _abi.BFS[bfs], fiterr, None if cp is None else _p(cp),
+                                None if ep is None else _p(ep), _p(phiL), _p(omaL), gL.shape[0], _p(phiS), _p(omaS), gS.shape[0], _p(out.n),
+                                _p(out.sstats), _p(out.raw), _p(out.w))
+    if rc != 0:
+        raise RuntimeError("oracle: %s" % lib().orc_last_error().decode())
+    return out
+
+
+def hybrid_pair_stats(hc, g, p, fiterr, covars_pos=None, expl_pair=None):
+    """b [S], Sigma [S][S] and Vg [S][S] of pair p (of gene g) under
+    --error hybrid."""
+    S = hc.S
+    b, Sig, Vg = np.zeros(S), np.zeros((S, S)), np.zeros((S, S))
+    cp = None if covars_pos is None else np.ascontiguousarray(covars_pos, dtype=np.float64)
+    st = hc.struct()
+    ep = None if expl_pair is None else np.ascontiguousarray(expl_pair, dtype=np.float64)
+    rc = lib().orc_hybrid_pair_stats(C.addressof(st), g, p, fiterr, None if cp is None else _p(cp),
+                                     None if ep is None else _p(ep), _p(b), _p(Sig), _p(Vg))
+    if rc != 0:
+        raise RuntimeError("oracle: %s" % lib().orc_last_error().decode())
+    return b, Sig, Vg
 
 
 def mvlr_abfs(Y, x, Xc, fiterr, gamma, grid):
