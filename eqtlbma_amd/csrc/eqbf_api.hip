@@ -19,6 +19,8 @@ static int check_flag(eqbf_ctx* ctx, hipStream_t st) {
   if (flag & FLAG_MVLR_SAMPLES)
     return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED,
                     "--error mvlr needs the same samples (genotyped, with expression levels) in every subgroup");
+  if (flag & FLAG_HYB_NO_COMMON)
+    return ctx_fail(ctx, EQBF_ERR_DATA, "two subgroups have no individuals in common (gene_snp_pair.cpp:895-900)");
   return EQBF_OK;
 }
 
@@ -297,6 +299,54 @@ extern "C" int eqbf_assoc_mvlr(eqbf_ctx* ctx, const eqbf_chunk* c, const eqbf_mv
       EQBF_HIP(ctx, hipMemcpyAsync(out->sstats, dout.sstats, 5 * S * NP * sizeof(double), hipMemcpyDeviceToHost, st));
     if (out->stdsstats)
       EQBF_HIP(ctx, hipMemcpyAsync(out->stdsstats, dout.stdsstats, 3 * S * NP * sizeof(double), hipMemcpyDeviceToHost, st));
+    EQBF_HIP(ctx, hipMemcpyAsync(out->raw_abf, dout.raw_abf, n_raw * NP * sizeof(double), hipMemcpyDeviceToHost, st));
+    EQBF_HIP(ctx, hipMemcpyAsync(out->w_abf, dout.w_abf, n_w * NP * sizeof(double), hipMemcpyDeviceToHost, st));
+  }
+  EQBF_HIP(ctx, hipStreamSynchronize(st));
+  return EQBF_OK;
+}
+
+extern "C" int eqbf_assoc_hybrid_device(eqbf_ctx* ctx, const eqbf_chunk* c, const eqbf_hybrid_opts* o,
+                                        eqbf_assoc_out* out, void* stream) {
+  if (!ctx || !c || !o || !out) return ctx_fail(ctx, EQBF_ERR_ARG, "null argument");
+  EQBF_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  return hybrid_run(ctx, c, o, out, st);
+}
+
+extern "C" int eqbf_assoc_hybrid(eqbf_ctx* ctx, const eqbf_chunk* c, const eqbf_hybrid_opts* o, eqbf_assoc_out* out) {
+  if (!ctx || !c || !o || !out || !out->n || !out->raw_abf || !out->w_abf)
+    return ctx_fail(ctx, EQBF_ERR_ARG, "null argument");
+  EQBF_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const int64_t G = c->n_genes, S = c->n_subgroups, N = c->n_samples, NP = c->n_pairs;
+  Stager sg{ctx, st};
+  eqbf_chunk d;
+  int rc = stage_chunk(ctx, sg, c, &d);
+  if (rc) return rc;
+  eqbf_hybrid_opts od = *o;
+  od.covars_by_position =
+      c->n_covars_max > 0 ? sg.put("h_hyb_cp", o->covars_by_position, (size_t)(S * c->n_covars_max * N)) : nullptr;
+  od.explevels_pairs = sg.put("h_hyb_yp", o->explevels_pairs, (size_t)(G * S * N));
+  const int GL = (int)ctx->phiL.size(), GS = (int)ctx->phiS.size();
+  const int64_t n_cfg = eqbf_n_configs(o->bfs, (int)S);
+  if (n_cfg < 0) return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED, "--bfs all with %d subgroups", (int)S);
+  const int64_t n_raw = eqbf_n_raw(o->bfs, (int)S, GL, GS), n_w = eqbf_n_weighted(o->bfs, (int)S);
+  eqbf_assoc_out dout;
+  memset(&dout, 0, sizeof dout);
+  dout.n = sg.alloc<int32_t>("o_n", (size_t)(S * NP), true);
+  dout.sstats = sg.alloc<double>("o_ss", (size_t)(5 * S * NP), out->sstats != nullptr);
+  dout.raw_abf = sg.alloc<double>("o_raw", (size_t)(n_raw * NP), true);
+  dout.w_abf = sg.alloc<double>("o_w", (size_t)(n_w * NP), true);
+  if (sg.rc) return ctx_fail(ctx, sg.rc, "allocating outputs failed");
+  rc = hybrid_run(ctx, &d, &od, &dout, st);
+  if (rc) return rc;
+  rc = check_flag(ctx, st);
+  if (rc) return rc;
+  if (NP > 0) {
+    EQBF_HIP(ctx, hipMemcpyAsync(out->n, dout.n, S * NP * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    if (out->sstats)
+      EQBF_HIP(ctx, hipMemcpyAsync(out->sstats, dout.sstats, 5 * S * NP * sizeof(double), hipMemcpyDeviceToHost, st));
     EQBF_HIP(ctx, hipMemcpyAsync(out->raw_abf, dout.raw_abf, n_raw * NP * sizeof(double), hipMemcpyDeviceToHost, st));
     EQBF_HIP(ctx, hipMemcpyAsync(out->w_abf, dout.w_abf, n_w * NP * sizeof(double), hipMemcpyDeviceToHost, st));
   }

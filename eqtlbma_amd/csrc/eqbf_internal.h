@@ -124,6 +124,7 @@ struct PrepRec {
 #define PREP_TOO_FEW 1
 #define FLAG_MISSING_COVAR 1
 #define FLAG_MVLR_SAMPLES 2  /* --error mvlr: subgroups with different samples */
+#define FLAG_HYB_NO_COMMON 4 /* --error hybrid: two subgroups without a common individual */
 
 /* ---- device special functions (FP64) ------------------------------------- */
 namespace dm {
@@ -379,6 +380,7 @@ int uvlr_i8_sums_run(eqbf_ctx* ctx, const eqbf_chunk& c, hipStream_t st, const P
                      const uint64_t* memb, const int4* tiles, const int* n_tiles, int KU, double* sx2,
                      uint32_t* redo);
 int mvlr_run(eqbf_ctx* ctx, const eqbf_chunk* c, const eqbf_mvlr_opts* o, eqbf_assoc_out* out, hipStream_t st);
+int hybrid_run(eqbf_ctx* ctx, const eqbf_chunk* c, const eqbf_hybrid_opts* o, eqbf_assoc_out* out, hipStream_t st);
 /* --inss: Bayes factors from given summary statistics (device pointers) */
 int abf_from_sstats_run(eqbf_ctx* ctx, const eqbf_sstats_in* in, int bfs, int raw_dtype, eqbf_assoc_out* out,
                         hipStream_t st);

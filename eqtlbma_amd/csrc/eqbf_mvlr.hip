@@ -606,11 +606,66 @@ __global__ void __launch_bounds__(256) mvlr_wavg_kernel(const double* __restrict
   w[wr * NP + p] = res;
 }
 
+/* Raw and grid-averaged rows of every configuration (the consistent one
+ * first), then the BMA rows.  fill(c0, nb, par, PSZ, pair_blocks) enqueues
+ * the kernel writing the parameters of configurations [c0, c0 + nb) in the
+ * layout mvlr_grid_kernel reads: [cfg - c0][PSZ][NP] with Sigma^-1 [S][S],
+ * Sigma^-1 t, t, eVb, k and a flag (0 = computed).  Configurations go in
+ * batches bounded by the parameter buffer and by the 65535 limit of the
+ * grid's y dimension. */
+template <int SM, class Fill>
+int mv_rows_run(eqbf_ctx* ctx, const eqbf_chunk& c, int bfs, eqbf_assoc_out* out, hipStream_t st, Fill fill) {
+  const int S = c.n_subgroups;
+  const int64_t NP = c.n_pairs;
+  const int GL = (int)ctx->phiL.size(), GS = (int)ctx->phiS.size();
+  const int64_t n_cfg = eqbf_n_configs(bfs, S);
+  const int PSZ = S * S + 3 * S + 2;
+  const int64_t total = 1 + n_cfg;
+  int64_t per = std::max<int64_t>(1, ((int64_t)1 << 30) / std::max<int64_t>(1, NP * PSZ * 8));
+  per = std::min<int64_t>(std::min<int64_t>(per, 65535), total);
+  double* par = (double*)ctx_buf(ctx, "mv_par", (size_t)per * NP * PSZ * sizeof(double));
+  if (!par) return ctx_fail(ctx, EQBF_ERR_HIP, "allocating mvlr buffers failed");
+  const int64_t pair_blocks = (NP + 63) / 64;
+  for (int64_t c0 = 0; c0 < total; c0 += per) {
+    const int nb = (int)std::min<int64_t>(per, total - c0);
+    fill(c0, nb, par, PSZ, pair_blocks);
+    /* raw rows of configurations [c0, c0 + nb) and their weighted rows */
+    const int64_t c1 = c0 + nb;
+    const int64_t row0 = c0 == 0 ? 0 : 3 * (int64_t)GL + (c0 - 1) * GS;
+    const int64_t row1 = 3 * (int64_t)GL + (c1 - 1) * GS;
+    const int64_t wrow0 = c0 == 0 ? 0 : 3 + (c0 - 1);
+    const int64_t wrow1 = 3 + (c1 - 1);
+    for (int64_t r0 = row0; r0 < row1;) {
+      const int64_t nr = std::min<int64_t>(row1 - r0, std::max<int64_t>(1, ((int64_t)1 << 30) / pair_blocks));
+      MvGrid mg;
+      mg.par = par;
+      mg.grid = (const double*)ctx->d_grid.ptr;
+      mg.raw = (double*)out->raw_abf;
+      mg.GL = GL;
+      mg.GS = GS;
+      mg.cfg0 = (int)c0;
+      mg.PSZ = PSZ;
+      mg.row0 = r0;
+      mg.nrows = nr;
+      mg.pair_blocks = pair_blocks;
+      {
+        ProfScope ps(ctx, "mvlr_grid_kernel", st);
+        hipLaunchKernelGGL(mvlr_grid_kernel<SM>, dim3((unsigned)(nr * pair_blocks)), dim3(64), 0, st, c, mg);
+      }
+      r0 += nr;
+    }
+    const int64_t nw = wrow1 - wrow0;
+    hipLaunchKernelGGL(mvlr_wavg_kernel, dim3((unsigned)((nw * NP + 255) / 256)), dim3(256), 0, st,
+                       (const double*)out->raw_abf, out->w_abf, NP, GL, GS, wrow0, nw);
+    EQBF_HIP(ctx, hipGetLastError());
+  }
+  return launch_bma(ctx, out->w_abf, NP, S, bfs, st);
+}
+
 template <int SM>
 int mvlr_run_sm(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_mvlr_opts* o, eqbf_assoc_out* out, hipStream_t st) {
   const int S = c.n_subgroups, N = c.n_samples, G = c.n_genes;
   const int64_t NP = c.n_pairs;
-  const int GL = (int)ctx->phiL.size(), GS = (int)ctx->phiS.size();
   const int64_t n_cfg = eqbf_n_configs(o->bfs, S);
   const int KU = 1 + c.n_covars_max;
   MvPrep pp;
@@ -658,17 +713,7 @@ int mvlr_run_sm(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_mvlr_opts* o, eqb
     hipLaunchKernelGGL(mvlr_sums_kernel<SM>, dim3((unsigned)((NP + 255) / 256)), dim3(256), 0, st, c, ms);
   }
   EQBF_HIP(ctx, hipGetLastError());
-  /* configurations in batches bounded by the parameter buffer (and by the
-   * 65535 limit of the grid's y dimension) */
-  const int PSZ = S * S + 3 * S + 2;
-  const int64_t total = 1 + n_cfg;
-  int64_t per = std::max<int64_t>(1, ((int64_t)1 << 30) / std::max<int64_t>(1, NP * PSZ * 8));
-  per = std::min<int64_t>(std::min<int64_t>(per, 65535), total);
-  double* par = (double*)ctx_buf(ctx, "mv_par", (size_t)per * NP * PSZ * sizeof(double));
-  if (!par) return ctx_fail(ctx, EQBF_ERR_HIP, "allocating mvlr buffers failed");
-  const int64_t pair_blocks = (NP + 63) / 64;
-  for (int64_t c0 = 0; c0 < total; c0 += per) {
-    const int nb = (int)std::min<int64_t>(per, total - c0);
+  return mv_rows_run<SM>(ctx, c, o->bfs, out, st, [&](int64_t c0, int nb, double* par, int PSZ, int64_t pair_blocks) {
     MvCfg mc;
     mc.mrec = pp.mrec;
     mc.YtY = pp.YtY;
@@ -682,41 +727,574 @@ int mvlr_run_sm(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_mvlr_opts* o, eqb
     mc.fiterr = o->fiterr;
     mc.cfg0 = (int)c0;
     mc.PSZ = PSZ;
-    {
-      ProfScope ps(ctx, "mvlr_cfg_kernel", st);
-      hipLaunchKernelGGL(mvlr_cfg_kernel<SM>, dim3((unsigned)pair_blocks, (unsigned)nb), dim3(64), 0, st, c, mc);
-    }
-    /* raw rows of configurations [c0, c0 + nb) and their weighted rows */
-    const int64_t c1 = c0 + nb;
-    const int64_t row0 = c0 == 0 ? 0 : 3 * (int64_t)GL + (c0 - 1) * GS;
-    const int64_t row1 = 3 * (int64_t)GL + (c1 - 1) * GS;
-    const int64_t wrow0 = c0 == 0 ? 0 : 3 + (c0 - 1);
-    const int64_t wrow1 = 3 + (c1 - 1);
-    for (int64_t r0 = row0; r0 < row1;) {
-      const int64_t nr = std::min<int64_t>(row1 - r0, std::max<int64_t>(1, ((int64_t)1 << 30) / pair_blocks));
-      MvGrid mg;
-      mg.par = par;
-      mg.grid = (const double*)ctx->d_grid.ptr;
-      mg.raw = (double*)out->raw_abf;
-      mg.GL = GL;
-      mg.GS = GS;
-      mg.cfg0 = (int)c0;
-      mg.PSZ = PSZ;
-      mg.row0 = r0;
-      mg.nrows = nr;
-      mg.pair_blocks = pair_blocks;
-      {
-        ProfScope ps(ctx, "mvlr_grid_kernel", st);
-        hipLaunchKernelGGL(mvlr_grid_kernel<SM>, dim3((unsigned)(nr * pair_blocks)), dim3(64), 0, st, c, mg);
-      }
-      r0 += nr;
-    }
-    const int64_t nw = wrow1 - wrow0;
-    hipLaunchKernelGGL(mvlr_wavg_kernel, dim3((unsigned)((nw * NP + 255) / 256)), dim3(256), 0, st,
-                       (const double*)out->raw_abf, out->w_abf, NP, GL, GS, wrow0, nw);
-    EQBF_HIP(ctx, hipGetLastError());
+    ProfScope ps(ctx, "mvlr_cfg_kernel", st);
+    hipLaunchKernelGGL(mvlr_cfg_kernel<SM>, dim3((unsigned)pair_blocks, (unsigned)nb), dim3(64), 0, st, c, mc);
+  });
+}
+
+/* ==== --error hybrid ================================================================
+ * GeneSnpPair::CalcAbfsHybrid (gene_snp_pair.cpp:1384-1423) and what it
+ * calls (:760-1163), with the Bayes factor of CalcLog10AbfMvlr
+ * (:1165-1256).  Per pair of a gene expressed in every subgroup and a SNP
+ * genotyped in every subgroup:
+ *   b_s, Sigma_ss, Vg_ss from the subgroup's own regression over its members
+ *     (present: genotyped, expression level not NaN) on [1, x, covariates],
+ *     sigma2 = rss / n, Sigma_ss = f s2_full + (1 - f) s2_null and
+ *     Vg_ss = Sigma_ss [(X'X)^-1]_xx;
+ *   for s1 < s2, over the individuals present in both (rows [1, x, covariates
+ *     of s1 read at the individual's index]): Sigma_12 = f full_01 + (1 - f)
+ *     null_01 from the ML error covariances Y'(I - H)Y / n12, and
+ *     Vg_12 = Sigma_12 [M1^-1 G12 M2^-1]_xx with G12 = X12'X12 and M_s the
+ *     same sums over every individual present in s (the reference's
+ *     pinv(G12 + Xs'Xs) X12' products, :977-1131);
+ *   log10 ABF = (b'Vg^-1 W (I + Vg^-1 W)^-1 Vg^-1 b - ln|det(I + Vg^-1 W)|)
+ *     / (2 ln 10), W = (oma2 1 1' + phi2 I) o gamma gamma' o sd sd',
+ *     sd_s = sqrt(Sigma_ss): the MVLR row kernel with k = 1 and Vg in place
+ *     of Sigma.
+ * Every sum over individuals splits into a covariate block that does not
+ * depend on the SNP (per gene: A = sum c c', its inverse, the null
+ * regression coefficients psi = A^-1 sum c y and the null residual
+ * cross-product, hyb_gene_kernel) and SNP terms (u = sum c x, sum x^2,
+ * sum x y, hyb_pair_kernel).  With w = A^-1 u and k = sum x^2 - u'w (the
+ * Schur complement, x~'x~ of Frisch-Waugh), the genotype column of M^-1 is
+ * (-w, 1) / k, so
+ *   [(X'X)^-1]_xx = 1 / k,  x~'y~ = sum x y - u'psi,
+ *   rss_full = rss_null - (x~'y~)^2 / k,
+ *   [M1^-1 G12 M2^-1]_xx = (w1'A12 w2 - w1'u12 - u12'w2 + sum_12 x^2) / (k1 k2)
+ * and no (Q+2) x (Q+2) system is solved per pair.  The reference's
+ * pseudo-inverses invert every singular value, so on full-rank designs they
+ * are these inverses; the covariate blocks are inverted by Gauss-Jordan
+ * elimination with partial pivoting. */
+struct HybGene {
+  const double* cp;  /* pairwise covariates [S][Qmax][N] (by index in the samples) */
+  const double* yp;  /* pairwise expression levels [G][S][N]                       */
+  double* rec;       /* [G][NSET][RSZ]                                             */
+  int NSET, RSZ, Q1M;
+};
+
+/* set j of a gene: 0 <= j < S: members of s with its own covariates and
+ * levels; S + s1: members of s1 with the pairwise covariates of s1;
+ * 2S + pi: present in s1 and s2 (pair pi = (s1, s2), s1 < s2 in
+ * lexicographic order) with the pairwise covariates and levels;
+ * 2S + npr + pi: present in s2 with the pairwise covariates of s1. */
+__device__ inline void hyb_pair_of(int pi, int S, int& s1, int& s2) {
+  s1 = 0;
+  while (pi >= S - 1 - s1) {
+    pi -= S - 1 - s1;
+    ++s1;
   }
-  return launch_bma(ctx, out->w_abf, NP, S, o->bfs, st);
+  s2 = s1 + 1 + pi;
+}
+
+__device__ inline bool hyb_present(const eqbf_chunk& c, int g, int s, int i) {
+  const int S = c.n_subgroups, N = c.n_samples;
+  return c.geno_in_sbgrp[(int64_t)s * N + i] && !isnan(c.explevels[((int64_t)g * S + s) * N + i]);
+}
+
+/* record of a set: A^-1 [Q1M][Q1M], A [Q1M][Q1M], psi1 [Q1M], psi2 [Q1M],
+ * r (null residual cross-product y1'(I - Hc)y2), n */
+__global__ void __launch_bounds__(256) hyb_gene_kernel(const eqbf_chunk c, const HybGene hg) {
+  const int g = blockIdx.x, j = blockIdx.y, t = threadIdx.x;
+  const int S = c.n_subgroups, N = c.n_samples, QM = c.n_covars_max;
+  for (int s = 0; s < S; ++s)
+    if (!c.gene_in_sbgrp[(int64_t)g * S + s]) return;
+  const int npr = S * (S - 1) / 2;
+  int type, sa, sb = -1;
+  if (j < S) {
+    type = 0;
+    sa = j;
+  } else if (j < 2 * S) {
+    type = 1;
+    sa = j - S;
+  } else if (j < 2 * S + npr) {
+    type = 2;
+    hyb_pair_of(j - 2 * S, S, sa, sb);
+  } else {
+    type = 3;
+    hyb_pair_of(j - 2 * S - npr, S, sa, sb);
+  }
+  const int Q1 = min(max(c.covar_count[sa], 0), QM) + 1, Q1M = hg.Q1M;
+  const double* cov = (type == 0) ? c.covars + (int64_t)sa * QM * N : hg.cp + (int64_t)sa * QM * N;
+  const bool has_y = type == 0 || type == 2;
+  const double* y1 = (type == 0) ? c.explevels + ((int64_t)g * S + sa) * N : hg.yp + ((int64_t)g * S + sa) * N;
+  const double* y2 = (type == 0) ? y1 : (type == 2 ? hg.yp + ((int64_t)g * S + sb) * N : nullptr);
+  /* entries: the upper triangle of A, then sum c y1, sum c y2, sum y1 y2, n */
+  const int nA = Q1 * (Q1 + 1) / 2;
+  const int nE = nA + (has_y ? 2 * Q1 + 1 : 0) + 1;
+  __shared__ double ent[17 * 18 / 2 + 2 * 17 + 2];
+  __shared__ double M[17][34];
+  __shared__ double fac[17];
+  __shared__ int pivrow;
+  if (t < nE) {
+    int kind, a = 0, b = 0;
+    if (t < nA) {
+      kind = 0;
+      int r = t;
+      while (r >= Q1 - a) {
+        r -= Q1 - a;
+        ++a;
+      }
+      b = a + r;
+    } else if (has_y && t < nA + Q1) {
+      kind = 1;
+      a = t - nA;
+    } else if (has_y && t < nA + 2 * Q1) {
+      kind = 2;
+      a = t - nA - Q1;
+    } else if (has_y && t == nA + 2 * Q1) {
+      kind = 3;
+    } else {
+      kind = 4;
+    }
+    double acc = 0.0;
+    for (int i = 0; i < N; ++i) {
+      bool in;
+      if (type == 0 || type == 1)
+        in = hyb_present(c, g, sa, i);
+      else if (type == 2)
+        in = hyb_present(c, g, sa, i) && hyb_present(c, g, sb, i);
+      else
+        in = hyb_present(c, g, sb, i);
+      if (!in) continue;
+      const double va = a == 0 ? 1.0 : cov[(int64_t)(a - 1) * N + i];
+      switch (kind) {
+        case 0: acc += va * (b == 0 ? 1.0 : cov[(int64_t)(b - 1) * N + i]); break;
+        case 1: acc += va * y1[i]; break;
+        case 2: acc += va * y2[i]; break;
+        case 3: acc += y1[i] * y2[i]; break;
+        default: acc += 1.0; break;
+      }
+    }
+    ent[t] = acc;
+  }
+  __syncthreads();
+  /* [A | I] */
+  for (int e = t; e < Q1 * 2 * Q1; e += blockDim.x) {
+    const int r = e / (2 * Q1), col = e - r * 2 * Q1;
+    double v;
+    if (col < Q1) {
+      const int a = min(r, col), b = max(r, col);
+      v = ent[a * Q1 - a * (a - 1) / 2 + (b - a)];
+    } else {
+      v = (col - Q1 == r) ? 1.0 : 0.0;
+    }
+    M[r][col] = v;
+  }
+  __syncthreads();
+  for (int jc = 0; jc < Q1; ++jc) {
+    if (t == 0) {
+      int p = jc;
+      double best = fabs(M[jc][jc]);
+      for (int r = jc + 1; r < Q1; ++r)
+        if (fabs(M[r][jc]) > best) {
+          best = fabs(M[r][jc]);
+          p = r;
+        }
+      pivrow = p;
+    }
+    __syncthreads();
+    const int p = pivrow;
+    if (p != jc && t < 2 * Q1) {
+      const double tmp = M[jc][t];
+      M[jc][t] = M[p][t];
+      M[p][t] = tmp;
+    }
+    __syncthreads();
+    if (t < Q1) fac[t] = (t == jc) ? M[jc][jc] : M[t][jc];
+    __syncthreads();
+    if (t < 2 * Q1) M[jc][t] /= fac[jc];
+    __syncthreads();
+    for (int e = t; e < Q1 * 2 * Q1; e += blockDim.x) {
+      const int r = e / (2 * Q1), col = e - r * 2 * Q1;
+      if (r != jc) M[r][col] -= fac[r] * M[jc][col];
+    }
+    __syncthreads();
+  }
+  const int RSZ = hg.RSZ;
+  double* out = hg.rec + ((int64_t)g * hg.NSET + j) * RSZ;
+  for (int e = t; e < Q1M * Q1M; e += blockDim.x) {
+    const int r = e / Q1M, col = e - r * Q1M;
+    const bool in = r < Q1 && col < Q1;
+    out[e] = in ? M[r][Q1 + col] : 0.0;
+    if (in) {
+      const int a = min(r, col), b = max(r, col);
+      out[Q1M * Q1M + e] = ent[a * Q1 - a * (a - 1) / 2 + (b - a)];
+    } else {
+      out[Q1M * Q1M + e] = 0.0;
+    }
+  }
+  if (t < Q1M) {
+    double p1 = 0.0, p2 = 0.0;
+    if (has_y && t < Q1)
+      for (int k = 0; k < Q1; ++k) {
+        p1 += M[t][Q1 + k] * ent[nA + k];
+        p2 += M[t][Q1 + k] * ent[nA + Q1 + k];
+      }
+    out[2 * Q1M * Q1M + t] = p1;
+    out[2 * Q1M * Q1M + Q1M + t] = p2;
+  }
+  if (t == 0) {
+    double r = 0.0;
+    if (has_y) {
+      r = ent[nA + 2 * Q1];
+      for (int k = 0; k < Q1; ++k) {
+        double p2 = 0.0;
+        for (int l = 0; l < Q1; ++l) p2 += M[k][Q1 + l] * ent[nA + Q1 + l];
+        r -= ent[nA + k] * p2;
+      }
+    }
+    out[2 * Q1M * Q1M + 2 * Q1M] = r;
+    out[2 * Q1M * Q1M + 2 * Q1M + 1] = ent[nE - 1];
+  }
+}
+
+struct HybTile {
+  int64_t p0;
+  int32_t g, n;
+};
+
+struct HybPair {
+  const HybTile* tiles;
+  const double* cp;
+  const double* yp;
+  const double* rec;
+  double* b;      /* [S][NP]    */
+  double* Sig;    /* [S*S][NP]  */
+  double* Vg;     /* [S*S][NP]  */
+  int32_t* pgene; /* [NP]       */
+  int32_t* n_out;
+  double* sstats;
+  int* flag;
+  double fiterr;
+  int NSET, RSZ;
+};
+
+template <int Q1M>
+__device__ inline double hyb_quad(const double* __restrict__ Ai, const double (&u)[Q1M], const double (&v)[Q1M],
+                                  int Q1) {
+  double s = 0.0;
+#pragma unroll
+  for (int a = 0; a < Q1M; ++a) {
+    if (a >= Q1) break;
+    double r = 0.0;
+#pragma unroll
+    for (int b = 0; b < Q1M; ++b)
+      if (b < Q1) r += Ai[a * Q1M + b] * v[b];
+    s += u[a] * r;
+  }
+  return s;
+}
+
+__device__ inline double hyb_geno(const eqbf_chunk& c, int s, int i, int64_t snp) {
+  const int64_t off = (int64_t)c.geno_set_of_sbgrp[s] * c.geno_set_stride + (int64_t)i * c.geno_ld + snp;
+  return c.geno_dtype == EQBF_F32 ? (double)((const float*)c.genotypes)[off] : ((const double*)c.genotypes)[off];
+}
+
+/* one lane per pair; the 64 pairs of a block belong to one gene, so every
+ * test of membership and every covariate and level read is uniform */
+template <int Q1M>
+__global__ void __launch_bounds__(64) hyb_pair_kernel(const eqbf_chunk c, const HybPair hp) {
+  const HybTile tl = hp.tiles[blockIdx.x];
+  const int g = tl.g;
+  if ((int)threadIdx.x >= tl.n) return;
+  const int64_t p = tl.p0 + threadIdx.x;
+  const int64_t NP = c.n_pairs;
+  const int S = c.n_subgroups, N = c.n_samples, QM = c.n_covars_max;
+  const int64_t snp = c.pair_snp[p];
+  bool ok = true;
+  for (int s = 0; s < S; ++s) ok &= c.gene_in_sbgrp[(int64_t)g * S + s] && c.snp_in_sbgrp[snp * S + s];
+  if (!ok) {
+    /* gene.cpp:314-321 skips the pair: no sample size, no statistics */
+    for (int s = 0; s < S; ++s) {
+      hp.n_out[(int64_t)s * NP + p] = 0;
+      if (hp.sstats)
+        for (int k = 0; k < 5; ++k) hp.sstats[((int64_t)k * S + s) * NP + p] = NAN;
+    }
+    hp.pgene[p] = -1;
+    return;
+  }
+  const int npr = S * (S - 1) / 2;
+  const int RSZ = hp.RSZ;
+  const double* R = hp.rec + (int64_t)g * hp.NSET * RSZ;
+  const double f = hp.fiterr;
+  int pi = 0;
+  for (int s1 = 0; s1 < S; ++s1) {
+    const int Q1 = min(max(c.covar_count[s1], 0), QM) + 1;
+    const double* cov0 = c.covars + (int64_t)s1 * QM * N;
+    const double* covp = hp.cp + (int64_t)s1 * QM * N;
+    const double* y = c.explevels + ((int64_t)g * S + s1) * N;
+    double u0[Q1M], u1[Q1M], sxx = 0.0, sxy = 0.0;
+#pragma unroll
+    for (int a = 0; a < Q1M; ++a) u0[a] = u1[a] = 0.0;
+    for (int i = 0; i < N; ++i) {
+      if (!hyb_present(c, g, s1, i)) continue;
+      const double x = hyb_geno(c, s1, i, snp);
+      sxx += x * x;
+      sxy += x * y[i];
+      u0[0] += x;
+      u1[0] += x;
+#pragma unroll
+      for (int a = 1; a < Q1M; ++a)
+        if (a < Q1) {
+          u0[a] += cov0[(int64_t)(a - 1) * N + i] * x;
+          u1[a] += covp[(int64_t)(a - 1) * N + i] * x;
+        }
+    }
+    const double* R0 = R + (int64_t)s1 * RSZ;
+    const double* R1 = R + (int64_t)(S + s1) * RSZ;
+    const double k0 = sxx - hyb_quad<Q1M>(R0, u0, u0, Q1);
+    double xty = sxy;
+#pragma unroll
+    for (int a = 0; a < Q1M; ++a)
+      if (a < Q1) xty -= u0[a] * R0[2 * Q1M * Q1M + a];
+    const double rss0 = R0[2 * Q1M * Q1M + 2 * Q1M], n0 = R0[2 * Q1M * Q1M + 2 * Q1M + 1];
+    const double rss1 = rss0 - xty * xty / k0;
+    const double sg = f * rss1 / n0 + (1.0 - f) * rss0 / n0;
+    hp.b[(int64_t)s1 * NP + p] = xty / k0;
+    hp.Sig[(int64_t)(s1 * S + s1) * NP + p] = sg;
+    hp.Vg[(int64_t)(s1 * S + s1) * NP + p] = sg / k0;
+    /* w1 = A1^-1 u1, k1: the genotype column of M1^-1 is (-w1, 1) / k1 */
+    double w1[Q1M];
+#pragma unroll
+    for (int a = 0; a < Q1M; ++a) {
+      double r = 0.0;
+#pragma unroll
+      for (int b2 = 0; b2 < Q1M; ++b2)
+        if (a < Q1 && b2 < Q1) r += R1[a * Q1M + b2] * u1[b2];
+      w1[a] = r;
+    }
+    double k1 = sxx;
+#pragma unroll
+    for (int a = 0; a < Q1M; ++a)
+      if (a < Q1) k1 -= u1[a] * w1[a];
+    const double* yp1 = hp.yp + ((int64_t)g * S + s1) * N;
+    for (int s2 = s1 + 1; s2 < S; ++s2, ++pi) {
+      const double* yp2 = hp.yp + ((int64_t)g * S + s2) * N;
+      double u12[Q1M], u2[Q1M], sxx12 = 0.0, sxx2 = 0.0, sy1 = 0.0, sy2 = 0.0;
+#pragma unroll
+      for (int a = 0; a < Q1M; ++a) u12[a] = u2[a] = 0.0;
+      for (int i = 0; i < N; ++i) {
+        if (!hyb_present(c, g, s2, i)) continue;
+        const bool both = hyb_present(c, g, s1, i);
+        const double x = hyb_geno(c, s1, i, snp);
+        double cv[Q1M];
+        cv[0] = 1.0;
+#pragma unroll
+        for (int a = 1; a < Q1M; ++a) cv[a] = (a < Q1) ? covp[(int64_t)(a - 1) * N + i] : 0.0;
+        sxx2 += x * x;
+#pragma unroll
+        for (int a = 0; a < Q1M; ++a) u2[a] += cv[a] * x;
+        if (both) {
+          sxx12 += x * x;
+          sy1 += x * yp1[i];
+          sy2 += x * yp2[i];
+#pragma unroll
+          for (int a = 0; a < Q1M; ++a) u12[a] += cv[a] * x;
+        }
+      }
+      const double* R2 = R + (int64_t)(2 * S + pi) * RSZ;
+      const double* R3 = R + (int64_t)(2 * S + npr + pi) * RSZ;
+      const double n12 = R2[2 * Q1M * Q1M + 2 * Q1M + 1];
+      if (!(n12 > 0.0)) atomicOr(hp.flag, FLAG_HYB_NO_COMMON);
+      const double k12 = sxx12 - hyb_quad<Q1M>(R2, u12, u12, Q1);
+      double xty1 = sy1, xty2 = sy2;
+#pragma unroll
+      for (int a = 0; a < Q1M; ++a)
+        if (a < Q1) {
+          xty1 -= u12[a] * R2[2 * Q1M * Q1M + a];
+          xty2 -= u12[a] * R2[2 * Q1M * Q1M + Q1M + a];
+        }
+      const double s12 = R2[2 * Q1M * Q1M + 2 * Q1M] / n12 - f * xty1 * xty2 / (k12 * n12);
+      double w2[Q1M];
+#pragma unroll
+      for (int a = 0; a < Q1M; ++a) {
+        double r = 0.0;
+#pragma unroll
+        for (int b2 = 0; b2 < Q1M; ++b2)
+          if (a < Q1 && b2 < Q1) r += R3[a * Q1M + b2] * u2[b2];
+        w2[a] = r;
+      }
+      double k2 = sxx2, wu = 0.0;
+#pragma unroll
+      for (int a = 0; a < Q1M; ++a)
+        if (a < Q1) {
+          k2 -= u2[a] * w2[a];
+          wu += w1[a] * u12[a] + u12[a] * w2[a];
+        }
+      const double aw = hyb_quad<Q1M>(R2 + Q1M * Q1M, w1, w2, Q1);
+      const double a11 = (aw - wu + sxx12) / (k1 * k2);
+      hp.Sig[(int64_t)(s1 * S + s2) * NP + p] = s12;
+      hp.Sig[(int64_t)(s2 * S + s1) * NP + p] = s12;
+      hp.Vg[(int64_t)(s1 * S + s2) * NP + p] = s12 * a11;
+      hp.Vg[(int64_t)(s2 * S + s1) * NP + p] = s12 * a11;
+    }
+  }
+  hp.pgene[p] = g;
+}
+
+/* per (pair, configuration): Vg^-1, Vg^-1 t, t = gamma o sqrt(diag Sigma),
+ * Vg^-1 b and k = 1, in the layout of mvlr_grid_kernel */
+struct HybCfg {
+  const double* b;
+  const double* Sig;
+  const double* Vg;
+  const int32_t* pgene;
+  const uint64_t* masks;
+  double* par;
+  int cfg0, PSZ;
+};
+
+template <int SM>
+__global__ void __launch_bounds__(64) hyb_cfg_kernel(const eqbf_chunk c, const HybCfg hc) {
+  const int64_t NP = c.n_pairs;
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= NP) return;
+  const int ci = blockIdx.y;
+  const int S = c.n_subgroups, PSZ = hc.PSZ;
+  double* out = hc.par + (int64_t)ci * PSZ * NP + p;
+  if (hc.pgene[p] < 0) {
+    out[(int64_t)(PSZ - 1) * NP] = 1.0;
+    return;
+  }
+  const uint64_t gm = hc.masks[hc.cfg0 + ci];
+  double A[SM][SM], b[SM], tv[SM];
+#pragma unroll
+  for (int i = 0; i < SM; ++i) {
+    b[i] = (i < S) ? hc.b[(int64_t)i * NP + p] : 0.0;
+    tv[i] = (i < S && ((gm >> i) & 1ull)) ? sqrt(hc.Sig[(int64_t)(i * S + i) * NP + p]) : 0.0;
+#pragma unroll
+    for (int j = 0; j < SM; ++j) A[i][j] = (i < S && j < S) ? hc.Vg[(int64_t)(i * S + j) * NP + p] : (i == j ? 1.0 : 0.0);
+  }
+  int piv[SM];
+  (void)lu_factor<SM>(A, piv);
+  double u[SM], eVb[SM];
+#pragma unroll
+  for (int i = 0; i < SM; ++i) u[i] = eVb[i] = 0.0;
+  for (int col = 0; col < S; ++col) {
+    double e[SM];
+#pragma unroll
+    for (int i = 0; i < SM; ++i) e[i] = (i == col) ? 1.0 : 0.0;
+    lu_apply<SM>(A, piv, e);
+    double tc = 0, bc = 0;
+#pragma unroll
+    for (int j = 0; j < SM; ++j)
+      if (j == col) {
+        tc = tv[j];
+        bc = b[j];
+      }
+#pragma unroll
+    for (int i = 0; i < SM; ++i)
+      if (i < S) {
+        out[(int64_t)(i * S + col) * NP] = e[i];
+        u[i] += e[i] * tc;
+        eVb[i] += e[i] * bc;
+      }
+  }
+#pragma unroll
+  for (int i = 0; i < SM; ++i)
+    if (i < S) {
+      out[(int64_t)(S * S + i) * NP] = u[i];
+      out[(int64_t)(S * S + S + i) * NP] = tv[i];
+      out[(int64_t)(S * S + 2 * S + i) * NP] = eVb[i];
+    }
+  out[(int64_t)(S * S + 3 * S) * NP] = 1.0;
+  out[(int64_t)(PSZ - 1) * NP] = 0.0;
+}
+
+template <int SM, int Q1M>
+int hybrid_run_sm(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_hybrid_opts* o, eqbf_assoc_out* out,
+                  hipStream_t st) {
+  const int S = c.n_subgroups, N = c.n_samples, G = c.n_genes;
+  const int64_t NP = c.n_pairs;
+  const int64_t n_cfg = eqbf_n_configs(o->bfs, S);
+  /* sample sizes and summary statistics: the separate analysis (FP64) */
+  eqbf_assoc_opts so{EQBF_ANALYSIS_SEP, EQBF_BFS_GEN, EQBF_PREC_FP64_BATCHED, EQBF_F64};
+  eqbf_assoc_out sout;
+  memset(&sout, 0, sizeof sout);
+  sout.n = out->n;
+  sout.sstats = out->sstats;
+  int rc = assoc_uvlr_run(ctx, &c, &so, &sout, st);
+  if (rc) return rc;
+  const int npr = S * (S - 1) / 2;
+  const int NSET = 2 * S + 2 * npr;
+  const int RSZ = 2 * Q1M * Q1M + 2 * Q1M + 2;
+  /* tiles of at most 64 pairs of one gene */
+  std::vector<int64_t> off(G + 1);
+  EQBF_HIP(ctx, hipMemcpyAsync(off.data(), c.pair_offset, (G + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  EQBF_HIP(ctx, hipStreamSynchronize(st));
+  std::vector<HybTile> tiles;
+  for (int g = 0; g < G; ++g)
+    for (int64_t p0 = off[g]; p0 < off[g + 1]; p0 += 64)
+      tiles.push_back(HybTile{p0, g, (int32_t)std::min<int64_t>(64, off[g + 1] - p0)});
+  if (tiles.empty()) return EQBF_OK;
+  HybTile* dtiles = (HybTile*)ctx_buf(ctx, "hyb_tiles", tiles.size() * sizeof(HybTile));
+  double* rec = (double*)ctx_buf(ctx, "hyb_rec", (size_t)G * NSET * RSZ * sizeof(double));
+  double* bb = (double*)ctx_buf(ctx, "hyb_b", (size_t)S * NP * sizeof(double));
+  double* sig = (double*)ctx_buf(ctx, "hyb_sig", (size_t)S * S * NP * sizeof(double));
+  double* vg = (double*)ctx_buf(ctx, "hyb_vg", (size_t)S * S * NP * sizeof(double));
+  int32_t* pgene = (int32_t*)ctx_buf(ctx, "mv_pgene", (size_t)NP * sizeof(int32_t));
+  std::vector<uint64_t> hm(1 + n_cfg);
+  hm[0] = (S >= 64) ? ~0ull : ((1ull << S) - 1ull);
+  if (n_cfg > 0) eqbf_config_masks(o->bfs, S, hm.data() + 1);
+  uint64_t* dmask = (uint64_t*)ctx_buf(ctx, "mv_masks", hm.size() * sizeof(uint64_t));
+  if (!dtiles || !rec || !bb || !sig || !vg || !pgene || !dmask)
+    return ctx_fail(ctx, EQBF_ERR_HIP, "allocating hybrid buffers failed");
+  EQBF_HIP(ctx, hipMemcpyAsync(dtiles, tiles.data(), tiles.size() * sizeof(HybTile), hipMemcpyHostToDevice, st));
+  EQBF_HIP(ctx, hipMemcpyAsync(dmask, hm.data(), hm.size() * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+  EQBF_HIP(ctx, hipStreamSynchronize(st)); /* the host vectors go out of scope */
+  HybGene hg;
+  hg.cp = o->covars_by_position ? o->covars_by_position : c.covars;
+  hg.yp = o->explevels_pairs ? o->explevels_pairs : c.explevels;
+  hg.rec = rec;
+  hg.NSET = NSET;
+  hg.RSZ = RSZ;
+  hg.Q1M = Q1M;
+  {
+    ProfScope ps(ctx, "hyb_gene_kernel", st);
+    hipLaunchKernelGGL(hyb_gene_kernel, dim3(G, NSET), dim3(256), 0, st, c, hg);
+  }
+  HybPair hp;
+  hp.tiles = dtiles;
+  hp.cp = hg.cp;
+  hp.yp = hg.yp;
+  hp.rec = rec;
+  hp.b = bb;
+  hp.Sig = sig;
+  hp.Vg = vg;
+  hp.pgene = pgene;
+  hp.n_out = out->n;
+  hp.sstats = out->sstats;
+  hp.flag = (int*)ctx->d_flag.ptr;
+  hp.fiterr = o->fiterr;
+  hp.NSET = NSET;
+  hp.RSZ = RSZ;
+  {
+    ProfScope ps(ctx, "hyb_pair_kernel", st);
+    hipLaunchKernelGGL(hyb_pair_kernel<Q1M>, dim3((unsigned)tiles.size()), dim3(64), 0, st, c, hp);
+  }
+  EQBF_HIP(ctx, hipGetLastError());
+  return mv_rows_run<SM>(ctx, c, o->bfs, out, st, [&](int64_t c0, int nb, double* par, int PSZ, int64_t pair_blocks) {
+    HybCfg hc;
+    hc.b = bb;
+    hc.Sig = sig;
+    hc.Vg = vg;
+    hc.pgene = pgene;
+    hc.masks = dmask;
+    hc.par = par;
+    hc.cfg0 = (int)c0;
+    hc.PSZ = PSZ;
+    ProfScope ps(ctx, "hyb_cfg_kernel", st);
+    hipLaunchKernelGGL(hyb_cfg_kernel<SM>, dim3((unsigned)pair_blocks, (unsigned)nb), dim3(64), 0, st, c, hc);
+  });
+}
+
+template <int SM>
+int hybrid_run_q(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_hybrid_opts* o, eqbf_assoc_out* out,
+                 hipStream_t st) {
+  const int Q1 = c.n_covars_max + 1;
+  if (Q1 <= 4) return hybrid_run_sm<SM, 4>(ctx, c, o, out, st);
+  if (Q1 <= 8) return hybrid_run_sm<SM, 8>(ctx, c, o, out, st);
+  return hybrid_run_sm<SM, 17>(ctx, c, o, out, st);
 }
 
 }  // namespace
@@ -739,4 +1317,24 @@ int mvlr_run(eqbf_ctx* ctx, const eqbf_chunk* cp, const eqbf_mvlr_opts* o, eqbf_
   if (S <= 8) return mvlr_run_sm<8>(ctx, c, o, out, st);
   if (S <= 12) return mvlr_run_sm<12>(ctx, c, o, out, st);
   return mvlr_run_sm<16>(ctx, c, o, out, st);
+}
+
+int hybrid_run(eqbf_ctx* ctx, const eqbf_chunk* cp, const eqbf_hybrid_opts* o, eqbf_assoc_out* out, hipStream_t st) {
+  const eqbf_chunk& c = *cp;
+  const int S = c.n_subgroups, N = c.n_samples, G = c.n_genes;
+  const int GL = (int)ctx->phiL.size(), GS = (int)ctx->phiS.size();
+  if (S < 1 || S > 16) return ctx_fail(ctx, EQBF_ERR_UNSUPPORTED, "--error hybrid supports 1 to 16 subgroups (S=%d)", S);
+  if (N < 1 || G < 0 || c.n_pairs < 0 || c.n_covars_max < 0 || c.n_covars_max > 16)
+    return ctx_fail(ctx, EQBF_ERR_ARG, "chunk sizes out of range (N=%d G=%d Q=%d)", N, G, c.n_covars_max);
+  if (o->bfs != EQBF_BFS_GEN && o->bfs != EQBF_BFS_SIN && o->bfs != EQBF_BFS_ALL)
+    return ctx_fail(ctx, EQBF_ERR_ARG, "unknown --bfs value %d", o->bfs);
+  if (!(o->fiterr >= 0.0 && o->fiterr <= 1.0)) return ctx_fail(ctx, EQBF_ERR_ARG, "--fiterr must be in [0, 1]");
+  if (GL < 1 || (o->bfs != EQBF_BFS_GEN && GS < 1))
+    return ctx_fail(ctx, EQBF_ERR_ARG, "--error hybrid needs the grids (eqbf_set_grids)");
+  if (!out || !out->n || !out->w_abf || !out->raw_abf) return ctx_fail(ctx, EQBF_ERR_ARG, "missing output buffers");
+  if (G == 0 || c.n_pairs == 0) return EQBF_OK;
+  if (S <= 4) return hybrid_run_q<4>(ctx, c, o, out, st);
+  if (S <= 8) return hybrid_run_q<8>(ctx, c, o, out, st);
+  if (S <= 12) return hybrid_run_q<12>(ctx, c, o, out, st);
+  return hybrid_run_q<16>(ctx, c, o, out, st);
 }

@@ -44,6 +44,8 @@ def lib():
         L.eqbf_assoc_uvlr_device_noalloc.argtypes = [vp, vp, vp, vp, vp]
         L.eqbf_assoc_mvlr.argtypes = [vp, vp, vp, vp]
         L.eqbf_assoc_mvlr_device.argtypes = [vp, vp, vp, vp, vp]
+        L.eqbf_assoc_hybrid.argtypes = [vp, vp, vp, vp]
+        L.eqbf_assoc_hybrid_device.argtypes = [vp, vp, vp, vp, vp]
         L.eqbf_abf_from_sstats.argtypes = [vp, vp, i32, i32, vp]
         L.eqbf_abf_from_sstats_device.argtypes = [vp, vp, i32, i32, vp, vp]
         L.eqbf_debug_ttail.argtypes = [vp, vp, vp, i32, vp, vp, vp, vp]
@@ -96,6 +98,14 @@ class PermOpts(C.Structure):
 
 class MvlrOpts(C.Structure):
     _fields_ = [("bfs", C.c_int32), ("fiterr", C.c_double)]
+
+
+class HybridOpts(C.Structure):
+    """eqbf_hybrid_opts: bfs, fiterr, pairwise covariates by sample index
+    [S][Qmax][N] and pairwise expression levels [G][S][N] (NULL: the
+    chunk's)."""
+    _fields_ = [("bfs", C.c_int32), ("fiterr", C.c_double), ("covars_by_position", C.c_void_p),
+                ("explevels_pairs", C.c_void_p)]
 
 
 class HmOpts(C.Structure):
@@ -238,6 +248,28 @@ class Engine:
         o = out.struct()
         self._check(lib().eqbf_assoc_mvlr(self._ctx, C.byref(st), C.byref(opts), C.byref(o)))
         return out
+
+    def assoc_hybrid(self, hc, bfs="gen", fiterr=0.0, covars_pos=None, expl_pair=None, want_sstats=True):
+        """--error hybrid Bayes factors (FP64), with the sample sizes and
+        summary statistics of the separate analysis."""
+        out = _abi.HostAssocOut(hc.S, hc.n_pairs, bfs, self.GL, self.GS, raw_dtype=np.float64,
+                                want_sstats=want_sstats, want_std=False, want_raw=True)
+        cp = None if covars_pos is None else np.ascontiguousarray(covars_pos, dtype=np.float64)
+        ep = None if expl_pair is None else np.ascontiguousarray(expl_pair, dtype=np.float64)
+        if cp is not None:
+            assert cp.shape == hc.covars.shape
+        if ep is not None:
+            assert ep.shape == hc.explevels.shape
+        opts = HybridOpts(_abi.BFS[bfs], float(fiterr), None if cp is None else cp.ctypes.data,
+                          None if ep is None else ep.ctypes.data)
+        st = hc.struct()
+        o = out.struct()
+        self._check(lib().eqbf_assoc_hybrid(self._ctx, C.byref(st), C.byref(opts), C.byref(o)))
+        return out
+
+    def assoc_hybrid_device(self, chunk_struct, opts, out_struct, stream=None):
+        self._check(lib().eqbf_assoc_hybrid_device(self._ctx, C.byref(chunk_struct), C.byref(opts),
+                                                   C.byref(out_struct), stream))
 
     def assoc_mvlr_device(self, chunk_struct, opts, out_struct, stream=None):
         self._check(lib().eqbf_assoc_mvlr_device(self._ctx, C.byref(chunk_struct), C.byref(opts),

@@ -149,3 +149,31 @@ def mvlr(seed=1003, S=9, N=150, n_genes=4, snps_per_gene=40, rho=0.5, r_eff=0.8,
         pair_offset=np.arange(n_genes + 1, dtype=np.int64) * P,
         pair_snp=np.arange(n_snps, dtype=np.int32), covars=covars,
         covar_count=np.full(S, Q, np.int32), covar_in_sbgrp=np.ones((S, N), np.uint8))
+
+
+def hybrid(seed=1005, S=3, N=70, n_genes=2, P=4, Q=2, miss=0.25, same_covars=False, q_per_sbgrp=None,
+           dosages=False):
+    """--error hybrid shape: one genotype column per individual in every
+    subgroup (one genotype file), expression levels missing at random per
+    subgroup, covariates of their own per subgroup (q_per_sbgrp: a count per
+    subgroup, at most Q); dosages: FP32 genotypes with noise."""
+    rng = np.random.default_rng(seed)
+    n_snps = n_genes * P
+    g = rng.binomial(2, rng.uniform(0.1, 0.5, n_snps)[:, None], size=(n_snps, N)).astype(float)
+    if dosages:
+        g = np.clip(g + rng.normal(0, 0.05, g.shape), 0, 2).astype(np.float32).astype(np.float64)
+    covars = rng.normal(size=(S, max(Q, 1), N))
+    if same_covars:
+        covars[:] = covars[0:1]
+    y = rng.normal(size=(n_genes, S, N)) + 0.4 * covars[None, :, 0, :]
+    for gi in range(n_genes):
+        j = gi * P + rng.integers(P)
+        y[gi] += rng.normal(0, 0.6, size=(S, 1)) * g[j][None, :]
+    y[rng.random(y.shape) < miss] = np.nan
+    qc = np.full(S, Q, np.int32) if q_per_sbgrp is None else np.asarray(q_per_sbgrp, np.int32)
+    return _abi.HostChunk(
+        explevels=y, gene_in_sbgrp=np.ones((n_genes, S), np.uint8), geno_in_sbgrp=np.ones((S, N), np.uint8),
+        genotypes=np.ascontiguousarray(g.T).astype(np.float32 if dosages else np.float64),
+        snp_in_sbgrp=np.ones((n_snps, S), np.uint8),
+        pair_offset=np.arange(n_genes + 1, dtype=np.int64) * P, pair_snp=np.arange(n_snps, dtype=np.int32),
+        covars=covars, covar_count=qc, covar_in_sbgrp=np.ones((S, N), np.uint8))

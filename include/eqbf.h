@@ -203,6 +203,40 @@ int eqbf_assoc_mvlr(eqbf_ctx* ctx, const eqbf_chunk* chunk, const eqbf_mvlr_opts
 int eqbf_assoc_mvlr_device(eqbf_ctx* ctx, const eqbf_chunk* chunk_dev, const eqbf_mvlr_opts* opts,
                            eqbf_assoc_out* out_dev, void* hip_stream);
 
+/* ---- hybrid error model (--error hybrid) -----------------------------------------
+ * GeneSnpPair::CalcAbfsHybrid (gene_snp_pair.cpp:1384-1423): per-subgroup
+ * regressions give the effects and the diagonals of the error and effect
+ * covariances (CalcBetahatsAndDiagsPerSubgroup :760-880), each pair of
+ * subgroups the off-diagonal terms over the individuals present in both and
+ * in either (CalcOffDiagCovarsFromPairsOfSubgroups :1035-1131), and the Bayes
+ * factors take the MVLR form (CalcLog10AbfMvlr :1165-1256).  Subgroups may
+ * have different samples (expression levels missing per subgroup).  Genes
+ * without expression in every subgroup and SNPs without genotypes in every
+ * subgroup are skipped (gene.cpp:314-321): n = 0, NaN statistics and Bayes
+ * factors.  n and sstats are those of the separate analysis, as the
+ * reference records them (:835-870).  Computed in FP64; raw_abf in FP64,
+ * required like n and w_abf.  Up to 16 subgroups and 16 covariates.
+ * EQBF_ERR_DATA when two subgroups of an analysed pair share no individual
+ * (the reference exits, :895-900) or a member lacks a covariate. */
+typedef struct eqbf_hybrid_opts {
+  int32_t bfs;                       /* EQBF_BFS_*                                      */
+  double fiterr;                     /* --fiterr                                        */
+  /* covariates of the pairwise designs [S][Qmax][N]: the reference reads the
+   * first subgroup's covariate at the individual's index in the union of
+   * samples, not at its column of the covariate file (:920-927); NULL: the
+   * chunk's covariates (the same when the files list the samples in the
+   * same order) */
+  const double* covars_by_position;
+  /* expression levels of the pairwise step [G][S][N]: the reference reads
+   * them unnormalised under --qnorm (Gene::GetExplevel, :908-915); NULL: the
+   * chunk's */
+  const double* explevels_pairs;
+} eqbf_hybrid_opts;
+int eqbf_assoc_hybrid(eqbf_ctx* ctx, const eqbf_chunk* chunk, const eqbf_hybrid_opts* opts, eqbf_assoc_out* out);
+/* Device-pointer variant: chunk, opts' arrays and outputs on the device. */
+int eqbf_assoc_hybrid_device(eqbf_ctx* ctx, const eqbf_chunk* chunk_dev, const eqbf_hybrid_opts* opts,
+                             eqbf_assoc_out* out_dev, void* hip_stream);
+
 /* ---- summary statistics as input (--inss) ----------------------------------------
  * Bayes factors of pairs whose summary statistics are given rather than
  * computed: replaces, for the pairs loadSummaryStats / fillGeneSnpPairsWith-

@@ -22,31 +22,14 @@ import numpy as np
 import pytest
 from scipy import stats
 
-from eqtlbma_amd import _abi, synth
+from eqtlbma_amd import synth
 from eqtlbma_amd._abi import config_masks
 
 LN10 = np.log(10.0)
 
 
-def hybrid_chunk(seed, S=3, N=70, n_genes=2, P=4, Q=2, miss=0.25, same_covars=False):
-    """One genotype column per individual in every subgroup (one genotype
-    file), expression levels missing at random per subgroup."""
-    rng = np.random.default_rng(seed)
-    n_snps = n_genes * P
-    g = rng.binomial(2, rng.uniform(0.1, 0.5, n_snps)[:, None], size=(n_snps, N)).astype(float)
-    covars = rng.normal(size=(S, max(Q, 1), N))
-    if same_covars:
-        covars[:] = covars[0:1]
-    y = rng.normal(size=(n_genes, S, N)) + 0.4 * covars[None, :, 0, :]
-    for gi in range(n_genes):
-        j = gi * P + rng.integers(P)
-        y[gi] += rng.normal(0, 0.6, size=(S, 1)) * g[j][None, :]
-    y[rng.random(y.shape) < miss] = np.nan
-    return _abi.HostChunk(
-        explevels=y, gene_in_sbgrp=np.ones((n_genes, S), np.uint8), geno_in_sbgrp=np.ones((S, N), np.uint8),
-        genotypes=np.ascontiguousarray(g.T), snp_in_sbgrp=np.ones((n_snps, S), np.uint8),
-        pair_offset=np.arange(n_genes + 1, dtype=np.int64) * P, pair_snp=np.arange(n_snps, dtype=np.int32),
-        covars=covars, covar_count=np.full(S, Q, np.int32), covar_in_sbgrp=np.ones((S, N), np.uint8))
+def hybrid_chunk(seed, **kw):
+    return synth.hybrid(seed=seed, **kw)
 
 
 def _lsq_resid(X, y):
