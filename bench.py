@@ -90,6 +90,9 @@ def parse():
     ap.add_argument("--only-perm", action="store_true", help="run only the permutation workload")
     ap.add_argument("--mvlr-genes", type=int, default=2000)
     ap.add_argument("--no-mvlr", action="store_true", help="skip the MVLR workload")
+    ap.add_argument("--hybrid-genes", type=int, default=2000)
+    ap.add_argument("--no-hybrid", action="store_true", help="skip the --error hybrid workload")
+    ap.add_argument("--only-hybrid", action="store_true", help="run only the --error hybrid workload")
     ap.add_argument("--no-c0", action="store_true", help="skip the plain-method workload")
     ap.add_argument("--hm-genes", type=int, default=20000)
     ap.add_argument("--hm-iters", type=int, default=6, help="EM iterations timed in the eqtlbma_hm workload")
@@ -848,6 +851,149 @@ def mvlr_leg(args, torch, result):
 # solve and quadratic form (4 S^2), S = 9
 MVLR_FLOPS_PER_SYSTEM = 2 * 9 ** 3 + (2 * 9 ** 3) // 3 + 4 * 9 ** 2
 
+class HybridWorkload(MvlrWorkload):
+    """The MVLR configuration's shape with what --error hybrid is for: each
+    subgroup misses the expression levels of 15 % of the individuals (NaN,
+    drawn per gene and subgroup) and has 3 Gaussian covariates of its own."""
+
+    Q = 3
+
+    def __init__(self, torch, G, seed, miss=0.15):
+        super().__init__(torch, G, seed)
+        dev, S, N = self.dev, self.S, self.N
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(seed + 1)
+        self.covars = torch.randn((S, self.Q, N), device=dev, generator=gen, dtype=torch.float64)
+        self.covar_count = torch.full((S,), self.Q, dtype=torch.int32, device=dev)
+        self.y += 0.3 * self.covars[None, :, 0, :]
+        drop = torch.rand(self.y.shape, device=dev, generator=gen) < miss
+        self.y[drop] = float("nan")
+        self.y = self.y.contiguous()
+        torch.cuda.synchronize()
+
+    def chunk(self, abi, g0, g1):
+        c = PermWorkload.chunk(self, abi, g0, g1)
+        c.n_covars_max = self.Q
+        return c
+
+    def host_chunk(self, abi, g0, g1):
+        h = PermWorkload.host_chunk(self, abi, g0, g1)
+        return abi.HostChunk(h.explevels, h.gene_in_sbgrp, h.geno_in_sbgrp, h.genotypes, h.snp_in_sbgrp,
+                             h.pair_offset, h.pair_snp, covars=self.covars.cpu().numpy(),
+                             covar_count=self.covar_count.cpu().numpy(), covar_in_sbgrp=self.covar_in.cpu().numpy())
+
+
+def hybrid_flops(n, n12, S, Q):
+    """Algorithmic FP64 flops of the per-pair sums of hyb_pair_kernel: per
+    subgroup s, over its n[s] members, x^2, x y and (1, covariates) x for the
+    subgroup's own and the pairwise covariates (2 + 2 (Q + 1) products); per
+    pair of subgroups, over the n[s2] members of the second, x^2 and
+    (1, covariates) x (Q + 2), and over the n12 common ones x^2, two x y and
+    (1, covariates) x (Q + 4); a multiply-add counts 2."""
+    f = 0
+    k = 0
+    for s1 in range(S):
+        f += n[s1] * (2 + 2 * (Q + 1))
+        for s2 in range(s1 + 1, S):
+            f += n[s2] * (Q + 2) + n12[k] * (Q + 4)
+            k += 1
+    return 2 * f
+
+
+def hybrid_leg(args, torch, result):
+    """--error hybrid Bayes factors (--fiterr 0.5 --bfs gen, 50-point grid):
+    2,000 genes x ~800 cis SNPs, 9 subgroups over 150 individuals, 15 % of
+    the levels missing per subgroup, 3 covariates per subgroup
+    (gene_snp_pair.cpp:1384-1423)."""
+    from eqtlbma_amd import _abi, synth
+    from eqtlbma_amd.engine import Engine, HybridOpts
+    from oracle import oracle as orc
+    grid = synth.make_grid("general", n_aes=10)
+    GLn = grid.shape[0]
+    G = args.hybrid_genes
+    wl = HybridWorkload(torch, G, 1007)
+    S, N, dev = wl.S, wl.N, wl.dev
+    NP = int(wl.offs[-1])
+    eng = Engine(0)
+    eng.set_grids(grid, grid)
+    out_n = torch.empty((S, NP), dtype=torch.int32, device=dev)
+    out_ss = torch.empty((5, S, NP), dtype=torch.float64, device=dev)
+    out_raw = torch.empty((3 * GLn, NP), dtype=torch.float64, device=dev)
+    out_w = torch.empty((5, NP), dtype=torch.float64, device=dev)
+    o = _abi.AssocOut()
+    o.n, o.sstats, o.stdsstats = out_n.data_ptr(), out_ss.data_ptr(), None
+    o.raw_abf, o.w_abf = out_raw.data_ptr(), out_w.data_ptr()
+    ho = HybridOpts(_abi.BFS["gen"], 0.5, None, None)
+    sptr = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    c = wl.chunk(_abi, 0, G)
+    eng.assoc_hybrid_device(c, ho, o, sptr)  # warm-up
+    torch.cuda.synchronize()
+    eng.profile(True)
+    before = eng.profile_read()
+    reps = 3
+    t = time.perf_counter()
+    for _ in range(reps):
+        eng.assoc_hybrid_device(c, ho, o, sptr)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / reps
+    after = eng.profile_read()
+    eng.profile(False)
+    kern = {k: ((v[0] - before.get(k, (0.0, 0))[0]) / reps, (v[1] - before.get(k, (0.0, 0))[1]) // reps)
+            for k, v in after.items() if v[1] > before.get(k, (0, 0))[1]}
+    # members per (gene, subgroup) and common members per (gene, pair of subgroups)
+    pres = (~torch.isnan(wl.y)).double()  # [G][S][N]
+    n_gs = pres.sum(2).cpu().numpy()
+    common = torch.einsum("gsn,gtn->gst", pres, pres).cpu().numpy()
+    iu = np.triu_indices(S, 1)
+    flops = 0.0
+    for g in range(G):
+        flops += float(wl.P[g]) * hybrid_flops(n_gs[g], common[g][iu], S, wl.Q)
+    pk = kern.get("hyb_pair_kernel", (float("nan"), 0))
+    gk = kern.get("mvlr_grid_kernel", (float("nan"), 0))
+    systems = float(NP) * 3 * GLn
+    res = {"metric": "SNP-gene hybrid Bayes factors/s", "value": NP / dt, "unit": "pairs/s",
+           "seconds_per_pass": dt,
+           "workload": "hybrid: %d genes x ~800 cis SNPs (%d pairs), 9 subgroups over N=150 with 15%% of the "
+                       "levels missing per subgroup, 3 covariates per subgroup, --fiterr 0.5 --bfs gen, "
+                       "50-point grid, FP64" % (G, NP),
+           "kernels_ms_per_pass": {k: v[0] for k, v in kern.items()},
+           "roofline": {"bound": "fp64", "kernel": "hyb_pair_kernel", "alg_flops": flops,
+                        "achieved": flops / (pk[0] / 1e3) / 1e12, "peak": 78.6, "unit": "TFLOP/s",
+                        "frac": flops / (pk[0] / 1e3) / 1e12 / 78.6, "avg_launch_ms": pk[0] / max(pk[1], 1)},
+           "roofline_rows": {"bound": "fp64", "kernel": "mvlr_grid_kernel", "systems": systems,
+                             "achieved": systems * MVLR_FLOPS_PER_SYSTEM / (gk[0] / 1e3) / 1e12, "peak": 78.6,
+                             "unit": "TFLOP/s",
+                             "frac": systems * MVLR_FLOPS_PER_SYSTEM / (gk[0] / 1e3) / 1e12 / 78.6}}
+    model, T = host_cpu()
+    h0 = wl.host_chunk(_abi, 0, 1)
+    t = time.perf_counter()
+    r0 = orc.assoc_hybrid(h0, "gen", 0.5, grid, grid)
+    d1 = time.perf_counter() - t
+    raw0 = out_raw[:, :h0.n_pairs].cpu().numpy()
+    w0 = out_w[:, :h0.n_pairs].cpu().numpy()
+    fr, fw = np.isfinite(r0.raw), np.isfinite(r0.w)
+    res["error_vs_fp64_oracle"] = {"sample_pairs": int(h0.n_pairs),
+                                   "log10_abf_raw_abs": float(np.max(np.abs(raw0[fr] - r0.raw[fr]))),
+                                   "log10_abf_weighted_abs": float(np.max(np.abs(w0[fw] - r0.w[fw]))),
+                                   "n_equal": bool(np.array_equal(out_n[:, :h0.n_pairs].cpu().numpy(), r0.n))}
+    if not args.no_cpu_baseline:
+        k = min(G, T * max(1, int(args.cpu_seconds / max(d1, 1e-3))))
+        hcs = [wl.host_chunk(_abi, g, g + 1) for g in range(k)]
+        t = time.perf_counter()
+        threaded(lambda h: orc.assoc_hybrid(h, "gen", 0.5, grid, grid), hcs, T)
+        tc = time.perf_counter() - t
+        cp = sum(h.n_pairs for h in hcs)
+        res["cpu_baseline"] = dict(value=cp / tc, unit="pairs/s",
+                                   sample="first %d genes (%d pairs), one gene per task, %.1f s wall" % (k, cp, tc),
+                                   **cpu_label(model, T, "genes sharded over threads"))
+        log("hybrid: cpu %.3e pairs/s" % (cp / tc))
+    log("hybrid: %d pairs in %.3f s = %.3e pairs/s; kernels %s; pair-sums %.1f%% of FP64 peak" %
+        (NP, dt, NP / dt, {k: round(v[0], 2) for k, v in kern.items()}, 100 * res["roofline"]["frac"]))
+    eng.close()
+    del wl, out_n, out_ss, out_raw, out_w
+    torch.cuda.empty_cache()
+    result["hybrid"] = res
+
 
 def hm_leg(args, torch, result):
     """eqtlbma_hm's EM (configs model) on device-resident raw log10 ABFs: the
@@ -1014,6 +1160,11 @@ def main():
             log("calibration copy: %d MiB read + %d MiB written" % (args.calib_copy_mib, args.calib_copy_mib))
             del src, dst
         hm_leg(args, torch, result)
+        print(json.dumps(result), flush=True)
+        return
+    if args.only_hybrid:
+        result = {"metric": "--error hybrid only"}
+        hybrid_leg(args, torch, result)
         print(json.dumps(result), flush=True)
         return
     if args.only_perm:
@@ -1272,6 +1423,8 @@ def main():
         c0_leg(args, torch, result)
     if not args.no_mvlr:
         mvlr_leg(args, torch, result)
+    if not args.no_hybrid:
+        hybrid_leg(args, torch, result)
     if not args.no_hm:
         hm_leg(args, torch, result)
     if not args.no_perm:

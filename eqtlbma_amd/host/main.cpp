@@ -115,6 +115,11 @@ struct Shared {
   std::vector<uint8_t> geno_in, covar_in;
   std::vector<int32_t> covar_count, gset_of_sbgrp;
   std::vector<double> covars;
+  /* --error hybrid's pairwise designs read covariate q of subgroup s at the
+   * sample's index in the sorted union of samples, not at its column of the
+   * covariate file (gene_snp_pair.cpp:920-927, covariates.cpp:79-84): the
+   * value in column i, NaN past the file's columns */
+  std::vector<double> covars_pos;
   /* genotype sets of the chunk: (loaded set, column map of its subgroups) */
   std::vector<int32_t> set_src;                   /* loaded set per chunk set */
   std::vector<std::vector<int32_t>> set_cols;     /* [chunk set][union sample] */
@@ -148,6 +153,10 @@ Shared make_shared(const Data& d) {
     }
     sh.gset_of_sbgrp.push_back(it->second);
     sh.covar_count[s] = (int)sg.covar_names.size();
+    if (sh.covars_pos.empty()) sh.covars_pos.assign(sh.covars.size(), kNaN);
+    for (int q = 0; q < sh.covar_count[s]; ++q)
+      for (int i = 0; i < sh.N && i < (int)sg.covar_vals[q].size(); ++i)
+        sh.covars_pos[((size_t)s * std::max(sh.Qmax, 1) + q) * sh.N + i] = sg.covar_vals[q][i];
     for (int i = 0; i < sh.N; ++i) {
       const int32_t c = sg.covar_col[i];
       if (c < 0) continue;
@@ -165,6 +174,7 @@ struct Chunk {
   std::vector<std::vector<const Snp*>> cis;  /* pairs of each gene           */
   std::vector<const Snp*> snps;              /* distinct SNPs of the chunk   */
   std::vector<double> expl, geno;
+  std::vector<double> expl_raw; /* --error hybrid with --qnorm: levels before normalisation */
   std::vector<uint8_t> gene_in, snp_in;
   std::vector<int64_t> pair_offset;
   std::vector<int32_t> pair_snp;
@@ -197,7 +207,10 @@ void build_chunk(const Data& d, const Shared& sh, const Options& o, std::vector<
   ch.pair_offset.assign(G + 1, 0);
   ch.gene_in.assign((size_t)G * S, 0);
   ch.expl.assign((size_t)G * S * N, kNaN);
-  const bool mvlr = o.analysis == "join" && o.error_model == "mvlr";
+  const bool multi = o.analysis == "join" && o.error_model != "uvlr";
+  const bool keep_raw = o.analysis == "join" && o.error_model == "hybrid" && o.need_qnorm;
+  ch.expl_raw.clear();
+  if (keep_raw) ch.expl_raw.assign((size_t)G * S * N, kNaN);
   std::map<const Snp*, int32_t> col;
   ch.snps.clear();
   ch.pair_snp.clear();
@@ -211,6 +224,7 @@ void build_chunk(const Data& d, const Shared& sh, const Options& o, std::vector<
       double* y = &ch.expl[((size_t)g * S + s) * N];
       for (int i = 0; i < N; ++i)
         if (sg.exp_col[i] >= 0) y[i] = sg.exp_vals[(size_t)row * sg.exp_ncols + sg.exp_col[i]];
+      if (keep_raw) std::copy(y, y + N, &ch.expl_raw[((size_t)g * S + s) * N]);
       if (o.need_qnorm) {
         std::vector<int32_t> mem;
         for (int i = 0; i < N; ++i)
@@ -221,14 +235,14 @@ void build_chunk(const Data& d, const Shared& sh, const Options& o, std::vector<
     std::vector<const Snp*> cis = cis_snps(d, gn, o.anchor, o.radius);
     /* a gene is tested only when a cis SNP has genotypes in a subgroup where
      * the gene has expression levels (eqtlbma_bf.cpp:747-753); --error mvlr
-     * also needs expression levels in every subgroup (:755-762) */
+     * and hybrid also need expression levels in every subgroup (:755-762) */
     bool any = false, all_expr = true;
     for (int s = 0; s < S; ++s) all_expr &= gn.exp_row[s] >= 0;
     for (const Snp* sn : cis) {
       for (int s = 0; s < S && !any; ++s) any = gn.exp_row[s] >= 0 && sn->row[d.sbgrps[s].geno_set] >= 0;
       if (any) break;
     }
-    if (!any || (mvlr && !all_expr)) cis.clear();
+    if (!any || (multi && !all_expr)) cis.clear();
     for (const Snp* sn : cis) {
       auto it = col.find(sn);
       if (it == col.end()) {
@@ -721,6 +735,7 @@ int main(int argc, char** argv) {
   load_grid(o.gridS, phiS, omaS, verbose);
   const bool join = o.analysis == "join";
   const bool mvlr = join && o.error_model == "mvlr";
+  const bool hybrid = join && o.error_model == "hybrid";
   const int S = (int)d.sbgrps.size();
   const int bfs = bfs_code(o.bfs);
   Engine eng(o.device);
@@ -821,10 +836,12 @@ int main(int argc, char** argv) {
   write_all(nullptr, nullptr, nullptr, nullptr);
   wr.mode = "ab";
 
-  if (verbose > 0)
+  if (verbose > 0) {
     std::cout << "test for association between each pair gene-SNP ..." << std::endl
-              << "analysis=" << o.analysis << " likelihood=" << o.likelihood << " error_model=" << o.error_model
-              << " anchor=" << o.anchor << " radius=" << o.radius << " precision=" << o.precision << std::endl;
+              << "analysis=" << o.analysis << " likelihood=" << o.likelihood << " error_model=" << o.error_model;
+    if (o.error_model != "uvlr") std::cout << " prop_cov_errors=" << o.prop_cov_errors;
+    std::cout << " anchor=" << o.anchor << " radius=" << o.radius << " precision=" << o.precision << std::endl;
+  }
   std::vector<const Gene*> all;
   for (auto& kv : d.genes) all.push_back(&kv.second);
   int64_t nb_perm = (int64_t)o.nb_permutations;
@@ -858,6 +875,10 @@ int main(int argc, char** argv) {
         eqbf_mvlr_opts mo{bfs, o.prop_cov_errors};
         out.sstats = nullptr;
         eng.check(eqbf_assoc_mvlr(eng.ctx, &ch.c, &mo, &out));
+      } else if (hybrid) {
+        eqbf_hybrid_opts ho{bfs, o.prop_cov_errors, sh.Qmax > 0 ? sh.covars_pos.data() : nullptr,
+                            ch.expl_raw.empty() ? nullptr : ch.expl_raw.data()};
+        eng.check(eqbf_assoc_hybrid(eng.ctx, &ch.c, &ho, &out));
       } else {
         eqbf_assoc_opts ao{join ? EQBF_ANALYSIS_JOIN : EQBF_ANALYSIS_SEP, bfs, prec, EQBF_F64};
         eng.check(eqbf_assoc_uvlr(eng.ctx, &ch.c, &ao, &out));

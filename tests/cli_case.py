@@ -22,8 +22,11 @@ def _w(path, lines):
             f.write(l + "\n")
 
 
-def make_case(root, seed=3, S=3, N=36, mvlr=False, covariates=True, near_mono=None):
+def make_case(root, seed=3, S=3, N=36, mvlr=False, covariates=True, near_mono=None, one_geno=False):
     """Writes the input files under root; returns a dict describing them.
+    one_geno: one genotype file of every sample for all subgroups (as
+    --error mvlr / hybrid require) while the expression files keep their own
+    sample subsets and NA levels.
     near_mono: index of a SNP made monomorphic but for one carrier (its
     statistics are NaN whenever that carrier is not among the members)."""
     rng = np.random.default_rng(seed)
@@ -47,6 +50,9 @@ def make_case(root, seed=3, S=3, N=36, mvlr=False, covariates=True, near_mono=No
     for s in range(S):
         if mvlr:
             e = np.ones(N, bool)
+            g = np.ones(N, bool)
+        elif one_geno:
+            e = rng.random(N) < 0.85
             g = np.ones(N, bool)
         else:
             e = rng.random(N) < 0.85
@@ -78,13 +84,13 @@ def make_case(root, seed=3, S=3, N=36, mvlr=False, covariates=True, near_mono=No
         p = os.path.join(root, "exp_%s.txt" % name)
         _w(p, rows)
         exp_lines.append("%s %s" % (name, p))
-        if mvlr:
+        if mvlr or one_geno:
             gorder = list(range(N))
             gpath = os.path.join(root, "geno.txt")
         else:
             gorder = [i for i in rng.permutation(N) if geno_in[s][i]]
             gpath = os.path.join(root, "geno_%s.txt" % name)
-        if not mvlr or s == 0:
+        if not (mvlr or one_geno) or s == 0:
             rows = ["Id\t" + "\t".join(samples[i] for i in gorder)]
             for j, (sname, _c, _p) in enumerate(snps):
                 vals = ["%g" % G[j, i] for i in gorder]
@@ -204,6 +210,7 @@ def load_expected_inputs(case):
             files[geno[sg]] = d
         geno_vals.append(files[geno[sg]])
     covars = np.zeros((S, 2, N))
+    covars_pos = np.full((S, 2, N), np.nan)  # covariate value at file column i (hybrid's pairwise designs)
     covar_count = np.zeros(S, np.int32)
     covar_in = np.zeros((S, N), np.uint8)
     for s, sg in enumerate(sbgrps):
@@ -219,10 +226,12 @@ def load_expected_inputs(case):
             for k, v in enumerate(rows[name]):
                 covars[s, q, idx[c[k]]] = v
                 covar_in[s, idx[c[k]]] = 1
+                if k < N:
+                    covars_pos[s, q, k] = v
         covar_count[s] = len(rows)
     return dict(sbgrps=sbgrps, samples=samples, gcoord=gcoord, explev=explev, scoord=scoord,
                 geno_vals=geno_vals, geno_in=geno_in, covars=covars, covar_count=covar_count,
-                covar_in=covar_in)
+                covar_in=covar_in, covars_pos=covars_pos)
 
 
 def expected_chunks(inp, radius, wrtsize, mvlr=False, anchor_tes=False):
