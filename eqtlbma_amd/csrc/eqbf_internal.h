@@ -381,6 +381,12 @@ int uvlr_i8_sums_run(eqbf_ctx* ctx, const eqbf_chunk& c, hipStream_t st, const P
                      uint32_t* redo);
 int mvlr_run(eqbf_ctx* ctx, const eqbf_chunk* c, const eqbf_mvlr_opts* o, eqbf_assoc_out* out, hipStream_t st);
 int hybrid_run(eqbf_ctx* ctx, const eqbf_chunk* c, const eqbf_hybrid_opts* o, eqbf_assoc_out* out, hipStream_t st);
+/* eqtlbma_avg_bfs kernels (device pointers; weight tables already staged) */
+int avg_bfs_run(eqbf_ctx* ctx, const double* raw, int64_t NP, const int64_t* off, int64_t G, const eqbf_avg_opts* o,
+                const double* d_gw, const int32_t* d_keep, const double* d_dw, const uint64_t* d_cs,
+                const double* d_sw, eqbf_avg_out* out, hipStream_t st);
+int avg_grid_run(eqbf_ctx* ctx, const double* raw, int64_t n_rows, int L, const double* gw, const int32_t* keep,
+                 int nk, double* out, hipStream_t st);
 /* --inss: Bayes factors from given summary statistics (device pointers) */
 int abf_from_sstats_run(eqbf_ctx* ctx, const eqbf_sstats_in* in, int bfs, int raw_dtype, eqbf_assoc_out* out,
                         hipStream_t st);

@@ -45,6 +45,9 @@ def lib():
         L.eqbf_assoc_mvlr.argtypes = [vp, vp, vp, vp]
         L.eqbf_assoc_mvlr_device.argtypes = [vp, vp, vp, vp, vp]
         L.eqbf_assoc_hybrid.argtypes = [vp, vp, vp, vp]
+        L.eqbf_avg_bfs.argtypes = [vp, vp, i64, vp, i64, vp, vp]
+        L.eqbf_avg_bfs_device.argtypes = [vp, vp, i64, vp, i64, vp, vp, vp]
+        L.eqbf_avg_grid.argtypes = [vp, vp, i64, i32, vp, vp, i32, vp]
         L.eqbf_assoc_hybrid_device.argtypes = [vp, vp, vp, vp, vp]
         L.eqbf_abf_from_sstats.argtypes = [vp, vp, i32, i32, vp]
         L.eqbf_abf_from_sstats_device.argtypes = [vp, vp, i32, i32, vp, vp]
@@ -106,6 +109,22 @@ class HybridOpts(C.Structure):
     chunk's)."""
     _fields_ = [("bfs", C.c_int32), ("fiterr", C.c_double), ("covars_by_position", C.c_void_p),
                 ("explevels_pairs", C.c_void_p)]
+
+
+class AvgOpts(C.Structure):
+    """eqbf_avg_opts (eqtlbma_avg_bfs)."""
+    _fields_ = [("model", C.c_int32), ("dim", C.c_int32), ("n_subgroups", C.c_int32), ("n_grid", C.c_int32),
+                ("n_keep", C.c_int32), ("keep", C.c_void_p), ("grid_weights", C.c_void_p),
+                ("dim_weights", C.c_void_p), ("cfg_subgroups", C.c_void_p), ("sub_weights", C.c_void_p),
+                ("pi0", C.c_double), ("want", C.c_int32)]
+
+
+class AvgOut(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("gene_bf", "gene_post", "snp_bf", "dim_bf", "post_the", "post_an",
+                                         "post_dim", "post_sub", "best_dim")]
+
+
+AVG_POST = {"a": 1, "b": 2, "c": 4, "d": 8, "dims": 16}
 
 
 class HmOpts(C.Structure):
@@ -266,6 +285,59 @@ class Engine:
         o = out.struct()
         self._check(lib().eqbf_assoc_hybrid(self._ctx, C.byref(st), C.byref(opts), C.byref(o)))
         return out
+
+    def avg_grid(self, raw, grid_w, keep=None):
+        """eqtlbma_avg_bfs without configuration weights: each raw row
+        [n_rows][L] averaged over the kept grid points."""
+        raw = np.ascontiguousarray(raw, dtype=np.float64)
+        L = raw.shape[1]
+        gw = np.ascontiguousarray(grid_w, dtype=np.float64)
+        kp = np.ascontiguousarray(np.arange(L) if keep is None else keep, dtype=np.int32)
+        out = np.zeros(raw.shape[0])
+        self._check(lib().eqbf_avg_grid(self._ctx, raw.ctypes.data, raw.shape[0], L, gw.ctypes.data,
+                                        kp.ctypes.data, kp.shape[0], out.ctypes.data))
+        return out
+
+    def avg_bfs(self, raw, off, grid_w, model="configs", cfg_w=None, cfg_sub=None, type_w=None, sub_w=None,
+                pi0=float("nan"), want=(), keep=None, n_subgroups=None):
+        """eqtlbma_avg_bfs averaging over the grid and the configurations
+        or types, with the posteriors named in want; raw [n_pairs][R][L].
+        Returns a dict of arrays (the layout of the oracle's avg_bfs)."""
+        raw = np.ascontiguousarray(raw, dtype=np.float64)
+        NP, R, L = raw.shape
+        off = np.ascontiguousarray(off, dtype=np.int64)
+        G = off.shape[0] - 1
+        types = model == "types"
+        gw = np.ascontiguousarray(grid_w, dtype=np.float64)
+        kp = np.ascontiguousarray(np.arange(L) if keep is None else keep, dtype=np.int32)
+        if types:
+            dw = np.ascontiguousarray(type_w, dtype=np.float64)
+            sw = np.ascontiguousarray(sub_w, dtype=np.float64)
+            D, S, cs = dw.shape[0], R, None
+        else:
+            dw = np.ascontiguousarray(cfg_w, dtype=np.float64)
+            D, sw = R, None
+            S = n_subgroups if n_subgroups is not None else 1
+            cs = np.ascontiguousarray(cfg_sub if cfg_sub is not None else np.zeros(D), dtype=np.uint64)
+        bits = 0
+        for k in want:
+            bits |= AVG_POST[k]
+        o = dict(gene_bf=np.full(G, np.nan), gene_post=np.full(G, np.nan), snp_bf=np.full(NP, np.nan),
+                 dim_bf=np.full((NP, D), np.nan), post_the=np.full(NP, np.nan), post_an=np.full(NP, np.nan),
+                 post_dim=np.full((NP, D), np.nan), post_sub=np.full((NP, S), np.nan),
+                 best_dim=np.full(NP, -1, np.int32))
+        opts = AvgOpts(int(types), D, S, L, kp.shape[0], kp.ctypes.data, gw.ctypes.data, dw.ctypes.data,
+                       None if cs is None else cs.ctypes.data, None if sw is None else sw.ctypes.data, float(pi0),
+                       bits)
+        ao = AvgOut(*[o[n].ctypes.data for n in ("gene_bf", "gene_post", "snp_bf", "dim_bf", "post_the", "post_an",
+                                                  "post_dim", "post_sub", "best_dim")])
+        self._check(lib().eqbf_avg_bfs(self._ctx, raw.ctypes.data, NP, off.ctypes.data, G, C.byref(opts),
+                                       C.byref(ao)))
+        return o
+
+    def avg_bfs_device(self, raw_dev, n_pairs, off_dev, n_genes, opts, out_struct, stream=None):
+        self._check(lib().eqbf_avg_bfs_device(self._ctx, raw_dev, n_pairs, off_dev, n_genes, C.byref(opts),
+                                              C.byref(out_struct), stream))
 
     def assoc_hybrid_device(self, chunk_struct, opts, out_struct, stream=None):
         self._check(lib().eqbf_assoc_hybrid_device(self._ctx, C.byref(chunk_struct), C.byref(opts),

@@ -513,6 +513,61 @@ int eqbf_hm_bfs(eqbf_ctx* ctx, const double* raw, int64_t n_pairs, const int64_t
 int eqbf_hm_ci(eqbf_ctx* ctx, const double* raw, int64_t n_pairs, const int64_t* gene_offset, int64_t n_genes,
                const eqbf_hm_opts* opts, const eqbf_hm_params* params, double* left, double* right);
 
+/* ---- eqtlbma_avg_bfs: averaging raw ABFs, posteriors ------------------------------
+ * Replaces the per-gene loop of averageBFs (src/eqtlbma_avg_bfs.cpp:1025-1101)
+ * over Snp::avg_raw_bfs (:418-475), Gene::avg_raw_bfs (:598-622) and the
+ * posteriors (:634-796); the host program keeps the reference's file
+ * parsing, filters, SNP ordering and writers.  Raw log10 ABFs [n_pairs][R]
+ * [n_grid] in FP64: R = dim for the configs model (the j-th configuration
+ * row read for the SNP goes with the j-th weight, as in the reference),
+ * R = n_subgroups for the types model (subgroup-specific rows).  Every sum
+ * is utils::log10_weighted_sum (utils_math.cpp:100-159).  Outputs (FP64,
+ * [n_genes] or [n_pairs][...]): gene_bf, snp_bf, dim_bf [n_pairs][dim],
+ * best_dim (configs: first strict maximum of dim_bf, replaced by that of
+ * post_dim when computed; types: -1 unless post_dim is computed), and the
+ * posteriors selected by want: EQBF_AVG_POST_A gene_post, _B post_the,
+ * _C post_an (never capped: the reference tests post_the, :677-678), _D
+ * post_sub [n_pairs][n_subgroups] (configs: sums of post_dim over the
+ * configurations whose subgroup bit is set in cfg_subgroups), _DIMS
+ * post_dim [n_pairs][dim] (--bestdim / --alldim; the configs model computes
+ * it also for _D).  Outputs not selected may be NULL. */
+enum { EQBF_AVG_CONFIGS = 0, EQBF_AVG_TYPES = 1 };
+enum { EQBF_AVG_POST_A = 1, EQBF_AVG_POST_B = 2, EQBF_AVG_POST_C = 4, EQBF_AVG_POST_D = 8, EQBF_AVG_POST_DIMS = 16 };
+typedef struct eqbf_avg_opts {
+  int32_t model;               /* EQBF_AVG_CONFIGS / EQBF_AVG_TYPES (--model)          */
+  int32_t dim;                 /* --dim: configurations or types                       */
+  int32_t n_subgroups;         /* types: raw rows per pair; configs: post_sub columns  */
+  int32_t n_grid;              /* raw values per row (grid weights read, --gwts)       */
+  int32_t n_keep;              /* grid points kept (--gtk)                             */
+  const int32_t* keep;         /* [n_keep] indices < n_grid                            */
+  const double* grid_weights;  /* [n_grid]                                             */
+  const double* dim_weights;   /* [dim] configuration (--cwts) or type (--tswts) weights */
+  const uint64_t* cfg_subgroups; /* [dim] configs model: bit s = subgroup s+1 in the name */
+  const double* sub_weights;   /* [dim][n_subgroups] types model                       */
+  double pi0;                  /* --pi0 (EQBF_AVG_POST_A)                              */
+  int32_t want;                /* EQBF_AVG_POST_* bits                                 */
+} eqbf_avg_opts;
+typedef struct eqbf_avg_out {
+  double* gene_bf;
+  double* gene_post;
+  double* snp_bf;
+  double* dim_bf;
+  double* post_the;
+  double* post_an;
+  double* post_dim;
+  double* post_sub;
+  int32_t* best_dim;
+} eqbf_avg_out;
+int eqbf_avg_bfs(eqbf_ctx* ctx, const double* raw, int64_t n_pairs, const int64_t* gene_offset, int64_t n_genes,
+                 const eqbf_avg_opts* opts, eqbf_avg_out* out);
+/* raw_dev, gene_offset_dev and out_dev in HBM; opts' arrays stay host arrays */
+int eqbf_avg_bfs_device(eqbf_ctx* ctx, const double* raw_dev, int64_t n_pairs, const int64_t* gene_offset_dev,
+                        int64_t n_genes, const eqbf_avg_opts* opts, eqbf_avg_out* out_dev, void* hip_stream);
+/* Without configuration or type weights (averageRawBFsOverGrid, :834-879):
+ * each raw row [n_rows][n_grid] averaged over the kept grid points. */
+int eqbf_avg_grid(eqbf_ctx* ctx, const double* raw, int64_t n_rows, int32_t n_grid, const double* grid_weights,
+                  const int32_t* keep, int32_t n_keep, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -82,6 +82,9 @@ def lib():
         L.orc_assoc_hybrid.restype = i32
         L.orc_hybrid_pair_stats.argtypes = [vp, i64, i64, d, vp, vp, vp, vp, vp]
         L.orc_hybrid_pair_stats.restype = i32
+        L.orc_avg_grid.argtypes = [vp, i64, i32, vp, vp, i32, vp]
+        L.orc_avg_bfs.argtypes = [vp, vp, i64, i32, i32, i32, i32, vp, i32, vp, vp, vp, vp, vp, d, i32,
+                                  vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.orc_hm_ci.argtypes = [vp, vp, i64, i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp]
         L.orc_hm_ci.restype = i32
         L.orc_abf_from_sstats.argtypes = [i32, i64, vp, vp, vp, vp, i32, vp, vp, i32, vp, vp, i32, vp, vp, vp]
@@ -467,3 +470,56 @@ def hm_ci(raw, off, prm):
     if rc:
         raise RuntimeError("oracle: NaN or infinite log10 likelihood")
     return left, right
+
+
+def avg_grid(raw, grid_w, keep=None):
+    """eqtlbma_avg_bfs without configuration weights: each row of raw
+    [n_rows][L] averaged over the kept grid points (default: all)."""
+    raw = np.ascontiguousarray(raw, dtype=np.float64)
+    L = raw.shape[1]
+    gw = np.ascontiguousarray(grid_w, dtype=np.float64)
+    kp = np.ascontiguousarray(np.arange(L) if keep is None else keep, dtype=np.int32)
+    out = np.zeros(raw.shape[0])
+    lib().orc_avg_grid(_p(raw), raw.shape[0], L, _p(gw), _p(kp), kp.shape[0], _p(out))
+    return out
+
+
+AVG_POST = {"a": 1, "b": 2, "c": 4, "d": 8, "dims": 16}
+
+
+def avg_bfs(raw, off, grid_w, model="configs", cfg_w=None, cfg_sub=None, type_w=None, sub_w=None, pi0=np.nan,
+            want=(), keep=None, n_subgroups=None):
+    """eqtlbma_avg_bfs averaging over the grid and the configurations or
+    types, with the posteriors named in want ('a', 'b', 'c', 'd', 'dims').
+    raw [n_pairs][R][L] (R = configurations or subgroups); off [G + 1].
+    Returns a dict of arrays."""
+    raw = np.ascontiguousarray(raw, dtype=np.float64)
+    NP, R, L = raw.shape
+    off = np.ascontiguousarray(off, dtype=np.int64)
+    G = off.shape[0] - 1
+    types = model == "types"
+    gw = np.ascontiguousarray(grid_w, dtype=np.float64)
+    kp = np.ascontiguousarray(np.arange(L) if keep is None else keep, dtype=np.int32)
+    if types:
+        tw = np.ascontiguousarray(type_w, dtype=np.float64)
+        sw = np.ascontiguousarray(sub_w, dtype=np.float64)
+        D, S = tw.shape[0], R
+        cw, cs = np.zeros(D), np.zeros(D, np.uint64)
+    else:
+        cw = np.ascontiguousarray(cfg_w, dtype=np.float64)
+        D = R
+        S = n_subgroups if n_subgroups is not None else 1
+        cs = np.ascontiguousarray(cfg_sub if cfg_sub is not None else np.zeros(D), dtype=np.uint64)
+        tw, sw = np.zeros(1), np.zeros(1)
+    bits = 0
+    for k in want:
+        bits |= AVG_POST[k]
+    o = dict(gene_bf=np.full(G, np.nan), gene_post=np.full(G, np.nan), snp_bf=np.full(NP, np.nan),
+             dim_bf=np.full((NP, D), np.nan), post_the=np.full(NP, np.nan), post_an=np.full(NP, np.nan),
+             post_dim=np.full((NP, D), np.nan), post_sub=np.full((NP, S), np.nan),
+             best_dim=np.full(NP, -1, np.int32))
+    lib().orc_avg_bfs(_p(raw), _p(off), G, int(types), D, S, L, _p(kp), kp.shape[0], _p(gw), _p(cw), _p(cs),
+                      _p(tw), _p(sw), float(pi0), bits, _p(o["gene_bf"]), _p(o["gene_post"]), _p(o["snp_bf"]),
+                      _p(o["dim_bf"]), _p(o["post_the"]), _p(o["post_an"]), _p(o["post_dim"]), _p(o["post_sub"]),
+                      _p(o["best_dim"]))
+    return o
