@@ -12,3 +12,4 @@ REPO_DIR = os.path.dirname(PKG_DIR)
 LIB_PATH = os.path.join(PKG_DIR, "libeqbf_hip.so")
 CLI_PATH = os.path.join(PKG_DIR, "eqtlbma_bf")
 CLI_HM_PATH = os.path.join(PKG_DIR, "eqtlbma_hm")
+CLI_AVG_PATH = os.path.join(PKG_DIR, "eqtlbma_avg_bfs")

@@ -19,6 +19,8 @@ LIB = os.path.join(HERE, "libeqbf_hip.so")
 CLI = os.path.join(HERE, "eqtlbma_bf")
 HOST_HM = os.path.join(HERE, "host_hm")
 CLI_HM = os.path.join(HERE, "eqtlbma_hm")
+HOST_AVG = os.path.join(HERE, "host_avg")
+CLI_AVG = os.path.join(HERE, "eqtlbma_avg_bfs")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("EQBF_ARCH", "gfx950")
@@ -82,10 +84,21 @@ def build_hm():
     return CLI_HM
 
 
+def build_avg():
+    srcs = sorted(glob.glob(os.path.join(HOST_AVG, "*.cpp"))) + [os.path.join(HOST, "util.cpp")]
+    tmp = CLI_AVG + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I" + INCLUDE, "-o", tmp] + srcs +
+         ["-L" + HERE, "-leqbf_hip", "-Wl,-rpath,$ORIGIN", "-L/opt/rocm/lib", "-Wl,-rpath-link,/opt/rocm/lib",
+          "-lz"])
+    os.replace(tmp, CLI_AVG)
+    return CLI_AVG
+
+
 def build(jobs=8):
     build_lib(jobs)
     build_cli()
     build_hm()
+    build_avg()
 
 
 if __name__ == "__main__":

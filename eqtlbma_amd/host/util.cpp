@@ -112,4 +112,23 @@ bool file_exists(const std::string& path) {
   return stat(path.c_str(), &st) == 0;
 }
 
+std::vector<std::string> read_file_lines(const std::string& path) {
+  gzFile f = gzopen(path.c_str(), "rb");
+  if (!f) fail("can't open file " + path);
+  gzbuffer(f, 1 << 20);
+  std::string buf;
+  std::vector<char> chunk(1 << 20);
+  while (true) {
+    const int n = gzread(f, chunk.data(), (unsigned)chunk.size());
+    if (n < 0) fail("gzread failed on " + path);
+    if (n == 0) break;
+    buf.append(chunk.data(), (size_t)n);
+  }
+  gzclose(f);
+  if (!buf.empty()) buf.pop_back();
+  std::vector<std::string> out;
+  tokenize(buf, "\n", out);
+  return out;
+}
+
 }  // namespace host

@@ -67,6 +67,11 @@ void put_int(std::string& out, int64_t v);
 
 bool file_exists(const std::string& path);
 
+/* Whole-file read with the rules of the reference's readFile
+ * (utils_io.cpp:290-376): the last byte of the file is dropped (normally
+ * its final newline), lines split at '\n' with empty ones removed. */
+std::vector<std::string> read_file_lines(const std::string& path);
+
 }  // namespace host
 
 #endif

@@ -203,24 +203,7 @@ struct Data {
   std::vector<double> raw; /* [pairs][D][L] */
 };
 
-/* Whole-file read with readFile's rules: the last byte is dropped, lines
- * split at '\n' with empty ones removed. */
-std::vector<std::string> read_lines(const std::string& path) {
-  gzFile f = gzopen(path.c_str(), "rb");
-  if (!f) host::fail("can't open file " + path);
-  gzbuffer(f, 1 << 20);
-  std::string buf;
-  std::vector<char> chunk(1 << 20);
-  while (true) {
-    const int n = gzread(f, chunk.data(), (unsigned)chunk.size());
-    if (n < 0) host::fail("gzread failed on " + path);
-    if (n == 0) break;
-    buf.append(chunk.data(), (size_t)n);
-  }
-  gzclose(f);
-  if (!buf.empty()) buf.pop_back();
-  return split(buf, "\n");
-}
+std::vector<std::string> read_lines(const std::string& path) { return host::read_file_lines(path); }
 
 void load_file(const std::string& path, const Options& o, size_t D, Data& d) {
   const std::vector<std::string> lines = read_lines(path);
