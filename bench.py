@@ -1066,6 +1066,7 @@ def hm_leg(args, torch, result):
                         "achieved": alg / (pair_ms / 1e3) / 1e9, "peak": 8000.0, "unit": "GB/s",
                         "frac": alg / (pair_ms / 1e3) / 1e9 / 8000.0, "avg_launch_ms": pair_ms,
                         "traffic": _load_traffic_hm("hm_pair_kernel", G)}}
+    avg_leg(args, torch, result, eng, raw, off, off_d, K, L, 5, p.pi0, grid, cfg)
     if args.no_cpu_baseline:
         eng.close()
         del raw, bump, off_d
@@ -1115,6 +1116,100 @@ def hm_leg(args, torch, result):
     del raw, bump, off_d
     torch.cuda.empty_cache()
     result["hm"] = res
+
+
+def avg_leg(args, torch, result, eng, raw, off, off_d, K, L, S, pi0, grid, cfg):
+    """eqtlbma_avg_bfs after the EM on the same device-resident raw log10
+    ABFs (configs model, 31 configurations of 5 subgroups, 10 grid points,
+    the EM's weights): per-configuration and SNP BFs, gene BFs and every
+    posterior (--save bf+post --post a+b+c+d --alldim --bestdim)."""
+    from eqtlbma_amd import _abi
+    from eqtlbma_amd.engine import AvgOpts, AvgOut
+    from oracle import oracle as orc
+    G, NP = off.shape[0] - 1, int(off[-1])
+    dev = raw.device
+    masks = np.array(list(_abi.config_masks("all", S)), np.uint64)
+    keep = np.arange(L, dtype=np.int32)
+    gw, cw = np.ascontiguousarray(grid, dtype=np.float64), np.ascontiguousarray(cfg, dtype=np.float64)
+    want = 1 | 2 | 4 | 8 | 16
+    outs = {"gene_bf": torch.empty(G, dtype=torch.float64, device=dev),
+            "gene_post": torch.empty(G, dtype=torch.float64, device=dev),
+            "snp_bf": torch.empty(NP, dtype=torch.float64, device=dev),
+            "dim_bf": torch.empty((NP, K), dtype=torch.float64, device=dev),
+            "post_the": torch.empty(NP, dtype=torch.float64, device=dev),
+            "post_an": torch.empty(NP, dtype=torch.float64, device=dev),
+            "post_dim": torch.empty((NP, K), dtype=torch.float64, device=dev),
+            "post_sub": torch.empty((NP, S), dtype=torch.float64, device=dev),
+            "best_dim": torch.empty(NP, dtype=torch.int32, device=dev)}
+    names = ("gene_bf", "gene_post", "snp_bf", "dim_bf", "post_the", "post_an", "post_dim", "post_sub", "best_dim")
+    ao = AvgOut(*[outs[n].data_ptr() for n in names])
+    opts = AvgOpts(0, K, S, L, L, keep.ctypes.data, gw.ctypes.data, cw.ctypes.data, masks.ctypes.data, None,
+                   float(pi0), want)
+    sptr = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    eng.avg_bfs_device(raw.data_ptr(), NP, off_d.data_ptr(), G, opts, ao, sptr)  # warm-up
+    torch.cuda.synchronize()
+    eng.profile(True)
+    before = eng.profile_read()
+    reps = 3
+    t = time.perf_counter()
+    for _ in range(reps):
+        eng.avg_bfs_device(raw.data_ptr(), NP, off_d.data_ptr(), G, opts, ao, sptr)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / reps
+    after = eng.profile_read()
+    eng.profile(False)
+    kern = {k: ((v[0] - before.get(k, (0.0, 0))[0]) / reps, (v[1] - before.get(k, (0, 0))[1]) // reps)
+            for k, v in after.items() if v[1] > before.get(k, (0, 0))[1]}
+    pk = kern.get("avg_pair_kernel", (float("nan"), 1))
+    # algorithmic bytes of avg_pair_kernel: the raw ABFs read once, the
+    # per-configuration BFs, the SNP BF and the best configuration written once
+    alg = NP * (K * L * 8.0 + K * 8.0 + 8.0 + 4.0)
+    res = {"metric": "SNP-gene pairs averaged per second (eqtlbma_avg_bfs)", "value": NP / dt, "unit": "pairs/s",
+           "seconds_per_pass": dt,
+           "workload": "eqtlbma_avg_bfs configs model on the eqtlbma_hm workload's raw ABFs (%d pairs, 31 "
+                       "configurations x 10 grid points, FP64, resident in HBM), the EM's weights, every "
+                       "posterior" % NP,
+           "kernels_ms_per_pass": {k: v[0] for k, v in kern.items()},
+           "roofline": {"bound": "hbm", "kernel": "avg_pair_kernel", "alg_bytes_per_launch": alg,
+                        "achieved": alg / (pk[0] / 1e3) / 1e9, "peak": 8000.0, "unit": "GB/s",
+                        "frac": alg / (pk[0] / 1e3) / 1e9 / 8000.0, "avg_launch_ms": pk[0] / max(pk[1], 1)}}
+    gs = 40
+    raw_h = raw[:int(off[gs])].cpu().numpy()
+    off_h = off[:gs + 1].copy()
+    kw = dict(cfg_w=cw, cfg_sub=masks, pi0=float(pi0), want=("a", "b", "c", "d", "dims"), n_subgroups=S)
+    t = time.perf_counter()
+    ref = orc.avg_bfs(raw_h, off_h, gw, **kw)
+    d1 = time.perf_counter() - t
+    np_h = int(off[gs])
+    err = {}
+    for n in names[:-1]:
+        a = outs[n][:gs].cpu().numpy() if n in ("gene_bf", "gene_post") else outs[n][:np_h].cpu().numpy()
+        b = ref[n]
+        fin = np.isfinite(b)
+        err[n] = float(np.max(np.abs(a[fin] - b[fin]), initial=0))
+    err["best_dim_equal"] = bool(np.array_equal(outs["best_dim"][:np_h].cpu().numpy(), ref["best_dim"]))
+    res["error_vs_fp64_oracle"] = dict(sample_genes=gs, **err)
+    if not args.no_cpu_baseline:
+        model, T = host_cpu()
+        k = min(G, max(gs, int(gs * args.cpu_seconds * T / max(d1, 1e-3))))
+        if off[k] * K * L * 8.0 > 4e9:
+            k = max(gs, int(np.searchsorted(off, 4e9 / (K * L * 8.0), side="right")) - 1)
+        bounds = [(int(c[0]), int(c[-1]) + 1) for c in np.array_split(np.arange(k), min(T, k))]
+        shards = [(raw[int(off[a]):int(off[b])].cpu().numpy(), off[a:b + 1] - off[a]) for a, b in bounds]
+        t = time.perf_counter()
+        threaded(lambda sh: orc.avg_bfs(sh[0], sh[1], gw, **kw), shards, T)
+        tc = time.perf_counter() - t
+        cp = int(off[k])
+        res["cpu_baseline"] = dict(value=cp / tc, unit="pairs/s",
+                                   sample="first %d genes (%d pairs), one averaging pass with every posterior, "
+                                          "%.1f s wall" % (k, cp, tc),
+                                   **cpu_label(model, T, "gene ranges sharded over threads"))
+        del shards
+    log("avg_bfs: %d pairs in %.1f ms; kernels %s; avg_pair_kernel %.1f%% of HBM peak" %
+        (NP, dt * 1e3, {k: round(v[0], 2) for k, v in kern.items()}, 100 * res["roofline"]["frac"]))
+    del outs
+    torch.cuda.empty_cache()
+    result["avg_bfs"] = res
 
 
 def alg_bytes(P_g, S, N, GL, GS):

@@ -13,14 +13,16 @@
  * Kernels:
  *   avg_grid_kernel  one thread per raw row: the row averaged over the kept
  *                    grid points (BayesFactor::avg_raw_bfs, :371-382)
- *   avg_pair_kernel  one wave per pair, lanes over the dimensions: the
+ *   avg_pair_kernel  one wave per pair (four waves a block striding over
+ *                    the pairs), lanes over the dimensions: the
  *                    per-dimension BFs (Snp::avg_raw_bfs, :418-475), the
- *                    SNP BF over them and the first strict maximum
+ *                    SNP BF over them and the first strict maximum; then,
+ *                    when asked, post_dim and its argmax (:683-718) and the
+ *                    configs model's subgroup posteriors (:722-745)
  *   avg_gene_kernel  one wave per gene, lanes over its SNPs: the gene BF
  *                    (Gene::avg_raw_bfs, :598-622), gene_post (:634-643),
  *                    post_the (:647-662), post_an (:664-680)
- *   avg_post_kernel  one wave per pair: post_dim and its argmax (:683-718),
- *                    subgroup posteriors (:722-796)
+ *   avg_post_kernel  the types model's subgroup posteriors (:749-796)
  * The roofline is HBM: each raw value is read once by avg_pair_kernel (and
  * again by avg_post_kernel for the types model's subgroup posteriors).
  */
@@ -112,45 +114,108 @@ __global__ void __launch_bounds__(256) avg_grid_kernel(const double* __restrict_
   out[r] = l10_round(mx + log10(sum));
 }
 
-__global__ void __launch_bounds__(64) avg_pair_kernel(const AvgArgs a) {
-  const int64_t p = blockIdx.x;
-  const int lane = threadIdx.x;
+/* reductions over the aligned group of w lanes (w a power of two <= 64) */
+__device__ inline double gmax(double v, int w) {
+  for (int o = w >> 1; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ inline double gsum(double v, int w) {
+  for (int o = w >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+/* first index of the strict maximum scanning from index 0 (the reference's
+ * "j == 0 || v[j] > v[best]"): v0 is the first value, (lmax, larg) the
+ * lane's largest value and its first index (larg = n when it has none) */
+__device__ inline int first_max(double v0, double lmax, int larg, int n, int w) {
+  const double mx = gmax(lmax, w);
+  int cand = (larg < n && lmax == mx) ? larg : n;
+  for (int o = w >> 1; o > 0; o >>= 1) cand = min(cand, __shfl_xor(cand, o, 64));
+  if (isnan(v0) || !(mx > v0)) return 0;
+  return cand >= n ? 0 : cand;
+}
+
+constexpr int kAvgWaves = 4;
+
+/* A group of w lanes per pair (w = the power of two >= dim, at most 64:
+ * 64 / w pairs a wave), four waves a block striding over the pairs; lanes
+ * over the dimensions.  Also writes post_dim, its best dimension and the
+ * configs model's subgroup posteriors when asked: they need only the pair's
+ * own values. */
+__global__ void __launch_bounds__(64 * kAvgWaves) avg_pair_kernel(const AvgArgs a, int w) {
+  const int lane = threadIdx.x & 63;
+  const int gl = lane & (w - 1), g0 = lane - gl; /* lane in the group, group's first lane */
+  const int per_wave = 64 / w;
   const int D = a.D;
-  double* dims = a.dim_bf + (size_t)p * D;
-  /* per-dimension BFs; the lane's largest value and its first index */
-  double lmax = -INFINITY;
-  int larg = D;
-  for (int k = lane; k < D; k += 64) {
-    double d;
-    if (!a.types) {
-      const double* r = a.raw + ((size_t)p * D + k) * a.L;
-      d = grid_lws(a, [&](int i) { return r[a.keep[i]]; });
-    } else {
-      d = grid_lws(a, [&](int i) { return types_point(a, p, k, i, -1); });
+  const bool dims_post = (a.want & 16) || (!a.types && (a.want & 8));
+  const bool sub_post = !a.types && (a.want & 8);
+  const int64_t step = (int64_t)gridDim.x * kAvgWaves * per_wave;
+  for (int64_t pb = ((int64_t)blockIdx.x * kAvgWaves + (threadIdx.x >> 6)) * per_wave; pb < a.NP; pb += step) {
+    const int64_t p = pb + lane / w;
+    const bool on = p < a.NP; /* every lane still takes part in the shuffles */
+    double* dims = a.dim_bf + (size_t)p * D;
+    /* per-dimension BFs; the lane's largest value and its first index */
+    double lmax = -INFINITY, dk = NAN;
+    int larg = D;
+    for (int k = gl; on && k < D; k += w) {
+      double d;
+      if (!a.types) {
+        const double* r = a.raw + ((size_t)p * D + k) * a.L;
+        d = grid_lws(a, [&](int i) { return r[a.keep[i]]; });
+      } else {
+        d = grid_lws(a, [&](int i) { return types_point(a, p, k, i, -1); });
+      }
+      dims[k] = d;
+      if (k == gl) dk = d;
+      if (d > lmax) {
+        lmax = d;
+        larg = k;
+      }
     }
-    dims[k] = d;
-    if (d > lmax) {
-      lmax = d;
-      larg = k;
+    const double d0 = __shfl(dk, g0, 64); /* the group's first lane holds dims[0] */
+    /* maximum as the reference scans it: from the first value, strict > */
+    double mx = gmax(lmax, w);
+    if (isnan(d0)) mx = NAN;
+    if (!(mx > d0)) mx = d0;
+    const int best0 = first_max(d0, lmax, larg, D, w);
+    double sum = 0.0;
+    for (int k = gl; on && k < D; k += w) {
+      const double d = (k == gl) ? dk : dims[k];
+      if (!isnan(d)) sum += a.dw[k] * exp10(d - mx);
     }
-  }
-  const double d0 = __shfl(lane == 0 ? dims[0] : 0.0, 0, 64);
-  /* maximum as the reference scans it: from the first value, strict > */
-  double mx = wmax(lmax);
-  if (isnan(d0)) mx = NAN;
-  if (!(mx > d0)) mx = d0;
-  /* first index reaching the maximum (configs model's best dimension) */
-  int cand = (larg < D && lmax == mx) ? larg : D;
-  for (int o = 32; o > 0; o >>= 1) cand = min(cand, __shfl_xor(cand, o, 64));
-  double sum = 0.0;
-  for (int k = lane; k < D; k += 64) {
-    const double d = dims[k];
-    if (!isnan(d)) sum += a.dw[k] * exp10(d - mx);
-  }
-  sum = wsum(sum);
-  if (lane == 0) {
-    a.snp_bf[p] = l10_round(mx + log10(sum));
-    a.best_dim[p] = a.types ? -1 : (isnan(d0) || cand >= D ? 0 : cand);
+    sum = gsum(sum, w);
+    const double sbp = l10_round(mx + log10(sum));
+    int best = a.types ? -1 : best0;
+    if (dims_post) {
+      double* pd = a.post_dim + (size_t)p * D;
+      double pmax = -INFINITY, pk = NAN;
+      int parg = D;
+      for (int k = gl; on && k < D; k += w) {
+        const double d = (k == gl) ? dk : dims[k];
+        double v = exp10(log10(a.dw[k]) + d - sbp);
+        if (v > 1.0) v = 1.0;
+        pd[k] = v;
+        if (k == gl) pk = v;
+        if (v > pmax) {
+          pmax = v;
+          parg = k;
+        }
+      }
+      best = first_max(__shfl(pk, g0, 64), pmax, parg, D, w);
+      if (sub_post) {
+        for (int s = 0; s < a.S; ++s) {
+          double acc = 0.0;
+          for (int k = gl; on && k < D; k += w)
+            if ((a.cs[k] >> s) & 1ull) acc += (k == gl) ? pk : pd[k];
+          acc = gsum(acc, w);
+          if (on && gl == 0) a.post_sub[(size_t)p * a.S + s] = acc > 1.0 ? 1.0 : acc;
+        }
+      }
+    }
+    if (on && gl == 0) {
+      a.snp_bf[p] = sbp;
+      a.best_dim[p] = best;
+    }
   }
 }
 
@@ -215,49 +280,21 @@ __global__ void __launch_bounds__(64) avg_gene_kernel(const AvgArgs a) {
   }
 }
 
-__global__ void __launch_bounds__(64) avg_post_kernel(const AvgArgs a) {
-  const int64_t p = blockIdx.x;
-  const int lane = threadIdx.x;
+/* types model's subgroup posteriors (:749-796): one wave per pair
+ * (grid-stride), lanes over the subgroups */
+__global__ void __launch_bounds__(64 * kAvgWaves) avg_post_kernel(const AvgArgs a) {
+  const int lane = threadIdx.x & 63;
   const int D = a.D, S = a.S;
-  const double sbp = a.snp_bf[p];
-  const bool dims_post = (a.want & 16) || (!a.types && (a.want & 8));
-  if (dims_post) {
-    const double* dims = a.dim_bf + (size_t)p * D;
-    double* pd = a.post_dim + (size_t)p * D;
-    double lmax = -INFINITY;
-    int larg = D;
-    for (int k = lane; k < D; k += 64) {
-      double v = exp10(log10(a.dw[k]) + dims[k] - sbp);
-      if (v > 1.0) v = 1.0;
-      pd[k] = v;
-      if (v > lmax) {
-        lmax = v;
-        larg = k;
-      }
-    }
-    /* best: index 0, then any strictly larger value (the first of the maximum) */
-    const double v0 = __shfl(lane == 0 ? pd[0] : 0.0, 0, 64);
-    double mx = wmax(lmax);
-    int cand = (larg < D && lmax == mx && mx > v0) ? larg : D;
-    for (int o = 32; o > 0; o >>= 1) cand = min(cand, __shfl_xor(cand, o, 64));
-    if (lane == 0) a.best_dim[p] = (isnan(v0) || cand >= D) ? 0 : cand;
-  }
-  if (a.want & 8) {
-    /* every lane reads post_dim written above by the other lanes */
-    __syncthreads();
+  for (int64_t p = (int64_t)blockIdx.x * kAvgWaves + (threadIdx.x >> 6); p < a.NP;
+       p += (int64_t)gridDim.x * kAvgWaves) {
+    const double sbp = a.snp_bf[p];
     for (int s = lane; s < S; s += 64) {
       double acc = 0.0;
-      if (!a.types) {
-        const double* pd = a.post_dim + (size_t)p * D;
-        for (int j = 0; j < D; ++j)
-          if ((a.cs[j] >> s) & 1ull) acc += pd[j];
-      } else {
-        for (int k = 0; k < D; ++k) {
-          double t = log10(a.dw[k]) + log10(a.sw[(size_t)k * S + s]) - sbp;
-          const double* rs = a.raw + ((size_t)p * S + s) * a.L;
-          t += grid_lws(a, [&](int i) { return rs[a.keep[i]] + types_point(a, p, k, i, s); });
-          acc += exp10(t);
-        }
+      for (int k = 0; k < D; ++k) {
+        double t = log10(a.dw[k]) + log10(a.sw[(size_t)k * S + s]) - sbp;
+        const double* rs = a.raw + ((size_t)p * S + s) * a.L;
+        t += grid_lws(a, [&](int i) { return rs[a.keep[i]] + types_point(a, p, k, i, s); });
+        acc += exp10(t);
       }
       a.post_sub[(size_t)p * S + s] = acc > 1.0 ? 1.0 : acc;
     }
@@ -296,18 +333,24 @@ int avg_bfs_run(eqbf_ctx* ctx, const double* raw, int64_t NP, const int64_t* off
   a.nk = o->n_keep;
   a.types = o->model == EQBF_AVG_TYPES;
   a.want = o->want;
+  /* lanes per pair: the power of two >= dim, at most 64; enough blocks to
+   * fill every CU several times over, each wave then strides over the pairs */
+  int w = 1;
+  while (w < a.D && w < 64) w <<= 1;
+  const int64_t per_block = (int64_t)kAvgWaves * (64 / w);
+  const unsigned pblocks = (unsigned)std::min<int64_t>((NP + per_block - 1) / per_block, (int64_t)256 * 64);
   if (NP > 0) {
     ProfScope ps(ctx, "avg_pair_kernel", st);
-    hipLaunchKernelGGL(avg_pair_kernel, dim3((unsigned)NP), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(avg_pair_kernel, dim3(pblocks), dim3(64 * kAvgWaves), 0, st, a, w);
   }
   if (G > 0) {
     ProfScope ps(ctx, "avg_gene_kernel", st);
     hipLaunchKernelGGL(avg_gene_kernel, dim3((unsigned)G), dim3(64), 0, st, a);
   }
-  const bool dims_post = (a.want & 16) || (!a.types && (a.want & 8));
-  if (NP > 0 && (dims_post || (a.want & 8))) {
+  if (NP > 0 && a.types && (a.want & 8)) {
     ProfScope ps(ctx, "avg_post_kernel", st);
-    hipLaunchKernelGGL(avg_post_kernel, dim3((unsigned)NP), dim3(64), 0, st, a);
+    const unsigned qblocks = (unsigned)std::min<int64_t>((NP + kAvgWaves - 1) / kAvgWaves, (int64_t)256 * 64);
+    hipLaunchKernelGGL(avg_post_kernel, dim3(qblocks), dim3(64 * kAvgWaves), 0, st, a);
   }
   EQBF_HIP(ctx, hipGetLastError());
   return EQBF_OK;
