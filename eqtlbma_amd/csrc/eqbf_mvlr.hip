@@ -512,8 +512,11 @@ struct MvGrid {
 template <int SM>
 __global__ void __launch_bounds__(64) mvlr_grid_kernel(const eqbf_chunk c, const MvGrid mg) {
   const int64_t NP = c.n_pairs;
-  const int64_t rr = blockIdx.x / mg.pair_blocks;
-  const int64_t p = (blockIdx.x - rr * mg.pair_blocks) * 64 + threadIdx.x;
+  /* rows fastest: the blocks in flight share one tile of 64 pairs, whose
+   * parameters stay in L2 across the rows */
+  const int64_t pb = blockIdx.x / mg.nrows;
+  const int64_t rr = blockIdx.x - pb * mg.nrows;
+  const int64_t p = pb * 64 + threadIdx.x;
   if (p >= NP || rr >= mg.nrows) return;
   const int64_t row = mg.row0 + rr;
   const int S = c.n_subgroups;
@@ -581,6 +584,242 @@ __global__ void __launch_bounds__(64) mvlr_grid_kernel(const eqbf_chunk c, const
   mg.raw[row * NP + p] = l;
 }
 
+/* ---- per (pair, group of rows): Hessenberg form, every grid point ----------------
+ * With B = k Sigma^-1 T^2 (T = diag t) and the rank-one term o2 k u t'
+ * (u = Sigma^-1 t), the matrix of every row is M = I + p2 B + o2 (k u) t'.
+ * An orthogonal similarity Q'BQ = H (Householder reflections, upper
+ * Hessenberg) is computed once per (pair, configuration); then for each grid
+ * point A = I + p2 H is factorised with partial pivoting in O(S^2) (only
+ * adjacent rows can swap), the rank-one term is added by the matrix
+ * determinant lemma and Sherman-Morrison, and
+ *   ln|det M| = ln|det A| + ln|1 + o2 t~'A^-1 u~|,
+ *   M^-1 eVb  = Q (A^-1 z~ - o2 A^-1 u~ (t~'A^-1 z~) / (1 + o2 t~'A^-1 u~)),
+ * with u~ = k Q'u, z~ = Q'eVb, t~ = Q't; the quadratic form eVb'W M^-1 eVb
+ * = o2 (eVb't)(t~'x~) + p2 (Q'T^2 eVb)'x~.  Rows of the gen-fix type
+ * (p2 = 0) reduce to scalars.  The row kernel above builds and factorises
+ * each M from scratch (O(S^3) per row); this one gives the same values to
+ * rounding.  Groups: for the consistent configuration gen, gen-fix and
+ * gen-maxh (GL rows each), for the others their GS rows. */
+template <int SM>
+struct HessIdx {
+  /* position of entry (i, j), j >= i - 1, of an upper Hessenberg SM x SM matrix */
+  static constexpr int at(int i, int j) { return i == 0 ? j : SM + (i - 1) * SM - (i - 1) * (i - 2) / 2 + (j - (i - 1)); }
+  static constexpr int size = SM == 1 ? 1 : SM + (SM - 1) * SM - (SM - 1) * (SM - 2) / 2;
+};
+
+template <int SM>
+__global__ void __launch_bounds__(64) mvlr_rows_kernel(const eqbf_chunk c, const MvGrid mg) {
+  const int64_t NP = c.n_pairs;
+  const int64_t p = blockIdx.x * (int64_t)64 + threadIdx.x;
+  const int lane = threadIdx.x;
+  const int g = blockIdx.y;
+  /* group g -> configuration (relative to the batch) and row type */
+  int cfg, type;
+  if (mg.cfg0 == 0) {
+    cfg = g < 3 ? 0 : g - 2;
+    type = g < 3 ? g : 3;
+  } else {
+    cfg = g;
+    type = 3;
+  }
+  __shared__ double Hs[HessIdx<SM>::size][64];
+  const bool on = p < NP;
+  const int PSZ = mg.PSZ, S = SM;
+  const double* pr = mg.par + (int64_t)cfg * PSZ * NP + (on ? p : 0);
+  const bool ok = on && pr[(int64_t)(PSZ - 1) * NP] == 0.0;
+  const int n_rows = type == 3 ? mg.GS : mg.GL;
+  const int64_t row_base = type < 3 ? (int64_t)type * mg.GL : 3 * (int64_t)mg.GL + (int64_t)(mg.cfg0 + cfg - 1) * mg.GS;
+  const double* gp = type == 3 ? mg.grid + 2 * mg.GL : mg.grid;
+  const double* go = type == 3 ? mg.grid + 2 * mg.GL + mg.GS : mg.grid + mg.GL;
+  const double ln10 = 2.302585092994045684;
+  double t[SM], u[SM], z[SM];
+  double kk = 0.0;
+  if (ok) {
+    kk = pr[(int64_t)(S * S + 3 * S) * NP];
+#pragma unroll
+    for (int i = 0; i < SM; ++i) {
+      t[i] = pr[(int64_t)(S * S + S + i) * NP];
+      u[i] = pr[(int64_t)(S * S + i) * NP];
+      z[i] = pr[(int64_t)(S * S + 2 * S + i) * NP];
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < SM; ++i) t[i] = u[i] = z[i] = 0.0;
+  }
+  double et = 0.0;
+#pragma unroll
+  for (int i = 0; i < SM; ++i) et += z[i] * t[i];
+  if (type == 1) { /* gen-fix: M = I + o2 k u t' */
+    double tu = 0.0, tz = 0.0;
+#pragma unroll
+    for (int i = 0; i < SM; ++i) {
+      tu += t[i] * u[i];
+      tz += t[i] * z[i];
+    }
+    for (int gi = 0; gi < n_rows; ++gi) {
+      const double o2 = gp[gi] + go[gi];
+      const double sig = 1.0 + o2 * kk * tu;
+      const double quad = o2 * et * (tz / sig);
+      if (on) mg.raw[(row_base + gi) * NP + p] = ok ? (0.5 * quad - 0.5 * log(fabs(sig))) / ln10 : NAN;
+    }
+    return;
+  }
+  /* B = k Sigma^-1 T^2 and the vectors, then the Householder reduction */
+  double ut[SM], zt[SM], tt[SM], vt[SM];
+  {
+    double B[SM][SM];
+#pragma unroll
+    for (int i = 0; i < SM; ++i) {
+#pragma unroll
+      for (int j = 0; j < SM; ++j) B[i][j] = ok ? kk * pr[(int64_t)(i * S + j) * NP] * t[j] * t[j] : 0.0;
+      ut[i] = kk * u[i];
+      zt[i] = z[i];
+      tt[i] = t[i];
+      vt[i] = t[i] * t[i] * z[i];
+    }
+#pragma unroll
+    for (int k = 0; k + 2 < SM; ++k) {
+      double nrm2 = 0.0;
+#pragma unroll
+      for (int i = k + 1; i < SM; ++i) nrm2 += B[i][k] * B[i][k];
+      const double x0 = B[k + 1][k];
+      double alpha = sqrt(nrm2);
+      if (x0 > 0) alpha = -alpha;
+      double v[SM];
+      double vv = 0.0;
+#pragma unroll
+      for (int i = 0; i < SM; ++i) {
+        v[i] = i == k + 1 ? x0 - alpha : (i > k + 1 ? B[i][k] : 0.0);
+        vv += v[i] * v[i];
+      }
+      const double beta = vv > 0.0 ? 2.0 / vv : 0.0;
+#pragma unroll
+      for (int j = k; j < SM; ++j) {
+        double s = 0.0;
+#pragma unroll
+        for (int i = k + 1; i < SM; ++i) s += v[i] * B[i][j];
+        s *= beta;
+#pragma unroll
+        for (int i = k + 1; i < SM; ++i) B[i][j] -= v[i] * s;
+      }
+#pragma unroll
+      for (int i = 0; i < SM; ++i) {
+        double s = 0.0;
+#pragma unroll
+        for (int j = k + 1; j < SM; ++j) s += B[i][j] * v[j];
+        s *= beta;
+#pragma unroll
+        for (int j = k + 1; j < SM; ++j) B[i][j] -= s * v[j];
+      }
+      double s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
+#pragma unroll
+      for (int i = k + 1; i < SM; ++i) {
+        s1 += v[i] * ut[i];
+        s2 += v[i] * zt[i];
+        s3 += v[i] * tt[i];
+        s4 += v[i] * vt[i];
+      }
+#pragma unroll
+      for (int i = k + 1; i < SM; ++i) {
+        ut[i] -= beta * v[i] * s1;
+        zt[i] -= beta * v[i] * s2;
+        tt[i] -= beta * v[i] * s3;
+        vt[i] -= beta * v[i] * s4;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < SM; ++i)
+#pragma unroll
+      for (int j = (i == 0 ? 0 : i - 1); j < SM; ++j) Hs[HessIdx<SM>::at(i, j)][lane] = B[i][j];
+  }
+  for (int gi = 0; gi < n_rows; ++gi) {
+    double p2, o2;
+    if (type == 2) { /* gen-maxh */
+      p2 = gp[gi] + go[gi];
+      o2 = 0.0;
+    } else {
+      p2 = gp[gi];
+      o2 = go[gi];
+    }
+    double W[SM][SM], r1[SM], r2[SM];
+#pragma unroll
+    for (int i = 0; i < SM; ++i) {
+#pragma unroll
+      for (int j = (i == 0 ? 0 : i - 1); j < SM; ++j) W[i][j] = (i == j ? 1.0 : 0.0) + p2 * Hs[HessIdx<SM>::at(i, j)][lane];
+      r1[i] = ut[i];
+      r2[i] = zt[i];
+    }
+    /* LU of the Hessenberg matrix, partial pivoting between rows j and j+1 */
+    double mant = 1.0;
+    int ex = 0;
+#pragma unroll
+    for (int j = 0; j + 1 < SM; ++j) {
+      const bool sw = fabs(W[j + 1][j]) > fabs(W[j][j]);
+#pragma unroll
+      for (int col = j; col < SM; ++col) {
+        const double a = W[j][col], b = W[j + 1][col];
+        W[j][col] = sw ? b : a;
+        W[j + 1][col] = sw ? a : b;
+      }
+      {
+        const double a = r1[j], b = r1[j + 1];
+        r1[j] = sw ? b : a;
+        r1[j + 1] = sw ? a : b;
+        const double c2 = r2[j], d2 = r2[j + 1];
+        r2[j] = sw ? d2 : c2;
+        r2[j + 1] = sw ? c2 : d2;
+      }
+      const double piv = W[j][j];
+      const double f = W[j + 1][j] / piv;
+#pragma unroll
+      for (int col = j + 1; col < SM; ++col) W[j + 1][col] -= f * W[j][col];
+      r1[j + 1] -= f * r1[j];
+      r2[j + 1] -= f * r2[j];
+      int e;
+      mant = frexp(mant * piv, &e);
+      ex += e;
+    }
+    {
+      int e;
+      mant = frexp(mant * W[SM - 1][SM - 1], &e);
+      ex += e;
+    }
+    double lnd = log(fabs(mant)) + (double)ex * 0.69314718055994530942;
+#pragma unroll
+    for (int i = SM - 1; i >= 0; --i) {
+      double a = r1[i], b = r2[i];
+#pragma unroll
+      for (int col = i + 1; col < SM; ++col) {
+        a -= W[i][col] * r1[col];
+        b -= W[i][col] * r2[col];
+      }
+      r1[i] = a / W[i][i];
+      r2[i] = b / W[i][i];
+    }
+    if (o2 != 0.0) {
+      double tau = 0.0, tz = 0.0;
+#pragma unroll
+      for (int i = 0; i < SM; ++i) {
+        tau += tt[i] * r1[i];
+        tz += tt[i] * r2[i];
+      }
+      const double sig = 1.0 + o2 * tau;
+      const double coef = o2 * tz / sig;
+#pragma unroll
+      for (int i = 0; i < SM; ++i) r2[i] -= coef * r1[i];
+      lnd += log(fabs(sig));
+    }
+    double txd = 0.0, vxd = 0.0;
+#pragma unroll
+    for (int i = 0; i < SM; ++i) {
+      txd += tt[i] * r2[i];
+      vxd += vt[i] * r2[i];
+    }
+    const double quad = o2 * et * txd + p2 * vxd;
+    if (on) mg.raw[(row_base + gi) * NP + p] = ok ? (0.5 * quad - 0.5 * lnd) / ln10 : NAN;
+  }
+}
+
 /* ---- per (pair, weighted row): utils::log10_weighted_sum over the grid ------------ */
 __global__ void __launch_bounds__(256) mvlr_wavg_kernel(const double* __restrict__ raw, double* __restrict__ w,
                                                          int64_t NP, int GL, int GS, int64_t wrow0, int64_t nw) {
@@ -613,6 +852,57 @@ __global__ void __launch_bounds__(256) mvlr_wavg_kernel(const double* __restrict
  * Sigma^-1 t, t, eVb, k and a flag (0 = computed).  Configurations go in
  * batches bounded by the parameter buffer and by the 65535 limit of the
  * grid's y dimension. */
+/* the row kernel at the exact subgroup count (no padded rows in the LU) */
+void launch_grid_kernel(int S, unsigned blocks, hipStream_t st, const eqbf_chunk& c, const MvGrid& mg) {
+#define EQBF_GRID_CASE(n) \
+  case n:                 \
+    hipLaunchKernelGGL(mvlr_grid_kernel<n>, dim3(blocks), dim3(64), 0, st, c, mg); \
+    break;
+  switch (S) {
+    EQBF_GRID_CASE(1)
+    EQBF_GRID_CASE(2)
+    EQBF_GRID_CASE(3)
+    EQBF_GRID_CASE(4)
+    EQBF_GRID_CASE(5)
+    EQBF_GRID_CASE(6)
+    EQBF_GRID_CASE(7)
+    EQBF_GRID_CASE(8)
+    EQBF_GRID_CASE(9)
+    EQBF_GRID_CASE(10)
+    EQBF_GRID_CASE(11)
+    EQBF_GRID_CASE(12)
+    EQBF_GRID_CASE(13)
+    EQBF_GRID_CASE(14)
+    EQBF_GRID_CASE(15)
+    default:
+      hipLaunchKernelGGL(mvlr_grid_kernel<16>, dim3(blocks), dim3(64), 0, st, c, mg);
+      break;
+  }
+#undef EQBF_GRID_CASE
+}
+
+/* the Hessenberg row kernel at the exact subgroup count, 2 <= S <= 10 */
+void launch_rows_kernel(int S, dim3 grid, hipStream_t st, const eqbf_chunk& c, const MvGrid& mg) {
+#define EQBF_ROWS_CASE(n) \
+  case n:                 \
+    hipLaunchKernelGGL(mvlr_rows_kernel<n>, grid, dim3(64), 0, st, c, mg); \
+    break;
+  switch (S) {
+    EQBF_ROWS_CASE(2)
+    EQBF_ROWS_CASE(3)
+    EQBF_ROWS_CASE(4)
+    EQBF_ROWS_CASE(5)
+    EQBF_ROWS_CASE(6)
+    EQBF_ROWS_CASE(7)
+    EQBF_ROWS_CASE(8)
+    EQBF_ROWS_CASE(9)
+    default:
+      hipLaunchKernelGGL(mvlr_rows_kernel<10>, grid, dim3(64), 0, st, c, mg);
+      break;
+  }
+#undef EQBF_ROWS_CASE
+}
+
 template <int SM, class Fill>
 int mv_rows_run(eqbf_ctx* ctx, const eqbf_chunk& c, int bfs, eqbf_assoc_out* out, hipStream_t st, Fill fill) {
   const int S = c.n_subgroups;
@@ -622,7 +912,11 @@ int mv_rows_run(eqbf_ctx* ctx, const eqbf_chunk& c, int bfs, eqbf_assoc_out* out
   const int PSZ = S * S + 3 * S + 2;
   const int64_t total = 1 + n_cfg;
   int64_t per = std::max<int64_t>(1, ((int64_t)1 << 30) / std::max<int64_t>(1, NP * PSZ * 8));
-  per = std::min<int64_t>(std::min<int64_t>(per, 65535), total);
+  /* 2 to 10 subgroups: one thread per (pair, group of rows), the grid
+   * points of a group from one Hessenberg reduction (mvlr_rows_kernel);
+   * otherwise one thread per (pair, row) */
+  const bool hess = S >= 2 && S <= 10;
+  per = std::min<int64_t>(std::min<int64_t>(per, hess ? 65533 : 65535), total);
   double* par = (double*)ctx_buf(ctx, "mv_par", (size_t)per * NP * PSZ * sizeof(double));
   if (!par) return ctx_fail(ctx, EQBF_ERR_HIP, "allocating mvlr buffers failed");
   const int64_t pair_blocks = (NP + 63) / 64;
@@ -635,7 +929,23 @@ int mv_rows_run(eqbf_ctx* ctx, const eqbf_chunk& c, int bfs, eqbf_assoc_out* out
     const int64_t row1 = 3 * (int64_t)GL + (c1 - 1) * GS;
     const int64_t wrow0 = c0 == 0 ? 0 : 3 + (c0 - 1);
     const int64_t wrow1 = 3 + (c1 - 1);
-    for (int64_t r0 = row0; r0 < row1;) {
+    if (hess) {
+      MvGrid mg;
+      mg.par = par;
+      mg.grid = (const double*)ctx->d_grid.ptr;
+      mg.raw = (double*)out->raw_abf;
+      mg.GL = GL;
+      mg.GS = GS;
+      mg.cfg0 = (int)c0;
+      mg.PSZ = PSZ;
+      mg.row0 = row0;
+      mg.nrows = row1 - row0;
+      mg.pair_blocks = pair_blocks;
+      const unsigned groups = (unsigned)(nb + (c0 == 0 ? 2 : 0));
+      ProfScope ps(ctx, "mvlr_rows_kernel", st);
+      launch_rows_kernel(S, dim3((unsigned)pair_blocks, groups), st, c, mg);
+    }
+    for (int64_t r0 = row0; !hess && r0 < row1;) {
       const int64_t nr = std::min<int64_t>(row1 - r0, std::max<int64_t>(1, ((int64_t)1 << 30) / pair_blocks));
       MvGrid mg;
       mg.par = par;
@@ -650,7 +960,7 @@ int mv_rows_run(eqbf_ctx* ctx, const eqbf_chunk& c, int bfs, eqbf_assoc_out* out
       mg.pair_blocks = pair_blocks;
       {
         ProfScope ps(ctx, "mvlr_grid_kernel", st);
-        hipLaunchKernelGGL(mvlr_grid_kernel<SM>, dim3((unsigned)(nr * pair_blocks)), dim3(64), 0, st, c, mg);
+        launch_grid_kernel(S, (unsigned)(nr * pair_blocks), st, c, mg);
       }
       r0 += nr;
     }
