@@ -1265,6 +1265,7 @@ struct HybTile {
 
 struct HybPair {
   const HybTile* tiles;
+  const uint64_t* pres; /* [G][S][W] presence bits */
   const double* cp;
   const double* yp;
   const double* rec;
@@ -1295,151 +1296,215 @@ __device__ inline double hyb_quad(const double* __restrict__ Ai, const double (&
   return s;
 }
 
-__device__ inline double hyb_geno(const eqbf_chunk& c, int s, int i, int64_t snp) {
-  const int64_t off = (int64_t)c.geno_set_of_sbgrp[s] * c.geno_set_stride + (int64_t)i * c.geno_ld + snp;
-  return c.geno_dtype == EQBF_F32 ? (double)((const float*)c.genotypes)[off] : ((const double*)c.genotypes)[off];
+/* presence bits: bit i of word w of (gene, subgroup) = individual 64 w + i
+ * is genotyped in the subgroup and has a level (not NaN) for the gene */
+__global__ void __launch_bounds__(64) hyb_pres_kernel(const eqbf_chunk c, uint64_t* __restrict__ pres, int W) {
+  const int g = blockIdx.x, s = blockIdx.y, lane = threadIdx.x;
+  const int S = c.n_subgroups, N = c.n_samples;
+  for (int w = 0; w < W; ++w) {
+    const int i = w * 64 + lane;
+    const bool in = i < N && c.geno_in_sbgrp[(int64_t)s * N + i] && !isnan(c.explevels[((int64_t)g * S + s) * N + i]);
+    const uint64_t m = __ballot(in);
+    if (lane == 0) pres[((int64_t)g * S + s) * W + w] = m;
+  }
 }
 
-/* one lane per pair; the 64 pairs of a block belong to one gene, so every
- * test of membership and every covariate and level read is uniform */
-template <int Q1M>
+/* one lane per pair; the 64 pairs of a block belong to one gene.  The
+ * individuals go by chunks of 64: the block stages the chunk's covariates
+ * and levels in LDS (every lane reads them by broadcast), the presence bits
+ * come as one uniform word a chunk, and each lane loads its genotypes eight
+ * individuals at a time so that the loads overlap. */
+template <int Q1M, class GT>
 __global__ void __launch_bounds__(64) hyb_pair_kernel(const eqbf_chunk c, const HybPair hp) {
+  __shared__ double st[(2 * (Q1M - 1) + 2) * 64];
   const HybTile tl = hp.tiles[blockIdx.x];
-  const int g = tl.g;
-  if ((int)threadIdx.x >= tl.n) return;
-  const int64_t p = tl.p0 + threadIdx.x;
+  const int g = tl.g, lane = threadIdx.x;
+  const bool act = lane < tl.n;
+  const int64_t p = tl.p0 + (act ? lane : 0);
   const int64_t NP = c.n_pairs;
   const int S = c.n_subgroups, N = c.n_samples, QM = c.n_covars_max;
+  const int W = (N + 63) >> 6;
   const int64_t snp = c.pair_snp[p];
-  bool ok = true;
-  for (int s = 0; s < S; ++s) ok &= c.gene_in_sbgrp[(int64_t)g * S + s] && c.snp_in_sbgrp[snp * S + s];
-  if (!ok) {
-    /* gene.cpp:314-321 skips the pair: no sample size, no statistics */
-    for (int s = 0; s < S; ++s) {
-      hp.n_out[(int64_t)s * NP + p] = 0;
-      if (hp.sstats)
-        for (int k = 0; k < 5; ++k) hp.sstats[((int64_t)k * S + s) * NP + p] = NAN;
-    }
-    hp.pgene[p] = -1;
-    return;
+  bool gene_ok = true, ok = act;
+  for (int s = 0; s < S; ++s) {
+    gene_ok &= c.gene_in_sbgrp[(int64_t)g * S + s] != 0;
+    ok &= c.snp_in_sbgrp[snp * S + s] != 0;
   }
-  const int npr = S * (S - 1) / 2;
-  const int RSZ = hp.RSZ;
-  const double* R = hp.rec + (int64_t)g * hp.NSET * RSZ;
-  const double f = hp.fiterr;
-  int pi = 0;
-  for (int s1 = 0; s1 < S; ++s1) {
-    const int Q1 = min(max(c.covar_count[s1], 0), QM) + 1;
-    const double* cov0 = c.covars + (int64_t)s1 * QM * N;
-    const double* covp = hp.cp + (int64_t)s1 * QM * N;
-    const double* y = c.explevels + ((int64_t)g * S + s1) * N;
-    double u0[Q1M], u1[Q1M], sxx = 0.0, sxy = 0.0;
+  ok &= gene_ok;
+  if (gene_ok) {
+    const uint64_t* pres = hp.pres + (int64_t)g * S * W;
+    const int npr = S * (S - 1) / 2;
+    const int RSZ = hp.RSZ;
+    const double* R = hp.rec + (int64_t)g * hp.NSET * RSZ;
+    const double f = hp.fiterr;
+    int pi = 0;
+    for (int s1 = 0; s1 < S; ++s1) {
+      const int Q1 = min(max(c.covar_count[s1], 0), QM) + 1, Q = Q1 - 1;
+      const double* cov0 = c.covars + (int64_t)s1 * QM * N;
+      const double* covp = hp.cp + (int64_t)s1 * QM * N;
+      const double* y = c.explevels + ((int64_t)g * S + s1) * N;
+      const GT* xb = (const GT*)c.genotypes + (int64_t)c.geno_set_of_sbgrp[s1] * c.geno_set_stride + snp;
+      const int64_t ld = c.geno_ld;
+      double u0[Q1M], u1[Q1M], sxx = 0.0, sxy = 0.0;
 #pragma unroll
-    for (int a = 0; a < Q1M; ++a) u0[a] = u1[a] = 0.0;
-    for (int i = 0; i < N; ++i) {
-      if (!hyb_present(c, g, s1, i)) continue;
-      const double x = hyb_geno(c, s1, i, snp);
-      sxx += x * x;
-      sxy += x * y[i];
-      u0[0] += x;
-      u1[0] += x;
-#pragma unroll
-      for (int a = 1; a < Q1M; ++a)
-        if (a < Q1) {
-          u0[a] += cov0[(int64_t)(a - 1) * N + i] * x;
-          u1[a] += covp[(int64_t)(a - 1) * N + i] * x;
+      for (int a = 0; a < Q1M; ++a) u0[a] = u1[a] = 0.0;
+      for (int c0 = 0; c0 < N; c0 += 64) {
+        const int n = min(64, N - c0);
+        __syncthreads();
+        if (lane < n) {
+          for (int q = 0; q < Q; ++q) {
+            st[q * 64 + lane] = cov0[(int64_t)q * N + c0 + lane];
+            st[(Q + q) * 64 + lane] = covp[(int64_t)q * N + c0 + lane];
+          }
+          st[2 * Q * 64 + lane] = y[c0 + lane];
         }
-    }
-    const double* R0 = R + (int64_t)s1 * RSZ;
-    const double* R1 = R + (int64_t)(S + s1) * RSZ;
-    const double k0 = sxx - hyb_quad<Q1M>(R0, u0, u0, Q1);
-    double xty = sxy;
+        __syncthreads();
+        const uint64_t m = pres[(int64_t)s1 * W + (c0 >> 6)];
+        for (int j0 = 0; j0 < n; j0 += 8) {
+          double xv[8];
 #pragma unroll
-    for (int a = 0; a < Q1M; ++a)
-      if (a < Q1) xty -= u0[a] * R0[2 * Q1M * Q1M + a];
-    const double rss0 = R0[2 * Q1M * Q1M + 2 * Q1M], n0 = R0[2 * Q1M * Q1M + 2 * Q1M + 1];
-    const double rss1 = rss0 - xty * xty / k0;
-    const double sg = f * rss1 / n0 + (1.0 - f) * rss0 / n0;
-    hp.b[(int64_t)s1 * NP + p] = xty / k0;
-    hp.Sig[(int64_t)(s1 * S + s1) * NP + p] = sg;
-    hp.Vg[(int64_t)(s1 * S + s1) * NP + p] = sg / k0;
-    /* w1 = A1^-1 u1, k1: the genotype column of M1^-1 is (-w1, 1) / k1 */
-    double w1[Q1M];
+          for (int k = 0; k < 8; ++k) xv[k] = (ok && j0 + k < n) ? (double)xb[(int64_t)(c0 + j0 + k) * ld] : 0.0;
 #pragma unroll
-    for (int a = 0; a < Q1M; ++a) {
-      double r = 0.0;
+          for (int k = 0; k < 8; ++k) {
+            const int j = j0 + k;
+            if (j >= n || !((m >> j) & 1ull)) continue;
+            const double x = xv[k];
+            sxx += x * x;
+            sxy += x * st[2 * Q * 64 + j];
+            u0[0] += x;
+            u1[0] += x;
 #pragma unroll
-      for (int b2 = 0; b2 < Q1M; ++b2)
-        if (a < Q1 && b2 < Q1) r += R1[a * Q1M + b2] * u1[b2];
-      w1[a] = r;
-    }
-    double k1 = sxx;
-#pragma unroll
-    for (int a = 0; a < Q1M; ++a)
-      if (a < Q1) k1 -= u1[a] * w1[a];
-    const double* yp1 = hp.yp + ((int64_t)g * S + s1) * N;
-    for (int s2 = s1 + 1; s2 < S; ++s2, ++pi) {
-      const double* yp2 = hp.yp + ((int64_t)g * S + s2) * N;
-      double u12[Q1M], u2[Q1M], sxx12 = 0.0, sxx2 = 0.0, sy1 = 0.0, sy2 = 0.0;
-#pragma unroll
-      for (int a = 0; a < Q1M; ++a) u12[a] = u2[a] = 0.0;
-      for (int i = 0; i < N; ++i) {
-        if (!hyb_present(c, g, s2, i)) continue;
-        const bool both = hyb_present(c, g, s1, i);
-        const double x = hyb_geno(c, s1, i, snp);
-        double cv[Q1M];
-        cv[0] = 1.0;
-#pragma unroll
-        for (int a = 1; a < Q1M; ++a) cv[a] = (a < Q1) ? covp[(int64_t)(a - 1) * N + i] : 0.0;
-        sxx2 += x * x;
-#pragma unroll
-        for (int a = 0; a < Q1M; ++a) u2[a] += cv[a] * x;
-        if (both) {
-          sxx12 += x * x;
-          sy1 += x * yp1[i];
-          sy2 += x * yp2[i];
-#pragma unroll
-          for (int a = 0; a < Q1M; ++a) u12[a] += cv[a] * x;
+            for (int a = 1; a < Q1M; ++a)
+              if (a < Q1) {
+                u0[a] += st[(a - 1) * 64 + j] * x;
+                u1[a] += st[(Q + a - 1) * 64 + j] * x;
+              }
+          }
         }
       }
-      const double* R2 = R + (int64_t)(2 * S + pi) * RSZ;
-      const double* R3 = R + (int64_t)(2 * S + npr + pi) * RSZ;
-      const double n12 = R2[2 * Q1M * Q1M + 2 * Q1M + 1];
-      if (!(n12 > 0.0)) atomicOr(hp.flag, FLAG_HYB_NO_COMMON);
-      const double k12 = sxx12 - hyb_quad<Q1M>(R2, u12, u12, Q1);
-      double xty1 = sy1, xty2 = sy2;
+      const double* R0 = R + (int64_t)s1 * RSZ;
+      const double* R1 = R + (int64_t)(S + s1) * RSZ;
+      const double k0 = sxx - hyb_quad<Q1M>(R0, u0, u0, Q1);
+      double xty = sxy;
 #pragma unroll
       for (int a = 0; a < Q1M; ++a)
-        if (a < Q1) {
-          xty1 -= u12[a] * R2[2 * Q1M * Q1M + a];
-          xty2 -= u12[a] * R2[2 * Q1M * Q1M + Q1M + a];
-        }
-      const double s12 = R2[2 * Q1M * Q1M + 2 * Q1M] / n12 - f * xty1 * xty2 / (k12 * n12);
-      double w2[Q1M];
+        if (a < Q1) xty -= u0[a] * R0[2 * Q1M * Q1M + a];
+      const double rss0 = R0[2 * Q1M * Q1M + 2 * Q1M], n0 = R0[2 * Q1M * Q1M + 2 * Q1M + 1];
+      const double rss1 = rss0 - xty * xty / k0;
+      const double sg = f * rss1 / n0 + (1.0 - f) * rss0 / n0;
+      if (ok) {
+        hp.b[(int64_t)s1 * NP + p] = xty / k0;
+        hp.Sig[(int64_t)(s1 * S + s1) * NP + p] = sg;
+        hp.Vg[(int64_t)(s1 * S + s1) * NP + p] = sg / k0;
+      }
+      /* w1 = A1^-1 u1, k1: the genotype column of M1^-1 is (-w1, 1) / k1 */
+      double w1[Q1M];
 #pragma unroll
       for (int a = 0; a < Q1M; ++a) {
         double r = 0.0;
 #pragma unroll
         for (int b2 = 0; b2 < Q1M; ++b2)
-          if (a < Q1 && b2 < Q1) r += R3[a * Q1M + b2] * u2[b2];
-        w2[a] = r;
+          if (a < Q1 && b2 < Q1) r += R1[a * Q1M + b2] * u1[b2];
+        w1[a] = r;
       }
-      double k2 = sxx2, wu = 0.0;
+      double k1 = sxx;
 #pragma unroll
       for (int a = 0; a < Q1M; ++a)
-        if (a < Q1) {
-          k2 -= u2[a] * w2[a];
-          wu += w1[a] * u12[a] + u12[a] * w2[a];
+        if (a < Q1) k1 -= u1[a] * w1[a];
+      const double* yp1 = hp.yp + ((int64_t)g * S + s1) * N;
+      for (int s2 = s1 + 1; s2 < S; ++s2, ++pi) {
+        const double* yp2 = hp.yp + ((int64_t)g * S + s2) * N;
+        double u12[Q1M], u2[Q1M], sxx12 = 0.0, sxx2 = 0.0, sy1 = 0.0, sy2 = 0.0;
+#pragma unroll
+        for (int a = 0; a < Q1M; ++a) u12[a] = u2[a] = 0.0;
+        for (int c0 = 0; c0 < N; c0 += 64) {
+          const int n = min(64, N - c0);
+          __syncthreads();
+          if (lane < n) {
+            for (int q = 0; q < Q; ++q) st[q * 64 + lane] = covp[(int64_t)q * N + c0 + lane];
+            st[Q * 64 + lane] = yp1[c0 + lane];
+            st[(Q + 1) * 64 + lane] = yp2[c0 + lane];
+          }
+          __syncthreads();
+          const uint64_t m1 = pres[(int64_t)s1 * W + (c0 >> 6)], m2 = pres[(int64_t)s2 * W + (c0 >> 6)];
+          for (int j0 = 0; j0 < n; j0 += 8) {
+            double xv[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) xv[k] = (ok && j0 + k < n) ? (double)xb[(int64_t)(c0 + j0 + k) * ld] : 0.0;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+              const int j = j0 + k;
+              if (j >= n || !((m2 >> j) & 1ull)) continue;
+              const double x = xv[k];
+              double cv[Q1M];
+              cv[0] = 1.0;
+#pragma unroll
+              for (int a = 1; a < Q1M; ++a) cv[a] = (a < Q1) ? st[(a - 1) * 64 + j] : 0.0;
+              sxx2 += x * x;
+#pragma unroll
+              for (int a = 0; a < Q1M; ++a) u2[a] += cv[a] * x;
+              if ((m1 >> j) & 1ull) {
+                sxx12 += x * x;
+                sy1 += x * st[Q * 64 + j];
+                sy2 += x * st[(Q + 1) * 64 + j];
+#pragma unroll
+                for (int a = 0; a < Q1M; ++a) u12[a] += cv[a] * x;
+              }
+            }
+          }
         }
-      const double aw = hyb_quad<Q1M>(R2 + Q1M * Q1M, w1, w2, Q1);
-      const double a11 = (aw - wu + sxx12) / (k1 * k2);
-      hp.Sig[(int64_t)(s1 * S + s2) * NP + p] = s12;
-      hp.Sig[(int64_t)(s2 * S + s1) * NP + p] = s12;
-      hp.Vg[(int64_t)(s1 * S + s2) * NP + p] = s12 * a11;
-      hp.Vg[(int64_t)(s2 * S + s1) * NP + p] = s12 * a11;
+        const double* R2 = R + (int64_t)(2 * S + pi) * RSZ;
+        const double* R3 = R + (int64_t)(2 * S + npr + pi) * RSZ;
+        const double n12 = R2[2 * Q1M * Q1M + 2 * Q1M + 1];
+        if (ok && !(n12 > 0.0)) atomicOr(hp.flag, FLAG_HYB_NO_COMMON);
+        const double k12 = sxx12 - hyb_quad<Q1M>(R2, u12, u12, Q1);
+        double xty1 = sy1, xty2 = sy2;
+#pragma unroll
+        for (int a = 0; a < Q1M; ++a)
+          if (a < Q1) {
+            xty1 -= u12[a] * R2[2 * Q1M * Q1M + a];
+            xty2 -= u12[a] * R2[2 * Q1M * Q1M + Q1M + a];
+          }
+        const double s12 = R2[2 * Q1M * Q1M + 2 * Q1M] / n12 - f * xty1 * xty2 / (k12 * n12);
+        double w2[Q1M];
+#pragma unroll
+        for (int a = 0; a < Q1M; ++a) {
+          double r = 0.0;
+#pragma unroll
+          for (int b2 = 0; b2 < Q1M; ++b2)
+            if (a < Q1 && b2 < Q1) r += R3[a * Q1M + b2] * u2[b2];
+          w2[a] = r;
+        }
+        double k2 = sxx2, wu = 0.0;
+#pragma unroll
+        for (int a = 0; a < Q1M; ++a)
+          if (a < Q1) {
+            k2 -= u2[a] * w2[a];
+            wu += w1[a] * u12[a] + u12[a] * w2[a];
+          }
+        const double aw = hyb_quad<Q1M>(R2 + Q1M * Q1M, w1, w2, Q1);
+        const double a11 = (aw - wu + sxx12) / (k1 * k2);
+        if (ok) {
+          hp.Sig[(int64_t)(s1 * S + s2) * NP + p] = s12;
+          hp.Sig[(int64_t)(s2 * S + s1) * NP + p] = s12;
+          hp.Vg[(int64_t)(s1 * S + s2) * NP + p] = s12 * a11;
+          hp.Vg[(int64_t)(s2 * S + s1) * NP + p] = s12 * a11;
+        }
+      }
     }
   }
-  hp.pgene[p] = g;
+  if (!act) return;
+  if (ok) {
+    hp.pgene[p] = g;
+    return;
+  }
+  /* gene.cpp:314-321 skips the pair: no sample size, no statistics */
+  for (int s = 0; s < S; ++s) {
+    hp.n_out[(int64_t)s * NP + p] = 0;
+    if (hp.sstats)
+      for (int k = 0; k < 5; ++k) hp.sstats[((int64_t)k * S + s) * NP + p] = NAN;
+  }
+  hp.pgene[p] = -1;
 }
 
 /* per (pair, configuration): Vg^-1, Vg^-1 t, t = gamma o sqrt(diag Sigma),
@@ -1517,8 +1582,13 @@ int hybrid_run_sm(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_hybrid_opts* o,
   const int S = c.n_subgroups, N = c.n_samples, G = c.n_genes;
   const int64_t NP = c.n_pairs;
   const int64_t n_cfg = eqbf_n_configs(o->bfs, S);
-  /* sample sizes and summary statistics: the separate analysis (FP64) */
-  eqbf_assoc_opts so{EQBF_ANALYSIS_SEP, EQBF_BFS_GEN, EQBF_PREC_FP64_BATCHED, EQBF_F64};
+  /* sample sizes and summary statistics: the separate analysis; the batched
+   * path (FP64 sums when summary statistics are asked for) where it applies,
+   * else the FP64 plain path */
+  int KU = 1;
+  while (KU < c.n_covars_max + 1) KU = (KU == 1) ? 4 : KU * 2;
+  const int prec = uvlr_fast_eligible(c, KU) ? EQBF_PREC_MIXED : EQBF_PREC_FP64_BATCHED;
+  eqbf_assoc_opts so{EQBF_ANALYSIS_SEP, EQBF_BFS_GEN, prec, EQBF_F64};
   eqbf_assoc_out sout;
   memset(&sout, 0, sizeof sout);
   sout.n = out->n;
@@ -1563,8 +1633,13 @@ int hybrid_run_sm(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_hybrid_opts* o,
     ProfScope ps(ctx, "hyb_gene_kernel", st);
     hipLaunchKernelGGL(hyb_gene_kernel, dim3(G, NSET), dim3(256), 0, st, c, hg);
   }
+  const int W = (N + 63) / 64;
+  uint64_t* pres = (uint64_t*)ctx_buf(ctx, "hyb_pres", (size_t)G * S * W * sizeof(uint64_t));
+  if (!pres) return ctx_fail(ctx, EQBF_ERR_HIP, "allocating hybrid buffers failed");
+  hipLaunchKernelGGL(hyb_pres_kernel, dim3(G, S), dim3(64), 0, st, c, pres, W);
   HybPair hp;
   hp.tiles = dtiles;
+  hp.pres = pres;
   hp.cp = hg.cp;
   hp.yp = hg.yp;
   hp.rec = rec;
@@ -1580,7 +1655,10 @@ int hybrid_run_sm(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_hybrid_opts* o,
   hp.RSZ = RSZ;
   {
     ProfScope ps(ctx, "hyb_pair_kernel", st);
-    hipLaunchKernelGGL(hyb_pair_kernel<Q1M>, dim3((unsigned)tiles.size()), dim3(64), 0, st, c, hp);
+    if (c.geno_dtype == EQBF_F32)
+      hipLaunchKernelGGL((hyb_pair_kernel<Q1M, float>), dim3((unsigned)tiles.size()), dim3(64), 0, st, c, hp);
+    else
+      hipLaunchKernelGGL((hyb_pair_kernel<Q1M, double>), dim3((unsigned)tiles.size()), dim3(64), 0, st, c, hp);
   }
   EQBF_HIP(ctx, hipGetLastError());
   return mv_rows_run<SM>(ctx, c, o->bfs, out, st, [&](int64_t c0, int nb, double* par, int PSZ, int64_t pair_blocks) {
