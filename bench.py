@@ -849,16 +849,16 @@ def mvlr_leg(args, torch, result):
 
 def mvlr_rows_flops(NP, S, GL):
     """Algorithmic FP64 flops of mvlr_rows_kernel for --bfs gen: per pair,
-    the gen and gen-maxh groups each reduce B = k Sigma^-1 T^2 to Hessenberg
-    form (2 S^2 to form it, 10/3 S^3 for the reflections, 8 S^2 to carry
-    four vectors) and take GL rows of: forming I + p2 H over its h = S(S+1)/2
-    + S - 1 entries (2h), the pivoted Hessenberg LU with two right-hand
-    sides (S^2 + 4S), two back substitutions (2 S^2), the rank-one update
-    and the dot products (16 S); the gen-fix rows are scalar (4 flops)."""
+    the gen and gen-maxh groups take GL rows each of: forming I + p2 H over
+    the h = S(S+1)/2 + S - 1 entries of the Hessenberg form (2h), the
+    pivoted Hessenberg LU with two right-hand sides (S^2 + 4S), two back
+    substitutions (2 S^2), the rank-one update and the dot products (20 S);
+    the gen-fix rows are scalar (4 flops).  The reduction to Hessenberg form
+    (mvlr_hess_kernel, once per pair: 2 S^2 + 10/3 S^3 + 8 S^2) is not
+    counted here."""
     h = S * (S + 1) // 2 + S - 1
-    setup = 2 * S * S + (10 * S ** 3) // 3 + 8 * S * S
     row = 2 * h + 3 * S * S + 20 * S
-    return float(NP) * (2 * (setup + GL * row) + 4 * GL)
+    return float(NP) * (2 * GL * row + 4 * GL)
 
 
 class HybridWorkload(MvlrWorkload):

@@ -607,8 +607,105 @@ struct HessIdx {
   static constexpr int size = SM == 1 ? 1 : SM + (SM - 1) * SM - (SM - 1) * (SM - 2) / 2;
 };
 
+/* per (pair, configuration of the batch): the Hessenberg form and the
+ * carried vectors, [cfg][E][NP] with E = HessIdx::size + 4 SM + 1:
+ * H's entries, u~ = k Q'u, z~ = Q'eVb, t~ = Q't, v~ = Q'T^2 eVb, eVb't */
 template <int SM>
-__global__ void __launch_bounds__(64) mvlr_rows_kernel(const eqbf_chunk c, const MvGrid mg) {
+__global__ void __launch_bounds__(64) mvlr_hess_kernel(const eqbf_chunk c, const MvGrid mg, double* __restrict__ hess) {
+  const int64_t NP = c.n_pairs;
+  const int64_t p = blockIdx.x * (int64_t)64 + threadIdx.x;
+  if (p >= NP) return;
+  const int cfg = blockIdx.y;
+  const int PSZ = mg.PSZ, S = SM;
+  constexpr int HS = HessIdx<SM>::size, E = HS + 4 * SM + 1;
+  const double* pr = mg.par + (int64_t)cfg * PSZ * NP + p;
+  double* out = hess + (int64_t)cfg * E * NP + p;
+  if (pr[(int64_t)(PSZ - 1) * NP] != 0.0) return; /* no result: the row kernel writes NaN */
+  const double kk = pr[(int64_t)(S * S + 3 * S) * NP];
+  double t[SM], ut[SM], zt[SM], tt[SM], vt[SM], B[SM][SM];
+  double et = 0.0;
+#pragma unroll
+  for (int i = 0; i < SM; ++i) t[i] = pr[(int64_t)(S * S + S + i) * NP];
+#pragma unroll
+  for (int i = 0; i < SM; ++i) {
+    const double u = pr[(int64_t)(S * S + i) * NP], z = pr[(int64_t)(S * S + 2 * S + i) * NP];
+#pragma unroll
+    for (int j = 0; j < SM; ++j) B[i][j] = kk * pr[(int64_t)(i * S + j) * NP] * t[j] * t[j];
+    ut[i] = kk * u;
+    zt[i] = z;
+    tt[i] = t[i];
+    vt[i] = t[i] * t[i] * z;
+    et += z * t[i];
+  }
+#pragma unroll
+  for (int k = 0; k + 2 < SM; ++k) {
+    double nrm2 = 0.0;
+#pragma unroll
+    for (int i = k + 1; i < SM; ++i) nrm2 += B[i][k] * B[i][k];
+    const double x0 = B[k + 1][k];
+    double alpha = sqrt(nrm2);
+    if (x0 > 0) alpha = -alpha;
+    double v[SM];
+    double vv = 0.0;
+#pragma unroll
+    for (int i = 0; i < SM; ++i) {
+      v[i] = i == k + 1 ? x0 - alpha : (i > k + 1 ? B[i][k] : 0.0);
+      vv += v[i] * v[i];
+    }
+    const double beta = vv > 0.0 ? 2.0 / vv : 0.0;
+#pragma unroll
+    for (int j = k; j < SM; ++j) {
+      double s = 0.0;
+#pragma unroll
+      for (int i = k + 1; i < SM; ++i) s += v[i] * B[i][j];
+      s *= beta;
+#pragma unroll
+      for (int i = k + 1; i < SM; ++i) B[i][j] -= v[i] * s;
+    }
+#pragma unroll
+    for (int i = 0; i < SM; ++i) {
+      double s = 0.0;
+#pragma unroll
+      for (int j = k + 1; j < SM; ++j) s += B[i][j] * v[j];
+      s *= beta;
+#pragma unroll
+      for (int j = k + 1; j < SM; ++j) B[i][j] -= s * v[j];
+    }
+    double s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
+#pragma unroll
+    for (int i = k + 1; i < SM; ++i) {
+      s1 += v[i] * ut[i];
+      s2 += v[i] * zt[i];
+      s3 += v[i] * tt[i];
+      s4 += v[i] * vt[i];
+    }
+#pragma unroll
+    for (int i = k + 1; i < SM; ++i) {
+      ut[i] -= beta * v[i] * s1;
+      zt[i] -= beta * v[i] * s2;
+      tt[i] -= beta * v[i] * s3;
+      vt[i] -= beta * v[i] * s4;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < SM; ++i)
+#pragma unroll
+    for (int j = (i == 0 ? 0 : i - 1); j < SM; ++j) out[(int64_t)HessIdx<SM>::at(i, j) * NP] = B[i][j];
+#pragma unroll
+  for (int i = 0; i < SM; ++i) {
+    out[(int64_t)(HS + i) * NP] = ut[i];
+    out[(int64_t)(HS + SM + i) * NP] = zt[i];
+    out[(int64_t)(HS + 2 * SM + i) * NP] = tt[i];
+    out[(int64_t)(HS + 3 * SM + i) * NP] = vt[i];
+  }
+  out[(int64_t)(HS + 4 * SM) * NP] = et;
+}
+
+/* per (pair, group of rows): every grid point of the group from the
+ * Hessenberg form (H staged in LDS, the carried vectors in registers) */
+template <int SM>
+__global__ void __launch_bounds__(64) mvlr_rows_kernel(const eqbf_chunk c, const MvGrid mg,
+                                                       const double* __restrict__ hess) {
   const int64_t NP = c.n_pairs;
   const int64_t p = blockIdx.x * (int64_t)64 + threadIdx.x;
   const int lane = threadIdx.x;
@@ -622,116 +719,52 @@ __global__ void __launch_bounds__(64) mvlr_rows_kernel(const eqbf_chunk c, const
     cfg = g;
     type = 3;
   }
-  __shared__ double Hs[HessIdx<SM>::size][64];
-  const bool on = p < NP;
+  constexpr int HS = HessIdx<SM>::size, E = HS + 4 * SM + 1;
+  __shared__ double Hs[HS][64];
+  if (p >= NP) return; /* no barrier below: each lane uses its own LDS column */
   const int PSZ = mg.PSZ, S = SM;
-  const double* pr = mg.par + (int64_t)cfg * PSZ * NP + (on ? p : 0);
-  const bool ok = on && pr[(int64_t)(PSZ - 1) * NP] == 0.0;
+  const double* pr = mg.par + (int64_t)cfg * PSZ * NP + p;
+  const bool ok = pr[(int64_t)(PSZ - 1) * NP] == 0.0;
   const int n_rows = type == 3 ? mg.GS : mg.GL;
   const int64_t row_base = type < 3 ? (int64_t)type * mg.GL : 3 * (int64_t)mg.GL + (int64_t)(mg.cfg0 + cfg - 1) * mg.GS;
   const double* gp = type == 3 ? mg.grid + 2 * mg.GL : mg.grid;
   const double* go = type == 3 ? mg.grid + 2 * mg.GL + mg.GS : mg.grid + mg.GL;
   const double ln10 = 2.302585092994045684;
-  double t[SM], u[SM], z[SM];
-  double kk = 0.0;
-  if (ok) {
-    kk = pr[(int64_t)(S * S + 3 * S) * NP];
-#pragma unroll
-    for (int i = 0; i < SM; ++i) {
-      t[i] = pr[(int64_t)(S * S + S + i) * NP];
-      u[i] = pr[(int64_t)(S * S + i) * NP];
-      z[i] = pr[(int64_t)(S * S + 2 * S + i) * NP];
-    }
-  } else {
-#pragma unroll
-    for (int i = 0; i < SM; ++i) t[i] = u[i] = z[i] = 0.0;
+  double* raw = mg.raw + row_base * NP + p;
+  if (!ok) {
+    for (int gi = 0; gi < n_rows; ++gi) raw[(int64_t)gi * NP] = NAN;
+    return;
   }
-  double et = 0.0;
-#pragma unroll
-  for (int i = 0; i < SM; ++i) et += z[i] * t[i];
   if (type == 1) { /* gen-fix: M = I + o2 k u t' */
-    double tu = 0.0, tz = 0.0;
+    const double kk = pr[(int64_t)(S * S + 3 * S) * NP];
+    double tu = 0.0, tz = 0.0, et = 0.0;
 #pragma unroll
     for (int i = 0; i < SM; ++i) {
-      tu += t[i] * u[i];
-      tz += t[i] * z[i];
+      const double t = pr[(int64_t)(S * S + S + i) * NP];
+      const double u = pr[(int64_t)(S * S + i) * NP], z = pr[(int64_t)(S * S + 2 * S + i) * NP];
+      tu += t * u;
+      tz += t * z;
+      et += z * t;
     }
     for (int gi = 0; gi < n_rows; ++gi) {
       const double o2 = gp[gi] + go[gi];
       const double sig = 1.0 + o2 * kk * tu;
       const double quad = o2 * et * (tz / sig);
-      if (on) mg.raw[(row_base + gi) * NP + p] = ok ? (0.5 * quad - 0.5 * log(fabs(sig))) / ln10 : NAN;
+      raw[(int64_t)gi * NP] = (0.5 * quad - 0.5 * log(fabs(sig))) / ln10;
     }
     return;
   }
-  /* B = k Sigma^-1 T^2 and the vectors, then the Householder reduction */
+  const double* hv = hess + (int64_t)cfg * E * NP + p;
+  for (int e = 0; e < HS; ++e) Hs[e][lane] = hv[(int64_t)e * NP];
   double ut[SM], zt[SM], tt[SM], vt[SM];
-  {
-    double B[SM][SM];
 #pragma unroll
-    for (int i = 0; i < SM; ++i) {
-#pragma unroll
-      for (int j = 0; j < SM; ++j) B[i][j] = ok ? kk * pr[(int64_t)(i * S + j) * NP] * t[j] * t[j] : 0.0;
-      ut[i] = kk * u[i];
-      zt[i] = z[i];
-      tt[i] = t[i];
-      vt[i] = t[i] * t[i] * z[i];
-    }
-#pragma unroll
-    for (int k = 0; k + 2 < SM; ++k) {
-      double nrm2 = 0.0;
-#pragma unroll
-      for (int i = k + 1; i < SM; ++i) nrm2 += B[i][k] * B[i][k];
-      const double x0 = B[k + 1][k];
-      double alpha = sqrt(nrm2);
-      if (x0 > 0) alpha = -alpha;
-      double v[SM];
-      double vv = 0.0;
-#pragma unroll
-      for (int i = 0; i < SM; ++i) {
-        v[i] = i == k + 1 ? x0 - alpha : (i > k + 1 ? B[i][k] : 0.0);
-        vv += v[i] * v[i];
-      }
-      const double beta = vv > 0.0 ? 2.0 / vv : 0.0;
-#pragma unroll
-      for (int j = k; j < SM; ++j) {
-        double s = 0.0;
-#pragma unroll
-        for (int i = k + 1; i < SM; ++i) s += v[i] * B[i][j];
-        s *= beta;
-#pragma unroll
-        for (int i = k + 1; i < SM; ++i) B[i][j] -= v[i] * s;
-      }
-#pragma unroll
-      for (int i = 0; i < SM; ++i) {
-        double s = 0.0;
-#pragma unroll
-        for (int j = k + 1; j < SM; ++j) s += B[i][j] * v[j];
-        s *= beta;
-#pragma unroll
-        for (int j = k + 1; j < SM; ++j) B[i][j] -= s * v[j];
-      }
-      double s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
-#pragma unroll
-      for (int i = k + 1; i < SM; ++i) {
-        s1 += v[i] * ut[i];
-        s2 += v[i] * zt[i];
-        s3 += v[i] * tt[i];
-        s4 += v[i] * vt[i];
-      }
-#pragma unroll
-      for (int i = k + 1; i < SM; ++i) {
-        ut[i] -= beta * v[i] * s1;
-        zt[i] -= beta * v[i] * s2;
-        tt[i] -= beta * v[i] * s3;
-        vt[i] -= beta * v[i] * s4;
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < SM; ++i)
-#pragma unroll
-      for (int j = (i == 0 ? 0 : i - 1); j < SM; ++j) Hs[HessIdx<SM>::at(i, j)][lane] = B[i][j];
+  for (int i = 0; i < SM; ++i) {
+    ut[i] = hv[(int64_t)(HS + i) * NP];
+    zt[i] = hv[(int64_t)(HS + SM + i) * NP];
+    tt[i] = hv[(int64_t)(HS + 2 * SM + i) * NP];
+    vt[i] = hv[(int64_t)(HS + 3 * SM + i) * NP];
   }
+  const double et = hv[(int64_t)(HS + 4 * SM) * NP];
   for (int gi = 0; gi < n_rows; ++gi) {
     double p2, o2;
     if (type == 2) { /* gen-maxh */
@@ -816,7 +849,7 @@ __global__ void __launch_bounds__(64) mvlr_rows_kernel(const eqbf_chunk c, const
       vxd += vt[i] * r2[i];
     }
     const double quad = o2 * et * txd + p2 * vxd;
-    if (on) mg.raw[(row_base + gi) * NP + p] = ok ? (0.5 * quad - 0.5 * lnd) / ln10 : NAN;
+    raw[(int64_t)gi * NP] = (0.5 * quad - 0.5 * lnd) / ln10;
   }
 }
 
@@ -881,26 +914,34 @@ void launch_grid_kernel(int S, unsigned blocks, hipStream_t st, const eqbf_chunk
 #undef EQBF_GRID_CASE
 }
 
-/* the Hessenberg row kernel at the exact subgroup count, 2 <= S <= 10 */
-void launch_rows_kernel(int S, dim3 grid, hipStream_t st, const eqbf_chunk& c, const MvGrid& mg) {
-#define EQBF_ROWS_CASE(n) \
-  case n:                 \
-    hipLaunchKernelGGL(mvlr_rows_kernel<n>, grid, dim3(64), 0, st, c, mg); \
-    break;
-  switch (S) {
-    EQBF_ROWS_CASE(2)
-    EQBF_ROWS_CASE(3)
-    EQBF_ROWS_CASE(4)
-    EQBF_ROWS_CASE(5)
-    EQBF_ROWS_CASE(6)
-    EQBF_ROWS_CASE(7)
-    EQBF_ROWS_CASE(8)
-    EQBF_ROWS_CASE(9)
-    default:
-      hipLaunchKernelGGL(mvlr_rows_kernel<10>, grid, dim3(64), 0, st, c, mg);
-      break;
+/* the Hessenberg kernels at the exact subgroup count, 2 <= S <= 10 */
+template <int n>
+void launch_hess_rows(unsigned pair_blocks, unsigned nb, unsigned groups, hipStream_t st, const eqbf_chunk& c,
+                      const MvGrid& mg, double* hess, eqbf_ctx* ctx) {
+  {
+    ProfScope ps(ctx, "mvlr_hess_kernel", st);
+    hipLaunchKernelGGL(mvlr_hess_kernel<n>, dim3(pair_blocks, nb), dim3(64), 0, st, c, mg, hess);
   }
-#undef EQBF_ROWS_CASE
+  ProfScope ps(ctx, "mvlr_rows_kernel", st);
+  hipLaunchKernelGGL(mvlr_rows_kernel<n>, dim3(pair_blocks, groups), dim3(64), 0, st, c, mg, (const double*)hess);
+}
+
+constexpr bool hess_ok(int S) { return S >= 2 && S <= 10; }
+constexpr int hess_entries(int S) { return (S == 1 ? 1 : S + (S - 1) * S - (S - 1) * (S - 2) / 2) + 4 * S + 1; }
+
+void launch_rows_kernel(int S, unsigned pair_blocks, unsigned nb, unsigned groups, hipStream_t st,
+                        const eqbf_chunk& c, const MvGrid& mg, double* hess, eqbf_ctx* ctx) {
+  switch (S) {
+    case 2: launch_hess_rows<2>(pair_blocks, nb, groups, st, c, mg, hess, ctx); break;
+    case 3: launch_hess_rows<3>(pair_blocks, nb, groups, st, c, mg, hess, ctx); break;
+    case 4: launch_hess_rows<4>(pair_blocks, nb, groups, st, c, mg, hess, ctx); break;
+    case 5: launch_hess_rows<5>(pair_blocks, nb, groups, st, c, mg, hess, ctx); break;
+    case 6: launch_hess_rows<6>(pair_blocks, nb, groups, st, c, mg, hess, ctx); break;
+    case 7: launch_hess_rows<7>(pair_blocks, nb, groups, st, c, mg, hess, ctx); break;
+    case 8: launch_hess_rows<8>(pair_blocks, nb, groups, st, c, mg, hess, ctx); break;
+    case 9: launch_hess_rows<9>(pair_blocks, nb, groups, st, c, mg, hess, ctx); break;
+    default: launch_hess_rows<10>(pair_blocks, nb, groups, st, c, mg, hess, ctx); break;
+  }
 }
 
 template <int SM, class Fill>
@@ -915,10 +956,11 @@ int mv_rows_run(eqbf_ctx* ctx, const eqbf_chunk& c, int bfs, eqbf_assoc_out* out
   /* 2 to 10 subgroups: one thread per (pair, group of rows), the grid
    * points of a group from one Hessenberg reduction (mvlr_rows_kernel);
    * otherwise one thread per (pair, row) */
-  const bool hess = S >= 2 && S <= 10;
-  per = std::min<int64_t>(std::min<int64_t>(per, hess ? 65533 : 65535), total);
+  per = std::min<int64_t>(std::min<int64_t>(per, hess_ok(S) ? 65533 : 65535), total);
   double* par = (double*)ctx_buf(ctx, "mv_par", (size_t)per * NP * PSZ * sizeof(double));
-  if (!par) return ctx_fail(ctx, EQBF_ERR_HIP, "allocating mvlr buffers failed");
+  double* hess = hess_ok(S) ? (double*)ctx_buf(ctx, "mv_hess", (size_t)per * NP * hess_entries(S) * sizeof(double))
+                            : nullptr;
+  if (!par || (hess_ok(S) && !hess)) return ctx_fail(ctx, EQBF_ERR_HIP, "allocating mvlr buffers failed");
   const int64_t pair_blocks = (NP + 63) / 64;
   for (int64_t c0 = 0; c0 < total; c0 += per) {
     const int nb = (int)std::min<int64_t>(per, total - c0);
@@ -929,7 +971,7 @@ int mv_rows_run(eqbf_ctx* ctx, const eqbf_chunk& c, int bfs, eqbf_assoc_out* out
     const int64_t row1 = 3 * (int64_t)GL + (c1 - 1) * GS;
     const int64_t wrow0 = c0 == 0 ? 0 : 3 + (c0 - 1);
     const int64_t wrow1 = 3 + (c1 - 1);
-    if (hess) {
+    if (hess_ok(S)) {
       MvGrid mg;
       mg.par = par;
       mg.grid = (const double*)ctx->d_grid.ptr;
@@ -942,10 +984,9 @@ int mv_rows_run(eqbf_ctx* ctx, const eqbf_chunk& c, int bfs, eqbf_assoc_out* out
       mg.nrows = row1 - row0;
       mg.pair_blocks = pair_blocks;
       const unsigned groups = (unsigned)(nb + (c0 == 0 ? 2 : 0));
-      ProfScope ps(ctx, "mvlr_rows_kernel", st);
-      launch_rows_kernel(S, dim3((unsigned)pair_blocks, groups), st, c, mg);
+      launch_rows_kernel(S, (unsigned)pair_blocks, (unsigned)nb, groups, st, c, mg, hess, ctx);
     }
-    for (int64_t r0 = row0; !hess && r0 < row1;) {
+    for (int64_t r0 = row0; !hess_ok(S) && r0 < row1;) {
       const int64_t nr = std::min<int64_t>(row1 - r0, std::max<int64_t>(1, ((int64_t)1 << 30) / pair_blocks));
       MvGrid mg;
       mg.par = par;
