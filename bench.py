@@ -1172,8 +1172,9 @@ def avg_leg(args, torch, result, eng, raw, off, off_d, K, L, S, pi0, grid, cfg):
             for k, v in after.items() if v[1] > before.get(k, (0, 0))[1]}
     pk = kern.get("avg_pair_kernel", (float("nan"), 1))
     # algorithmic bytes of avg_pair_kernel: the raw ABFs read once, the
-    # per-configuration BFs, the SNP BF and the best configuration written once
-    alg = NP * (K * L * 8.0 + K * 8.0 + 8.0 + 4.0)
+    # per-configuration BFs and posteriors, the subgroup posteriors, the SNP
+    # BF and the best configuration written once
+    alg = NP * (K * L * 8.0 + 2 * K * 8.0 + S * 8.0 + 8.0 + 4.0)
     res = {"metric": "SNP-gene pairs averaged per second (eqtlbma_avg_bfs)", "value": NP / dt, "unit": "pairs/s",
            "seconds_per_pass": dt,
            "workload": "eqtlbma_avg_bfs configs model on the eqtlbma_hm workload's raw ABFs (%d pairs, 31 "
@@ -1182,7 +1183,8 @@ def avg_leg(args, torch, result, eng, raw, off, off_d, K, L, S, pi0, grid, cfg):
            "kernels_ms_per_pass": {k: v[0] for k, v in kern.items()},
            "roofline": {"bound": "hbm", "kernel": "avg_pair_kernel", "alg_bytes_per_launch": alg,
                         "achieved": alg / (pk[0] / 1e3) / 1e9, "peak": 8000.0, "unit": "GB/s",
-                        "frac": alg / (pk[0] / 1e3) / 1e9 / 8000.0, "avg_launch_ms": pk[0] / max(pk[1], 1)}}
+                        "frac": alg / (pk[0] / 1e3) / 1e9 / 8000.0, "avg_launch_ms": pk[0] / max(pk[1], 1),
+                        "traffic": _load_traffic_hm("avg_pair_kernel", G)}}
     gs = 40
     raw_h = raw[:int(off[gs])].cpu().numpy()
     off_h = off[:gs + 1].copy()
