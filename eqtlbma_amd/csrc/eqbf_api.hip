@@ -194,6 +194,21 @@ extern "C" int eqbf_perm_stats_device(eqbf_ctx* ctx, const eqbf_chunk* c, const 
   return perm_stats_run(ctx, c, po, perm_idx_dev, n_tables, gene_table, nperm, stats_dev, st);
 }
 
+/* the hybrid permutation options' arrays on the device */
+static eqbf_perm_opts stage_perm_opts(Stager& sg, const eqbf_chunk* c, const eqbf_perm_opts* po) {
+  eqbf_perm_opts d = *po;
+  if (po->error_model == EQBF_ERROR_HYBRID) {
+    const int64_t G = c->n_genes, S = c->n_subgroups, N = c->n_samples;
+    d.covars_by_position =
+        c->n_covars_max > 0 ? sg.put("h_pcp", po->covars_by_position, (size_t)(S * c->n_covars_max * N)) : nullptr;
+    d.explevels_pairs = sg.put("h_pyp", po->explevels_pairs, (size_t)(G * S * N));
+  } else {
+    d.covars_by_position = nullptr;
+    d.explevels_pairs = nullptr;
+  }
+  return d;
+}
+
 extern "C" int eqbf_perm_stats(eqbf_ctx* ctx, const eqbf_chunk* c, const eqbf_perm_opts* po, const int32_t* perm_idx,
                                int32_t n_tables, const int32_t* gene_table, int64_t nperm, double* stats) {
   if (!ctx || !c || !po || !perm_idx || !gene_table || !stats || n_tables < 0)
@@ -206,8 +221,9 @@ extern "C" int eqbf_perm_stats(eqbf_ctx* ctx, const eqbf_chunk* c, const eqbf_pe
   if (rc) return rc;
   const int32_t* dtab = sg.put("h_ptab", perm_idx, (size_t)n_tables * nperm * c->n_samples);
   double* dstats = sg.alloc<double>("o_pstats", (size_t)c->n_genes * nperm, true);
+  const eqbf_perm_opts pd = stage_perm_opts(sg, c, po);
   if (sg.rc || (n_tables > 0 && !dtab)) return ctx_fail(ctx, EQBF_ERR_HIP, "staging the tables failed");
-  rc = perm_stats_run(ctx, &d, po, dtab, n_tables, gene_table, nperm, dstats, st);
+  rc = perm_stats_run(ctx, &d, &pd, dtab, n_tables, gene_table, nperm, dstats, st);
   if (rc) return rc;
   rc = check_flag(ctx, st);
   if (rc) return rc;
@@ -244,9 +260,10 @@ extern "C" int eqbf_perm_stats_fp64_ties(eqbf_ctx* ctx, const eqbf_chunk* c, con
   const int32_t* dtab = sg.put("h_ptab", perm_idx, (size_t)n_tables * std::max<int64_t>(nperm, 0) * c->n_samples);
   double* dstats = sg.alloc<double>("o_pstats", (size_t)c->n_genes * std::max<int64_t>(nperm, 1), true);
   double* dtrue = c->n_genes > 0 ? sg.put("h_ptrue", true_stats, (size_t)c->n_genes) : nullptr;
+  const eqbf_perm_opts pd = stage_perm_opts(sg, c, po);
   if (sg.rc || (n_tables > 0 && nperm > 0 && !dtab) || !dstats || (c->n_genes > 0 && !dtrue))
     return ctx_fail(ctx, EQBF_ERR_HIP, "staging the tables failed");
-  rc = perm_ties_run(ctx, &d, po, dtab ? dtab : (const int32_t*)dstats, n_tables, gene_table, nperm, tie_bound, dstats,
+  rc = perm_ties_run(ctx, &d, &pd, dtab ? dtab : (const int32_t*)dstats, n_tables, gene_table, nperm, tie_bound, dstats,
                      dtrue, n_rechecked, st);
   if (rc) return rc;
   rc = check_flag(ctx, st);

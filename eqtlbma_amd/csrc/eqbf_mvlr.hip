@@ -1137,9 +1137,13 @@ __device__ inline void hyb_pair_of(int pi, int S, int& s1, int& s2) {
   s2 = s1 + 1 + pi;
 }
 
-__device__ inline bool hyb_present(const eqbf_chunk& c, int g, int s, int i) {
+/* present in subgroup s: genotyped there and a level (not NaN) in lev
+ * [G][S][N]; the per-subgroup sets read the chunk's levels, the pairwise
+ * sets the pairwise levels, which permutations leave as loaded
+ * (samples.cpp:148-192) */
+__device__ inline bool hyb_present(const eqbf_chunk& c, const double* lev, int g, int s, int i) {
   const int S = c.n_subgroups, N = c.n_samples;
-  return c.geno_in_sbgrp[(int64_t)s * N + i] && !isnan(c.explevels[((int64_t)g * S + s) * N + i]);
+  return c.geno_in_sbgrp[(int64_t)s * N + i] && !isnan(lev[((int64_t)g * S + s) * N + i]);
 }
 
 /* record of a set: A^-1 [Q1M][Q1M], A [Q1M][Q1M], psi1 [Q1M], psi2 [Q1M],
@@ -1198,14 +1202,15 @@ __global__ void __launch_bounds__(256) hyb_gene_kernel(const eqbf_chunk c, const
       kind = 4;
     }
     double acc = 0.0;
+    const double* lev = (type == 0) ? c.explevels : hg.yp;
     for (int i = 0; i < N; ++i) {
       bool in;
       if (type == 0 || type == 1)
-        in = hyb_present(c, g, sa, i);
+        in = hyb_present(c, lev, g, sa, i);
       else if (type == 2)
-        in = hyb_present(c, g, sa, i) && hyb_present(c, g, sb, i);
+        in = hyb_present(c, lev, g, sa, i) && hyb_present(c, lev, g, sb, i);
       else
-        in = hyb_present(c, g, sb, i);
+        in = hyb_present(c, lev, g, sb, i);
       if (!in) continue;
       const double va = a == 0 ? 1.0 : cov[(int64_t)(a - 1) * N + i];
       switch (kind) {
@@ -1306,7 +1311,8 @@ struct HybTile {
 
 struct HybPair {
   const HybTile* tiles;
-  const uint64_t* pres; /* [G][S][W] presence bits */
+  const uint64_t* pres;  /* [G][S][W] presence bits from the chunk's levels  */
+  const uint64_t* presp; /* the same from the pairwise levels (may be pres) */
   const double* cp;
   const double* yp;
   const double* rec;
@@ -1338,13 +1344,14 @@ __device__ inline double hyb_quad(const double* __restrict__ Ai, const double (&
 }
 
 /* presence bits: bit i of word w of (gene, subgroup) = individual 64 w + i
- * is genotyped in the subgroup and has a level (not NaN) for the gene */
-__global__ void __launch_bounds__(64) hyb_pres_kernel(const eqbf_chunk c, uint64_t* __restrict__ pres, int W) {
+ * is genotyped in the subgroup and has a level (not NaN) in lev [G][S][N] */
+__global__ void __launch_bounds__(64) hyb_pres_kernel(const eqbf_chunk c, const double* __restrict__ lev,
+                                                      uint64_t* __restrict__ pres, int W) {
   const int g = blockIdx.x, s = blockIdx.y, lane = threadIdx.x;
   const int S = c.n_subgroups, N = c.n_samples;
   for (int w = 0; w < W; ++w) {
     const int i = w * 64 + lane;
-    const bool in = i < N && c.geno_in_sbgrp[(int64_t)s * N + i] && !isnan(c.explevels[((int64_t)g * S + s) * N + i]);
+    const bool in = i < N && c.geno_in_sbgrp[(int64_t)s * N + i] && !isnan(lev[((int64_t)g * S + s) * N + i]);
     const uint64_t m = __ballot(in);
     if (lane == 0) pres[((int64_t)g * S + s) * W + w] = m;
   }
@@ -1374,6 +1381,8 @@ __global__ void __launch_bounds__(64) hyb_pair_kernel(const eqbf_chunk c, const 
   ok &= gene_ok;
   if (gene_ok) {
     const uint64_t* pres = hp.pres + (int64_t)g * S * W;
+    const uint64_t* presp = hp.presp + (int64_t)g * S * W;
+    const bool split = hp.presp != hp.pres;
     const int npr = S * (S - 1) / 2;
     const int RSZ = hp.RSZ;
     const double* R = hp.rec + (int64_t)g * hp.NSET * RSZ;
@@ -1386,7 +1395,7 @@ __global__ void __launch_bounds__(64) hyb_pair_kernel(const eqbf_chunk c, const 
       const double* y = c.explevels + ((int64_t)g * S + s1) * N;
       const GT* xb = (const GT*)c.genotypes + (int64_t)c.geno_set_of_sbgrp[s1] * c.geno_set_stride + snp;
       const int64_t ld = c.geno_ld;
-      double u0[Q1M], u1[Q1M], sxx = 0.0, sxy = 0.0;
+      double u0[Q1M], u1[Q1M], sxx = 0.0, sxxp = 0.0, sxy = 0.0;
 #pragma unroll
       for (int a = 0; a < Q1M; ++a) u0[a] = u1[a] = 0.0;
       for (int c0 = 0; c0 < N; c0 += 64) {
@@ -1401,6 +1410,7 @@ __global__ void __launch_bounds__(64) hyb_pair_kernel(const eqbf_chunk c, const 
         }
         __syncthreads();
         const uint64_t m = pres[(int64_t)s1 * W + (c0 >> 6)];
+        const uint64_t mp = split ? presp[(int64_t)s1 * W + (c0 >> 6)] : m;
         for (int j0 = 0; j0 < n; j0 += 8) {
           double xv[8];
 #pragma unroll
@@ -1408,18 +1418,24 @@ __global__ void __launch_bounds__(64) hyb_pair_kernel(const eqbf_chunk c, const 
 #pragma unroll
           for (int k = 0; k < 8; ++k) {
             const int j = j0 + k;
-            if (j >= n || !((m >> j) & 1ull)) continue;
+            const bool in0 = (m >> j) & 1ull, inp = (mp >> j) & 1ull;
+            if (j >= n || !(in0 || inp)) continue;
             const double x = xv[k];
-            sxx += x * x;
-            sxy += x * st[2 * Q * 64 + j];
-            u0[0] += x;
-            u1[0] += x;
+            if (in0) {
+              sxx += x * x;
+              sxy += x * st[2 * Q * 64 + j];
+              u0[0] += x;
 #pragma unroll
-            for (int a = 1; a < Q1M; ++a)
-              if (a < Q1) {
-                u0[a] += st[(a - 1) * 64 + j] * x;
-                u1[a] += st[(Q + a - 1) * 64 + j] * x;
-              }
+              for (int a = 1; a < Q1M; ++a)
+                if (a < Q1) u0[a] += st[(a - 1) * 64 + j] * x;
+            }
+            if (inp) {
+              sxxp += x * x;
+              u1[0] += x;
+#pragma unroll
+              for (int a = 1; a < Q1M; ++a)
+                if (a < Q1) u1[a] += st[(Q + a - 1) * 64 + j] * x;
+            }
           }
         }
       }
@@ -1448,7 +1464,7 @@ __global__ void __launch_bounds__(64) hyb_pair_kernel(const eqbf_chunk c, const 
           if (a < Q1 && b2 < Q1) r += R1[a * Q1M + b2] * u1[b2];
         w1[a] = r;
       }
-      double k1 = sxx;
+      double k1 = sxxp;
 #pragma unroll
       for (int a = 0; a < Q1M; ++a)
         if (a < Q1) k1 -= u1[a] * w1[a];
@@ -1467,7 +1483,7 @@ __global__ void __launch_bounds__(64) hyb_pair_kernel(const eqbf_chunk c, const 
             st[(Q + 1) * 64 + lane] = yp2[c0 + lane];
           }
           __syncthreads();
-          const uint64_t m1 = pres[(int64_t)s1 * W + (c0 >> 6)], m2 = pres[(int64_t)s2 * W + (c0 >> 6)];
+          const uint64_t m1 = presp[(int64_t)s1 * W + (c0 >> 6)], m2 = presp[(int64_t)s2 * W + (c0 >> 6)];
           for (int j0 = 0; j0 < n; j0 += 8) {
             double xv[8];
 #pragma unroll
@@ -1675,12 +1691,16 @@ int hybrid_run_sm(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_hybrid_opts* o,
     hipLaunchKernelGGL(hyb_gene_kernel, dim3(G, NSET), dim3(256), 0, st, c, hg);
   }
   const int W = (N + 63) / 64;
-  uint64_t* pres = (uint64_t*)ctx_buf(ctx, "hyb_pres", (size_t)G * S * W * sizeof(uint64_t));
+  const bool split = hg.yp != c.explevels;
+  uint64_t* pres = (uint64_t*)ctx_buf(ctx, "hyb_pres", (size_t)(split ? 2 : 1) * G * S * W * sizeof(uint64_t));
   if (!pres) return ctx_fail(ctx, EQBF_ERR_HIP, "allocating hybrid buffers failed");
-  hipLaunchKernelGGL(hyb_pres_kernel, dim3(G, S), dim3(64), 0, st, c, pres, W);
+  uint64_t* presp = split ? pres + (size_t)G * S * W : pres;
+  hipLaunchKernelGGL(hyb_pres_kernel, dim3(G, S), dim3(64), 0, st, c, c.explevels, pres, W);
+  if (split) hipLaunchKernelGGL(hyb_pres_kernel, dim3(G, S), dim3(64), 0, st, c, hg.yp, presp, W);
   HybPair hp;
   hp.tiles = dtiles;
   hp.pres = pres;
+  hp.presp = presp;
   hp.cp = hg.cp;
   hp.yp = hg.yp;
   hp.rec = rec;

@@ -48,6 +48,16 @@ __global__ void perm_gather_kernel(const double* __restrict__ y, const uint8_t* 
   if (threadIdx.x == 0) gene_in_v[(int64_t)v * S + s] = gene_in[(int64_t)g * S + s];
 }
 
+/* the levels as loaded, per virtual gene (the hybrid model's pairwise step) */
+__global__ void perm_levels_kernel(const double* __restrict__ y, const int32_t* __restrict__ vg_gene, int S, int N,
+                                   double* __restrict__ y_v) {
+  const int v = blockIdx.x / S, s = blockIdx.x - (blockIdx.x / S) * S;
+  const int g = vg_gene[v];
+  const double* src = y + ((int64_t)g * S + s) * N;
+  double* dst = y_v + ((int64_t)v * S + s) * N;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) dst[i] = src[i];
+}
+
 __global__ void perm_pairs_kernel(const int64_t* __restrict__ pair_offset, const int32_t* __restrict__ pair_snp,
                                   const int32_t* __restrict__ vg_gene, const int64_t* __restrict__ off_v,
                                   int32_t* __restrict__ pair_snp_v) {
@@ -196,8 +206,9 @@ static int general_items_run(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_perm
   const int64_t n_w = 3 + n_cfg + 2;
   const int w_row = !join ? 0 : (po->pbf == EQBF_PBF_GEN ? 0 : (po->pbf == EQBF_PBF_GENSIN ? (int)(3 + n_cfg) : (int)(3 + n_cfg + 1)));
   const bool mvlr = join && po->error_model == EQBF_ERROR_MVLR;
+  const bool hybrid = join && po->error_model == EQBF_ERROR_HYBRID;
   const int GL = (int)ctx->phiL.size(), GS = (int)ctx->phiS.size();
-  const int64_t n_raw = mvlr ? 3 * (int64_t)GL + n_cfg * GS : 0;
+  const int64_t n_raw = (mvlr || hybrid) ? 3 * (int64_t)GL + n_cfg * GS : 0;
   /* batch: at most 64k virtual genes and a bounded number of virtual pairs */
   const int64_t per_pair_bytes = (int64_t)S * 4 + (join ? (n_w + n_raw) * 8 : (int64_t)5 * S * 8) + 4;
   const int64_t pmax = std::max<int64_t>(1 << 16, std::min<int64_t>(8 << 20, ((int64_t)3 << 30) / per_pair_bytes));
@@ -220,8 +231,10 @@ static int general_items_run(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_perm
     int32_t* d_n = (int32_t*)ctx_buf(ctx, "p_n", (size_t)S * std::max<int64_t>(NPv, 1) * sizeof(int32_t));
     double* d_w = join ? (double*)ctx_buf(ctx, "p_w", (size_t)n_w * std::max<int64_t>(NPv, 1) * sizeof(double)) : nullptr;
     double* d_ss = !join ? (double*)ctx_buf(ctx, "p_ss", (size_t)5 * S * std::max<int64_t>(NPv, 1) * sizeof(double)) : nullptr;
-    double* d_raw = mvlr ? (double*)ctx_buf(ctx, "p_raw", (size_t)n_raw * std::max<int64_t>(NPv, 1) * sizeof(double)) : nullptr;
-    if (!d_vg || !d_off || !d_y || !d_gin || !d_psnp || !d_n || (join && !d_w) || (!join && !d_ss) || (mvlr && !d_raw))
+    double* d_raw = (mvlr || hybrid) ? (double*)ctx_buf(ctx, "p_raw", (size_t)n_raw * std::max<int64_t>(NPv, 1) * sizeof(double)) : nullptr;
+    double* d_yp = hybrid ? (double*)ctx_buf(ctx, "p_yp", (size_t)VB * S * N * sizeof(double)) : nullptr;
+    if (!d_vg || !d_off || !d_y || !d_gin || !d_psnp || !d_n || (join && !d_w) || (!join && !d_ss) ||
+        ((mvlr || hybrid) && !d_raw) || (hybrid && !d_yp))
       return EQBF_ERR_HIP;
     std::vector<int32_t> packed;
     packed.insert(packed.end(), vg_gene.begin(), vg_gene.end());
@@ -237,6 +250,9 @@ static int general_items_run(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_perm
                          tables_dev, d_vg, d_vg + VB, d_vg + 2 * VB, nperm, S, N, d_y, d_gin);
       hipLaunchKernelGGL(perm_pairs_kernel, dim3(VB), dim3(256), 0, st, c.pair_offset, c.pair_snp, d_vg, d_off,
                          d_psnp);
+      if (hybrid)
+        hipLaunchKernelGGL(perm_levels_kernel, dim3(VB * S), dim3(256), 0, st,
+                           po->explevels_pairs ? po->explevels_pairs : c.explevels, d_vg, S, N, d_yp);
     }
     EQBF_HIP(ctx, hipGetLastError());
     eqbf_chunk vc = c;
@@ -258,6 +274,14 @@ static int general_items_run(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_perm
       mo.fiterr = po->fiterr;
       vo.raw_abf = d_raw;
       rc = mvlr_run(ctx, &vc, &mo, &vo, st);
+    } else if (hybrid) {
+      eqbf_hybrid_opts ho;
+      ho.bfs = bfs;
+      ho.fiterr = po->fiterr;
+      ho.covars_by_position = po->covars_by_position;
+      ho.explevels_pairs = d_yp;
+      vo.raw_abf = d_raw;
+      rc = hybrid_run(ctx, &vc, &ho, &vo, st);
     } else {
       rc = assoc_uvlr_run(ctx, &vc, &ao, &vo, st);
     }
@@ -266,7 +290,7 @@ static int general_items_run(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_perm
       ProfScope ps(ctx, fp64_check ? "fp64_check_reduce_kernel" : "perm_reduce_kernel", st);
       hipLaunchKernelGGL(perm_reduce_kernel, dim3(VB), dim3(64), 0, st, d_off, d_vg, d_vg + 2 * VB, d_psnp, d_n,
                          d_ss, d_w, c.gene_in_sbgrp, c.snp_in_sbgrp, NPv, S, (int)join, w_row, po->use_max_bf,
-                         po->sep_sbgrp, out_stride, (int)mvlr, out);
+                         po->sep_sbgrp, out_stride, (int)(mvlr || hybrid), out);
     }
     EQBF_HIP(ctx, hipGetLastError());
     /* the host vectors are reused for the next batch */
@@ -322,11 +346,12 @@ int perm_stats_run(eqbf_ctx* ctx, const eqbf_chunk* cp, const eqbf_perm_opts* po
   nan_fill(stats_dev, (int64_t)G * nperm, st);
   EQBF_HIP(ctx, hipGetLastError());
   /* genes that qualify go through the contraction path first */
-  const bool mvlr = po->analysis == EQBF_ANALYSIS_JOIN && po->error_model == EQBF_ERROR_MVLR;
-  if (po->error_model != EQBF_ERROR_UVLR && !mvlr)
+  const bool join = po->analysis == EQBF_ANALYSIS_JOIN;
+  const bool multi = join && (po->error_model == EQBF_ERROR_MVLR || po->error_model == EQBF_ERROR_HYBRID);
+  if (po->error_model != EQBF_ERROR_UVLR && !multi)
     return ctx_fail(ctx, EQBF_ERR_ARG, "error_model %d is not valid for this analysis", po->error_model);
   std::vector<uint8_t> done(G, 0);
-  if (!mvlr) { /* the contraction path is the uvlr model's */
+  if (!multi) { /* the contraction path is the uvlr model's */
     rc = perm_fast_run(ctx, cp, po, tables_dev, n_tables, gene_table, nperm, stats_dev, st, done);
     if (rc) return rc;
   }
@@ -348,9 +373,10 @@ int perm_ties_run(eqbf_ctx* ctx, const eqbf_chunk* cp, const eqbf_perm_opts* po,
   if (!(tie_bound >= 0)) return ctx_fail(ctx, EQBF_ERR_ARG, "tie_bound must be >= 0");
   int rc = perm_stats_run(ctx, cp, po, tables_dev, n_tables, gene_table, nperm, stats_dev, st);
   if (rc) return rc;
-  /* FP64 already: the FP64 precisions and the MVLR engine */
+  /* FP64 already: the FP64 precisions and the MVLR and hybrid engines */
   const bool fp64 = po->precision == EQBF_PREC_FP64_PLAIN || po->precision == EQBF_PREC_FP64_BATCHED ||
-                    (po->analysis == EQBF_ANALYSIS_JOIN && po->error_model == EQBF_ERROR_MVLR);
+                    (po->analysis == EQBF_ANALYSIS_JOIN &&
+                     (po->error_model == EQBF_ERROR_MVLR || po->error_model == EQBF_ERROR_HYBRID));
   if (fp64 || nperm <= 0) return EQBF_OK;
   /* permutations within the bound of the caller's true statistic */
   const int64_t n = (int64_t)G * nperm;

@@ -96,7 +96,7 @@ def lib():
 class PermOpts(C.Structure):
     _fields_ = [("analysis", C.c_int32), ("pbf", C.c_int32), ("use_max_bf", C.c_int32),
                 ("sep_sbgrp", C.c_int32), ("precision", C.c_int32), ("error_model", C.c_int32),
-                ("fiterr", C.c_double)]
+                ("fiterr", C.c_double), ("covars_by_position", C.c_void_p), ("explevels_pairs", C.c_void_p)]
 
 
 class MvlrOpts(C.Structure):
@@ -372,16 +372,20 @@ class Engine:
                                                       C.byref(out_struct), stream))
 
     def perm_stats(self, hc, tables, gene_table, analysis="join", pbf="all", use_max_bf=True, sep_sbgrp=-1,
-                   precision="mixed", error_model="uvlr", fiterr=0.5):
+                   precision="mixed", error_model="uvlr", fiterr=0.5, covars_pos=None, expl_pair=None):
         """Statistic of every permutation: tables [n_tables][nperm][N],
         gene_table [n_genes] (-1 skips a gene); returns [n_genes][nperm].
-        error_model 'mvlr': MVLR Bayes factors (join)."""
+        error_model 'mvlr' / 'hybrid': MVLR / hybrid Bayes factors (join);
+        covars_pos, expl_pair: the hybrid model's pairwise inputs."""
         tables = np.ascontiguousarray(tables, dtype=np.int32)
         gt = np.ascontiguousarray(gene_table, dtype=np.int32)
         nperm = tables.shape[1]
         out = np.full((hc.G, nperm), np.nan)
+        cp = None if covars_pos is None else np.ascontiguousarray(covars_pos, dtype=np.float64)
+        ep = None if expl_pair is None else np.ascontiguousarray(expl_pair, dtype=np.float64)
         po = PermOpts(_abi.ANALYSIS[analysis], _abi.PBF[pbf], int(use_max_bf), sep_sbgrp, _abi.PREC[precision],
-                      {"uvlr": 0, "mvlr": 1}[error_model], float(fiterr))
+                      {"uvlr": 0, "mvlr": 1, "hybrid": 2}[error_model], float(fiterr),
+                      None if cp is None else cp.ctypes.data, None if ep is None else ep.ctypes.data)
         st = hc.struct()
         self._check(lib().eqbf_perm_stats(self._ctx, C.byref(st), C.byref(po), _p(tables), tables.shape[0],
                                           _p(gt), nperm, _p(out)))

@@ -895,13 +895,13 @@ int main(int argc, char** argv) {
     if (perm_join || perm_sep) {
       const int64_t N = sh.N;
       /* makePermutationsJoin (eqtlbma_bf.cpp:851-858): genes with a cis SNP;
-       * with --error mvlr also expression in every subgroup */
+       * with --error mvlr or hybrid also expression in every subgroup */
       std::vector<int> genes;
       for (int g = 0; g < G; ++g) {
         if (ch.pair_offset[g + 1] <= ch.pair_offset[g]) continue;
         bool all_expr = true;
         for (int s = 0; s < S; ++s) all_expr &= ch.genes[g]->exp_row[s] >= 0;
-        if (mvlr && !all_expr) continue;
+        if ((mvlr || hybrid) && !all_expr) continue;
         genes.push_back(g);
       }
       eqbf_rng rng, rngT;
@@ -911,8 +911,12 @@ int main(int argc, char** argv) {
       };
       eqbf_perm_opts po{};
       po.precision = prec;
-      po.error_model = mvlr ? EQBF_ERROR_MVLR : EQBF_ERROR_UVLR;
+      po.error_model = mvlr ? EQBF_ERROR_MVLR : (hybrid ? EQBF_ERROR_HYBRID : EQBF_ERROR_UVLR);
       po.fiterr = o.prop_cov_errors;
+      if (hybrid) {
+        po.covars_by_position = sh.Qmax > 0 ? sh.covars_pos.data() : nullptr;
+        po.explevels_pairs = ch.expl_raw.empty() ? nullptr : ch.expl_raw.data();
+      }
       std::vector<double> truth;
       if (perm_join) {
         reseed();

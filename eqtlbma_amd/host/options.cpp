@@ -197,8 +197,6 @@ Options parse_options(int argc, char** argv) {
   if (o.bfs != "gen" && o.bfs != "sin" && o.bfs != "all") bad(o, prog, "--bfs " + o.bfs + " is not valid");
   if (o.error_model != "uvlr" && o.error_model != "mvlr" && o.error_model != "hybrid")
     bad(o, prog, "--error " + o.error_model + " is not valid");
-  if (o.error_model == "hybrid" && o.analysis == "join" && o.nb_permutations > 0)
-    bad(o, prog, "--error hybrid with --nperm is not supported by this build");
   if (o.trick != 0 && o.trick != 1 && o.trick != 2) bad(o, prog, "--trick " + std::to_string(o.trick) + " is not valid");
   if (o.trick != 0 && o.trick_cutoff > o.nb_permutations)
     bad(o, prog, "--tricut " + std::to_string(o.trick_cutoff) + " is larger than --nperm " +

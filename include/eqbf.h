@@ -288,9 +288,20 @@ typedef struct eqbf_perm_opts {
    * value in the reference's loop, gene.cpp:650, 663-664); the subgroups of
    * a gene must keep the same samples under every permutation. */
   int32_t error_model;
-  double fiterr;       /* --fiterr with EQBF_ERROR_MVLR                    */
+  double fiterr;       /* --fiterr with EQBF_ERROR_MVLR / EQBF_ERROR_HYBRID */
+  /* EQBF_ERROR_HYBRID (--error hybrid, gene.cpp:669-672): every
+   * permutation's Bayes factors come from the hybrid engine
+   * (eqbf_assoc_hybrid): the per-subgroup fits see the permuted levels, the
+   * pairwise step the levels as loaded (CalcOffDiagCovarsFromPairsOfSubgroups
+   * takes no permutation, gene_snp_pair.cpp:1157-1159); a SNP without
+   * genotypes in every subgroup contributes 0 as with EQBF_ERROR_MVLR.
+   * covars_by_position and explevels_pairs as in eqbf_hybrid_opts (NULL:
+   * the chunk's); host arrays for the host-pointer entries, device arrays
+   * for the _device ones. */
+  const double* covars_by_position; /* [S][Qmax][N]                          */
+  const double* explevels_pairs;    /* [G][S][N]                             */
 } eqbf_perm_opts;
-enum { EQBF_ERROR_UVLR = 0, EQBF_ERROR_MVLR = 1 };
+enum { EQBF_ERROR_UVLR = 0, EQBF_ERROR_MVLR = 1, EQBF_ERROR_HYBRID = 2 };
 
 /* Statistic of every permutation of every gene of the chunk.
  * perm_idx:   [n_tables][nperm][n_samples]; sample i of permutation p of

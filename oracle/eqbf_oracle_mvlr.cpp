@@ -470,8 +470,9 @@ extern "C" int orc_assoc_mvlr(const eqbf_chunk* c, int32_t bfs, double fiterr, c
 /* covars_pos: [S][Qmax][N], the covariate value at file column i of each   */
 /* subgroup (NaN past the file's columns); NULL: the chunk's covariates.    */
 /* expl_pair: [G][S][N] expression levels of the pairwise step, which the   */
-/* reference reads without --qnorm's normalisation (Gene::GetExplevel);     */
-/* NULL: the chunk's.                                                       */
+/* reference reads without --qnorm's normalisation (Gene::GetExplevel) and */
+/* without permutation; their NaN pattern also decides who is present in   */
+/* the pairwise step.  NULL: the chunk's.                                   */
 /* Pinned in tests/test_oracle_hybrid.py: per-subgroup quantities by numpy  */
 /* least squares, the pairwise covariances by residual cross-products, Vg   */
 /* off the diagonal by a different algebra, the Bayes factor by the ratio   */
@@ -552,8 +553,11 @@ static int hybrid_pair(const eqbf_chunk* c, int64_t g, int64_t p, double fiterr,
   const int64_t NP = c->n_pairs, N = c->n_samples, QM = c->n_covars_max;
   const int64_t snp = c->pair_snp[p];
   const ldouble f = fiterr;
+  /* the pairwise step picks its individuals from the levels it reads, which
+   * permutations leave as loaded (samples.cpp:148-192) */
   auto present = [&](int32_t s, int64_t i) {
-    return c->geno_in_sbgrp[(int64_t)s * N + i] && !std::isnan(expl_at(c, g, s, i));
+    const double y = expl_pair ? expl_pair[(g * S + s) * N + i] : expl_at(c, g, s, i);
+    return c->geno_in_sbgrp[(int64_t)s * N + i] && !std::isnan(y);
   };
   b.assign(S, 0);
   std::vector<ldouble> sd(S), vd(S);

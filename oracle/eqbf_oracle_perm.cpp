@@ -164,6 +164,58 @@ extern "C" int orc_perm_join_stats_mvlr(const eqbf_chunk* c, int64_t g, int32_t 
   return EQBF_OK;
 }
 
+/* --error hybrid permutation statistics of gene g (gene.cpp:669-672): the
+ * per-subgroup fits on the permuted levels, the pairwise step on the levels
+ * as loaded (expl_pair, NULL: the chunk's), a SNP without genotypes in
+ * every subgroup contributing 0.  Pinned by construction: the hybrid oracle
+ * (pinned in tests/test_oracle_hybrid.py) on explicitly permuted levels. */
+extern "C" int orc_assoc_hybrid(const eqbf_chunk* c, int32_t bfs, double fiterr, const double* covars_pos,
+                                const double* expl_pair, const double* phiL, const double* omaL, int32_t GL,
+                                const double* phiS, const double* omaS, int32_t GS, int32_t* n_out, double* sstats,
+                                double* raw, double* w_abf);
+extern "C" int orc_perm_join_stats_hybrid(const eqbf_chunk* c, int64_t g, int32_t pbf, int32_t use_max_bf,
+                                          double fiterr, const double* covars_pos, const double* expl_pair,
+                                          const double* phiL, const double* omaL, int32_t GL, const double* phiS,
+                                          const double* omaS, int32_t GS, const int32_t* table, int64_t nperm,
+                                          double* stats) {
+  const int32_t S = c->n_subgroups;
+  const int64_t N = c->n_samples;
+  const int64_t p0 = c->pair_offset[g], np = c->pair_offset[g + 1] - p0;
+  const int32_t bfs = pbf_to_bfs(pbf);
+  const int64_t nc = n_configs(bfs, S), n_raw = 3 * (int64_t)GL + nc * GS, n_w = 3 + nc + 2;
+  const int64_t row = pbf_index(pbf, S);
+  std::vector<double> y((size_t)S * N);
+  const double* yp = (expl_pair ? expl_pair : c->explevels) + (size_t)g * S * N;
+  std::vector<int64_t> poff = {0, np};
+  eqbf_chunk c1 = *c;
+  c1.n_genes = 1;
+  c1.n_pairs = np;
+  c1.explevels = y.data();
+  c1.gene_in_sbgrp = c->gene_in_sbgrp + g * S;
+  c1.pair_offset = poff.data();
+  c1.pair_snp = c->pair_snp + p0;
+  std::vector<int32_t> n_out((size_t)S * std::max<int64_t>(np, 1));
+  std::vector<double> ss((size_t)5 * S * std::max<int64_t>(np, 1));
+  std::vector<double> raw((size_t)n_raw * std::max<int64_t>(np, 1)), w((size_t)n_w * std::max<int64_t>(np, 1));
+  std::vector<double> v(np);
+  for (int64_t p = 0; p < nperm; ++p) {
+    const int32_t* perm = table + p * N;
+    for (int32_t s = 0; s < S; ++s)
+      for (int64_t i = 0; i < N; ++i) y[(size_t)s * N + i] = expl_at(c, g, s, perm[i]);
+    int rc = orc_assoc_hybrid(&c1, bfs, fiterr, covars_pos, yp, phiL, omaL, GL, phiS, omaS, GS, n_out.data(),
+                              ss.data(), raw.data(), w.data());
+    if (rc) return rc;
+    for (int64_t q = 0; q < np; ++q) {
+      const int64_t snp = c1.pair_snp[q];
+      bool all_geno = true;
+      for (int32_t s = 0; s < S; ++s) all_geno &= c->snp_in_sbgrp[snp * S + s] != 0;
+      v[q] = all_geno ? w[(size_t)row * np + q] : 0.0;
+    }
+    stats[p] = use_max_bf ? max_elem(v) : orc_log10_weighted_sum(v.data(), (int64_t)v.size());
+  }
+  return EQBF_OK;
+}
+
 /* True statistic of a gene from its pairs' weighted ABF[pbf]
  * (FindMaxTrueL10Abf gene.cpp:575-582 / AvgTrueL10Abfs :587-596). */
 extern "C" double orc_true_stat(const double* wpbf, int64_t n, int32_t use_max_bf) {

@@ -81,6 +81,9 @@ def lib():
         L.orc_assoc_hybrid.argtypes = [vp, i32, d, vp, vp, vp, vp, i32, vp, vp, i32, vp, vp, vp, vp]
         L.orc_assoc_hybrid.restype = i32
         L.orc_hybrid_pair_stats.argtypes = [vp, i64, i64, d, vp, vp, vp, vp, vp]
+        L.orc_perm_join_stats_hybrid.argtypes = [vp, i64, i32, i32, d, vp, vp, vp, vp, i32, vp, vp, i32, vp, i64,
+                                                 vp]
+        L.orc_perm_join_stats_hybrid.restype = i32
         L.orc_hybrid_pair_stats.restype = i32
         L.orc_avg_grid.argtypes = [vp, i64, i32, vp, vp, i32, vp]
         L.orc_avg_bfs.argtypes = [vp, vp, i64, i32, i32, i32, i32, vp, i32, vp, vp, vp, vp, vp, d, i32,
@@ -346,6 +349,27 @@ def perm_join_stats_mvlr(hc, g, pbf, use_max_bf, fiterr, gridL, gridS, table):
     rc = lib().orc_perm_join_stats_mvlr(C.addressof(st), g, _abi.PBF[pbf], int(use_max_bf), float(fiterr),
                                         _p(phiL), _p(omaL), gL.shape[0], _p(phiS), _p(omaS), gS.shape[0],
                                         _p(table), table.shape[0], _p(stats))
+    if rc != 0:
+        raise RuntimeError("oracle: %s" % lib().orc_last_error().decode())
+    return stats
+
+
+def perm_join_stats_hybrid(hc, g, pbf, use_max_bf, fiterr, gridL, gridS, table, covars_pos=None, expl_pair=None):
+    """--error hybrid permutation statistics of gene g for every row of
+    table (pairwise step on the levels as loaded)."""
+    gL = np.ascontiguousarray(gridL, dtype=np.float64)
+    gS = np.ascontiguousarray(gridS, dtype=np.float64)
+    phiL, omaL = np.ascontiguousarray(gL[:, 0]), np.ascontiguousarray(gL[:, 1])
+    phiS, omaS = np.ascontiguousarray(gS[:, 0]), np.ascontiguousarray(gS[:, 1])
+    table = np.ascontiguousarray(table, dtype=np.int32)
+    stats = np.zeros(table.shape[0])
+    cp = None if covars_pos is None else np.ascontiguousarray(covars_pos, dtype=np.float64)
+    ep = None if expl_pair is None else np.ascontiguousarray(expl_pair, dtype=np.float64)
+    st = hc.struct()
+    rc = lib().orc_perm_join_stats_hybrid(C.addressof(st), g, _abi.PBF[pbf], int(use_max_bf), float(fiterr),
+                                          None if cp is None else _p(cp), None if ep is None else _p(ep),
+                                          _p(phiL), _p(omaL), gL.shape[0], _p(phiS), _p(omaS), gS.shape[0],
+                                          _p(table), table.shape[0], _p(stats))
     if rc != 0:
         raise RuntimeError("oracle: %s" % lib().orc_last_error().decode())
     return stats

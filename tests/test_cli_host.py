@@ -50,8 +50,6 @@ def test_help_and_version():
      "--tricut 10 is larger than --nperm 5"),
     (["--analys", "sep", "--wrtsize", "0"], "--wrtsize should be greater than 1"),
     (["--analys", "sep", "--anchor", "TES"], "--anchor should be TSS or TSS+TES"),
-    (["--analys", "join", "--gridL", "GL", "--error", "hybrid", "--nperm", "10", "--pbf", "gen"],
-     "--error hybrid with --nperm is not supported by this build"),
 ])
 def test_option_checks(root, extra, msg):
     extra = [os.path.join(root, "gridL.txt") if a == "GL" else os.path.join(root, "gridS.txt") if a == "GS" else a

@@ -16,7 +16,11 @@ quantgen/gene_snp_pair.cpp:1384-1423, and what it calls).
   (Sigma = null residual covariance, Vg = Sigma / x~'x~ by Frisch-Waugh):
   the whole chunk output equals the oracle's MVLR output, itself pinned by
   tests/test_oracle_mvlr.py;
-* the summary statistics are those of the univariate separate analysis.
+* the summary statistics are those of the univariate separate analysis;
+* permutations (Gene::MakePermutationsJoin, gene.cpp:669-672) permute the
+  per-subgroup fits only: the pairwise step reads the levels and picks the
+  individuals as loaded (gene_snp_pair.cpp:1157-1159, samples.cpp:148-192);
+  each permuted statistic from numpy fits and scipy densities.
 """
 import numpy as np
 import pytest
@@ -37,12 +41,15 @@ def _lsq_resid(X, y):
     return coef, y - X @ coef
 
 
-def numpy_pair(hc, g, p, f, covars_pos=None, expl_pair=None):
+def numpy_pair(hc, g, p, f, covars_pos=None, expl_pair=None, perm=None):
+    """perm: individual i of the per-subgroup fits takes the level of
+    individual perm[i]; the pairwise step reads the levels as loaded."""
     S, N = hc.S, hc.N
     x = hc.genotypes[0][:, hc.pair_snp[p]]
-    y = hc.explevels[g]
-    yp = y if expl_pair is None else expl_pair[g]
+    yp = hc.explevels[g] if expl_pair is None else expl_pair[g]
+    y = hc.explevels[g] if perm is None else hc.explevels[g][:, perm]
     pres = ~np.isnan(y) & (hc.geno_in_sbgrp == 1)
+    presp = ~np.isnan(yp) & (hc.geno_in_sbgrp == 1)
     b, Sig, Vg = np.zeros(S), np.zeros((S, S)), np.zeros((S, S))
     for s in range(S):
         Q = hc.covar_count[s]
@@ -62,7 +69,7 @@ def numpy_pair(hc, g, p, f, covars_pos=None, expl_pair=None):
 
             def design(mask):
                 return np.column_stack([np.ones(mask.sum()), x[mask]] + [cov[s1, q, mask] for q in range(Q)])
-            both, only1, only2 = pres[s1] & pres[s2], pres[s1] & ~pres[s2], ~pres[s1] & pres[s2]
+            both, only1, only2 = presp[s1] & presp[s2], presp[s1] & ~presp[s2], ~presp[s1] & presp[s2]
             X12 = design(both)
             n12 = both.sum()
             Y12 = np.column_stack([yp[s1, both], yp[s2, both]])
@@ -179,3 +186,30 @@ def test_no_common_individual_is_an_error(orc):
     hc.explevels[0, 1, 35:] = np.nan
     with pytest.raises(RuntimeError, match="no individuals in common"):
         orc.assoc_hybrid(hc, "gen", 0.5, synth.make_grid("general"), None)
+
+
+def _l10_mean(v):
+    m = np.max(v)
+    return m + np.log10(np.mean(10.0 ** (np.asarray(v) - m)))
+
+
+@pytest.mark.parametrize("use_max", [True, False])
+def test_permutations_permute_the_subgroup_fits_only(orc, use_max):
+    hc = hybrid_chunk(71, S=3, N=48, n_genes=2, P=5, Q=1, miss=0.15)
+    gL, gS = synth.make_grid("general")[::3], synth.make_grid("small")
+    rng = np.random.default_rng(5)
+    table = np.array([rng.permutation(hc.N) for _ in range(3)] + [np.arange(hc.N)], np.int32)
+    one = np.ones(hc.S)
+    for g in range(hc.G):
+        got = orc.perm_join_stats_hybrid(hc, g, "gen", use_max, 0.5, gL, gS, table)
+        assert np.isfinite(got).all()
+        for r, perm in enumerate(table):
+            v = []
+            for p in range(hc.pair_offset[g], hc.pair_offset[g + 1]):
+                b, Sig, Vg = numpy_pair(hc, g, p, 0.5, perm=perm)
+                v.append(_l10_mean([mvn_l10abf(b, Sig, Vg, one, ph, om) for ph, om in gL]))
+            want = max(v) if use_max else _l10_mean(v)
+            assert got[r] == pytest.approx(want, abs=1e-8)
+        out = orc.assoc_hybrid(hc, "gen", 0.5, gL, gS)
+        w0 = out.w[0, hc.pair_offset[g]:hc.pair_offset[g + 1]]
+        assert got[-1] == pytest.approx(max(w0) if use_max else _l10_mean(w0), abs=1e-12)
