@@ -292,8 +292,9 @@ typedef struct eqbf_perm_opts {
   /* EQBF_ERROR_HYBRID (--error hybrid, gene.cpp:669-672): every
    * permutation's Bayes factors come from the hybrid engine
    * (eqbf_assoc_hybrid): the per-subgroup fits see the permuted levels, the
-   * pairwise step the levels as loaded (CalcOffDiagCovarsFromPairsOfSubgroups
-   * takes no permutation, gene_snp_pair.cpp:1157-1159); a SNP without
+   * pairwise step the levels as loaded, and its sets of individuals from
+   * them (CalcOffDiagCovarsFromPairsOfSubgroups takes no permutation,
+   * gene_snp_pair.cpp:1157-1159, samples.cpp:148-192); a SNP without
    * genotypes in every subgroup contributes 0 as with EQBF_ERROR_MVLR.
    * covars_by_position and explevels_pairs as in eqbf_hybrid_opts (NULL:
    * the chunk's); host arrays for the host-pointer entries, device arrays
