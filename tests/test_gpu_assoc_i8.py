@@ -117,3 +117,27 @@ def test_i8_sums_vs_fp64_product(eng, monkeypatch):
         err = float(np.max(np.abs(x[fin] - y[fin]), initial=0))
         _say("[gpu]   %s int8 vs FP64 product max abs diff %.2e" % (name, err))
         assert err <= 2e-5, name
+
+
+@pytest.mark.parametrize("n_all", [128, 192, 256, 320])
+def test_i8_persistent_half_tiles(eng, monkeypatch, n_all):
+    """~30,000 pairs, so each of the persistent workgroups runs about four
+    32-pair half tiles one after another (the centre and scale of each, the
+    digit buffers and the genotype registers rotating over 2, 3, 4 and 5
+    sample steps of 64).  Same chunk through the int8 and the FP64
+    products, as above."""
+    ns_hi = min(300, n_all - 8)
+    hc = synth.batched(seed=93, S=6, N_all=n_all, n_genes=60, p_mean=500, p_lo=400, p_hi=600,
+                       ns_lo=40, ns_hi=ns_hi, Q=2, nan_frac=0.01)
+    eng.set_grids(GL25, GS10)
+    _say("[gpu] int8 persistent half tiles N=%d pairs=%d" % (n_all, hc.n_pairs))
+    a = eng.assoc_uvlr(hc, "join", "sin", precision="mixed", want_sstats=False, want_std=False)
+    monkeypatch.setenv("EQBF_I8_SUMS", "0")
+    b = eng.assoc_uvlr(hc, "join", "sin", precision="mixed", want_sstats=False, want_std=False)
+    assert np.array_equal(a.n, b.n)
+    for x, y, name in [(a.raw.astype(np.float64), b.raw.astype(np.float64), "raw"), (a.w, b.w, "weighted")]:
+        assert np.array_equal(np.isnan(x), np.isnan(y)), name
+        fin = np.isfinite(y)
+        err = float(np.max(np.abs(x[fin] - y[fin]), initial=0))
+        _say("[gpu]   %s int8 vs FP64 product max abs diff %.2e" % (name, err))
+        assert err <= 2e-5, name
