@@ -1361,8 +1361,10 @@ __global__ void __launch_bounds__(64) hyb_pres_kernel(const eqbf_chunk c, const 
  * individuals go by chunks of 64: the block stages the chunk's covariates
  * and levels in LDS (every lane reads them by broadcast), the presence bits
  * come as one uniform word a chunk, and each lane loads its genotypes eight
- * individuals at a time so that the loads overlap. */
-template <int Q1M, class GT>
+ * individuals at a time so that the loads overlap.  SPLIT: the pairwise
+ * sets have presence bits of their own (hp.presp != hp.pres), so each
+ * subgroup's first pass keeps its per-subgroup and pairwise sums apart. */
+template <int Q1M, class GT, bool SPLIT>
 __global__ void __launch_bounds__(64) hyb_pair_kernel(const eqbf_chunk c, const HybPair hp) {
   __shared__ double st[(2 * (Q1M - 1) + 2) * 64];
   const HybTile tl = hp.tiles[blockIdx.x];
@@ -1382,7 +1384,6 @@ __global__ void __launch_bounds__(64) hyb_pair_kernel(const eqbf_chunk c, const 
   if (gene_ok) {
     const uint64_t* pres = hp.pres + (int64_t)g * S * W;
     const uint64_t* presp = hp.presp + (int64_t)g * S * W;
-    const bool split = hp.presp != hp.pres;
     const int npr = S * (S - 1) / 2;
     const int RSZ = hp.RSZ;
     const double* R = hp.rec + (int64_t)g * hp.NSET * RSZ;
@@ -1410,7 +1411,7 @@ __global__ void __launch_bounds__(64) hyb_pair_kernel(const eqbf_chunk c, const 
         }
         __syncthreads();
         const uint64_t m = pres[(int64_t)s1 * W + (c0 >> 6)];
-        const uint64_t mp = split ? presp[(int64_t)s1 * W + (c0 >> 6)] : m;
+        const uint64_t mp = SPLIT ? presp[(int64_t)s1 * W + (c0 >> 6)] : m;
         for (int j0 = 0; j0 < n; j0 += 8) {
           double xv[8];
 #pragma unroll
@@ -1421,6 +1422,19 @@ __global__ void __launch_bounds__(64) hyb_pair_kernel(const eqbf_chunk c, const 
             const bool in0 = (m >> j) & 1ull, inp = (mp >> j) & 1ull;
             if (j >= n || !(in0 || inp)) continue;
             const double x = xv[k];
+            if (!SPLIT) { /* one set: the per-subgroup and pairwise sums together */
+              sxx += x * x;
+              sxy += x * st[2 * Q * 64 + j];
+              u0[0] += x;
+              u1[0] += x;
+#pragma unroll
+              for (int a = 1; a < Q1M; ++a)
+                if (a < Q1) {
+                  u0[a] += st[(a - 1) * 64 + j] * x;
+                  u1[a] += st[(Q + a - 1) * 64 + j] * x;
+                }
+              continue;
+            }
             if (in0) {
               sxx += x * x;
               sxy += x * st[2 * Q * 64 + j];
@@ -1464,7 +1478,7 @@ __global__ void __launch_bounds__(64) hyb_pair_kernel(const eqbf_chunk c, const 
           if (a < Q1 && b2 < Q1) r += R1[a * Q1M + b2] * u1[b2];
         w1[a] = r;
       }
-      double k1 = sxxp;
+      double k1 = SPLIT ? sxxp : sxx;
 #pragma unroll
       for (int a = 0; a < Q1M; ++a)
         if (a < Q1) k1 -= u1[a] * w1[a];
@@ -1716,10 +1730,15 @@ int hybrid_run_sm(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_hybrid_opts* o,
   hp.RSZ = RSZ;
   {
     ProfScope ps(ctx, "hyb_pair_kernel", st);
-    if (c.geno_dtype == EQBF_F32)
-      hipLaunchKernelGGL((hyb_pair_kernel<Q1M, float>), dim3((unsigned)tiles.size()), dim3(64), 0, st, c, hp);
+    const dim3 grid((unsigned)tiles.size());
+    if (c.geno_dtype == EQBF_F32 && split)
+      hipLaunchKernelGGL((hyb_pair_kernel<Q1M, float, true>), grid, dim3(64), 0, st, c, hp);
+    else if (c.geno_dtype == EQBF_F32)
+      hipLaunchKernelGGL((hyb_pair_kernel<Q1M, float, false>), grid, dim3(64), 0, st, c, hp);
+    else if (split)
+      hipLaunchKernelGGL((hyb_pair_kernel<Q1M, double, true>), grid, dim3(64), 0, st, c, hp);
     else
-      hipLaunchKernelGGL((hyb_pair_kernel<Q1M, double>), dim3((unsigned)tiles.size()), dim3(64), 0, st, c, hp);
+      hipLaunchKernelGGL((hyb_pair_kernel<Q1M, double, false>), grid, dim3(64), 0, st, c, hp);
   }
   EQBF_HIP(ctx, hipGetLastError());
   return mv_rows_run<SM>(ctx, c, o->bfs, out, st, [&](int64_t c0, int nb, double* par, int PSZ, int64_t pair_blocks) {
