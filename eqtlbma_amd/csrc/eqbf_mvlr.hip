@@ -1483,21 +1483,43 @@ __global__ void __launch_bounds__(64) hyb_pair_kernel(const eqbf_chunk c, const 
       for (int a = 0; a < Q1M; ++a)
         if (a < Q1) k1 -= u1[a] * w1[a];
       const double* yp1 = hp.yp + ((int64_t)g * S + s1) * N;
-      for (int s2 = s1 + 1; s2 < S; ++s2, ++pi) {
-        const double* yp2 = hp.yp + ((int64_t)g * S + s2) * N;
-        double u12[Q1M], u2[Q1M], sxx12 = 0.0, sxx2 = 0.0, sy1 = 0.0, sy2 = 0.0;
+      /* PW second subgroups per pass over the individuals (the genotypes
+       * are read once for them); a missing second one repeats the first
+       * and is not written */
+      constexpr int PW = Q1M <= 8 ? 2 : 1;
+      for (int s2 = s1 + 1; s2 < S; s2 += PW) {
+        const int nb = min(PW, S - s2);
+        const double* yp2[PW];
+        int sb[PW];
 #pragma unroll
-        for (int a = 0; a < Q1M; ++a) u12[a] = u2[a] = 0.0;
+        for (int b = 0; b < PW; ++b) {
+          sb[b] = s2 + min(b, nb - 1);
+          yp2[b] = hp.yp + ((int64_t)g * S + sb[b]) * N;
+        }
+        double u12[PW][Q1M], u2[PW][Q1M], sxx12[PW], sxx2[PW], sy1[PW], sy2[PW];
+#pragma unroll
+        for (int b = 0; b < PW; ++b) {
+          sxx12[b] = sxx2[b] = sy1[b] = sy2[b] = 0.0;
+#pragma unroll
+          for (int a = 0; a < Q1M; ++a) u12[b][a] = u2[b][a] = 0.0;
+        }
         for (int c0 = 0; c0 < N; c0 += 64) {
           const int n = min(64, N - c0);
           __syncthreads();
           if (lane < n) {
             for (int q = 0; q < Q; ++q) st[q * 64 + lane] = covp[(int64_t)q * N + c0 + lane];
             st[Q * 64 + lane] = yp1[c0 + lane];
-            st[(Q + 1) * 64 + lane] = yp2[c0 + lane];
+#pragma unroll
+            for (int b = 0; b < PW; ++b) st[(Q + 1 + b) * 64 + lane] = yp2[b][c0 + lane];
           }
           __syncthreads();
-          const uint64_t m1 = presp[(int64_t)s1 * W + (c0 >> 6)], m2 = presp[(int64_t)s2 * W + (c0 >> 6)];
+          const uint64_t m1 = presp[(int64_t)s1 * W + (c0 >> 6)];
+          uint64_t m2[PW], any = 0ull;
+#pragma unroll
+          for (int b = 0; b < PW; ++b) {
+            m2[b] = presp[(int64_t)sb[b] * W + (c0 >> 6)];
+            any |= m2[b];
+          }
           for (int j0 = 0; j0 < n; j0 += 8) {
             double xv[8];
 #pragma unroll
@@ -1505,62 +1527,73 @@ __global__ void __launch_bounds__(64) hyb_pair_kernel(const eqbf_chunk c, const 
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
               const int j = j0 + k;
-              if (j >= n || !((m2 >> j) & 1ull)) continue;
+              if (j >= n || !((any >> j) & 1ull)) continue;
               const double x = xv[k];
               double cv[Q1M];
               cv[0] = 1.0;
 #pragma unroll
               for (int a = 1; a < Q1M; ++a) cv[a] = (a < Q1) ? st[(a - 1) * 64 + j] : 0.0;
-              sxx2 += x * x;
+              const bool in1 = (m1 >> j) & 1ull;
 #pragma unroll
-              for (int a = 0; a < Q1M; ++a) u2[a] += cv[a] * x;
-              if ((m1 >> j) & 1ull) {
-                sxx12 += x * x;
-                sy1 += x * st[Q * 64 + j];
-                sy2 += x * st[(Q + 1) * 64 + j];
+              for (int b = 0; b < PW; ++b) {
+                if (!((m2[b] >> j) & 1ull)) continue;
+                sxx2[b] += x * x;
 #pragma unroll
-                for (int a = 0; a < Q1M; ++a) u12[a] += cv[a] * x;
+                for (int a = 0; a < Q1M; ++a) u2[b][a] += cv[a] * x;
+                if (in1) {
+                  sxx12[b] += x * x;
+                  sy1[b] += x * st[Q * 64 + j];
+                  sy2[b] += x * st[(Q + 1 + b) * 64 + j];
+#pragma unroll
+                  for (int a = 0; a < Q1M; ++a) u12[b][a] += cv[a] * x;
+                }
               }
             }
           }
         }
-        const double* R2 = R + (int64_t)(2 * S + pi) * RSZ;
-        const double* R3 = R + (int64_t)(2 * S + npr + pi) * RSZ;
-        const double n12 = R2[2 * Q1M * Q1M + 2 * Q1M + 1];
-        if (ok && !(n12 > 0.0)) atomicOr(hp.flag, FLAG_HYB_NO_COMMON);
-        const double k12 = sxx12 - hyb_quad<Q1M>(R2, u12, u12, Q1);
-        double xty1 = sy1, xty2 = sy2;
 #pragma unroll
-        for (int a = 0; a < Q1M; ++a)
-          if (a < Q1) {
-            xty1 -= u12[a] * R2[2 * Q1M * Q1M + a];
-            xty2 -= u12[a] * R2[2 * Q1M * Q1M + Q1M + a];
+        for (int b = 0; b < PW; ++b) {
+          if (b >= nb) break;
+          const int s2b = s2 + b, pib = pi + b;
+          const double* R2 = R + (int64_t)(2 * S + pib) * RSZ;
+          const double* R3 = R + (int64_t)(2 * S + npr + pib) * RSZ;
+          const double n12 = R2[2 * Q1M * Q1M + 2 * Q1M + 1];
+          if (ok && !(n12 > 0.0)) atomicOr(hp.flag, FLAG_HYB_NO_COMMON);
+          const double k12 = sxx12[b] - hyb_quad<Q1M>(R2, u12[b], u12[b], Q1);
+          double xty1 = sy1[b], xty2 = sy2[b];
+#pragma unroll
+          for (int a = 0; a < Q1M; ++a)
+            if (a < Q1) {
+              xty1 -= u12[b][a] * R2[2 * Q1M * Q1M + a];
+              xty2 -= u12[b][a] * R2[2 * Q1M * Q1M + Q1M + a];
+            }
+          const double s12 = R2[2 * Q1M * Q1M + 2 * Q1M] / n12 - f * xty1 * xty2 / (k12 * n12);
+          double w2[Q1M];
+#pragma unroll
+          for (int a = 0; a < Q1M; ++a) {
+            double r = 0.0;
+#pragma unroll
+            for (int b2 = 0; b2 < Q1M; ++b2)
+              if (a < Q1 && b2 < Q1) r += R3[a * Q1M + b2] * u2[b][b2];
+            w2[a] = r;
           }
-        const double s12 = R2[2 * Q1M * Q1M + 2 * Q1M] / n12 - f * xty1 * xty2 / (k12 * n12);
-        double w2[Q1M];
+          double k2 = sxx2[b], wu = 0.0;
 #pragma unroll
-        for (int a = 0; a < Q1M; ++a) {
-          double r = 0.0;
-#pragma unroll
-          for (int b2 = 0; b2 < Q1M; ++b2)
-            if (a < Q1 && b2 < Q1) r += R3[a * Q1M + b2] * u2[b2];
-          w2[a] = r;
-        }
-        double k2 = sxx2, wu = 0.0;
-#pragma unroll
-        for (int a = 0; a < Q1M; ++a)
-          if (a < Q1) {
-            k2 -= u2[a] * w2[a];
-            wu += w1[a] * u12[a] + u12[a] * w2[a];
+          for (int a = 0; a < Q1M; ++a)
+            if (a < Q1) {
+              k2 -= u2[b][a] * w2[a];
+              wu += w1[a] * u12[b][a] + u12[b][a] * w2[a];
+            }
+          const double aw = hyb_quad<Q1M>(R2 + Q1M * Q1M, w1, w2, Q1);
+          const double a11 = (aw - wu + sxx12[b]) / (k1 * k2);
+          if (ok) {
+            hp.Sig[(int64_t)(s1 * S + s2b) * NP + p] = s12;
+            hp.Sig[(int64_t)(s2b * S + s1) * NP + p] = s12;
+            hp.Vg[(int64_t)(s1 * S + s2b) * NP + p] = s12 * a11;
+            hp.Vg[(int64_t)(s2b * S + s1) * NP + p] = s12 * a11;
           }
-        const double aw = hyb_quad<Q1M>(R2 + Q1M * Q1M, w1, w2, Q1);
-        const double a11 = (aw - wu + sxx12) / (k1 * k2);
-        if (ok) {
-          hp.Sig[(int64_t)(s1 * S + s2) * NP + p] = s12;
-          hp.Sig[(int64_t)(s2 * S + s1) * NP + p] = s12;
-          hp.Vg[(int64_t)(s1 * S + s2) * NP + p] = s12 * a11;
-          hp.Vg[(int64_t)(s2 * S + s1) * NP + p] = s12 * a11;
         }
+        pi += nb;
       }
     }
   }
