@@ -1180,30 +1180,32 @@ __global__ void __launch_bounds__(256) hyb_gene_kernel(const eqbf_chunk c, const
   __shared__ double M[17][34];
   __shared__ double fac[17];
   __shared__ int pivrow;
-  if (t < nE) {
+  /* one wave per entry, lanes striding the individuals */
+  const int lane = t & 63, nwv = blockDim.x >> 6;
+  const double* lev = (type == 0) ? c.explevels : hg.yp;
+  for (int e = t >> 6; e < nE; e += nwv) {
     int kind, a = 0, b = 0;
-    if (t < nA) {
+    if (e < nA) {
       kind = 0;
-      int r = t;
+      int r = e;
       while (r >= Q1 - a) {
         r -= Q1 - a;
         ++a;
       }
       b = a + r;
-    } else if (has_y && t < nA + Q1) {
+    } else if (has_y && e < nA + Q1) {
       kind = 1;
-      a = t - nA;
-    } else if (has_y && t < nA + 2 * Q1) {
+      a = e - nA;
+    } else if (has_y && e < nA + 2 * Q1) {
       kind = 2;
-      a = t - nA - Q1;
-    } else if (has_y && t == nA + 2 * Q1) {
+      a = e - nA - Q1;
+    } else if (has_y && e == nA + 2 * Q1) {
       kind = 3;
     } else {
       kind = 4;
     }
     double acc = 0.0;
-    const double* lev = (type == 0) ? c.explevels : hg.yp;
-    for (int i = 0; i < N; ++i) {
+    for (int i = lane; i < N; i += 64) {
       bool in;
       if (type == 0 || type == 1)
         in = hyb_present(c, lev, g, sa, i);
@@ -1221,7 +1223,8 @@ __global__ void __launch_bounds__(256) hyb_gene_kernel(const eqbf_chunk c, const
         default: acc += 1.0; break;
       }
     }
-    ent[t] = acc;
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if (lane == 0) ent[e] = acc;
   }
   __syncthreads();
   /* [A | I] */
