@@ -35,6 +35,7 @@
  * followed by the BMA kernel of the univariate path (gen-sin, all).
  */
 #include <algorithm>
+#include <type_traits>
 #include <cstring>
 #include <vector>
 
@@ -377,6 +378,24 @@ struct MvCfg {
   double fiterr;
   int cfg0, PSZ;
 };
+
+/* Calls launch(std::integral_constant<int, E>) with E the exact subgroup
+ * count where SM pads it (5..7 in 8, 9..11 in 12), else E = SM: the
+ * per-(pair, configuration) kernels then hold E x E matrices, fewer
+ * registers and no identity-padded LU. */
+template <int SM, class Launch>
+void with_exact_sm(int S, Launch&& launch) {
+  if constexpr (SM == 8) {
+    if (S == 5) return launch(std::integral_constant<int, 5>{});
+    if (S == 6) return launch(std::integral_constant<int, 6>{});
+    if (S == 7) return launch(std::integral_constant<int, 7>{});
+  } else if constexpr (SM == 12) {
+    if (S == 9) return launch(std::integral_constant<int, 9>{});
+    if (S == 10) return launch(std::integral_constant<int, 10>{});
+    if (S == 11) return launch(std::integral_constant<int, 11>{});
+  }
+  launch(std::integral_constant<int, SM>{});
+}
 
 template <int SM>
 __global__ void __launch_bounds__(64) mvlr_cfg_kernel(const eqbf_chunk c, const MvCfg mc) {
@@ -1079,7 +1098,10 @@ int mvlr_run_sm(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_mvlr_opts* o, eqb
     mc.cfg0 = (int)c0;
     mc.PSZ = PSZ;
     ProfScope ps(ctx, "mvlr_cfg_kernel", st);
-    hipLaunchKernelGGL(mvlr_cfg_kernel<SM>, dim3((unsigned)pair_blocks, (unsigned)nb), dim3(64), 0, st, c, mc);
+    with_exact_sm<SM>(c.n_subgroups, [&](auto e) {
+      hipLaunchKernelGGL(mvlr_cfg_kernel<decltype(e)::value>, dim3((unsigned)pair_blocks, (unsigned)nb), dim3(64), 0,
+                         st, c, mc);
+    });
   });
 }
 
@@ -1788,7 +1810,10 @@ int hybrid_run_sm(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_hybrid_opts* o,
     hc.cfg0 = (int)c0;
     hc.PSZ = PSZ;
     ProfScope ps(ctx, "hyb_cfg_kernel", st);
-    hipLaunchKernelGGL(hyb_cfg_kernel<SM>, dim3((unsigned)pair_blocks, (unsigned)nb), dim3(64), 0, st, c, hc);
+    with_exact_sm<SM>(c.n_subgroups, [&](auto e) {
+      hipLaunchKernelGGL(hyb_cfg_kernel<decltype(e)::value>, dim3((unsigned)pair_blocks, (unsigned)nb), dim3(64), 0,
+                         st, c, hc);
+    });
   });
 }
 
