@@ -1064,7 +1064,9 @@ int mvlr_run_sm(eqbf_ctx* ctx, const eqbf_chunk& c, const eqbf_mvlr_opts* o, eqb
   hipLaunchKernelGGL(mv_zero_kernel, dim3(1), dim3(1), 0, st, (int*)ctx->d_flag.ptr);
   {
     ProfScope ps(ctx, "mvlr_prep_kernel", st);
-    hipLaunchKernelGGL(mvlr_prep_kernel<SM>, dim3(G), dim3(64), 0, st, c, pp);
+    with_exact_sm<SM>(c.n_subgroups, [&](auto e) {
+      hipLaunchKernelGGL(mvlr_prep_kernel<decltype(e)::value>, dim3(G), dim3(64), 0, st, c, pp);
+    });
   }
   MvSums ms;
   ms.mrec = pp.mrec;
